@@ -1,0 +1,30 @@
+#!/bin/sh
+# Builds oracle/_ref/libstz_ref.so: the SZ3-free functions of the reference's
+# canonical variant, compiled from the reference source where it lies.
+#
+#   F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5.cpp
+#   F5:1-17      its standard-library #includes
+#   F5:24-1253   globals, split/merge, predictors, pre/de-preprocess, quantizer
+#   F5:1577-1839 ROI / slice planners
+#
+# F5:18-22 (#include of the SZ3-ncp headers) and every function that calls
+# SZ3-ncp (F5:1255-1576, F5:1841-) are left out: SZ3-ncp is not in this image,
+# so the entropy-coding half of the reference cannot be built here.  The lines
+# are read through a pipe; no reference source is written to disk.
+# ref_harness.cpp (appended) only adds extern "C" wrappers that call them.
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+REFROOT=${STZ_REFERENCE_ROOT:-/root/reference}
+F5="$REFROOT/src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5.cpp"
+if [ ! -f "$F5" ]; then
+    echo "reference source not present; oracle/_ref not built" >&2
+    exit 0
+fi
+mkdir -p "$HERE/_ref"
+{
+    echo "#line 1 \"F5\""; sed -n '1,17p' "$F5"
+    echo "#line 24 \"F5\""; sed -n '24,1253p' "$F5"
+    echo "#line 1577 \"F5\""; sed -n '1577,1839p' "$F5"
+    echo "#line 1 \"ref_harness.cpp\""; cat "$HERE/ref_harness.cpp"
+} | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w -o "$HERE/_ref/libstz_ref.so" -
+echo "built $HERE/_ref/libstz_ref.so"
