@@ -1,0 +1,162 @@
+/*
+ * stz.h -- C ABI of the MI355X-native STZ compress/decompress hot path.
+ *
+ * The reference (FabioGrosso/SC25-STZ2) has no library API: its boundary is a
+ * set of internal C++ functions of the canonical program
+ *   F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5.cpp
+ * driven by run_typed<T>() (F5:1954-3038).  Each entry point below names the
+ * reference function(s) it replaces.  All entry points are plain C: pointers,
+ * sizes and integer status codes; no C++ or torch types cross the boundary.
+ *
+ * Memory: every field / output pointer may be host memory or HIP device
+ * memory (detected with hipPointerGetAttributes).  Compressed streams
+ * returned to the caller are host memory owned by the caller (stz_free).
+ *
+ * Threading: a context owns one HIP stream and its memory pool; calls on one
+ * context are serialised by the caller, calls on different contexts are
+ * independent (no process-global state, unlike F5's HuffmanEncoder forced
+ * marker, F5:1264).  The library never prints or exits.
+ */
+#ifndef STZ_H
+#define STZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define STZ_OK 0
+#define STZ_E_ARG -1     /* bad argument (F5:1957-2070 usage errors) */
+#define STZ_E_DIMS -2    /* dims unsupported (F5:1976-2006) */
+#define STZ_E_STREAM -3  /* corrupt / wrong stream (F5:1477-1485, F5:1855-1861) */
+#define STZ_E_OOM -4     /* allocation failed */
+#define STZ_E_HIP -5     /* HIP runtime error */
+#define STZ_E_ZSTD -6    /* host zstd failure */
+#define STZ_E_NODEV -7   /* no usable GPU */
+
+typedef enum { STZ_F32 = 0, STZ_F64 = 1 } stz_dtype; /* F5:3057-3076 -f / -d */
+
+/* STZ_EB_ABS / STZ_EB_REL: eb is the maximum pointwise error (REL: times
+ * the value range, computed in double).  STZ_EB_REF: eb has the meaning of
+ * F5's argv[1]: level bounds are mult_* x eb (F5:2139, F5:2201, F5:2259). */
+typedef enum { STZ_EB_ABS = 0, STZ_EB_REL = 1, STZ_EB_REF = 2 } stz_eb_mode;
+
+typedef struct {
+    double mult_base, mult_low, mult_high; /* F5:2127-2129: 0.7 / 2.7 / 8.2 */
+    int chunk_high, chunk_low;             /* F5:34-35 marker chunks: 32 / 16 */
+    int interp;                            /* base codec: 0 linear, 1 cubic */
+    int order;                             /* base codec axis order 0..5 */
+    double alpha, beta;                    /* base codec level eb: eb/min(alpha^(l-1),beta) */
+    int zstd_level;                        /* host zstd level (F5:3048: SZ3 default 3) */
+    int tuner;                             /* 0 fixed params, 1 learned, 2 exhaustive trial */
+} stz_params;
+
+typedef struct {
+    size_t nx, ny, nz;        /* caller's dims, x fastest */
+    size_t px, py, pz;        /* padded dims (multiples of 4) */
+    stz_dtype dtype;
+    double eb_base, eb_low, eb_high;
+    size_t payload_bytes;     /* device payload (before zstd) */
+    int interp, order;
+    double mult_base, mult_low, mult_high, alpha, beta;
+} stz_info;
+
+typedef struct stz_ctx stz_ctx;
+typedef struct stz_dstream stz_dstream; /* compressed stream resident in HBM */
+
+/* ---- context ----------------------------------------------------------- */
+int stz_create(stz_ctx** ctx, int device);
+void stz_destroy(stz_ctx* ctx);
+/* run subsequent work on a caller-owned hipStream_t (NULL: the library's own) */
+int stz_set_stream(stz_ctx* ctx, void* hip_stream);
+void* stz_get_stream(stz_ctx* ctx);
+/* per-kernel timing with HIP events on the context stream */
+int stz_profile(stz_ctx* ctx, int enable);
+/* JSON {"kernel": [launches, total_ms], ...} of everything since enable */
+int stz_profile_report(stz_ctx* ctx, char* buf, size_t cap);
+void stz_default_params(stz_params* p);
+const char* stz_strerror(int code);
+void stz_free(void* p);
+
+/* ---- whole path ---------------------------------------------------------
+ * stz_compress replaces run_typed's compress half: slice_full_data(_parity)
+ * (F5:2102-2109), SZ_compress4De (F5:1439, base), preprocess_block_zslab +
+ * SZ_compress (F5:2182-2212, low), merge_sub_blocks_to_full (F5:2230),
+ * fused preprocess + quantize_range_nodec + SZ_compress_finish (F5:2252-2311,
+ * high).  The stream is a zstd container of the device payload; the input is
+ * never modified (unlike F5:1312). */
+int stz_compress(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny,
+                 size_t nz, double eb, stz_eb_mode mode, const stz_params* params,
+                 void** stream, size_t* bytes);
+/* replaces SZ_decompress4De + SZ_decompress_separated x14 + depreprocess /
+ * merge (F5:2339-2451) */
+int stz_decompress(stz_ctx* ctx, const void* stream, size_t bytes, void* out);
+/* 3D box, inclusive corners normalised like make_query (F5:1637-1644); a == b
+ * on an axis makes a slice.  Replaces the --auto-query path (F5:2460-2605)
+ * built on SZ_decompress_nopred_marked_chunks (F5:1841-1911). */
+int stz_decompress_roi(stz_ctx* ctx, const void* stream, size_t bytes, int x0, int y0, int z0,
+                       int x1, int y1, int z1, void* out, size_t out_dims[3]);
+/* axis-aligned 2D slice (axis 0=x,1=y,2=z); the disabled z-slice test of
+ * F5:2678-2929 generalised to any axis */
+int stz_decompress_slice(stz_ctx* ctx, const void* stream, size_t bytes, int axis, int index,
+                         void* out, size_t out_dims[3]);
+/* progressive resolution: 0 = 1/8 (base codec stride-2 points), 1 = 1/4
+ * (base, F5:2339), 2 = 1/2 (F5:2392-2394), 3 = full */
+int stz_decompress_level(stz_ctx* ctx, const void* stream, size_t bytes, int level, void* out,
+                         size_t out_dims[3]);
+int stz_query_info(const void* stream, size_t bytes, stz_info* info);
+
+/* ---- device-resident streams (benchmark, serving) ---------------------- */
+int stz_compress_device(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny,
+                        size_t nz, double eb, stz_eb_mode mode, const stz_params* params,
+                        stz_dstream** out);
+int stz_dstream_upload(stz_ctx* ctx, const void* stream, size_t bytes, stz_dstream** out);
+int stz_dstream_download(stz_ctx* ctx, const stz_dstream* ds, int zstd_level, void** stream,
+                         size_t* bytes);
+int stz_dstream_info(const stz_dstream* ds, stz_info* info);
+int stz_dstream_decompress(stz_ctx* ctx, stz_dstream* ds, void* out);
+int stz_dstream_decompress_roi(stz_ctx* ctx, stz_dstream* ds, int x0, int y0, int z0, int x1,
+                               int y1, int z1, void* out, size_t out_dims[3]);
+int stz_dstream_decompress_level(stz_ctx* ctx, stz_dstream* ds, int level, void* out,
+                                 size_t out_dims[3]);
+/* device payload pointer and size (pre-zstd layout) */
+int stz_dstream_payload(const stz_dstream* ds, const void** d_payload, size_t* bytes);
+void stz_dstream_free(stz_dstream* ds);
+
+/* ---- stage entry points (parity tests of single kernels) ----------------- */
+/* slice_full_data (F5:107-122): sub[k] gets dims (nx/2, ny/2, nz/2) */
+int stz_split(stz_ctx* ctx, stz_dtype dtype, const void* full, size_t nx, size_t ny, size_t nz,
+              void* const sub[8]);
+/* merge_sub_blocks_to_full (F5:68-90) */
+int stz_merge(stz_ctx* ctx, stz_dtype dtype, const void* const sub[8], size_t sx, size_t sy,
+              size_t sz, void* full);
+/* preprocess_block_zslab over all slabs (F5:1161-1175): diff = sub - pred(ref) */
+int stz_residual_predict(stz_ctx* ctx, stz_dtype dtype, int block, const void* sub,
+                         const void* ref, size_t dx, size_t dy, size_t dz, void* diff);
+/* depreprocess_block_zslab (F5:1177-1191): out = de + pred(ref) */
+int stz_residual_reconstruct(stz_ctx* ctx, stz_dtype dtype, int block, const void* de,
+                             const void* ref, size_t dx, size_t dy, size_t dz, void* out);
+/* quantize_range_nodec (F5:1204-1253) with radius 32768; codes are uint16;
+ * unpredictable values in element order */
+int stz_quantize(stz_ctx* ctx, stz_dtype dtype, const void* data, size_t n, double eb,
+                 uint16_t* codes, void* unpred, size_t* n_unpred);
+/* decode_marked_transform's recover LUT (F5:1531-1544) */
+int stz_dequantize(stz_ctx* ctx, stz_dtype dtype, const uint16_t* codes, size_t n, double eb,
+                   const void* unpred, void* out);
+/* Huffman code lengths (HuffmanEncoder::preprocess_encode_fast, F5:1271) --
+ * this project's canonical length-limited code */
+int stz_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len);
+/* build_residual_plan / build_low_residual_plan (F5:1689-1830) for full dims
+ * (fx,fy,fz): query[7] = x0 x1 y0 y1 z0 z1 is_slice; per block b (7 rows of
+ * 8 ints): valid x0 x1 y0 y1 z0 z1 nchunks, chunk ids at chunks[b*cap ..] */
+int stz_plan_query(int fx, int fy, int fz, int chunk_high, int chunk_low, int ax, int ay, int az,
+                   int bx, int by, int bz, int* query, int* high_plan, int64_t* high_chunks,
+                   int* low_plan, int64_t* low_chunks, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+// device_math.h -- the STZ per-point arithmetic on gfx950.
+//
+// Bit-exact with the reference's CPU code: the library is compiled with
+// -ffp-contract=off (no FMA contraction, F5 is built without -march so x86
+// has none) and -fgpu-flush-denormals-to-zero (F5:3046-3047 runs with
+// FTZ/DAZ on); every expression keeps the reference's operand order.
+// F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5.cpp
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace stz {
+
+// F5:141-146 cubic_line_pred
+template <typename T>
+__device__ __forceinline__ T cub(T a, T b, T c, T d)
+{
+    return T(-0.0625f) * a + T(0.5625f) * b + T(0.5625f) * c - T(0.0625f) * d;
+}
+
+// F5:163-175 cubic_line_pv_cse (default build)
+template <typename T>
+__device__ __forceinline__ void pv(T a, T b, T c, T d, T& p, T& v)
+{
+    const T eps = T(1e-12f);
+    const T c0 = a - T(2.0f) * b + c;
+    const T c1 = b - T(2.0f) * c + d;
+    v = T(0.5f) * (c0 * c0 + c1 * c1) + eps;
+    p = T(0.5f) * (b + c) - T(0.0625f) * (c0 + c1);
+}
+
+// F5:177-186
+template <typename T>
+__device__ __forceinline__ T blend2(T p0, T v0, T p1, T v1)
+{
+    return (p0 * v1 + p1 * v0) / (v0 + v1);
+}
+
+// F5:188-203
+template <typename T>
+__device__ __forceinline__ T blend4(T p0, T v0, T p1, T v1, T p2, T v2, T p3, T v3)
+{
+    const T w0 = v1 * v2 * v3;
+    const T w1 = v0 * v2 * v3;
+    const T w2 = v0 * v1 * v3;
+    const T w3 = v0 * v1 * v2;
+    return (w0 * p0 + w1 * p1 + w2 * p2 + w3 * p3) / (w0 + w1 + w2 + w3);
+}
+
+// F5:690-748 predict_block_const + F5:624-688 dynamic lines; interior points
+// evaluate exactly what predict_interior_const (F5:828-840) evaluates.
+template <int B, typename T>
+__device__ __forceinline__ T predict(const T* __restrict__ r, int i, int x, int y, int z, int dx,
+                                     int dy, int dz)
+{
+    const int sx = dx, sxy = dx * dy;
+    if constexpr (B == 0) {
+        if (x == dx - 1) return r[i];
+        if (x == dx - 2 || x == 0) return T(0.5f) * (r[i] + r[i + 1]);
+        return cub(r[i - 1], r[i], r[i + 1], r[i + 2]);
+    } else if constexpr (B == 1) {
+        if (y == dy - 1) return r[i];
+        if (y == dy - 2 || y == 0) return T(0.5f) * (r[i] + r[i + sx]);
+        return cub(r[i - sx], r[i], r[i + sx], r[i + 2 * sx]);
+    } else if constexpr (B == 3) {
+        if (z == dz - 1) return r[i];
+        if (z == dz - 2 || z == 0) return T(0.5f) * (r[i] + r[i + sxy]);
+        return cub(r[i - sxy], r[i], r[i + sxy], r[i + 2 * sxy]);
+    } else if constexpr (B == 2) {
+        if (x == dx - 1 || y == dy - 1) return r[i];
+        if (y == dy - 2 || y == 0 || x == dx - 2 || x == 0)
+            return T(0.25f) * (r[i] + r[i + sx + 1] + r[i + sx] + r[i + 1]);
+        T p0, v0, p1, v1;
+        pv(r[i - sx - 1], r[i], r[i + sx + 1], r[i + 2 * sx + 2], p0, v0);
+        pv(r[i - sx + 2], r[i + 1], r[i + sx], r[i + 2 * sx - 1], p1, v1);
+        return blend2(p0, v0, p1, v1);
+    } else if constexpr (B == 4) {
+        if (x == dx - 1 || z == dz - 1) return r[i];
+        if (z == dz - 2 || z == 0 || x == dx - 2 || x == 0)
+            return T(0.25f) * (r[i] + r[i + sxy + 1] + r[i + 1] + r[i + sxy]);
+        T p0, v0, p1, v1;
+        pv(r[i - sxy - 1], r[i], r[i + sxy + 1], r[i + 2 * sxy + 2], p0, v0);
+        pv(r[i - sxy + 2], r[i + 1], r[i + sxy], r[i + 2 * sxy - 1], p1, v1);
+        return blend2(p0, v0, p1, v1);
+    } else if constexpr (B == 5) {
+        if (z == dz - 1 || y == dy - 1) return r[i];
+        if (y == dy - 2 || y == 0 || z == dz - 2 || z == 0)
+            return T(0.25f) * (r[i] + r[i + sxy + sx] + r[i + sxy] + r[i + sx]);
+        T p0, v0, p1, v1;
+        pv(r[i - sxy - sx], r[i], r[i + sxy + sx], r[i + 2 * sxy + 2 * sx], p0, v0);
+        pv(r[i - sxy + 2 * sx], r[i + sx], r[i + sxy], r[i + 2 * sxy - sx], p1, v1);
+        return blend2(p0, v0, p1, v1);
+    } else {
+        if (x == dx - 1 || y == dy - 1 || z == dz - 1) return r[i];
+        if (z == dz - 2 || z == 0 || y == dy - 2 || y == 0 || x == dx - 2 || x == 0)
+            return T(0.125f) * (r[i] + r[i + sxy + sx + 1] + r[i + sxy + sx] + r[i + sxy + 1] +
+                                r[i + sx + 1] + r[i + 1] + r[i + sx] + r[i + sxy]);
+        T p0, v0, p1, v1, p2, v2, p3, v3;
+        pv(r[i - sxy - sx - 1], r[i], r[i + sxy + sx + 1], r[i + 2 * sxy + 2 * sx + 2], p0, v0);
+        pv(r[i - sxy - sx + 2], r[i + 1], r[i + sxy + sx], r[i + 2 * sxy + 2 * sx - 1], p1, v1);
+        pv(r[i - sxy + 2 * sx - 1], r[i + sx], r[i + sxy + 1], r[i + 2 * sxy - sx + 2], p2, v2);
+        pv(r[i + 2 * sxy - sx - 1], r[i + sxy], r[i + sx + 1], r[i - sxy + 2 * sx + 2], p3, v3);
+        return blend4(p0, v0, p1, v1, p2, v2, p3, v3);
+    }
+}
+
+// runtime block index -> compile-time predictor
+template <typename T>
+__device__ __forceinline__ T predict_rt(int b, const T* __restrict__ r, int i, int x, int y, int z,
+                                        int dx, int dy, int dz)
+{
+    switch (b) {
+    case 0: return predict<0>(r, i, x, y, z, dx, dy, dz);
+    case 1: return predict<1>(r, i, x, y, z, dx, dy, dz);
+    case 2: return predict<2>(r, i, x, y, z, dx, dy, dz);
+    case 3: return predict<3>(r, i, x, y, z, dx, dy, dz);
+    case 4: return predict<4>(r, i, x, y, z, dx, dy, dz);
+    case 5: return predict<5>(r, i, x, y, z, dx, dy, dz);
+    default: return predict<6>(r, i, x, y, z, dx, dy, dz);
+    }
+}
+
+// Linear quantizer, bit-exact with F5:1345-1397 / F5:1204-1253 (itself
+// bit-exact with SZ3 LinearQuantizer::quantize_and_overwrite(d, 0)).
+// Returns the code (0 = unpredictable) and the decompressed value (d itself
+// when unpredictable).
+template <typename T>
+__device__ __forceinline__ uint32_t quantize(T d, double eb, double recip, T& dec_out)
+{
+    const double ad = fabs((double)d) * recip;
+    if (!(ad < (double)(2 * kRadius - 1))) { dec_out = d; return 0u; }
+    const int half = ((int)ad + 1) >> 1;
+    T dec = (T)((double)(2 * half) * eb);  // F5:1353-1356 dec_lut[h]
+    if (d < T(0)) dec = -dec;
+    const T derr = dec - d;
+    if (!(fabs((double)derr) <= eb)) { dec_out = d; return 0u; }
+    dec_out = dec;
+    return (uint32_t)((d < T(0)) ? kRadius - half : kRadius + half);
+}
+
+// F5:1531-1535 rec_lut[q] = (T)(2*(q-r)*eb)
+template <typename T>
+__device__ __forceinline__ T dequantize(uint32_t q, double eb)
+{
+    return (T)((double)(2 * ((int)q - kRadius)) * eb);
+}
+
+}  // namespace stz

@@ -1,0 +1,557 @@
+// k_huff.hip -- entropy-coding kernels of the STZ device path (gfx950).
+//
+// Replaces SZ3-ncp's HuffmanEncoder<int> as the reference calls it:
+// preprocess_encode_fast (histogram + tree, F5:1271 / F5:1402),
+// encode_marked_fast (chunk-marked bit packing, F5:1292 / F5:1412),
+// decode_marked_transform / decode_marked_chunks_fast (F5:1536, F5:1872).
+// The tree itself is built on the host from the compacted histogram; all
+// per-symbol work runs here:
+//   histogram   fused into the quantize kernels (wave-aggregated LDS bins)
+//   encode      per-sub-chunk code-length sums -> per-stream chunk scan ->
+//               wavefront bit-pack (16 symbols per lane, LDS word assembly,
+//               atomics only on the two words a sub-chunk may share)
+//   decode      one lane per 1024-symbol sub-chunk, 12-bit LDS lookup table,
+//               canonical fallback for longer codes
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace stz {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t t = (uint64_t)__shfl_up((long long)v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) v += (uint32_t)__shfl_xor((int)v, d, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int find_stream_by_chunk(const StreamSet& ss, uint32_t gc)
+{
+    int s = 0;
+    for (int k = 1; k < (int)ss.nstreams; ++k)
+        if (gc >= ss.s[k].chunk_base) s = k;
+    return s;
+}
+
+__device__ __forceinline__ uint64_t chunk_start(const Geom& g, uint32_t c)
+{
+    if (g.chunk <= 0) return (uint64_t)c * kBaseChunk;
+    const int C = g.chunk;
+    const int cx = (int)(c % g.ncx), cy = (int)((c / g.ncx) % g.ncy), cz = (int)(c / ((uint32_t)g.ncx * g.ncy));
+    const int wy = imin_(C, g.dy - cy * C), wz = imin_(C, g.dz - cz * C);
+    return (uint64_t)cz * C * g.dx * g.dy + (uint64_t)cy * C * g.dx * wz + (uint64_t)cx * C * wy * wz;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_zero_bits(StreamSet ss, uint8_t* __restrict__ payload)
+{
+    const StreamDesc& d = ss.s[blockIdx.y];
+    uint32_t* w = (uint32_t*)(payload + d.off_bits);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < d.bits_words;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+// one workgroup per stream: nonzero histogram bins -> (symbol << 32 | freq), symbol order
+__global__ __launch_bounds__(1024) void k_hist_compact(const uint32_t* __restrict__ hist,
+                                                       uint64_t* __restrict__ out,
+                                                       uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t wsum[16];
+    const int s = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+    const uint32_t* h = hist + (size_t)s * 65536 + tid * 64;
+    uint32_t v[64];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 64; k += 4) {
+        const uint4 q = *(const uint4*)(h + k);
+        v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 64; ++k) cnt += v[k] != 0u;
+    const uint32_t inc = wave_incl_scan_u32(cnt);
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t t = lane < 16 ? wsum[lane] : 0u;
+        t = wave_incl_scan_u32(t);
+        if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    uint32_t idx = inc - cnt + (wid ? wsum[wid - 1] : 0u);
+    uint64_t* o = out + (size_t)s * 65536;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+        if (v[k]) o[idx++] = ((uint64_t)(tid * 64 + k) << 32) | v[k];
+    if (tid == 0) counts[s] = wsum[15];
+}
+
+__global__ void k_enc_scatter(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ enc)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = list[2 * i], val = list[2 * i + 1];
+    enc[(size_t)(key >> 16) * 65536 + (key & 0xFFFFu)] = val;
+}
+
+// per chunk: bits and unpredictable count of each sub-chunk, in-chunk prefixes, chunk totals
+__global__ __launch_bounds__(256) void k_huff_len(StreamSet ss, const uint16_t* __restrict__ codes,
+                                                  const uint32_t* __restrict__ enc,
+                                                  uint8_t* __restrict__ payload,
+                                                  uint32_t* __restrict__ sub_pre_bits,
+                                                  uint32_t* __restrict__ sub_pre_un,
+                                                  uint64_t* __restrict__ chunk_tot_bits,
+                                                  uint64_t* __restrict__ chunk_tot_un)
+{
+    __shared__ uint32_t lb[256], lu[256];
+    const uint32_t gc = blockIdx.x;
+    const int s = find_stream_by_chunk(ss, gc);
+    const StreamDesc& d = ss.s[s];
+    const uint32_t c = gc - d.chunk_base;
+    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
+    const uint32_t sb = csub[c], nsub = csub[c + 1] - sb;
+    const uint64_t pstart = chunk_start(d.g, c);
+    const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
+    const uint16_t* cs = codes + d.code_off;
+    const uint32_t* es = enc + (size_t)s * 65536;
+    uint16_t* sbits = (uint16_t*)(payload + d.off_sub_bits);
+    uint16_t* sun = (uint16_t*)(payload + d.off_sub_unpred);
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    for (uint32_t j = wid; j < nsub; j += 4) {
+        const uint64_t p0 = pstart + (uint64_t)j * kSub;
+        const uint64_t p1 = p0 + kSub < pend ? p0 + kSub : pend;
+        uint32_t bits = 0, un = 0;
+        for (uint64_t p = p0 + lane; p < p1; p += kWave) {
+            const uint32_t q = cs[p];
+            bits += es[q] >> 24;
+            un += q == 0u;
+        }
+        bits = wave_sum_u32(bits);
+        un = wave_sum_u32(un);
+        if (lane == 0) {
+            lb[j] = bits;
+            lu[j] = un;
+            sbits[sb + j] = (uint16_t)bits;
+            sun[sb + j] = (uint16_t)un;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ab = 0, au = 0;
+        for (uint32_t j = 0; j < nsub; ++j) {
+            sub_pre_bits[d.sub_base + sb + j] = ab;
+            sub_pre_un[d.sub_base + sb + j] = au;
+            ab += lb[j];
+            au += lu[j];
+        }
+        chunk_tot_bits[gc] = ab;
+        chunk_tot_un[gc] = au;
+    }
+}
+
+// per stream: exclusive scan of chunk totals -> marker table (chunk bit offsets)
+__global__ __launch_bounds__(1024) void k_huff_scan(StreamSet ss, const uint64_t* __restrict__ tb,
+                                                    const uint64_t* __restrict__ tu,
+                                                    uint8_t* __restrict__ payload)
+{
+    __shared__ uint64_t wsb[16], wsu[16];
+    const StreamDesc& d = ss.s[blockIdx.x];
+    const uint32_t n = d.n_chunks;
+    const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t c0 = tid * per < n ? tid * per : n;
+    const uint32_t c1 = c0 + per < n ? c0 + per : n;
+    uint64_t sbits = 0, sun = 0;
+    for (uint32_t c = c0; c < c1; ++c) {
+        sbits += tb[d.chunk_base + c];
+        sun += tu[d.chunk_base + c];
+    }
+    const uint64_t ib = wave_incl_scan_u64(sbits), iu = wave_incl_scan_u64(sun);
+    if (lane == kWave - 1) { wsb[wid] = ib; wsu[wid] = iu; }
+    __syncthreads();
+    if (wid == 0) {
+        uint64_t a = lane < 16 ? wsb[lane] : 0, b = lane < 16 ? wsu[lane] : 0;
+        a = wave_incl_scan_u64(a);
+        b = wave_incl_scan_u64(b);
+        if (lane < 16) { wsb[lane] = a; wsu[lane] = b; }
+    }
+    __syncthreads();
+    uint64_t ab = ib - sbits + (wid ? wsb[wid - 1] : 0), au = iu - sun + (wid ? wsu[wid - 1] : 0);
+    uint64_t* cbit = (uint64_t*)(payload + d.off_chunk_bit);
+    uint64_t* cun = (uint64_t*)(payload + d.off_chunk_unpred);
+    for (uint32_t c = c0; c < c1; ++c) {
+        cbit[c] = ab;
+        cun[c] = au;
+        ab += tb[d.chunk_base + c];
+        au += tu[d.chunk_base + c];
+    }
+    if (tid == 0) {
+        cbit[n] = wsb[15];
+        cun[n] = wsu[15];
+    }
+}
+
+// per chunk: wavefront bit-pack of each sub-chunk + its unpredictable values
+__global__ __launch_bounds__(256) void k_huff_pack(StreamSet ss, const uint16_t* __restrict__ codes,
+                                                   const uint8_t* __restrict__ side,
+                                                   const uint32_t* __restrict__ enc,
+                                                   const uint32_t* __restrict__ sub_pre_bits,
+                                                   const uint32_t* __restrict__ sub_pre_un,
+                                                   uint8_t* __restrict__ payload)
+{
+    constexpr int kWords = (31 + kSub * kLmax + 31) / 32 + 2;  // 772
+    __shared__ uint32_t wbuf[4][kWords];
+    const uint32_t gc = blockIdx.x;
+    const int s = find_stream_by_chunk(ss, gc);
+    const StreamDesc& d = ss.s[s];
+    const uint32_t c = gc - d.chunk_base;
+    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
+    const uint32_t sb = csub[c], nsub = csub[c + 1] - sb;
+    const uint64_t cbit0 = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
+    const uint64_t cun0 = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
+    const uint64_t pstart = chunk_start(d.g, c);
+    const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
+    const uint16_t* cs = codes + d.code_off;
+    const uint32_t* es = enc + (size_t)s * 65536;
+    uint32_t* bw = (uint32_t*)(payload + d.off_bits);
+    uint8_t* un_out = payload + d.off_unpred;
+    const uint8_t* side_s = side + d.code_off * d.tsize;
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    uint32_t* wb = wbuf[wid];
+    const uint32_t rounds = (nsub + 3) / 4;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t j = r * 4 + wid;
+        const bool active = j < nsub;
+        uint32_t q[16];
+        uint32_t e[16];
+        uint32_t lbits = 0, lzero = 0, nW = 0, off = 0;
+        uint64_t bit0 = 0, u0 = 0, pl = 0, p1 = 0;
+        uint32_t eb_ = 0, eu_ = 0;
+        if (active) {
+            const uint32_t gs = d.sub_base + sb + j;
+            bit0 = cbit0 + sub_pre_bits[gs];
+            u0 = cun0 + sub_pre_un[gs];
+            const uint64_t p0 = pstart + (uint64_t)j * kSub;
+            p1 = p0 + kSub < pend ? p0 + kSub : pend;
+            pl = p0 + (uint64_t)lane * 16;
+            if (((d.code_off + pl) & 7) == 0 && pl + 16 <= p1) {
+                const uint4 a = *(const uint4*)(cs + pl);
+                const uint4 b = *(const uint4*)(cs + pl + 8);
+                const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    q[2 * k] = w8[k] & 0xFFFFu;
+                    q[2 * k + 1] = w8[k] >> 16;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) q[k] = pl + k < p1 ? (uint32_t)cs[pl + k] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const bool in = pl + k < p1;
+                e[k] = in ? es[q[k]] : 0u;
+                lbits += e[k] >> 24;
+                lzero += (in && q[k] == 0u) ? 1u : 0u;
+            }
+            const uint32_t ib = wave_incl_scan_u32(lbits);
+            const uint32_t iu = wave_incl_scan_u32(lzero);
+            const uint32_t tot = __shfl((int)ib, kWave - 1, kWave);
+            eb_ = ib - lbits;
+            eu_ = iu - lzero;
+            off = (uint32_t)(bit0 & 31u);
+            nW = (off + tot + 31) >> 5;
+            for (uint32_t w = lane; w < nW; w += kWave) wb[w] = 0u;
+        }
+        __syncthreads();
+        if (active) {
+            // assemble whole words in registers; the first may be shared with the previous lane
+            uint32_t pos = off + eb_;
+            uint32_t wcur = pos >> 5;
+            uint64_t acc = 0;
+            int nacc = (int)(pos & 31u);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int l = (int)(e[k] >> 24);
+                acc = (acc << l) | (e[k] & 0xFFFFFFu);
+                nacc += l;
+                if (nacc >= 32) {
+                    atomicOr(&wb[wcur], (uint32_t)(acc >> (nacc - 32)));
+                    wcur++;
+                    nacc -= 32;
+                    acc &= nacc ? ((1ull << nacc) - 1ull) : 0ull;
+                }
+            }
+            if (nacc > 0) atomicOr(&wb[wcur], (uint32_t)(acc << (32 - nacc)));
+            // unpredictable values of this lane, in stream order
+            uint64_t u = u0 + eu_;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (pl + k < p1 && q[k] == 0u) {
+                    if (d.tsize == 4)
+                        ((uint32_t*)un_out)[u] = ((const uint32_t*)side_s)[pl + k];
+                    else
+                        ((uint64_t*)un_out)[u] = ((const uint64_t*)side_s)[pl + k];
+                    u++;
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const uint64_t W0 = bit0 >> 5;
+            for (uint32_t w = lane; w < nW; w += kWave) {
+                const uint32_t v = wb[w];
+                if (w == 0 || w == nW - 1) {
+                    if (v) atomicOr(&bw[W0 + w], v);
+                } else {
+                    bw[W0 + w] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int find_stream_by_item(const StreamSet& ss, uint32_t item)
+{
+    int s = -1;
+    for (int k = 0; k < (int)ss.nstreams; ++k)
+        if (ss.s[k].item_count && item >= ss.s[k].item_base &&
+            item < ss.s[k].item_base + ((ss.s[k].item_count + 255u) & ~255u))
+            s = k;
+    return s;
+}
+
+// expand chunks into 1024-symbol decode work items
+__global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
+                           const uint32_t* __restrict__ list, uint32_t nlist,
+                           DecItem* __restrict__ items)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s, c, ib;
+    if (list) {
+        if (i >= nlist) return;
+        s = list[3 * i];
+        c = list[3 * i + 1];
+        ib = list[3 * i + 2];
+    } else {
+        if (i >= ss.total_chunks) return;
+        s = (uint32_t)find_stream_by_chunk(ss, i);
+        c = i - ss.s[s].chunk_base;
+        ib = ss.s[s].item_base + ((const uint32_t*)(payload + ss.s[s].off_chunk_sub))[c];
+    }
+    const StreamDesc& d = ss.s[s];
+    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
+    const uint16_t* sbits = (const uint16_t*)(payload + d.off_sub_bits);
+    const uint16_t* sun = (const uint16_t*)(payload + d.off_sub_unpred);
+    uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
+    uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
+    const uint32_t sb = csub[c], se = csub[c + 1];
+    const uint64_t pstart = chunk_start(d.g, c);
+    const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
+    for (uint32_t j = 0; j < se - sb; ++j) {
+        DecItem it;
+        it.bit0 = bit;
+        it.u0 = u;
+        it.p0 = (uint32_t)(pstart + (uint64_t)j * kSub);
+        const uint64_t rem = vol - (uint64_t)j * kSub;
+        it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
+        it.stream = s;
+        it.pad_ = 0;
+        items[ib + j] = it;
+        bit += sbits[sb + j];
+        u += sun[sb + j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
+                                                     const DecTab* __restrict__ tabs,
+                                                     const uint16_t* __restrict__ sorted,
+                                                     const DecItem* __restrict__ items,
+                                                     uint16_t* __restrict__ codes,
+                                                     uint8_t* __restrict__ side,
+                                                     uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
+    const uint32_t first = blockIdx.x * 256u;
+    const int s = find_stream_by_item(ss, first);
+    if (s < 0) return;  // uniform across the workgroup: padding blocks do nothing
+    const DecTab* t = tabs + s;
+    for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
+    if (threadIdx.x < kLmax + 2) {
+        fc[threadIdx.x] = t->first_code[threadIdx.x];
+        cn[threadIdx.x] = t->count[threadIdx.x];
+        fi[threadIdx.x] = t->first_index[threadIdx.x];
+    }
+    __syncthreads();
+    const StreamDesc& d = ss.s[s];
+    const uint32_t item = first + threadIdx.x;
+    if (item >= d.item_base + d.item_count) return;
+    const DecItem it = items[item];
+    const uint32_t* bw = (const uint32_t*)(payload + d.off_bits);
+    const uint8_t* un = payload + d.off_unpred;
+    const uint16_t* srt = sorted + t->sorted_off;
+    uint16_t* co = codes + d.code_off + it.p0;
+    uint8_t* so = side + (d.code_off + it.p0) * d.tsize;
+    uint64_t wpos = it.bit0 >> 5;
+    const int skip = (int)(it.bit0 & 31u);
+    uint32_t bad = 0;
+    if (wpos + 1 >= d.bits_words) {  // offsets are validated on upload; stay in bounds regardless
+        atomicOr(err, 1u);
+        return;
+    }
+    uint64_t buf = (((uint64_t)bw[wpos] << 32) | bw[wpos + 1]) << skip;
+    int nb = 64 - skip;
+    wpos += 2;
+    uint64_t u = it.u0;
+    const uint64_t nwords = d.bits_words;
+    for (uint32_t i = 0; i < it.cnt; ++i) {
+        if (nb < 32) {
+            // reads past the end only happen on a corrupt stream: feed zeros and flag it
+            const uint32_t w = wpos < nwords ? bw[wpos] : 0u;
+            bad |= wpos >= nwords ? 1u : 0u;
+            wpos++;
+            buf |= (uint64_t)w << (32 - nb);
+            nb += 32;
+        }
+        const uint32_t e = lut[(uint32_t)(buf >> (64 - kLutBits))];
+        uint32_t sym;
+        int l;
+        if (e != 0xFFFFFFFFu) {
+            sym = e & 0xFFFFu;
+            l = (int)(e >> 16);
+        } else {
+            const uint32_t win = (uint32_t)(buf >> 32);
+            for (l = kLutBits + 1; l <= kLmax; ++l)
+                if ((win >> (32 - l)) - fc[l] < cn[l]) break;
+            if (l > kLmax) {
+                bad = 1;
+                l = 1;
+                sym = 0;
+            } else {
+                sym = srt[fi[l] + (win >> (32 - l)) - fc[l]];
+            }
+        }
+        buf <<= l;
+        nb -= l;
+        co[i] = (uint16_t)sym;
+        if (sym == 0u) {
+            if (u >= d.n_unpred) {
+                bad = 1;
+                u = 0;
+                if (d.n_unpred == 0) continue;
+            }
+            if (d.tsize == 4)
+                ((uint32_t*)so)[i] = ((const uint32_t*)un)[u];
+            else
+                ((uint64_t*)so)[i] = ((const uint64_t*)un)[u];
+            u++;
+        }
+    }
+    if (bad) atomicOr(err, 1u);
+}
+
+}  // namespace
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v)
+{
+    if (!n) return;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    k_fill_u32<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n, v);
+}
+
+void launch_zero_bits(hipStream_t s, const StreamSet& ss, uint8_t* payload)
+{
+    uint64_t mx = 0;
+    for (uint32_t k = 0; k < ss.nstreams; ++k) mx = ss.s[k].bits_words > mx ? ss.s[k].bits_words : mx;
+    uint64_t bx = (mx + 255) / 256;
+    if (bx > 2048) bx = 2048;
+    if (bx == 0) return;
+    k_zero_bits<<<dim3((unsigned)bx, ss.nstreams), dim3(256), 0, s>>>(ss, payload);
+}
+
+void launch_hist_compact(hipStream_t s, const uint32_t* hist, int nstreams, uint64_t* out, uint32_t* counts)
+{
+    k_hist_compact<<<dim3(nstreams), dim3(1024), 0, s>>>(hist, out, counts);
+}
+
+void launch_enc_scatter(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* enc)
+{
+    if (!n) return;
+    k_enc_scatter<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(list, n, enc);
+}
+
+void launch_huff_len(hipStream_t s, const StreamSet& ss, const uint16_t* codes, const uint32_t* enc,
+                     uint8_t* payload, uint32_t* sub_pre_bits, uint32_t* sub_pre_un,
+                     uint64_t* chunk_tot_bits, uint64_t* chunk_tot_un)
+{
+    k_huff_len<<<dim3(ss.total_chunks), dim3(256), 0, s>>>(ss, codes, enc, payload, sub_pre_bits,
+                                                            sub_pre_un, chunk_tot_bits, chunk_tot_un);
+}
+
+void launch_huff_scan(hipStream_t s, const StreamSet& ss, const uint64_t* tb, const uint64_t* tu,
+                      uint8_t* payload)
+{
+    k_huff_scan<<<dim3(ss.nstreams), dim3(1024), 0, s>>>(ss, tb, tu, payload);
+}
+
+void launch_huff_pack(hipStream_t s, const StreamSet& ss, const uint16_t* codes, const void* side,
+                      const uint32_t* enc, const uint32_t* sub_pre_bits, const uint32_t* sub_pre_un,
+                      uint8_t* payload)
+{
+    k_huff_pack<<<dim3(ss.total_chunks), dim3(256), 0, s>>>(ss, codes, (const uint8_t*)side, enc,
+                                                             sub_pre_bits, sub_pre_un, payload);
+}
+
+void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const uint32_t* list,
+                     uint32_t nlist, DecItem* items)
+{
+    const uint32_t n = list ? nlist : ss.total_chunks;
+    if (!n) return;
+    k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items);
+}
+
+void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,
+                        const uint16_t* sorted, const DecItem* items, uint16_t* codes, void* side,
+                        uint32_t* err)
+{
+    if (!ss.total_items) return;
+    k_huff_decode<<<dim3(ss.total_items / 256), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items,
+                                                                   codes, (uint8_t*)side, err);
+}
+
+}  // namespace stz

@@ -1,0 +1,146 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace stz {
+
+// Per-stream view used by the entropy-coding kernels.  Offsets *_off are byte
+// offsets into the payload buffer; code_off is the element offset of the
+// stream inside the codes / side arrays.
+struct StreamDesc {
+    uint64_t code_off;
+    uint64_t n_symbols;
+    uint64_t off_chunk_bit, off_chunk_unpred, off_chunk_sub;
+    uint64_t off_sub_bits, off_sub_unpred, off_unpred, off_bits;
+    uint64_t bits_words;     // words reserved for the bitstream (incl. padding)
+    Geom g;
+    uint32_t n_chunks, n_sub;
+    uint32_t chunk_base;     // first global chunk id of this stream
+    uint32_t sub_base;       // first global sub-chunk id of this stream
+    uint32_t item_base;      // first decode work item of this stream (256-aligned)
+    uint32_t item_count;     // decode work items of this stream
+    uint32_t tsize;          // bytes per raw value (4 or 8)
+    uint32_t pad_;
+    uint64_t n_unpred;       // unpredictable values stored in the stream
+};
+
+struct StreamSet {
+    StreamDesc s[kStreams];
+    uint32_t total_chunks;
+    uint32_t total_sub;
+    uint32_t total_items;    // padded to 256 per stream
+    uint32_t nstreams;
+};
+
+// decode table of one stream (device memory)
+struct DecTab {
+    uint32_t lut[1 << kLutBits];       // (len << 16) | sym, or 0xFFFFFFFF
+    uint32_t first_code[kLmax + 2];
+    uint32_t count[kLmax + 2];
+    uint32_t first_index[kLmax + 2];
+    uint32_t sorted_off;               // element offset of the sorted-symbol list
+    uint32_t pad_[3];
+};
+
+struct DecItem {
+    uint64_t bit0;     // absolute bit offset in the stream's bitstream
+    uint64_t u0;       // index of the first unpredictable value
+    uint32_t p0;       // stream position of the first symbol
+    uint32_t cnt;      // symbols
+    uint32_t stream;
+    uint32_t pad_;
+};
+
+// ---- k_huff.hip ----
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
+void launch_hist_compact(hipStream_t s, const uint32_t* hist, int nstreams, uint64_t* out,
+                         uint32_t* counts);
+void launch_enc_scatter(hipStream_t s, const uint32_t* list, uint32_t n, uint32_t* enc);
+void launch_huff_len(hipStream_t s, const StreamSet& ss, const uint16_t* codes, const uint32_t* enc,
+                     uint8_t* payload, uint32_t* sub_pre_bits, uint32_t* sub_pre_un,
+                     uint64_t* chunk_tot_bits, uint64_t* chunk_tot_un);
+void launch_huff_scan(hipStream_t s, const StreamSet& ss, const uint64_t* chunk_tot_bits,
+                      const uint64_t* chunk_tot_un, uint8_t* payload);
+void launch_huff_pack(hipStream_t s, const StreamSet& ss, const uint16_t* codes, const void* side,
+                      const uint32_t* enc, const uint32_t* sub_pre_bits, const uint32_t* sub_pre_un,
+                      uint8_t* payload);
+void launch_zero_bits(hipStream_t s, const StreamSet& ss, uint8_t* payload);
+// chunk list: nullptr = every chunk of every stream; else (stream, chunk, item_base) triples
+void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
+                     const uint32_t* list, uint32_t nlist, DecItem* items);
+void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
+                        const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
+                        uint16_t* codes, void* side, uint32_t* err);
+
+// ---- k_field.hip ----
+template <typename T>
+void launch_minmax(hipStream_t s, const T* d, uint64_t n, double* partial, double* out);
+template <typename T>
+void launch_pad(hipStream_t s, const T* in, int nx, int ny, int nz, T* out, int px, int py, int pz);
+template <typename T>
+void launch_gather_low0(hipStream_t s, const T* full, int px, int py, int pz, T* v);
+template <typename T>
+void launch_split(hipStream_t s, const T* full, int nx, int ny, int nz, T* const* sub);
+template <typename T>
+void launch_merge(hipStream_t s, const T* const* sub, int sx, int sy, int sz, T* full);
+template <typename T>
+void launch_residual_predict(hipStream_t s, int block, const T* sub, const T* ref, int dx, int dy,
+                             int dz, T* out, int sign);
+template <typename T>
+void launch_quantize(hipStream_t s, const T* d, uint64_t n, double eb, uint16_t* codes);
+template <typename T>
+void launch_dequantize(hipStream_t s, const uint16_t* codes, const T* side, uint64_t n, double eb,
+                       T* out);
+template <typename T>
+void launch_crop(hipStream_t s, const T* in, int px, int py, int ox, int oy, int oz, int sx, int sy,
+                 int sz, int step, T* out);
+
+// ---- k_residual.hip ----
+struct LevelArgs {
+    double eb, recip;
+    uint64_t code_off[7];   // element offsets of the 7 block streams in codes/side
+};
+template <typename T>
+void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, const T* base,
+                       Geom g, LevelArgs a, uint16_t* codes, T* side, uint32_t* hist, T* mid);
+template <typename T>
+void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, const T* mid, Geom g,
+                        LevelArgs a, uint16_t* codes, T* side, uint32_t* hist);
+template <typename T>
+void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
+                       const T* side, T* mid);
+template <typename T>
+void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
+                        const T* side, T* out, int nx, int ny, int nz);
+// ROI: mid values on a box (mid coords, inclusive) -> mid array (full mid dims)
+template <typename T>
+void launch_low_decode_box(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
+                           const T* side, T* mid, int mx, int my, int x0, int x1, int y0, int y1,
+                           int z0, int z1);
+// ROI: full-resolution box (inclusive) -> dense output
+template <typename T>
+void launch_high_decode_box(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
+                            const T* side, T* out, int x0, int x1, int y0, int y1, int z0, int z1);
+
+// ---- k_interp.hip ----
+struct InterpPass {
+    int start[3], step[3], cnt[3];
+    int axis;        // -1: anchor (pred 0)
+    int s;           // stride along the axis
+    int interp;
+    int bx, by, bz;
+    double eb, recip;
+    uint64_t pos;    // first code position of the pass in the base stream
+    uint64_t n;      // points in the pass
+};
+template <typename T>
+void launch_interp_encode(hipStream_t s, const InterpPass& p, T* v, uint16_t* codes, T* side,
+                          uint32_t* hist);
+template <typename T>
+void launch_interp_decode(hipStream_t s, const InterpPass& p, T* v, const uint16_t* codes,
+                          const T* side);
+
+}  // namespace stz

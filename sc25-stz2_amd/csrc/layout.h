@@ -1,0 +1,110 @@
+// layout.h -- payload layout and block/chunk geometry of the STZ device path.
+//
+// Payload = file header (512 B) + 15 streams (stream 0: base, 1..7: low
+// blocks 0..6, 8..14: high blocks 0..6).  A residual stream holds the codes of
+// one parity block in chunk-major order (row-major chunk ids, F5:1577-1582;
+// chunk-local row-major inside a chunk, the order F5:1884-1905 reads them
+// back), cut into sub-chunks of 1024 symbols that decode in parallel.  The
+// per-chunk bit offsets are the reference's "markers" (F5:50-55); they also
+// drive random access.  Unpredictable values are kept per chunk, in stream
+// order, so a marked-chunk decode can place them (the reference keeps them in
+// global row-major order, F5:1201-1203, which a chunk decode cannot consume).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stz {
+
+constexpr uint32_t kMagic = 0x325A5453u;        // "STZ2"
+constexpr uint32_t kStreamMagic = 0x4D525453u;  // "STRM"
+constexpr uint32_t kVersion = 1;
+constexpr int kRadius = 32768;      // quantbinCnt/2 (F5:1334)
+constexpr int kLmax = 24;           // Huffman length limit
+constexpr int kSub = 1024;          // symbols per decode sub-chunk
+constexpr int kBaseChunk = 32768;   // symbols per base-stream chunk
+constexpr int kStreams = 15;
+constexpr int kLutBits = 12;        // primary decode table
+
+struct FileHeader {
+    uint32_t magic, version, dtype, eb_mode;
+    uint64_t nx, ny, nz;
+    uint64_t px, py, pz;
+    double eb_user, eb_unit;
+    double mult_base, mult_low, mult_high;
+    double eb_base, eb_low, eb_high;
+    double vmin, vmax;
+    uint32_t radius, chunk_high, chunk_low, sub_size;
+    uint32_t interp, order, lmax, n_streams;
+    double alpha, beta;
+    uint64_t total_bytes;
+    uint64_t reserved[3];
+    uint64_t stream_off[kStreams];
+    uint64_t stream_bytes[kStreams];
+    uint8_t pad[48];
+};
+static_assert(sizeof(FileHeader) == 512, "file header");
+
+struct StreamHeader {
+    uint32_t magic, kind, block, dtype;  // kind: 0 base, 1 low, 2 high
+    uint64_t n_symbols;
+    uint32_t dx, dy, dz, chunk;
+    uint32_t n_chunks, n_sub, n_used, lmax;
+    uint64_t n_unpred, total_bits;
+    double eb;
+    uint32_t radius, sub_size;
+    uint64_t off_codebook, off_chunk_bit, off_chunk_unpred, off_chunk_sub;
+    uint64_t off_sub_bits, off_sub_unpred, off_unpred, off_bits;
+    uint64_t stream_bytes;
+};
+static_assert(sizeof(StreamHeader) == 160, "stream header");
+
+// Geometry of one stream's block grid, passed to kernels by value.
+struct Geom {
+    int dx, dy, dz;
+    int chunk;  // C for residual streams, 0 for the base stream (linear chunks)
+    int ncx, ncy, ncz;
+    uint32_t n_chunks;
+};
+
+__host__ __device__ inline int imin_(int a, int b) { return a < b ? a : b; }
+
+// stream position of block point (x,y,z): chunk start + chunk-local row-major
+// index.  Chunks before (cz,cy,cx) are full along the axes they precede on.
+__host__ __device__ inline uint32_t stream_pos(const Geom& g, int x, int y, int z)
+{
+    const int C = g.chunk;
+    const int cx = x / C, cy = y / C, cz = z / C;
+    const int wx = imin_(C, g.dx - cx * C), wy = imin_(C, g.dy - cy * C), wz = imin_(C, g.dz - cz * C);
+    const uint32_t start = (uint32_t)cz * C * g.dx * g.dy + (uint32_t)cy * C * g.dx * wz +
+                           (uint32_t)cx * C * wy * wz;
+    return start + ((uint32_t)(z - cz * C) * wy + (uint32_t)(y - cy * C)) * wx + (uint32_t)(x - cx * C);
+}
+
+__host__ __device__ inline uint64_t chunk_volume(const Geom& g, uint32_t c, uint64_t n)
+{
+    if (g.chunk <= 0) {
+        const uint64_t p = (uint64_t)c * kBaseChunk;
+        return n - p < (uint64_t)kBaseChunk ? n - p : (uint64_t)kBaseChunk;
+    }
+    const int C = g.chunk;
+    const int cx = (int)(c % g.ncx), cy = (int)((c / g.ncx) % g.ncy), cz = (int)(c / ((uint32_t)g.ncx * g.ncy));
+    return (uint64_t)imin_(C, g.dx - cx * C) * imin_(C, g.dy - cy * C) * imin_(C, g.dz - cz * C);
+}
+
+inline Geom make_geom(int dx, int dy, int dz, int chunk)
+{
+    Geom g{};
+    g.dx = dx; g.dy = dy; g.dz = dz; g.chunk = chunk;
+    if (chunk > 0) {
+        g.ncx = (dx + chunk - 1) / chunk;
+        g.ncy = (dy + chunk - 1) / chunk;
+        g.ncz = (dz + chunk - 1) / chunk;
+        g.n_chunks = (uint32_t)g.ncx * g.ncy * g.ncz;
+    } else {
+        const uint64_t n = (uint64_t)dx * dy * dz;
+        g.n_chunks = (uint32_t)((n + kBaseChunk - 1) / kBaseChunk);
+    }
+    return g;
+}
+
+}  // namespace stz

@@ -1,0 +1,1518 @@
+// stz_api.cpp -- C ABI of the STZ device path (include/stz.h): context,
+// memory pool, the compress / decompress pipelines, the host zstd container
+// and the stage entry points.
+//
+// F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5.cpp
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "host_codec.h"
+#include "kernels.h"
+#include "layout.h"
+#include "stz.h"
+
+extern "C" {  // libzstd (host), the five entry points used
+size_t ZSTD_compress(void* dst, size_t cap, const void* src, size_t n, int level);
+size_t ZSTD_decompress(void* dst, size_t cap, const void* src, size_t n);
+size_t ZSTD_compressBound(size_t n);
+unsigned ZSTD_isError(size_t code);
+}
+
+using namespace stz;
+
+namespace {
+
+struct StzError : std::runtime_error {
+    int code;
+    StzError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) throw StzError(STZ_E_HIP, std::string(#x) + ": " +       \
+                                                            hipGetErrorString(e_));     \
+    } while (0)
+
+inline uint64_t align8(uint64_t v) { return (v + 7) & ~7ull; }
+inline uint64_t align_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------- memory pool
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_;
+    std::unordered_map<void*, size_t> size_;
+    int device = 0;
+    ~Pool()
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(device);
+        for (auto& kv : free_) (void)hipFree(kv.second);
+        (void)hipSetDevice(cur);
+    }
+    void* get(size_t n)
+    {
+        if (n == 0) n = 256;
+        n = align_up(n, 256);
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto it = free_.lower_bound(n);
+            if (it != free_.end() && it->first <= 2 * n + (64u << 20)) {
+                void* p = it->second;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            release_all();
+            e = hipMalloc(&p, n);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw StzError(STZ_E_OOM, "hipMalloc failed");
+            }
+        }
+        std::lock_guard<std::mutex> g(mu);
+        size_[p] = n;
+        return p;
+    }
+    void put(void* p)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> g(mu);
+        free_.insert({size_[p], p});
+    }
+    void release_all()
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (auto& kv : free_) {
+            size_.erase(kv.second);
+            (void)hipFree(kv.second);
+        }
+        free_.clear();
+    }
+};
+
+struct DBuf {
+    std::shared_ptr<Pool> pool;
+    void* p = nullptr;
+    DBuf() = default;
+    DBuf(std::shared_ptr<Pool> pl, size_t n) : pool(std::move(pl)), p(pool->get(n)) {}
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : pool(std::move(o.pool)), p(o.p) { o.p = nullptr; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        reset();
+        pool = std::move(o.pool);
+        p = o.p;
+        o.p = nullptr;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p && pool) pool->put(p);
+        p = nullptr;
+    }
+    template <typename U> U* as() const { return (U*)p; }
+};
+
+struct ProfRec { const char* name; hipEvent_t a, b; };
+
+}  // namespace
+
+struct stz_ctx {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    std::shared_ptr<Pool> pool;
+    void* pinned = nullptr;
+    size_t pinned_cap = 0;
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::map<std::string, std::pair<long, double>> agg;
+
+    void* host_stage(size_t n)
+    {
+        if (n > pinned_cap) {
+            if (pinned) (void)hipHostFree(pinned);
+            pinned = nullptr;
+            pinned_cap = 0;
+            const size_t cap = align_up(std::max<size_t>(n, 4u << 20), 1u << 20);
+            HIPCHK(hipHostMalloc(&pinned, cap, hipHostMallocDefault));
+            pinned_cap = cap;
+        }
+        return pinned;
+    }
+};
+
+// host-side metadata of a payload
+struct Meta {
+    FileHeader fh{};
+    StreamHeader sh[kStreams]{};
+    Codebook cb[kStreams];
+};
+
+struct stz_dstream {
+    int device = 0;
+    std::shared_ptr<Pool> pool;
+    DBuf payload;
+    size_t bytes = 0;
+    Meta meta;
+    // decode tables (built on first decode)
+    DBuf tabs, sorted;
+    bool have_tabs = false;
+};
+
+namespace {
+
+template <typename F>
+void K(stz_ctx* c, const char* name, F&& f)
+{
+    if (!c->prof) {
+        f();
+    } else {
+        ProfRec r{name, nullptr, nullptr};
+        HIPCHK(hipEventCreate(&r.a));
+        HIPCHK(hipEventCreate(&r.b));
+        HIPCHK(hipEventRecord(r.a, c->stream));
+        f();
+        HIPCHK(hipEventRecord(r.b, c->stream));
+        c->recs.push_back(r);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw StzError(STZ_E_HIP, std::string(name) + ": " + hipGetErrorString(e));
+}
+
+void resolve_prof(stz_ctx* c)
+{
+    if (c->recs.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& a = c->agg[r.name];
+        a.first += 1;
+        a.second += ms;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    c->recs.clear();
+}
+
+bool is_device_ptr(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
+}
+
+size_t tsize_of(int dtype) { return dtype == STZ_F32 ? 4 : 8; }
+size_t pad4(size_t n) { size_t p = (n + 3) & ~(size_t)3; return p < 4 ? 4 : p; }
+
+// ------------------------------------------------------------ geometry
+struct HostLayout {
+    size_t nx, ny, nz, px, py, pz;
+    int chunk_high, chunk_low;
+    Geom g[kStreams];
+    uint64_t n[kStreams];
+    uint32_t n_sub[kStreams];
+    std::vector<uint32_t> chunk_sub[kStreams];
+    uint64_t code_off[kStreams];
+    uint64_t total_codes;
+};
+
+void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
+{
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.px = px; L.py = py; L.pz = pz;
+    L.chunk_high = ch; L.chunk_low = cl;
+    const int lx = (int)(px / 4), ly = (int)(py / 4), lz = (int)(pz / 4);
+    const int mx = (int)(px / 2), my = (int)(py / 2), mz = (int)(pz / 2);
+    uint64_t off = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        if (s == 0) L.g[s] = make_geom(lx, ly, lz, 0);
+        else if (s < 8) L.g[s] = make_geom(lx, ly, lz, cl);
+        else L.g[s] = make_geom(mx, my, mz, ch);
+        const Geom& g = L.g[s];
+        L.n[s] = (uint64_t)g.dx * g.dy * g.dz;
+        L.chunk_sub[s].resize(g.n_chunks + 1);
+        uint32_t sub = 0;
+        for (uint32_t c = 0; c < g.n_chunks; ++c) {
+            L.chunk_sub[s][c] = sub;
+            sub += (uint32_t)((chunk_volume(g, c, L.n[s]) + kSub - 1) / kSub);
+        }
+        L.chunk_sub[s][g.n_chunks] = sub;
+        L.n_sub[s] = sub;
+        L.code_off[s] = off;
+        off += align_up(L.n[s], 64);
+    }
+    L.total_codes = off;
+}
+
+void check_dims(size_t nx, size_t ny, size_t nz, int ch, int cl)
+{
+    if (nx == 0 || ny == 0 || nz == 0) throw StzError(STZ_E_DIMS, "zero dimension");
+    if (ch <= 0 || cl <= 0 || ch > 64 || cl > 64) throw StzError(STZ_E_ARG, "chunk dims must be in 1..64");
+    const size_t px = pad4(nx), py = pad4(ny), pz = pad4(nz);
+    if ((px / 2) * (py / 2) * (pz / 2) >= (1ull << 31) || px * py >= (1ull << 31))
+        throw StzError(STZ_E_DIMS, "field too large for one stream grid");
+}
+
+void level_ebs(double eb, int mode, double vmin, double vmax, const stz_params& p, double ebl[3], double& unit)
+{
+    if (mode == STZ_EB_REF) {
+        unit = eb;
+    } else {
+        const double E = (mode == STZ_EB_REL) ? eb * (vmax - vmin) : eb;
+        double mm = p.mult_base;
+        if (p.mult_low > mm) mm = p.mult_low;
+        if (p.mult_high > mm) mm = p.mult_high;
+        unit = E / mm;
+    }
+    ebl[0] = p.mult_base * unit;
+    ebl[1] = p.mult_low * unit;
+    ebl[2] = p.mult_high * unit;
+}
+
+// section offsets of one stream, the same formula the payload reader uses
+void fill_stream_header(StreamHeader& sh, int s, const HostLayout& L, int dtype, double eb, uint32_t n_used,
+                        uint64_t n_unpred, uint64_t total_bits)
+{
+    std::memset(&sh, 0, sizeof(sh));
+    const Geom& g = L.g[s];
+    sh.magic = kStreamMagic;
+    sh.kind = s == 0 ? 0u : s < 8 ? 1u : 2u;
+    sh.block = s == 0 ? 0u : (uint32_t)((s - 1) % 7);
+    sh.dtype = (uint32_t)dtype;
+    sh.n_symbols = L.n[s];
+    sh.dx = (uint32_t)g.dx; sh.dy = (uint32_t)g.dy; sh.dz = (uint32_t)g.dz;
+    sh.chunk = g.chunk > 0 ? (uint32_t)g.chunk : (uint32_t)kBaseChunk;
+    sh.n_chunks = g.n_chunks;
+    sh.n_sub = L.n_sub[s];
+    sh.n_used = n_used;
+    sh.lmax = kLmax;
+    sh.n_unpred = n_unpred;
+    sh.total_bits = total_bits;
+    sh.eb = eb;
+    sh.radius = kRadius;
+    sh.sub_size = kSub;
+    const uint64_t ts = tsize_of(dtype);
+    uint64_t off = sizeof(StreamHeader);
+    sh.off_codebook = off;     off += align8(4ull * n_used);
+    sh.off_chunk_bit = off;    off += 8ull * (sh.n_chunks + 1);
+    sh.off_chunk_unpred = off; off += 8ull * (sh.n_chunks + 1);
+    sh.off_chunk_sub = off;    off += align8(4ull * (sh.n_chunks + 1));
+    sh.off_sub_bits = off;     off += align8(2ull * sh.n_sub);
+    sh.off_sub_unpred = off;   off += align8(2ull * sh.n_sub);
+    sh.off_unpred = off;       off += align8(ts * n_unpred);
+    sh.off_bits = off;         off += align8(4ull * ((total_bits + 31) / 32 + 2));
+    sh.stream_bytes = off;
+}
+
+StreamSet make_streamset(const HostLayout& L, const Meta& m, int dtype)
+{
+    StreamSet ss{};
+    ss.nstreams = kStreams;
+    uint32_t cb = 0, sb = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        StreamDesc& d = ss.s[s];
+        const StreamHeader& sh = m.sh[s];
+        const uint64_t so = m.fh.stream_off[s];
+        d.code_off = L.code_off[s];
+        d.n_symbols = L.n[s];
+        d.off_chunk_bit = so + sh.off_chunk_bit;
+        d.off_chunk_unpred = so + sh.off_chunk_unpred;
+        d.off_chunk_sub = so + sh.off_chunk_sub;
+        d.off_sub_bits = so + sh.off_sub_bits;
+        d.off_sub_unpred = so + sh.off_sub_unpred;
+        d.off_unpred = so + sh.off_unpred;
+        d.off_bits = so + sh.off_bits;
+        d.bits_words = (sh.stream_bytes - sh.off_bits) / 4;
+        d.g = L.g[s];
+        d.n_chunks = L.g[s].n_chunks;
+        d.n_sub = L.n_sub[s];
+        d.chunk_base = cb;
+        d.sub_base = sb;
+        d.tsize = (uint32_t)tsize_of(dtype);
+        d.n_unpred = sh.n_unpred;
+        cb += d.n_chunks;
+        sb += d.n_sub;
+    }
+    ss.total_chunks = cb;
+    ss.total_sub = sb;
+    return ss;
+}
+
+// ------------------------------------------------------------ compress
+template <typename T>
+void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
+                   const stz_params& prm, stz_dstream* ds)
+{
+    check_dims(nx, ny, nz, prm.chunk_high, prm.chunk_low);
+    if (!(eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
+        prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0))
+        throw StzError(STZ_E_ARG, "bad parameters");
+    hipStream_t st = c->stream;
+    auto pool = c->pool;
+    const size_t px = pad4(nx), py = pad4(ny), pz = pad4(nz);
+    const uint64_t nin = (uint64_t)nx * ny * nz;
+    HostLayout L;
+    make_layout(L, nx, ny, nz, px, py, pz, prm.chunk_high, prm.chunk_low);
+    const int lx = (int)(px / 4), ly = (int)(py / 4), lz = (int)(pz / 4);
+    const int mx = (int)(px / 2), my = (int)(py / 2), mz = (int)(pz / 2);
+
+    // input on device, 2-element aligned, padded to multiples of 4
+    DBuf in_copy, padded;
+    const T* din = field;
+    if (!is_device_ptr(field)) {
+        in_copy = DBuf(pool, nin * sizeof(T));
+        HIPCHK(hipMemcpyAsync(in_copy.p, field, nin * sizeof(T), hipMemcpyHostToDevice, st));
+        din = in_copy.as<T>();
+    } else if (((uintptr_t)field % (2 * sizeof(T))) != 0 && px == nx && py == ny && pz == nz) {
+        in_copy = DBuf(pool, nin * sizeof(T));
+        HIPCHK(hipMemcpyAsync(in_copy.p, field, nin * sizeof(T), hipMemcpyDeviceToDevice, st));
+        din = in_copy.as<T>();
+    }
+    const T* full = din;
+    if (px != nx || py != ny || pz != nz) {
+        padded = DBuf(pool, px * py * pz * sizeof(T));
+        K(c, "pad", [&] { launch_pad<T>(st, din, (int)nx, (int)ny, (int)nz, padded.as<T>(), (int)px, (int)py, (int)pz); });
+        full = padded.as<T>();
+    }
+
+    // value range (computeRange, F5:92-104) -> level error bounds
+    double vmin = 0, vmax = 0;
+    {
+        DBuf part(pool, 2 * 2048 * sizeof(double) + 64);
+        double* outp = part.as<double>() + 2 * 2048;
+        K(c, "minmax", [&] { launch_minmax<T>(st, din, nin, part.as<double>(), outp); });
+        double* h = (double*)c->host_stage(64);
+        HIPCHK(hipMemcpyAsync(h, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        vmin = h[0];
+        vmax = h[1];
+    }
+    double ebl[3], unit;
+    level_ebs(eb, mode, vmin, vmax, prm, ebl, unit);
+
+    // scratch
+    DBuf codes(pool, L.total_codes * 2), side(pool, L.total_codes * sizeof(T));
+    DBuf hist(pool, (size_t)kStreams * 65536 * 4);
+    DBuf vbase(pool, (size_t)lx * ly * lz * sizeof(T)), mid(pool, (size_t)mx * my * mz * sizeof(T));
+    K(c, "hist_zero", [&] { launch_fill_u32(st, hist.as<uint32_t>(), (uint64_t)kStreams * 65536, 0u); });
+
+    // base level (F5:2132-2145): gather low block 0, then the pass schedule
+    K(c, "gather_low0", [&] { launch_gather_low0<T>(st, full, (int)px, (int)py, (int)pz, vbase.as<T>()); });
+    const auto passes = make_passes(lx, ly, lz, ebl[0], prm.interp, prm.order, prm.alpha, prm.beta, 0);
+    for (const auto& p : passes)
+        K(c, "interp_encode", [&] {
+            launch_interp_encode<T>(st, p, vbase.as<T>(), codes.as<uint16_t>() + L.code_off[0],
+                                    side.as<T>() + L.code_off[0], hist.as<uint32_t>());
+        });
+    // low level (F5:2180-2230) fused with the mid merge
+    LevelArgs la{};
+    la.eb = ebl[1];
+    la.recip = 1.0 / ebl[1];
+    for (int b = 0; b < 7; ++b) la.code_off[b] = L.code_off[1 + b];
+    K(c, "low_encode", [&] {
+        launch_low_encode<T>(st, full, (int)px, (int)py, (int)pz, vbase.as<T>(), L.g[1], la, codes.as<uint16_t>(),
+                             side.as<T>(), hist.as<uint32_t>() + 65536, mid.as<T>());
+    });
+    // high level (F5:2252-2296)
+    LevelArgs ha{};
+    ha.eb = ebl[2];
+    ha.recip = 1.0 / ebl[2];
+    for (int b = 0; b < 7; ++b) ha.code_off[b] = L.code_off[8 + b];
+    K(c, "high_encode", [&] {
+        launch_high_encode<T>(st, full, (int)px, (int)py, (int)pz, mid.as<T>(), L.g[8], ha, codes.as<uint16_t>(),
+                              side.as<T>(), hist.as<uint32_t>() + 8 * 65536);
+    });
+
+    // histograms -> host, codebooks (F5:1271 / F5:1402 preprocess_encode_fast)
+    constexpr uint32_t kFirst = 4096;
+    DBuf compact(pool, (size_t)kStreams * 65536 * 8), counts(pool, 64 * 4);
+    K(c, "hist_compact", [&] { launch_hist_compact(st, hist.as<uint32_t>(), kStreams, compact.as<uint64_t>(), counts.as<uint32_t>()); });
+    uint8_t* hs = (uint8_t*)c->host_stage(64 * 4 + (size_t)kStreams * kFirst * 8);
+    uint32_t* hcounts = (uint32_t*)hs;
+    uint64_t* hfirst = (uint64_t*)(hs + 256);
+    HIPCHK(hipMemcpyAsync(hcounts, counts.p, kStreams * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpy2DAsync(hfirst, kFirst * 8, compact.p, 65536 * 8, kFirst * 8, kStreams, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<std::vector<uint64_t>> lists(kStreams);
+    for (int s = 0; s < kStreams; ++s) {
+        const uint32_t n = hcounts[s];
+        lists[s].resize(n);
+        if (n <= kFirst) std::memcpy(lists[s].data(), hfirst + (size_t)s * kFirst, n * 8);
+    }
+    for (int s = 0; s < kStreams; ++s)
+        if (hcounts[s] > kFirst)
+            HIPCHK(hipMemcpy(lists[s].data(), compact.as<uint64_t>() + (size_t)s * 65536, (size_t)hcounts[s] * 8,
+                             hipMemcpyDeviceToHost));
+
+    Meta& m = ds->meta;
+    std::memset(&m.fh, 0, sizeof(m.fh));
+    FileHeader& fh = m.fh;
+    fh.magic = kMagic;
+    fh.version = kVersion;
+    fh.dtype = (uint32_t)dtype;
+    fh.eb_mode = (uint32_t)mode;
+    fh.nx = nx; fh.ny = ny; fh.nz = nz;
+    fh.px = px; fh.py = py; fh.pz = pz;
+    fh.eb_user = eb;
+    fh.eb_unit = unit;
+    fh.mult_base = prm.mult_base; fh.mult_low = prm.mult_low; fh.mult_high = prm.mult_high;
+    fh.eb_base = ebl[0]; fh.eb_low = ebl[1]; fh.eb_high = ebl[2];
+    fh.vmin = vmin; fh.vmax = vmax;
+    fh.radius = kRadius;
+    fh.chunk_high = (uint32_t)prm.chunk_high;
+    fh.chunk_low = (uint32_t)prm.chunk_low;
+    fh.sub_size = kSub;
+    fh.interp = (uint32_t)prm.interp;
+    fh.order = (uint32_t)prm.order;
+    fh.lmax = kLmax;
+    fh.n_streams = kStreams;
+    fh.alpha = prm.alpha;
+    fh.beta = prm.beta;
+    uint64_t total = sizeof(FileHeader);
+    size_t enc_n = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        const uint32_t n = hcounts[s];
+        build_codebook(lists[s].data(), n, kLmax, m.cb[s]);
+        const uint64_t n_unpred = (n && (lists[s][0] >> 32) == 0) ? (lists[s][0] & 0xFFFFFFFFull) : 0;
+        const double ebs = s == 0 ? ebl[0] : s < 8 ? ebl[1] : ebl[2];
+        fill_stream_header(m.sh[s], s, L, dtype, ebs, n, n_unpred, m.cb[s].total_bits);
+        fh.stream_off[s] = total;
+        fh.stream_bytes[s] = m.sh[s].stream_bytes;
+        total += m.sh[s].stream_bytes;
+        enc_n += n;
+    }
+    fh.total_bytes = total;
+
+    // payload: zeroed, then host-written sections (headers, codebooks, chunk->sub tables)
+    ds->payload = DBuf(pool, align_up(total, 4));
+    ds->bytes = total;
+    ds->pool = pool;
+    ds->device = c->device;
+    uint8_t* pay = ds->payload.as<uint8_t>();
+    K(c, "payload_zero", [&] { launch_fill_u32(st, (uint32_t*)pay, align_up(total, 4) / 4, 0u); });
+    size_t host_bytes = sizeof(FileHeader) + enc_n * 8;
+    for (int s = 0; s < kStreams; ++s) host_bytes += m.sh[s].off_sub_bits;
+    uint8_t* stage = (uint8_t*)c->host_stage(host_bytes + 64);
+    std::memcpy(stage, &fh, sizeof(fh));
+    HIPCHK(hipMemcpyAsync(pay, stage, sizeof(fh), hipMemcpyHostToDevice, st));
+    size_t so = sizeof(fh);
+    for (int s = 0; s < kStreams; ++s) {
+        const StreamHeader& sh = m.sh[s];
+        uint8_t* b = stage + so;
+        std::memset(b, 0, sh.off_sub_bits);
+        std::memcpy(b, &sh, sizeof(sh));
+        uint32_t* cbw = (uint32_t*)(b + sh.off_codebook);
+        for (size_t i = 0; i < m.cb[s].sym.size(); ++i) cbw[i] = (uint32_t)m.cb[s].sym[i] | ((uint32_t)m.cb[s].len[i] << 16);
+        std::memcpy(b + sh.off_chunk_sub, L.chunk_sub[s].data(), 4 * L.chunk_sub[s].size());
+        HIPCHK(hipMemcpyAsync(pay + fh.stream_off[s], b, sh.off_sub_bits, hipMemcpyHostToDevice, st));
+        so += sh.off_sub_bits;
+    }
+    // encoder tables: (stream << 16 | symbol, code | len << 24) scattered into dense per-stream tables
+    uint32_t* encl = (uint32_t*)(stage + so);
+    {
+        size_t k = 0;
+        for (int s = 0; s < kStreams; ++s)
+            for (size_t i = 0; i < m.cb[s].sym.size(); ++i) {
+                encl[2 * k] = ((uint32_t)s << 16) | m.cb[s].sym[i];
+                encl[2 * k + 1] = m.cb[s].code[i] | ((uint32_t)m.cb[s].len[i] << 24);
+                k++;
+            }
+    }
+    DBuf encl_d(pool, enc_n * 8 + 8), enc(pool, (size_t)kStreams * 65536 * 4);
+    HIPCHK(hipMemcpyAsync(encl_d.p, encl, enc_n * 8, hipMemcpyHostToDevice, st));
+    K(c, "enc_scatter", [&] { launch_enc_scatter(st, encl_d.as<uint32_t>(), (uint32_t)enc_n, enc.as<uint32_t>()); });
+
+    // marked encode (F5:1412 encode_marked_fast): lengths -> chunk scan -> pack
+    const StreamSet ss = make_streamset(L, m, dtype);
+    DBuf pre_b(pool, (size_t)ss.total_sub * 4 + 16), pre_u(pool, (size_t)ss.total_sub * 4 + 16);
+    DBuf tot_b(pool, (size_t)ss.total_chunks * 8 + 16), tot_u(pool, (size_t)ss.total_chunks * 8 + 16);
+    K(c, "huff_len", [&] {
+        launch_huff_len(st, ss, codes.as<uint16_t>(), enc.as<uint32_t>(), pay, pre_b.as<uint32_t>(),
+                        pre_u.as<uint32_t>(), tot_b.as<uint64_t>(), tot_u.as<uint64_t>());
+    });
+    K(c, "huff_scan", [&] { launch_huff_scan(st, ss, tot_b.as<uint64_t>(), tot_u.as<uint64_t>(), pay); });
+    K(c, "huff_pack", [&] {
+        launch_huff_pack(st, ss, codes.as<uint16_t>(), side.p, enc.as<uint32_t>(), pre_b.as<uint32_t>(),
+                         pre_u.as<uint32_t>(), pay);
+    });
+    // the staging buffer feeds async copies: finish before it can be reused
+    HIPCHK(hipStreamSynchronize(st));
+    ds->have_tabs = false;
+}
+
+// ------------------------------------------------------------ decode helpers
+void ensure_tabs(stz_ctx* c, stz_dstream* ds)
+{
+    if (ds->have_tabs) return;
+    const Meta& m = ds->meta;
+    std::vector<DecTab> tabs(kStreams);
+    std::vector<uint16_t> all;
+    for (int s = 0; s < kStreams; ++s) {
+        std::vector<uint16_t> sorted;
+        build_dectab(m.cb[s].sym.data(), m.cb[s].len.data(), (uint32_t)m.cb[s].sym.size(), tabs[s], sorted);
+        tabs[s].sorted_off = (uint32_t)all.size();
+        all.insert(all.end(), sorted.begin(), sorted.end());
+    }
+    const size_t tb = sizeof(DecTab) * kStreams;
+    ds->tabs = DBuf(ds->pool, tb);
+    ds->sorted = DBuf(ds->pool, all.size() * 2 + 16);
+    uint8_t* stage = (uint8_t*)c->host_stage(tb + all.size() * 2 + 64);
+    std::memcpy(stage, tabs.data(), tb);
+    std::memcpy(stage + tb, all.data(), all.size() * 2);
+    HIPCHK(hipMemcpyAsync(ds->tabs.p, stage, tb, hipMemcpyHostToDevice, c->stream));
+    if (!all.empty()) HIPCHK(hipMemcpyAsync(ds->sorted.p, stage + tb, all.size() * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ds->have_tabs = true;
+}
+
+void layout_of(const Meta& m, HostLayout& L)
+{
+    make_layout(L, m.fh.nx, m.fh.ny, m.fh.nz, m.fh.px, m.fh.py, m.fh.pz, (int)m.fh.chunk_high, (int)m.fh.chunk_low);
+}
+
+// decoded symbols of the selected chunks (lists[s] empty + all[s]: every chunk)
+struct Decoded {
+    DBuf codes, side;
+};
+
+void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* want_all,
+                 const std::vector<int64_t>* lists, Decoded& out)
+{
+    ensure_tabs(c, ds);
+    hipStream_t st = c->stream;
+    const Meta& m = ds->meta;
+    StreamSet ss = make_streamset(L, m, (int)m.fh.dtype);
+    const size_t ts = tsize_of((int)m.fh.dtype);
+    out.codes = DBuf(c->pool, L.total_codes * 2);
+    out.side = DBuf(c->pool, L.total_codes * ts);
+    // work items: per stream, 256-aligned ranges
+    std::vector<uint32_t> list;  // (stream, chunk, item) triples for partial streams
+    bool all_full = true;
+    uint32_t item = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        StreamDesc& d = ss.s[s];
+        d.item_base = item;
+        if (want_all[s]) {
+            d.item_count = L.n_sub[s];
+            for (uint32_t ch = 0; ch < L.g[s].n_chunks; ++ch) {
+                list.push_back((uint32_t)s);
+                list.push_back(ch);
+                list.push_back(item + L.chunk_sub[s][ch]);
+            }
+        } else {
+            all_full = false;
+            uint32_t k = 0;
+            for (int64_t ch : lists[s]) {
+                list.push_back((uint32_t)s);
+                list.push_back((uint32_t)ch);
+                list.push_back(item + k);
+                k += L.chunk_sub[s][ch + 1] - L.chunk_sub[s][ch];
+            }
+            d.item_count = k;
+        }
+        item += (d.item_count + 255u) & ~255u;
+    }
+    ss.total_items = item;
+    if (!item) return;
+    DBuf items(c->pool, (size_t)item * sizeof(DecItem)), err(c->pool, 64);
+    uint8_t* pay = ds->payload.as<uint8_t>();
+    if (all_full) {
+        K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, nullptr, 0, items.as<DecItem>()); });
+    } else {
+        DBuf ld(c->pool, list.size() * 4 + 16);
+        uint32_t* stage = (uint32_t*)c->host_stage(list.size() * 4 + 16);
+        std::memcpy(stage, list.data(), list.size() * 4);
+        HIPCHK(hipMemcpyAsync(ld.p, stage, list.size() * 4, hipMemcpyHostToDevice, st));
+        K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, ld.as<uint32_t>(), (uint32_t)(list.size() / 3), items.as<DecItem>()); });
+        HIPCHK(hipStreamSynchronize(st));  // staging reuse
+    }
+    K(c, "err_zero", [&] { launch_fill_u32(st, err.as<uint32_t>(), 1, 0u); });
+    K(c, "huff_decode", [&] {
+        launch_huff_decode(st, ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
+                           out.codes.as<uint16_t>(), out.side.p, err.as<uint32_t>());
+    });
+    uint32_t* herr = (uint32_t*)c->host_stage(64);
+    HIPCHK(hipMemcpyAsync(herr, err.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (*herr) throw StzError(STZ_E_STREAM, "corrupt Huffman bitstream");
+}
+
+template <typename T>
+void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& dd, T* v, int stop_level)
+{
+    const auto passes = make_passes(L.g[0].dx, L.g[0].dy, L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
+                                    (int)m.fh.order, m.fh.alpha, m.fh.beta, stop_level);
+    for (const auto& p : passes)
+        K(c, "interp_decode", [&] {
+            launch_interp_decode<T>(c->stream, p, v, dd.codes.as<uint16_t>() + L.code_off[0],
+                                    dd.side.as<T>() + L.code_off[0]);
+        });
+}
+
+LevelArgs level_args(const Meta& m, const HostLayout& L, int level)
+{
+    LevelArgs a{};
+    a.eb = level == 1 ? m.fh.eb_low : m.fh.eb_high;
+    a.recip = 1.0 / a.eb;
+    for (int b = 0; b < 7; ++b) a.code_off[b] = L.code_off[(level == 1 ? 1 : 8) + b];
+    return a;
+}
+
+// output buffer: the caller's device pointer, or a pool buffer copied back
+struct OutBuf {
+    void* user;
+    bool dev;
+    DBuf tmp;
+    void* ptr;
+    size_t bytes;
+    OutBuf(stz_ctx* c, void* u, size_t n) : user(u), dev(is_device_ptr(u)), bytes(n)
+    {
+        if (dev) ptr = u;
+        else { tmp = DBuf(c->pool, n); ptr = tmp.p; }
+    }
+    void finish(stz_ctx* c)
+    {
+        if (!dev) HIPCHK(hipMemcpyAsync(user, ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+};
+
+template <typename T>
+void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
+{
+    const Meta& m = ds->meta;
+    HostLayout L;
+    layout_of(m, L);
+    bool all[kStreams];
+    for (int s = 0; s < kStreams; ++s) all[s] = true;
+    Decoded dd;
+    huff_decode(c, ds, L, all, nullptr, dd);
+    hipStream_t st = c->stream;
+    const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
+    DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));
+    decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
+    const LevelArgs la = level_args(m, L, 1), ha = level_args(m, L, 2);
+    K(c, "low_decode", [&] { launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
+    OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
+    K(c, "high_decode", [&] {
+        launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
+                              (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz);
+    });
+    ob.finish(c);
+}
+
+template <typename T>
+void decompress_level(stz_ctx* c, stz_dstream* ds, int level, void* out, size_t od[3])
+{
+    const Meta& m = ds->meta;
+    if (level == 3) {
+        od[0] = m.fh.nx; od[1] = m.fh.ny; od[2] = m.fh.nz;
+        decompress_full<T>(c, ds, out);
+        return;
+    }
+    HostLayout L;
+    layout_of(m, L);
+    bool all[kStreams];
+    std::vector<int64_t> lists[kStreams];
+    for (int s = 0; s < kStreams; ++s) all[s] = (s == 0) || (level == 2 && s < 8);
+    Decoded dd;
+    huff_decode(c, ds, L, all, lists, dd);
+    hipStream_t st = c->stream;
+    const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
+    DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T));
+    decode_base<T>(c, m, L, dd, vb.as<T>(), level == 0 ? 2 : 0);
+    const int sh = level == 0 ? 3 : level == 1 ? 2 : 1;
+    od[0] = (m.fh.nx + (1u << sh) - 1) >> sh;
+    od[1] = (m.fh.ny + (1u << sh) - 1) >> sh;
+    od[2] = (m.fh.nz + (1u << sh) - 1) >> sh;
+    OutBuf ob(c, out, od[0] * od[1] * od[2] * sizeof(T));
+    if (level == 2) {
+        const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
+        DBuf mid(c->pool, (size_t)mx * my * mz * sizeof(T));
+        const LevelArgs la = level_args(m, L, 1);
+        K(c, "low_decode", [&] { launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
+        K(c, "crop", [&] { launch_crop<T>(st, mid.as<T>(), mx, my, (int)od[0], (int)od[1], (int)od[2], 0, 0, 0, 1, (T*)ob.ptr); });
+        ob.finish(c);
+    } else {
+        const int step = level == 0 ? 2 : 1;
+        K(c, "crop", [&] { launch_crop<T>(st, vb.as<T>(), lx, ly, (int)od[0], (int)od[1], (int)od[2], 0, 0, 0, step, (T*)ob.ptr); });
+        ob.finish(c);
+    }
+}
+
+template <typename T>
+void decompress_roi(stz_ctx* c, stz_dstream* ds, int ax, int ay, int az, int bx, int by, int bz, void* out, size_t od[3])
+{
+    const Meta& m = ds->meta;
+    HostLayout L;
+    layout_of(m, L);
+    Plan pl;
+    make_plan(pl, (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, (int)m.fh.px, (int)m.fh.py, (int)m.fh.pz,
+              (int)m.fh.chunk_high, (int)m.fh.chunk_low, ax, ay, az, bx, by, bz);
+    bool all[kStreams];
+    std::vector<int64_t> lists[kStreams];
+    all[0] = true;  // the base is decoded in full (F5:2339)
+    for (int b = 0; b < 7; ++b) {
+        all[1 + b] = false;
+        all[8 + b] = false;
+        if (pl.low[b].valid) lists[1 + b] = pl.low[b].chunks;
+        if (pl.high[b].valid) lists[8 + b] = pl.high[b].chunks;
+    }
+    Decoded dd;
+    huff_decode(c, ds, L, all, lists, dd);
+    hipStream_t st = c->stream;
+    const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
+    const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
+    DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)mx * my * mz * sizeof(T));
+    decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
+    const LevelArgs la = level_args(m, L, 1), ha = level_args(m, L, 2);
+    for (const Box& R : pl.regions) {
+        if (R.x0 > R.x1 || R.y0 > R.y1 || R.z0 > R.z1) continue;
+        K(c, "low_decode_box", [&] {
+            launch_low_decode_box<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                                     mx, my, R.x0, R.x1, R.y0, R.y1, R.z0, R.z1);
+        });
+    }
+    od[0] = (size_t)(pl.q.x1 - pl.q.x0 + 1);
+    od[1] = (size_t)(pl.q.y1 - pl.q.y0 + 1);
+    od[2] = (size_t)(pl.q.z1 - pl.q.z0 + 1);
+    OutBuf ob(c, out, od[0] * od[1] * od[2] * sizeof(T));
+    K(c, "high_decode_box", [&] {
+        launch_high_decode_box<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
+                                  pl.q.x0, pl.q.x1, pl.q.y0, pl.q.y1, pl.q.z0, pl.q.z1);
+    });
+    ob.finish(c);
+}
+
+// ------------------------------------------------------------ payload parsing
+void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
+{
+    if (bytes < sizeof(FileHeader)) throw StzError(STZ_E_STREAM, "short payload");
+    std::memcpy(&m.fh, p, sizeof(FileHeader));
+    const FileHeader& fh = m.fh;
+    if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
+        (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
+        throw StzError(STZ_E_STREAM, "bad payload header");
+    if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))
+        throw StzError(STZ_E_STREAM, "bad dims");
+    check_dims(fh.nx, fh.ny, fh.nz, (int)fh.chunk_high, (int)fh.chunk_low);
+    HostLayout L;
+    make_layout(L, fh.nx, fh.ny, fh.nz, fh.px, fh.py, fh.pz, (int)fh.chunk_high, (int)fh.chunk_low);
+    for (int s = 0; s < kStreams; ++s) {
+        if (fh.stream_off[s] + sizeof(StreamHeader) > bytes || fh.stream_off[s] + fh.stream_bytes[s] > bytes)
+            throw StzError(STZ_E_STREAM, "stream out of bounds");
+        StreamHeader& sh = m.sh[s];
+        std::memcpy(&sh, p + fh.stream_off[s], sizeof(StreamHeader));
+        StreamHeader chk;
+        fill_stream_header(chk, s, L, (int)fh.dtype, sh.eb, sh.n_used, sh.n_unpred, sh.total_bits);
+        if (sh.magic != kStreamMagic || sh.n_symbols != L.n[s] || sh.n_chunks != L.g[s].n_chunks ||
+            sh.stream_bytes != chk.stream_bytes || sh.off_bits != chk.off_bits || sh.n_used > 65536 ||
+            sh.stream_bytes != fh.stream_bytes[s])
+            throw StzError(STZ_E_STREAM, "bad stream header");
+        const uint32_t* cb = (const uint32_t*)(p + fh.stream_off[s] + sh.off_codebook);
+        Codebook& b = m.cb[s];
+        b.sym.resize(sh.n_used);
+        b.len.resize(sh.n_used);
+        double kraft = 0;
+        for (uint32_t i = 0; i < sh.n_used; ++i) {
+            b.sym[i] = (uint16_t)(cb[i] & 0xFFFF);
+            b.len[i] = (uint8_t)(cb[i] >> 16);
+            if (b.len[i] == 0 || b.len[i] > kLmax || (i && b.sym[i] <= b.sym[i - 1]))
+                throw StzError(STZ_E_STREAM, "bad codebook");
+            kraft += std::ldexp(1.0, -b.len[i]);
+        }
+        if (kraft > 1.0 + 1e-12) throw StzError(STZ_E_STREAM, "codebook violates Kraft");
+        // chunk tables: geometry-derived sub-chunk map, monotone offsets that
+        // add up to the sub-chunk sizes (they steer every decode read)
+        const uint8_t* sb = p + fh.stream_off[s];
+        const uint64_t* cbit = (const uint64_t*)(sb + sh.off_chunk_bit);
+        const uint64_t* cun = (const uint64_t*)(sb + sh.off_chunk_unpred);
+        const uint32_t* csub = (const uint32_t*)(sb + sh.off_chunk_sub);
+        const uint16_t* sbits = (const uint16_t*)(sb + sh.off_sub_bits);
+        const uint16_t* sun = (const uint16_t*)(sb + sh.off_sub_unpred);
+        if (sh.n_sub != L.n_sub[s]) throw StzError(STZ_E_STREAM, "bad sub-chunk count");
+        uint64_t ab = 0, au = 0;
+        for (uint32_t c = 0; c < sh.n_chunks; ++c) {
+            if (csub[c] != L.chunk_sub[s][c] || cbit[c] != ab || cun[c] != au)
+                throw StzError(STZ_E_STREAM, "bad chunk table");
+            for (uint32_t j = csub[c]; j < L.chunk_sub[s][c + 1]; ++j) {
+                if (sbits[j] > (uint32_t)kSub * kLmax || sun[j] > (uint32_t)kSub) throw StzError(STZ_E_STREAM, "bad sub-chunk table");
+                ab += sbits[j];
+                au += sun[j];
+            }
+        }
+        if (csub[sh.n_chunks] != L.chunk_sub[s][sh.n_chunks] || cbit[sh.n_chunks] != ab || ab != sh.total_bits ||
+            cun[sh.n_chunks] != au || au != sh.n_unpred)
+            throw StzError(STZ_E_STREAM, "bad chunk table totals");
+    }
+}
+
+// ------------------------------------------------------------ host container
+constexpr uint32_t kContMagic = 0x435A5453u;  // "STZC"
+struct ContHeader {
+    uint32_t magic, version, n_frames, zstd_level;
+    uint64_t payload_bytes;
+    uint64_t raw[kStreams + 1], comp[kStreams + 1];
+};
+
+void to_container(const uint8_t* pay, size_t bytes, const Meta& m, int level, std::vector<uint8_t>& out)
+{
+    ContHeader ch{};
+    ch.magic = kContMagic;
+    ch.version = 1;
+    ch.n_frames = kStreams + 1;
+    ch.zstd_level = (uint32_t)level;
+    ch.payload_bytes = bytes;
+    std::vector<std::vector<uint8_t>> fr(kStreams + 1);
+    fr[0].assign(pay, pay + sizeof(FileHeader));
+    ch.raw[0] = ch.comp[0] = sizeof(FileHeader);
+    std::vector<std::thread> th;
+    std::vector<int> bad(kStreams + 1, 0);
+    for (int s = 0; s < kStreams; ++s) {
+        th.emplace_back([&, s] {
+            const uint8_t* src = pay + m.fh.stream_off[s];
+            const size_t n = m.fh.stream_bytes[s];
+            fr[s + 1].resize(ZSTD_compressBound(n));
+            const size_t r = ZSTD_compress(fr[s + 1].data(), fr[s + 1].size(), src, n, level);
+            if (ZSTD_isError(r)) { bad[s + 1] = 1; return; }
+            fr[s + 1].resize(r);
+        });
+    }
+    for (auto& t : th) t.join();
+    for (int s = 0; s <= kStreams; ++s)
+        if (bad[s]) throw StzError(STZ_E_ZSTD, "zstd compress failed");
+    for (int s = 0; s < kStreams; ++s) {
+        ch.raw[s + 1] = m.fh.stream_bytes[s];
+        ch.comp[s + 1] = fr[s + 1].size();
+    }
+    size_t total = sizeof(ch);
+    for (auto& f : fr) total += f.size();
+    out.resize(total);
+    std::memcpy(out.data(), &ch, sizeof(ch));
+    size_t o = sizeof(ch);
+    for (auto& f : fr) {
+        std::memcpy(out.data() + o, f.data(), f.size());
+        o += f.size();
+    }
+}
+
+// container -> payload bytes (host)
+void from_container(const uint8_t* in, size_t n, std::vector<uint8_t>& pay)
+{
+    if (n >= 4 && *(const uint32_t*)in == kMagic) {  // a bare payload
+        pay.assign(in, in + n);
+        return;
+    }
+    if (n < sizeof(ContHeader)) throw StzError(STZ_E_STREAM, "short container");
+    ContHeader ch;
+    std::memcpy(&ch, in, sizeof(ch));
+    if (ch.magic != kContMagic || ch.version != 1 || ch.n_frames != kStreams + 1 || ch.raw[0] != sizeof(FileHeader))
+        throw StzError(STZ_E_STREAM, "bad container");
+    size_t o = sizeof(ch), po = 0;
+    uint64_t sum = 0;
+    for (int f = 0; f <= kStreams; ++f) sum += ch.raw[f];
+    if (sum != ch.payload_bytes || ch.payload_bytes > (1ull << 40)) throw StzError(STZ_E_STREAM, "bad container sizes");
+    pay.resize(ch.payload_bytes);
+    std::vector<std::thread> th;
+    std::vector<int> bad(kStreams + 1, 0);
+    for (int f = 0; f <= kStreams; ++f) {
+        if (o + ch.comp[f] > n) throw StzError(STZ_E_STREAM, "truncated container");
+        const uint8_t* src = in + o;
+        uint8_t* dst = pay.data() + po;
+        const size_t cn = ch.comp[f], rn = ch.raw[f];
+        if (f == 0) {
+            if (cn != rn) throw StzError(STZ_E_STREAM, "bad header frame");
+            std::memcpy(dst, src, rn);
+        } else {
+            th.emplace_back([=, &bad] {
+                const size_t r = ZSTD_decompress(dst, rn, src, cn);
+                if (ZSTD_isError(r) || r != rn) bad[f] = 1;
+            });
+        }
+        o += cn;
+        po += rn;
+    }
+    for (auto& t : th) t.join();
+    for (int f = 0; f <= kStreams; ++f)
+        if (bad[f]) throw StzError(STZ_E_STREAM, "zstd frame failed");
+}
+
+stz_dstream* upload(stz_ctx* c, const uint8_t* pay, size_t bytes)
+{
+    auto ds = std::unique_ptr<stz_dstream>(new stz_dstream());
+    parse_meta(pay, bytes, ds->meta);
+    ds->pool = c->pool;
+    ds->device = c->device;
+    ds->bytes = ds->meta.fh.total_bytes;
+    ds->payload = DBuf(c->pool, align_up(ds->bytes, 4));
+    uint8_t* stage = (uint8_t*)c->host_stage(ds->bytes);
+    std::memcpy(stage, pay, ds->bytes);
+    HIPCHK(hipMemcpyAsync(ds->payload.p, stage, ds->bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ds.release();
+}
+
+template <typename F>
+int guard(stz_ctx* c, F&& f)
+{
+    try {
+        if (c) HIPCHK(hipSetDevice(c->device));
+        f();
+        return STZ_OK;
+    } catch (const StzError& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        return STZ_E_OOM;
+    } catch (...) {
+        return STZ_E_HIP;
+    }
+}
+
+stz_params defaults()
+{
+    stz_params p;
+    stz_default_params(&p);
+    return p;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+void stz_default_params(stz_params* p)
+{
+    p->mult_base = 0.7;   // F5:2127
+    p->mult_low = 2.7;    // F5:2128
+    p->mult_high = 8.2;   // F5:2129
+    p->chunk_high = 32;   // F5:34
+    p->chunk_low = 16;    // F5:35
+    p->interp = 1;
+    p->order = 0;
+    p->alpha = 1.0;
+    p->beta = 1.0;
+    p->zstd_level = 3;    // F5:3048 (SZ3 default)
+    p->tuner = 0;
+}
+
+const char* stz_strerror(int code)
+{
+    switch (code) {
+    case STZ_OK: return "ok";
+    case STZ_E_ARG: return "bad argument";
+    case STZ_E_DIMS: return "unsupported dimensions";
+    case STZ_E_STREAM: return "corrupt or unsupported stream";
+    case STZ_E_OOM: return "out of memory";
+    case STZ_E_HIP: return "HIP runtime error";
+    case STZ_E_ZSTD: return "zstd error";
+    case STZ_E_NODEV: return "no GPU device";
+    default: return "unknown error";
+    }
+}
+
+void stz_free(void* p) { std::free(p); }
+
+int stz_create(stz_ctx** out, int device)
+{
+    if (!out) return STZ_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return STZ_E_NODEV;
+    }
+    if (device < 0 || device >= n) return STZ_E_ARG;
+    auto c = std::unique_ptr<stz_ctx>(new stz_ctx());
+    c->device = device;
+    return guard(c.get(), [&] {
+        HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        c->stream = c->own;
+        c->pool = std::make_shared<Pool>();
+        c->pool->device = device;
+        *out = c.release();
+    });
+}
+
+void stz_destroy(stz_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+int stz_set_stream(stz_ctx* c, void* s)
+{
+    if (!c) return STZ_E_ARG;
+    c->stream = s ? (hipStream_t)s : c->own;
+    return STZ_OK;
+}
+
+void* stz_get_stream(stz_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int stz_profile(stz_ctx* c, int enable)
+{
+    if (!c) return STZ_E_ARG;
+    return guard(c, [&] {
+        resolve_prof(c);
+        if (enable) c->agg.clear();
+        c->prof = enable != 0;
+    });
+}
+
+int stz_profile_report(stz_ctx* c, char* buf, size_t cap)
+{
+    if (!c || !buf || !cap) return STZ_E_ARG;
+    return guard(c, [&] {
+        resolve_prof(c);
+        std::string s = "{";
+        bool first = true;
+        for (auto& kv : c->agg) {
+            char tmp[256];
+            std::snprintf(tmp, sizeof(tmp), "%s\"%s\": [%ld, %.6f]", first ? "" : ", ", kv.first.c_str(),
+                          kv.second.first, kv.second.second);
+            s += tmp;
+            first = false;
+        }
+        s += "}";
+        if (s.size() + 1 > cap) throw StzError(STZ_E_ARG, "report buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
+int stz_compress_device(stz_ctx* c, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
+                        stz_eb_mode mode, const stz_params* params, stz_dstream** out)
+{
+    if (!c || !field || !out || (dtype != STZ_F32 && dtype != STZ_F64) || mode < STZ_EB_ABS || mode > STZ_EB_REF)
+        return STZ_E_ARG;
+    *out = nullptr;
+    const stz_params p = params ? *params : defaults();
+    return guard(c, [&] {
+        auto ds = std::unique_ptr<stz_dstream>(new stz_dstream());
+        if (dtype == STZ_F32) compress_impl<float>(c, dtype, (const float*)field, nx, ny, nz, eb, mode, p, ds.get());
+        else compress_impl<double>(c, dtype, (const double*)field, nx, ny, nz, eb, mode, p, ds.get());
+        *out = ds.release();
+    });
+}
+
+int stz_dstream_upload(stz_ctx* c, const void* stream, size_t bytes, stz_dstream** out)
+{
+    if (!c || !stream || !out) return STZ_E_ARG;
+    *out = nullptr;
+    return guard(c, [&] {
+        std::vector<uint8_t> pay;
+        from_container((const uint8_t*)stream, bytes, pay);
+        *out = upload(c, pay.data(), pay.size());
+    });
+}
+
+int stz_dstream_download(stz_ctx* c, const stz_dstream* ds, int zstd_level, void** stream, size_t* bytes)
+{
+    if (!c || !ds || !stream || !bytes) return STZ_E_ARG;
+    *stream = nullptr;
+    *bytes = 0;
+    return guard(c, [&] {
+        std::vector<uint8_t> pay(ds->bytes);
+        HIPCHK(hipMemcpy(pay.data(), ds->payload.p, ds->bytes, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> out;
+        if (zstd_level < 0) out.swap(pay);  // bare payload, no zstd
+        else to_container(pay.data(), pay.size(), ds->meta, zstd_level, out);
+        void* m = std::malloc(out.size());
+        if (!m) throw StzError(STZ_E_OOM, "malloc");
+        std::memcpy(m, out.data(), out.size());
+        *stream = m;
+        *bytes = out.size();
+    });
+}
+
+int stz_dstream_info(const stz_dstream* ds, stz_info* info)
+{
+    if (!ds || !info) return STZ_E_ARG;
+    const FileHeader& fh = ds->meta.fh;
+    info->nx = fh.nx; info->ny = fh.ny; info->nz = fh.nz;
+    info->px = fh.px; info->py = fh.py; info->pz = fh.pz;
+    info->dtype = (stz_dtype)fh.dtype;
+    info->eb_base = fh.eb_base; info->eb_low = fh.eb_low; info->eb_high = fh.eb_high;
+    info->payload_bytes = ds->bytes;
+    info->interp = (int)fh.interp; info->order = (int)fh.order;
+    info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
+    info->alpha = fh.alpha; info->beta = fh.beta;
+    return STZ_OK;
+}
+
+int stz_dstream_payload(const stz_dstream* ds, const void** p, size_t* bytes)
+{
+    if (!ds || !p || !bytes) return STZ_E_ARG;
+    *p = ds->payload.p;
+    *bytes = ds->bytes;
+    return STZ_OK;
+}
+
+void stz_dstream_free(stz_dstream* ds)
+{
+    if (!ds) return;
+    (void)hipSetDevice(ds->device);
+    delete ds;
+}
+
+int stz_dstream_decompress(stz_ctx* c, stz_dstream* ds, void* out)
+{
+    if (!c || !ds || !out) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (ds->meta.fh.dtype == STZ_F32) decompress_full<float>(c, ds, out);
+        else decompress_full<double>(c, ds, out);
+    });
+}
+
+int stz_dstream_decompress_roi(stz_ctx* c, stz_dstream* ds, int x0, int y0, int z0, int x1, int y1, int z1, void* out,
+                               size_t od[3])
+{
+    if (!c || !ds || !out || !od) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (ds->meta.fh.dtype == STZ_F32) decompress_roi<float>(c, ds, x0, y0, z0, x1, y1, z1, out, od);
+        else decompress_roi<double>(c, ds, x0, y0, z0, x1, y1, z1, out, od);
+    });
+}
+
+int stz_dstream_decompress_level(stz_ctx* c, stz_dstream* ds, int level, void* out, size_t od[3])
+{
+    if (!c || !ds || !out || !od || level < 0 || level > 3) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (ds->meta.fh.dtype == STZ_F32) decompress_level<float>(c, ds, level, out, od);
+        else decompress_level<double>(c, ds, level, out, od);
+    });
+}
+
+int stz_compress(stz_ctx* c, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
+                 stz_eb_mode mode, const stz_params* params, void** stream, size_t* bytes)
+{
+    if (!stream || !bytes) return STZ_E_ARG;
+    stz_dstream* ds = nullptr;
+    int rc = stz_compress_device(c, dtype, field, nx, ny, nz, eb, mode, params, &ds);
+    if (rc != STZ_OK) return rc;
+    const int level = params ? params->zstd_level : 3;
+    rc = stz_dstream_download(c, ds, level, stream, bytes);
+    stz_dstream_free(ds);
+    return rc;
+}
+
+int stz_decompress(stz_ctx* c, const void* stream, size_t bytes, void* out)
+{
+    stz_dstream* ds = nullptr;
+    int rc = stz_dstream_upload(c, stream, bytes, &ds);
+    if (rc != STZ_OK) return rc;
+    rc = stz_dstream_decompress(c, ds, out);
+    stz_dstream_free(ds);
+    return rc;
+}
+
+int stz_decompress_roi(stz_ctx* c, const void* stream, size_t bytes, int x0, int y0, int z0, int x1, int y1, int z1,
+                       void* out, size_t od[3])
+{
+    stz_dstream* ds = nullptr;
+    int rc = stz_dstream_upload(c, stream, bytes, &ds);
+    if (rc != STZ_OK) return rc;
+    rc = stz_dstream_decompress_roi(c, ds, x0, y0, z0, x1, y1, z1, out, od);
+    stz_dstream_free(ds);
+    return rc;
+}
+
+int stz_decompress_slice(stz_ctx* c, const void* stream, size_t bytes, int axis, int index, void* out, size_t od[3])
+{
+    if (axis < 0 || axis > 2) return STZ_E_ARG;
+    stz_info info;
+    int rc = stz_query_info(stream, bytes, &info);
+    if (rc != STZ_OK) return rc;
+    if (index < 0 || (size_t)index >= (axis == 0 ? info.nx : axis == 1 ? info.ny : info.nz)) return STZ_E_ARG;
+    int a[3] = {0, 0, 0}, b[3] = {(int)info.nx - 1, (int)info.ny - 1, (int)info.nz - 1};
+    a[axis] = b[axis] = index;
+    return stz_decompress_roi(c, stream, bytes, a[0], a[1], a[2], b[0], b[1], b[2], out, od);
+}
+
+int stz_decompress_level(stz_ctx* c, const void* stream, size_t bytes, int level, void* out, size_t od[3])
+{
+    stz_dstream* ds = nullptr;
+    int rc = stz_dstream_upload(c, stream, bytes, &ds);
+    if (rc != STZ_OK) return rc;
+    rc = stz_dstream_decompress_level(c, ds, level, out, od);
+    stz_dstream_free(ds);
+    return rc;
+}
+
+int stz_query_info(const void* stream, size_t bytes, stz_info* info)
+{
+    if (!stream || !info) return STZ_E_ARG;
+    return guard(nullptr, [&] {
+        const uint8_t* p = (const uint8_t*)stream;
+        FileHeader fh;
+        if (bytes >= sizeof(FileHeader) && *(const uint32_t*)p == kMagic) {
+            std::memcpy(&fh, p, sizeof(fh));
+        } else {
+            if (bytes < sizeof(ContHeader) + sizeof(FileHeader)) throw StzError(STZ_E_STREAM, "short stream");
+            ContHeader ch;
+            std::memcpy(&ch, p, sizeof(ch));
+            if (ch.magic != kContMagic) throw StzError(STZ_E_STREAM, "bad container");
+            std::memcpy(&fh, p + sizeof(ContHeader), sizeof(fh));
+        }
+        if (fh.magic != kMagic || fh.version != kVersion) throw StzError(STZ_E_STREAM, "bad header");
+        info->nx = fh.nx; info->ny = fh.ny; info->nz = fh.nz;
+        info->px = fh.px; info->py = fh.py; info->pz = fh.pz;
+        info->dtype = (stz_dtype)fh.dtype;
+        info->eb_base = fh.eb_base; info->eb_low = fh.eb_low; info->eb_high = fh.eb_high;
+        info->payload_bytes = fh.total_bytes;
+        info->interp = (int)fh.interp; info->order = (int)fh.order;
+        info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
+        info->alpha = fh.alpha; info->beta = fh.beta;
+    });
+}
+
+}  // extern "C"
+
+// =================================================================== stage entry points
+namespace {
+
+// a device view of a caller buffer: copies in (and back) when it is host memory
+struct DView {
+    stz_ctx* c;
+    const void* user;
+    size_t bytes;
+    bool dev;
+    DBuf tmp;
+    void* ptr;
+    DView(stz_ctx* cc, const void* u, size_t n, bool copy_in) : c(cc), user(u), bytes(n), dev(is_device_ptr(u))
+    {
+        if (dev) {
+            ptr = (void*)u;
+        } else {
+            tmp = DBuf(c->pool, n);
+            ptr = tmp.p;
+            if (copy_in) HIPCHK(hipMemcpyAsync(ptr, u, n, hipMemcpyHostToDevice, c->stream));
+        }
+    }
+    void back()
+    {
+        if (!dev) HIPCHK(hipMemcpyAsync((void*)user, ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+};
+
+template <typename T>
+void split_impl(stz_ctx* c, const void* full, size_t nx, size_t ny, size_t nz, void* const sub[8])
+{
+    if (nx % 2 || ny % 2 || nz % 2 || !nx || !ny || !nz) throw StzError(STZ_E_DIMS, "split needs even dims");
+    const size_t hs = (nx / 2) * (ny / 2) * (nz / 2) * sizeof(T);
+    DView in(c, full, nx * ny * nz * sizeof(T), true);
+    std::vector<std::unique_ptr<DView>> outs;
+    T* ptrs[8];
+    for (int k = 0; k < 8; ++k) {
+        outs.emplace_back(new DView(c, sub[k], hs, false));
+        ptrs[k] = (T*)outs.back()->ptr;
+    }
+    K(c, "split", [&] { launch_split<T>(c->stream, (const T*)in.ptr, (int)nx, (int)ny, (int)nz, ptrs); });
+    for (auto& o : outs) o->back();
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+template <typename T>
+void merge_impl(stz_ctx* c, const void* const sub[8], size_t sx, size_t sy, size_t sz, void* full)
+{
+    const size_t hs = sx * sy * sz * sizeof(T);
+    std::vector<std::unique_ptr<DView>> ins;
+    const T* ptrs[8];
+    for (int k = 0; k < 8; ++k) {
+        ins.emplace_back(new DView(c, sub[k], hs, true));
+        ptrs[k] = (const T*)ins.back()->ptr;
+    }
+    DView out(c, full, 8 * hs, false);
+    K(c, "merge", [&] { launch_merge<T>(c->stream, ptrs, (int)sx, (int)sy, (int)sz, (T*)out.ptr); });
+    out.back();
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+template <typename T>
+void residual_impl(stz_ctx* c, int block, const void* a, const void* ref, size_t dx, size_t dy, size_t dz, void* out,
+                   int sign)
+{
+    if (block < 0 || block > 6 || !dx || !dy || !dz || dx * dy * dz >= (1ull << 31))
+        throw StzError(STZ_E_ARG, "bad block or dims");
+    const size_t n = dx * dy * dz * sizeof(T);
+    DView va(c, a, n, true), vr(c, ref, n, true), vo(c, out, n, false);
+    K(c, "residual", [&] {
+        launch_residual_predict<T>(c->stream, block, (const T*)va.ptr, (const T*)vr.ptr, (int)dx, (int)dy, (int)dz,
+                                   (T*)vo.ptr, sign);
+    });
+    vo.back();
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+namespace stz {
+// ordered compaction / expansion of unpredictable values (stage entry points)
+template <typename T>
+void launch_compact_unpred(hipStream_t s, const uint16_t* codes, const T* data, uint64_t n, T* out, uint64_t* count);
+template <typename T>
+void launch_dequant_list(hipStream_t s, const uint16_t* codes, const T* unpred, uint64_t n, double eb, T* out);
+}  // namespace stz
+
+extern "C" {
+
+int stz_split(stz_ctx* c, stz_dtype dtype, const void* full, size_t nx, size_t ny, size_t nz, void* const sub[8])
+{
+    if (!c || !full || !sub) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) split_impl<float>(c, full, nx, ny, nz, sub);
+        else split_impl<double>(c, full, nx, ny, nz, sub);
+    });
+}
+
+int stz_merge(stz_ctx* c, stz_dtype dtype, const void* const sub[8], size_t sx, size_t sy, size_t sz, void* full)
+{
+    if (!c || !full || !sub || !sx || !sy || !sz) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) merge_impl<float>(c, sub, sx, sy, sz, full);
+        else merge_impl<double>(c, sub, sx, sy, sz, full);
+    });
+}
+
+int stz_residual_predict(stz_ctx* c, stz_dtype dtype, int block, const void* sub, const void* ref, size_t dx,
+                         size_t dy, size_t dz, void* diff)
+{
+    if (!c || !sub || !ref || !diff) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) residual_impl<float>(c, block, sub, ref, dx, dy, dz, diff, 1);
+        else residual_impl<double>(c, block, sub, ref, dx, dy, dz, diff, 1);
+    });
+}
+
+int stz_residual_reconstruct(stz_ctx* c, stz_dtype dtype, int block, const void* de, const void* ref, size_t dx,
+                             size_t dy, size_t dz, void* out)
+{
+    if (!c || !de || !ref || !out) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) residual_impl<float>(c, block, de, ref, dx, dy, dz, out, -1);
+        else residual_impl<double>(c, block, de, ref, dx, dy, dz, out, -1);
+    });
+}
+
+int stz_quantize(stz_ctx* c, stz_dtype dtype, const void* data, size_t n, double eb, uint16_t* codes, void* unpred,
+                 size_t* n_unpred)
+{
+    if (!c || !data || !codes || !unpred || !n_unpred || !(eb >= 0) || n == 0) return STZ_E_ARG;
+    return guard(c, [&] {
+        const size_t ts = tsize_of(dtype);
+        DView vd(c, data, n * ts, true), vc(c, codes, n * 2, false), vu(c, unpred, n * ts, false);
+        DBuf cnt(c->pool, 64);
+        if (dtype == STZ_F32) {
+            K(c, "quantize", [&] { launch_quantize<float>(c->stream, (const float*)vd.ptr, n, eb, (uint16_t*)vc.ptr); });
+            K(c, "compact_unpred", [&] { launch_compact_unpred<float>(c->stream, (const uint16_t*)vc.ptr, (const float*)vd.ptr, n, (float*)vu.ptr, cnt.as<uint64_t>()); });
+        } else {
+            K(c, "quantize", [&] { launch_quantize<double>(c->stream, (const double*)vd.ptr, n, eb, (uint16_t*)vc.ptr); });
+            K(c, "compact_unpred", [&] { launch_compact_unpred<double>(c->stream, (const uint16_t*)vc.ptr, (const double*)vd.ptr, n, (double*)vu.ptr, cnt.as<uint64_t>()); });
+        }
+        vc.back();
+        vu.back();
+        uint64_t* h = (uint64_t*)c->host_stage(64);
+        HIPCHK(hipMemcpyAsync(h, cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *n_unpred = (size_t)h[0];
+    });
+}
+
+int stz_dequantize(stz_ctx* c, stz_dtype dtype, const uint16_t* codes, size_t n, double eb, const void* unpred, void* out)
+{
+    if (!c || !codes || !out || n == 0) return STZ_E_ARG;
+    return guard(c, [&] {
+        const size_t ts = tsize_of(dtype);
+        size_t nz = 0;
+        if (!is_device_ptr(codes)) {
+            for (size_t i = 0; i < n; ++i) nz += codes[i] == 0;
+        } else {
+            // device codes: the list must be device memory too (its length is unknown here)
+            if (unpred && !is_device_ptr(unpred)) throw StzError(STZ_E_ARG, "device codes need a device list");
+            nz = 1;
+        }
+        if (nz && !unpred) throw StzError(STZ_E_ARG, "unpredictable list missing");
+        DView vc(c, codes, n * 2, true), vo(c, out, n * ts, false);
+        DView vu(c, unpred ? unpred : (const void*)codes, (nz ? nz : 1) * ts, unpred != nullptr);
+        if (dtype == STZ_F32)
+            K(c, "dequant_list", [&] { launch_dequant_list<float>(c->stream, (const uint16_t*)vc.ptr, (const float*)vu.ptr, n, eb, (float*)vo.ptr); });
+        else
+            K(c, "dequant_list", [&] { launch_dequant_list<double>(c->stream, (const uint16_t*)vc.ptr, (const double*)vu.ptr, n, eb, (double*)vo.ptr); });
+        vo.back();
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stz_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len)
+{
+    if (!freq || !len || nsym <= 0 || nsym > 65536 || lmax < 1 || lmax > kLmax) return STZ_E_ARG;
+    int used = 0;
+    for (int s = 0; s < nsym; ++s) used += freq[s] != 0;
+    if (used > (1 << lmax) && used > 1) return STZ_E_ARG;
+    huff_lengths_dense(freq, nsym, lmax, len);
+    return STZ_OK;
+}
+
+int stz_plan_query(int fx, int fy, int fz, int chunk_high, int chunk_low, int ax, int ay, int az, int bx, int by,
+                   int bz, int* query, int* high_plan, int64_t* high_chunks, int* low_plan, int64_t* low_chunks,
+                   int64_t cap)
+{
+    if (fx <= 0 || fy <= 0 || fz <= 0 || chunk_high <= 0 || chunk_low <= 0 || !query || !high_plan || !low_plan ||
+        !high_chunks || !low_chunks || cap < 0)
+        return STZ_E_ARG;
+    Plan pl;
+    make_plan(pl, fx, fy, fz, (int)pad4((size_t)fx), (int)pad4((size_t)fy), (int)pad4((size_t)fz), chunk_high,
+              chunk_low, ax, ay, az, bx, by, bz);
+    query[0] = pl.q.x0; query[1] = pl.q.x1; query[2] = pl.q.y0; query[3] = pl.q.y1;
+    query[4] = pl.q.z0; query[5] = pl.q.z1; query[6] = pl.is_slice ? 1 : 0;
+    int rc = STZ_OK;
+    for (int b = 0; b < 7; ++b) {
+        const BlockPlan* ps[2] = {&pl.high[b], &pl.low[b]};
+        int* dst[2] = {high_plan + 8 * b, low_plan + 8 * b};
+        int64_t* cd[2] = {high_chunks + cap * b, low_chunks + cap * b};
+        for (int t = 0; t < 2; ++t) {
+            const BlockPlan& p = *ps[t];
+            dst[t][0] = p.valid ? 1 : 0;
+            dst[t][1] = p.r.x0; dst[t][2] = p.r.x1; dst[t][3] = p.r.y0;
+            dst[t][4] = p.r.y1; dst[t][5] = p.r.z0; dst[t][6] = p.r.z1;
+            dst[t][7] = (int)p.chunks.size();
+            if ((int64_t)p.chunks.size() > cap) { rc = STZ_E_ARG; continue; }
+            for (size_t i = 0; i < p.chunks.size(); ++i) cd[t][i] = p.chunks[i];
+        }
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Bar: bit-exact codes, payload bytes and reconstructions (integer/byte work
+and the FP predictors are reproduced operation for operation), max pointwise
+error <= eb (plus the reference quantizer's T-rounding of pred+dec, at most
+2 ulp of the largest magnitude), ROI / slice / level == the oracle's, which
+itself equals a crop of the full decode.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from payload_util import first_difference
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _field(kind, shape, dtype, seed):
+    from stz_amd import fields
+    if kind == "sin":
+        return fields.sinusoids(shape=shape, seed=seed, dtype=dtype)
+    if kind == "lognormal":
+        return fields.lognormal(shape=shape, seed=seed, dtype=dtype)
+    if kind == "grf":
+        return fields.grf(shape, -3.0, seed).astype(dtype)
+    raise ValueError(kind)
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+def _assert_bitexact(got, want, what):
+    g, w = _bits(got), _bits(want)
+    assert g.shape == w.shape, "%s: shape %s vs %s" % (what, g.shape, w.shape)
+    bad = np.nonzero(g.ravel() != w.ravel())[0]
+    assert bad.size == 0, "%s: %d of %d values differ, first at %d (%r vs %r)" % (
+        what, bad.size, g.size, bad[0], got.ravel()[bad[0]], want.ravel()[bad[0]])
+
+
+# ------------------------------------------------------------------ stages
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dims", [(6, 4, 10), (2, 2, 2), (8, 12, 6)])
+def test_split_merge(ctx, oracle, dtype, dims):
+    _say("split/merge", dtype.__name__, dims)
+    rng = np.random.default_rng(11)
+    full = rng.normal(size=dims).astype(dtype)
+    subs = ctx.split(full)
+    want = oracle.split(full)
+    for k in range(8):
+        _assert_bitexact(subs[k], want[k], "sub %d" % k)
+    back = ctx.merge(subs)
+    _assert_bitexact(back, full, "merge")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dims", [(9, 7, 11), (4, 4, 4), (3, 5, 6), (2, 3, 1), (17, 12, 20)])
+def test_residual_predict_reconstruct(ctx, oracle, dtype, dims):
+    rng = np.random.default_rng(5)
+    ref = np.cumsum(rng.normal(size=dims), axis=2).astype(dtype)
+    sub = (ref + rng.normal(scale=0.1, size=dims)).astype(dtype)
+    ref[0] = 0  # flat zero plane: exercises FTZ in the inverse-variance weights
+    for b in range(7):
+        _say("residual block", b, dtype.__name__, dims)
+        _assert_bitexact(ctx.residual_predict(b, sub, ref), oracle.preprocess(b, sub, ref), "predict b%d" % b)
+        _assert_bitexact(ctx.residual_reconstruct(b, sub, ref), oracle.depreprocess(b, sub, ref), "recon b%d" % b)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_quantize_dequantize(ctx, oracle, dtype):
+    rng = np.random.default_rng(9)
+    d = np.concatenate([rng.normal(size=7000) * rng.choice([1e-3, 1, 100, 1e5], size=7000),
+                        [np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-300]]).astype(dtype)
+    for eb in (1e-3, 0.37, 1e-7):
+        _say("quantize", dtype.__name__, eb)
+        codes, un = ctx.quantize(d, eb)
+        wc, wu, _ = oracle.quantize(d, eb)
+        np.testing.assert_array_equal(codes, wc)
+        _assert_bitexact(un, wu, "unpred")
+        out = ctx.dequantize(codes, un, eb, dtype)
+        _assert_bitexact(out, oracle.dequantize(wc, wu, eb, dtype), "dequant")
+
+
+# ------------------------------------------------------------------ whole path
+
+CASES = [
+    ("sin", (64, 64, 64), np.float32, 1e-3, "rel"),
+    ("sin", (40, 40, 40), np.float64, 1e-3, "rel"),
+    ("sin", (37, 35, 33), np.float32, 1e-3, "rel"),     # padding + partial chunks
+    ("lognormal", (64, 96, 32), np.float32, 1e-2, "rel"),
+    ("lognormal", (64, 64, 64), np.float32, 1e-4, "rel"),
+    ("grf", (48, 40, 56), np.float64, 1e-6, "rel"),
+    ("grf", (16, 16, 16), np.float32, 0.05, "abs"),
+    ("sin", (8, 8, 8), np.float32, 1e-3, "ref"),
+    ("sin", (5, 3, 7), np.float64, 1e-3, "rel"),        # dims below 4
+]
+
+
+def _compress_both(ctx, oracle, f, eb, mode, **kw):
+    ds = ctx.compress_device(f, eb, mode, **kw)
+    gpu = ds.download(zstd_level=-1)
+    want = oracle.compress(f, eb, {"abs": 0, "rel": 1, "ref": 2}[mode], **kw)
+    return ds, gpu, want
+
+
+@pytest.mark.parametrize("case", CASES, ids=[("%s-%s-%s-%g-%s" % (c[0], "x".join(map(str, c[1])), c[2].__name__, c[3], c[4])) for c in CASES])
+def test_payload_and_decode_bitexact(ctx, oracle, case):
+    kind, shape, dtype, eb, mode = case
+    _say("whole path", case)
+    f = _field(kind, shape, dtype, seed=1001)
+    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode)
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    rec = ds.decompress()
+    ref = oracle.decompress(want)
+    _assert_bitexact(rec, ref, "decompress")
+    h = oracle.header(want)
+    err = np.max(np.abs(rec.astype(np.float64) - f.astype(np.float64)))
+    bound = max(h["eb_base"], h["eb_low"], h["eb_high"])
+    slack = 2 * np.spacing(np.abs(f).max().astype(dtype)).astype(np.float64)
+    assert err <= bound + slack, (err, bound)
+    ds.free()
+
+
+@pytest.mark.parametrize("params", [dict(interp=0, order=3), dict(interp=1, order=5, alpha=1.5, beta=4.0),
+                                    dict(chunk_high=8, chunk_low=4), dict(mult_base=1.0, mult_low=2.5, mult_high=6.25)])
+def test_params_bitexact(ctx, oracle, params):
+    _say("params", params)
+    f = _field("sin", (48, 32, 40), np.float32, seed=7)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", **params)
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    _assert_bitexact(ds.decompress(), oracle.decompress(want), "decompress")
+
+
+def test_container_roundtrip(ctx, oracle):
+    _say("container")
+    f = _field("sin", (32, 48, 40), np.float32, seed=3)
+    s = ctx.compress(f, 1e-3, "rel")
+    rec = ctx.decompress(s)
+    want = oracle.compress(f, 1e-3, 1)
+    _assert_bitexact(rec, oracle.decompress(want), "container decode")
+    info = __import__("stz_amd").stream_info(s)
+    assert (info.nx, info.ny, info.nz) == (40, 48, 32)
+
+
+ROI_QUERIES = [(3, 5, 7, 20, 17, 30), (0, 0, 0, 63, 63, 63), (10, 10, 10, 10, 40, 50), (62, 0, 1, 63, 63, 2),
+               (40, 2, 9, 5, 30, 0), (-5, -5, -5, 100, 3, 3), (17, 9, 33, 17, 9, 33)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_roi_slice_level(ctx, oracle, dtype):
+    f = _field("lognormal", (64, 56, 64), dtype, seed=21)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel")
+    assert first_difference(gpu, want) is None
+    full = oracle.decompress(want)
+    for q in ROI_QUERIES:
+        _say("roi", dtype.__name__, q)
+        got = ds.decompress_roi(*q)
+        ref = oracle.decompress_roi(want, *q)
+        _assert_bitexact(got, ref, "roi %s" % (q,))
+        x0, x1 = sorted((max(0, min(q[0], 63)), max(0, min(q[3], 63))))
+        y0, y1 = sorted((max(0, min(q[1], 55)), max(0, min(q[4], 55))))
+        z0, z1 = sorted((max(0, min(q[2], 63)), max(0, min(q[5], 63))))
+        _assert_bitexact(got, full[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1], "roi crop %s" % (q,))
+    s = ds.download(3)
+    for axis, idx in [(0, 0), (1, 55), (2, 31), (0, 17)]:
+        _say("slice", axis, idx)
+        got = ctx.decompress_slice(s, axis, idx)
+        sl = [slice(None)] * 3
+        sl[2 - axis] = slice(idx, idx + 1)
+        _assert_bitexact(got, full[tuple(sl)], "slice %d %d" % (axis, idx))
+    for lv in range(4):
+        _say("level", lv)
+        _assert_bitexact(ds.decompress_level(lv), oracle.decompress_level(want, lv), "level %d" % lv)
+
+
+def test_c1_config_bitexact(ctx, oracle):
+    """128^3 fp32 sinusoid field (seed 1001), rel eb 1e-3, cubic, full path."""
+    from stz_amd import fields
+    _say("C1 128^3")
+    f = fields.sinusoids(128, seed=1001)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel")
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    _assert_bitexact(ds.decompress(), oracle.decompress(want), "C1 decompress")
+
+
+def test_device_pointers(ctx, oracle):
+    import torch
+    _say("device pointers")
+    f = _field("sin", (32, 32, 32), np.float32, seed=2)
+    t = torch.from_numpy(f).cuda()
+    ds = ctx.compress_device(t, 1e-3, "rel")
+    out = torch.empty_like(t)
+    ds.decompress(out)
+    torch.cuda.synchronize()
+    want = oracle.compress(f, 1e-3, 1)
+    _assert_bitexact(out.cpu().numpy(), oracle.decompress(want), "device decode")
+    assert first_difference(ds.download(-1), want) is None
+
+
+def test_corrupt_stream_rejected(ctx, oracle):
+    import stz_amd
+    _say("corrupt stream")
+    f = _field("sin", (16, 16, 16), np.float32, seed=4)
+    s = ctx.compress(f, 1e-3, "rel")
+    hdr = 280  # container header; the raw file-header frame follows it
+    bad = s.copy()
+    bad[hdr] ^= 0xFF  # payload magic
+    for b in (bad, s[:100], s[: len(s) - 9]):
+        with pytest.raises(stz_amd.StzError):
+            ctx.decompress(b)
+    # a payload whose chunk table lies about the bitstream is refused before any decode
+    raw = ctx.compress_device(f, 1e-3, "rel").download(-1)
+    from payload_util import stream_sections
+    base, offs = stream_sections(raw, 9)
+    raw2 = raw.copy()
+    raw2[base + offs[1] + 8] ^= 0x01  # chunk_bit[1]
+    with pytest.raises(stz_amd.StzError):
+        ctx.decompress(raw2)
