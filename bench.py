@@ -1,0 +1,205 @@
+"""Benchmark: STZ compress + decompress throughput on one MI355X.
+
+Workload (the metric's configuration): 512^3 fp32 log-normal density field,
+rho = exp(1.5 g), g a unit-variance Gaussian random field with P(k) ~ k^-3
+(seed 1002), value-range relative error bound 1e-3, cubic base interpolation,
+F5 level multipliers 0.7 / 2.7 / 8.2, marker chunks 32^3 / 16^3.
+
+A step = one compress (field resident in HBM -> device payload) plus one
+decompress (device payload -> field in HBM).  value = 2 * N * 4 bytes / (step
+time), GB/s.  Host zstd of the payload is timed separately (zstd_ms), as is
+the D2H copy it needs.  The dominant kernel's roofline figure comes from HIP
+events around each launch on the library's stream, in extra profiled steps
+after the timed ones.  cpu_baseline: the CPU oracle (C/OpenMP restatement of
+the reference) on a 256^3 corner of the same field, compress + decompress.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--n 512] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sc25-stz2_amd"))
+
+METRIC = "GB/s comp+decomp at rel eb 1e-3 (learned vs plain, same GPU); ROI p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, n, s, payload):
+    """Algorithmic HBM bytes of one launch of the per-point kernels at field size n
+    (values), value size s, device payload bytes (minimum each kernel must move)."""
+    n8 = n / 8.0
+    if name == "high_encode":      # read 7/8 targets + 1/8 mid reference, write 7/8 codes (2 B)
+        return n * (7 / 8.0) * s + n8 * s + n * (7 / 8.0) * 2
+    if name == "high_decode":      # read 7/8 codes + 1/8 mid, write the full field
+        return n * (7 / 8.0) * 2 + n8 * s + n * s
+    if name == "huff_decode":      # read the payload, write every code (2 B)
+        return payload + n * 2
+    if name == "huff_pack":        # read every code, write the payload
+        return n * 2 + payload
+    if name == "huff_len":         # read every code
+        return n * 2
+    if name == "low_encode":
+        return n / 64.0 * 7 * s + n / 64.0 * s + n / 64.0 * 7 * 2 + n8 * s
+    if name == "low_decode":
+        return n / 64.0 * 7 * 2 + n / 64.0 * s + n8 * s
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--eb", type=float, default=1e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the path is "
+              "single-GPU (multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import stz_amd
+    from stz_amd import fields
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.time()
+    f_np = fields.lognormal(n=n, seed=1002, dtype=np.float32, device=dev)
+    field = torch.from_numpy(f_np).to(dev)
+    torch.cuda.synchronize()
+    log("field %d^3 generated in %.1fs" % (n, time.time() - t0))
+    N = field.numel()
+    s = 4
+
+    ctx = stz_amd.Context(0)
+    stream = torch.cuda.Stream(device=dev)
+    ctx.set_stream(stream.cuda_stream)
+    out = torch.empty_like(field)
+
+    def step():
+        ds = ctx.compress_device(field, args.eb, "rel")
+        ds.decompress(out)
+        return ds
+
+    for _ in range(args.warmup):
+        step().free()
+    torch.cuda.synchronize()
+    # timed region: K steps, each compress + decompress, HIP events on the library stream
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    tc = td = 0.0
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    last = None
+    for _ in range(args.steps):
+        if last is not None:
+            last.free()
+        a = time.perf_counter()
+        last = ctx.compress_device(field, args.eb, "rel")
+        b = time.perf_counter()
+        last.decompress(out)
+        c = time.perf_counter()
+        tc += b - a
+        td += c - b
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    dev_ms = ev0.elapsed_time(ev1)
+    ms_per_step = dev_ms / args.steps
+    value = 2.0 * N * s / (ms_per_step * 1e-3) / 1e9
+
+    # correctness of the measured work
+    info = last.info
+    err = float((out.double() - field.double()).abs().max())
+    eb_high = info.eb_high
+    payload = last.payload_bytes
+    ok = err <= eb_high * (1 + 1e-12) + 2 * float(np.spacing(np.float32(f_np.max())))
+    log("step %.3f ms (device), wall %.3f ms; compress %.3f ms, decompress %.3f ms (host clock); max err %.3g "
+        "eb %.3g ok=%s" % (ms_per_step, wall * 1e3 / args.steps, tc * 1e3 / args.steps, td * 1e3 / args.steps,
+                           err, eb_high, ok))
+
+    # host zstd of the payload (timed separately, D2H included)
+    z0 = time.perf_counter()
+    host_stream = last.download(3)
+    zstd_ms = (time.perf_counter() - z0) * 1e3
+    cr_payload = N * s / payload
+    cr_zstd = N * s / host_stream.size
+
+    # per-kernel profile (HIP events around every launch on the library stream)
+    ctx.profile(True)
+    for _ in range(args.profile_steps):
+        step().free()
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    per_step = {k: (v[0] / args.profile_steps, v[1] / args.profile_steps) for k, v in prof.items()}
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_total_ms) = dom
+    dom_avg_ms = dom_total_ms / dom_launches
+    kb = kernel_bytes(dom_name, N, s, payload)
+    roof = None
+    if kb is not None:
+        ach = kb / (dom_avg_ms * 1e-3) / 1e9
+        roof = {"kernel": dom_name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(dom_avg_ms, 4), "algorithmic_bytes": int(kb)}
+    step_alg_bytes = 2.0 * (N * s + payload)
+    whole = {"algorithmic_bytes_per_step": int(step_alg_bytes),
+             "achieved_GBps": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+             "frac_of_hbm_peak": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        m = min(256, n)
+        sample = np.ascontiguousarray(f_np[:m, :m, :m])
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        c0 = time.perf_counter()
+        p = pyoracle.compress(sample, args.eb, pyoracle.EB_REL)
+        c1 = time.perf_counter()
+        pyoracle.decompress(p)
+        c2 = time.perf_counter()
+        cpu = {"value": round(2.0 * sample.nbytes / (c2 - c0) / 1e9, 4), "unit": "GB/s", "cores": threads,
+               "kind": "port",
+               "sample": "%d^3 corner of the same field, one compress + one decompress (pre-zstd payload) "
+                         "by the C/OpenMP oracle; compress %.2fs, decompress %.2fs" % (m, c1 - c0, c2 - c1)}
+        log("cpu baseline", cpu)
+
+    rec = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d^3 fp32 log-normal GRF (k^-3, sigma 1.5, seed 1002), rel eb %g, plain tuning, "
+                               "compress+decompress device path (zstd excluded)" % (n, args.eb),
+                   "field": "%dx%dx%d" % (n, n, n), "eb_rel": args.eb, "tuner": "plain",
+                   "quantizer_arith": "f64"},
+        "compress_ms_host_clock": round(tc * 1e3 / args.steps, 3),
+        "decompress_ms_host_clock": round(td * 1e3 / args.steps, 3),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "payload_bytes": int(payload), "cr_payload": round(cr_payload, 3), "cr_zstd": round(cr_zstd, 3),
+        "zstd_ms": round(zstd_ms, 2), "max_err": err, "eb_abs": eb_high, "error_bound_ok": bool(ok),
+        "roofline": roof, "whole_step": whole,
+        "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1][1])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+    last.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -938,23 +938,30 @@ void from_container(const uint8_t* in, size_t n, std::vector<uint8_t>& pay)
     uint64_t sum = 0;
     for (int f = 0; f <= kStreams; ++f) sum += ch.raw[f];
     if (sum != ch.payload_bytes || ch.payload_bytes > (1ull << 40)) throw StzError(STZ_E_STREAM, "bad container sizes");
+    // validate every frame before starting any thread: a throw with joinable
+    // std::threads alive would terminate the process
+    {
+        size_t oo = sizeof(ch);
+        for (int f = 0; f <= kStreams; ++f) {
+            if (ch.comp[f] > n || oo + ch.comp[f] > n) throw StzError(STZ_E_STREAM, "truncated container");
+            if (f == 0 && ch.comp[f] != ch.raw[f]) throw StzError(STZ_E_STREAM, "bad header frame");
+            oo += ch.comp[f];
+        }
+    }
     pay.resize(ch.payload_bytes);
+    std::memcpy(pay.data(), in + o, ch.raw[0]);
+    o += ch.comp[0];
+    po += ch.raw[0];
     std::vector<std::thread> th;
     std::vector<int> bad(kStreams + 1, 0);
-    for (int f = 0; f <= kStreams; ++f) {
-        if (o + ch.comp[f] > n) throw StzError(STZ_E_STREAM, "truncated container");
+    for (int f = 1; f <= kStreams; ++f) {
         const uint8_t* src = in + o;
         uint8_t* dst = pay.data() + po;
         const size_t cn = ch.comp[f], rn = ch.raw[f];
-        if (f == 0) {
-            if (cn != rn) throw StzError(STZ_E_STREAM, "bad header frame");
-            std::memcpy(dst, src, rn);
-        } else {
-            th.emplace_back([=, &bad] {
-                const size_t r = ZSTD_decompress(dst, rn, src, cn);
-                if (ZSTD_isError(r) || r != rn) bad[f] = 1;
-            });
-        }
+        th.emplace_back([=, &bad] {
+            const size_t r = ZSTD_decompress(dst, rn, src, cn);
+            if (ZSTD_isError(r) || r != rn) bad[f] = 1;
+        });
         o += cn;
         po += rn;
     }

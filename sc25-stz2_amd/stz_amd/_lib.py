@@ -62,6 +62,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libstz_hip.so not built (%s): run python sc25-stz2_amd/build.py" % LIB_PATH)
+    # torch ships its own libamdhip64 (same SONAME as /opt/rocm's).  Loading
+    # torch first makes this library bind to that one runtime, so device
+    # pointers from torch tensors are valid here; the other order would put
+    # two HIP runtimes in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, sz, i32, dbl = C.c_void_p, C.c_size_t, C.c_int, C.c_double
     P = C.POINTER
