@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--eb", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--calib", action="store_true",
+                    help="run a 512 MiB device-to-device copy first (known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the path is "
@@ -75,6 +77,14 @@ def main():
     from stz_amd import fields
 
     dev = torch.device("cuda:0")
+    if args.calib:
+        a = torch.empty(128 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 512 MiB read + 512 MiB written")
+        del a, b
     n = args.n
     t0 = time.time()
     f_np = fields.lognormal(n=n, seed=1002, dtype=np.float32, device=dev)
@@ -140,17 +150,21 @@ def main():
     cr_zstd = N * s / host_stream.size
 
     # per-kernel profile (HIP events around every launch on the library stream)
-    ctx.profile(True)
-    for _ in range(args.profile_steps):
-        step().free()
-    prof = ctx.profile_report()
-    ctx.profile(False)
+    prof = {}
+    if args.profile_steps > 0:
+        ctx.profile(True)
+        for _ in range(args.profile_steps):
+            step().free()
+        prof = ctx.profile_report()
+        ctx.profile(False)
     per_step = {k: (v[0] / args.profile_steps, v[1] / args.profile_steps) for k, v in prof.items()}
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_launches, dom_total_ms) = dom
-    dom_avg_ms = dom_total_ms / dom_launches
-    kb = kernel_bytes(dom_name, N, s, payload)
     roof = None
+    kb = None
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        dom_name, (dom_launches, dom_total_ms) = dom
+        dom_avg_ms = dom_total_ms / dom_launches
+        kb = kernel_bytes(dom_name, N, s, payload)
     if kb is not None:
         ach = kb / (dom_avg_ms * 1e-3) / 1e9
         roof = {"kernel": dom_name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,

@@ -11,15 +11,15 @@ namespace {
 
 // computeRange (F5:92-104) in double; NaN never wins a comparison
 template <typename T>
-__global__ __launch_bounds__(256) void k_minmax(const T* __restrict__ d, uint64_t n, double* __restrict__ partial)
+__device__ __forceinline__ void mm_upd(double v, double& mn, double& mx)
 {
-    __shared__ double smn[256], smx[256];
-    double mn = __builtin_inf(), mx = -__builtin_inf();
-    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) {
-        const double v = (double)d[i];
-        if (v < mn) mn = v;
-        if (v > mx) mx = v;
-    }
+    if (v < mn) mn = v;
+    if (v > mx) mx = v;
+}
+
+template <typename T>
+__device__ __forceinline__ void block_minmax(double mn, double mx, double* smn, double* smx)
+{
     smn[threadIdx.x] = mn;
     smx[threadIdx.x] = mx;
     __syncthreads();
@@ -30,22 +30,44 @@ __global__ __launch_bounds__(256) void k_minmax(const T* __restrict__ d, uint64_
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax(const T* __restrict__ d, uint64_t n, double* __restrict__ partial)
+{
+    __shared__ double smn[256], smx[256];
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    constexpr int V = 16 / sizeof(T);  // 16-byte vectors
+    typedef T vec __attribute__((ext_vector_type(V)));
+    const uint64_t nv = ((uintptr_t)d % 16 == 0) ? n / V : 0;
+    const vec* dv = (const vec*)d;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < nv; i += (uint64_t)gridDim.x * 256ull) {
+        const vec v = dv[i];
+#pragma unroll
+        for (int k = 0; k < V; ++k) mm_upd<T>((double)v[k], mn, mx);
+    }
+    for (uint64_t i = nv * V + blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull)
+        mm_upd<T>((double)d[i], mn, mx);
+    block_minmax<T>(mn, mx, smn, smx);
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = smn[0];
         partial[2 * blockIdx.x + 1] = smx[0];
     }
 }
 
-__global__ void k_minmax_final(const double* __restrict__ partial, int nb, double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_minmax_final(const double* __restrict__ partial, int nb, double* __restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double smn[256], smx[256];
     double mn = __builtin_inf(), mx = -__builtin_inf();
-    for (int b = 0; b < nb; ++b) {
+    for (int b = threadIdx.x; b < nb; b += 256) {
         if (partial[2 * b] < mn) mn = partial[2 * b];
         if (partial[2 * b + 1] > mx) mx = partial[2 * b + 1];
     }
-    out[0] = mn;
-    out[1] = mx;
+    block_minmax<double>(mn, mx, smn, smx);
+    if (threadIdx.x == 0) {
+        out[0] = smn[0];
+        out[1] = smx[0];
+    }
 }
 
 // edge-replicate padding to multiples of 4
@@ -223,7 +245,7 @@ void launch_minmax(hipStream_t s, const T* d, uint64_t n, double* partial, doubl
 {
     const unsigned nb = grid_for(n, 2048);
     k_minmax<T><<<dim3(nb), dim3(256), 0, s>>>(d, n, partial);
-    k_minmax_final<<<dim3(1), dim3(64), 0, s>>>(partial, (int)nb, out);
+    k_minmax_final<<<dim3(1), dim3(256), 0, s>>>(partial, (int)nb, out);
 }
 
 template <typename T>

@@ -21,6 +21,7 @@ namespace stz {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kDecStageWords = 12288;  // 48 KiB decode stage
 
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
 {
@@ -344,13 +345,13 @@ __global__ __launch_bounds__(256) void k_huff_pack(StreamSet ss, const uint16_t*
     }
 }
 
-__device__ __forceinline__ int find_stream_by_item(const StreamSet& ss, uint32_t item)
+__device__ __forceinline__ int find_stream_by_wg(const StreamSet& ss, uint32_t wg)
 {
     int s = -1;
-    for (int k = 0; k < (int)ss.nstreams; ++k)
-        if (ss.s[k].item_count && item >= ss.s[k].item_base &&
-            item < ss.s[k].item_base + ((ss.s[k].item_count + 255u) & ~255u))
-            s = k;
+    for (int k = 0; k < (int)ss.nstreams; ++k) {
+        const uint32_t nwg = (ss.s[k].item_count + ss.s[k].ipw - 1) / ss.s[k].ipw;
+        if (ss.s[k].item_count && wg >= ss.s[k].wg_base && wg < ss.s[k].wg_base + nwg) s = k;
+    }
     return s;
 }
 
@@ -389,64 +390,46 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
         const uint64_t rem = vol - (uint64_t)j * kSub;
         it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
         it.stream = s;
-        it.pad_ = 0;
+        it.bits = sbits[sb + j];
         items[ib + j] = it;
         bit += sbits[sb + j];
         u += sun[sb + j];
     }
 }
 
-__global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
-                                                     const DecTab* __restrict__ tabs,
-                                                     const uint16_t* __restrict__ sorted,
-                                                     const DecItem* __restrict__ items,
-                                                     uint16_t* __restrict__ codes,
-                                                     uint8_t* __restrict__ side,
-                                                     uint32_t* __restrict__ err)
+// Sequential Huffman decode of one lane's sub-chunk.  Bits come from `W`
+// (the workgroup's LDS stage or global memory; word w of the stream is
+// W[w - wbase]).  The refill is branch-free and its word is fetched one
+// symbol ahead, so the only serial latency per symbol is the table lookup.
+// Codes leave in 64-byte bursts (32 codes held in registers) when the
+// destination is 64-byte aligned, in 8-byte groups otherwise.
+template <int kGroup>
+__device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
+                                               const DecItem& it, const uint32_t* __restrict__ lut,
+                                               const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
+                                               const uint32_t* __restrict__ fi, const uint16_t* __restrict__ srt,
+                                               uint16_t* __restrict__ co, uint8_t* __restrict__ so,
+                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred)
 {
-    __shared__ uint32_t lut[1 << kLutBits];
-    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
-    const uint32_t first = blockIdx.x * 256u;
-    const int s = find_stream_by_item(ss, first);
-    if (s < 0) return;  // uniform across the workgroup: padding blocks do nothing
-    const DecTab* t = tabs + s;
-    for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
-    if (threadIdx.x < kLmax + 2) {
-        fc[threadIdx.x] = t->first_code[threadIdx.x];
-        cn[threadIdx.x] = t->count[threadIdx.x];
-        fi[threadIdx.x] = t->first_index[threadIdx.x];
-    }
-    __syncthreads();
-    const StreamDesc& d = ss.s[s];
-    const uint32_t item = first + threadIdx.x;
-    if (item >= d.item_base + d.item_count) return;
-    const DecItem it = items[item];
-    const uint32_t* bw = (const uint32_t*)(payload + d.off_bits);
-    const uint8_t* un = payload + d.off_unpred;
-    const uint16_t* srt = sorted + t->sorted_off;
-    uint16_t* co = codes + d.code_off + it.p0;
-    uint8_t* so = side + (d.code_off + it.p0) * d.tsize;
-    uint64_t wpos = it.bit0 >> 5;
+    uint64_t wpos = (it.bit0 >> 5) - wbase;
     const int skip = (int)(it.bit0 & 31u);
     uint32_t bad = 0;
-    if (wpos + 1 >= d.bits_words) {  // offsets are validated on upload; stay in bounds regardless
-        atomicOr(err, 1u);
-        return;
-    }
-    uint64_t buf = (((uint64_t)bw[wpos] << 32) | bw[wpos + 1]) << skip;
+    if (wpos + 2 > nwords) return 1u;
+    uint64_t buf = (((uint64_t)W[wpos] << 32) | W[wpos + 1]) << skip;
     int nb = 64 - skip;
     wpos += 2;
+    uint32_t nextw = wpos < nwords ? W[wpos] : 0u;
     uint64_t u = it.u0;
-    const uint64_t nwords = d.bits_words;
-    for (uint32_t i = 0; i < it.cnt; ++i) {
-        if (nb < 32) {
-            // reads past the end only happen on a corrupt stream: feed zeros and flag it
-            const uint32_t w = wpos < nwords ? bw[wpos] : 0u;
-            bad |= wpos >= nwords ? 1u : 0u;
-            wpos++;
-            buf |= (uint64_t)w << (32 - nb);
-            nb += 32;
-        }
+    const uint32_t cnt = it.cnt;
+    uint32_t i = 0;
+    auto step = [&]() -> uint32_t {
+        const bool need = nb < 32;
+        const int sh = need ? 32 - nb : 0;
+        buf |= need ? ((uint64_t)nextw << sh) : 0ull;
+        bad |= (need && wpos >= nwords) ? 1u : 0u;
+        wpos += need ? 1u : 0u;
+        nb += need ? 32 : 0;
+        nextw = wpos < nwords ? W[wpos] : 0u;
         const uint32_t e = lut[(uint32_t)(buf >> (64 - kLutBits))];
         uint32_t sym;
         int l;
@@ -467,20 +450,122 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
         }
         buf <<= l;
         nb -= l;
-        co[i] = (uint16_t)sym;
         if (sym == 0u) {
-            if (u >= d.n_unpred) {
+            if (u < n_unpred) {
+                if (tsize == 4) ((uint32_t*)so)[i] = ((const uint32_t*)un)[u];
+                else ((uint64_t*)so)[i] = ((const uint64_t*)un)[u];
+            } else {
                 bad = 1;
-                u = 0;
-                if (d.n_unpred == 0) continue;
             }
-            if (d.tsize == 4)
-                ((uint32_t*)so)[i] = ((const uint32_t*)un)[u];
-            else
-                ((uint64_t*)so)[i] = ((const uint64_t*)un)[u];
             u++;
         }
+        return sym;
+    };
+    if (kGroup == 32) {
+        // 32 codes -> 4 x 16-byte stores
+        for (; i + 32 <= cnt;) {
+            uint32_t r[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t a = step();
+                ++i;
+                const uint32_t b = step();
+                ++i;
+                r[k] = a | (b << 16);
+            }
+            uint4* d = (uint4*)(co + i - 32);
+            d[0] = make_uint4(r[0], r[1], r[2], r[3]);
+            d[1] = make_uint4(r[4], r[5], r[6], r[7]);
+            d[2] = make_uint4(r[8], r[9], r[10], r[11]);
+            d[3] = make_uint4(r[12], r[13], r[14], r[15]);
+        }
+    } else if (kGroup == 4) {
+        for (; i + 4 <= cnt;) {
+            const uint32_t a = step();
+            ++i;
+            const uint32_t b = step();
+            ++i;
+            const uint32_t c = step();
+            ++i;
+            const uint32_t d = step();
+            ++i;
+            *(uint2*)(co + i - 4) = make_uint2(a | (b << 16), c | (d << 16));
+        }
     }
+    for (; i < cnt; ++i) co[i] = (uint16_t)step();
+    return bad;
+}
+
+// One workgroup = `ipw` consecutive sub-chunks of one stream, one per lane.
+// Their bits form one contiguous range; when it fits, the workgroup first
+// copies it into LDS with coalesced loads, so the lanes' data-dependent
+// refills are LDS reads.  ipw is chosen per stream on the host so that the
+// expected range fits the stage.
+__global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
+                                                     const DecTab* __restrict__ tabs,
+                                                     const uint16_t* __restrict__ sorted,
+                                                     const DecItem* __restrict__ items,
+                                                     uint16_t* __restrict__ codes,
+                                                     uint8_t* __restrict__ side,
+                                                     uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
+    __shared__ uint32_t stage[kDecStageWords];
+    const int s = find_stream_by_wg(ss, blockIdx.x);
+    if (s < 0) return;  // uniform across the workgroup
+    const StreamDesc& d = ss.s[s];
+    const DecTab* t = tabs + s;
+    for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
+    if (threadIdx.x < kLmax + 2) {
+        fc[threadIdx.x] = t->first_code[threadIdx.x];
+        cn[threadIdx.x] = t->count[threadIdx.x];
+        fi[threadIdx.x] = t->first_index[threadIdx.x];
+    }
+    const uint32_t first = d.item_base + (blockIdx.x - d.wg_base) * d.ipw;
+    const uint32_t end = d.item_base + d.item_count;
+    const uint32_t lim = first + d.ipw < end ? first + d.ipw : end;
+    const uint32_t last = lim - 1u;
+    const uint32_t* bw = (const uint32_t*)(payload + d.off_bits);
+    // items of a stream are in increasing bit order, so the range is [first.bit0, last.bit0 + last.bits)
+    const uint64_t w0 = items[first].bit0 >> 5;
+    const uint64_t w1r = ((items[last].bit0 + items[last].bits + 31) >> 5) + 2;
+    const uint64_t w1 = w1r < d.bits_words ? w1r : d.bits_words;
+    const bool use_lds = w1 > w0 && (w1 - w0) <= (uint64_t)kDecStageWords;
+    if (use_lds) {
+        const uint64_t nw = w1 - w0;
+        const uint64_t a0 = (w0 + 3) & ~3ull;  // 16-byte aligned body
+        const uint64_t h = a0 - w0 < nw ? a0 - w0 : nw;
+        if (threadIdx.x < h) stage[threadIdx.x] = bw[w0 + threadIdx.x];
+        const uint64_t nb4 = (nw - h) / 4;
+        const uint4* src = (const uint4*)(bw + a0);
+        for (uint64_t k = threadIdx.x; k < nb4; k += 256) {
+            const uint4 v = src[k];
+            uint32_t* dst = stage + h + 4 * k;
+            dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
+        }
+        for (uint64_t k = h + 4 * nb4 + threadIdx.x; k < nw; k += 256) stage[k] = bw[w0 + k];
+    }
+    __syncthreads();
+    const uint32_t item = first + threadIdx.x;
+    if (item >= lim) return;
+    const DecItem it = items[item];
+    const uint16_t* srt = sorted + t->sorted_off;
+    uint16_t* co = codes + d.code_off + it.p0;
+    uint8_t* so = side + (d.code_off + it.p0) * d.tsize;
+    const uint8_t* un = payload + d.off_unpred;
+    const uint64_t al = (d.code_off + it.p0);
+    const bool in_stage = use_lds && it.bit0 >= (w0 << 5) && ((it.bit0 + it.bits + 31) >> 5) + 2 <= w1;
+    const uint32_t* W = in_stage ? stage : bw;
+    const uint64_t wb = in_stage ? w0 : 0;
+    const uint64_t nw = in_stage ? w1 - w0 : d.bits_words;
+    uint32_t bad;
+    if ((al & 31u) == 0)
+        bad = decode_run<32>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+    else if ((al & 3u) == 0)
+        bad = decode_run<4>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+    else
+        bad = decode_run<1>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
     if (bad) atomicOr(err, 1u);
 }
 
@@ -549,8 +634,8 @@ void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* paylo
                         const uint16_t* sorted, const DecItem* items, uint16_t* codes, void* side,
                         uint32_t* err)
 {
-    if (!ss.total_items) return;
-    k_huff_decode<<<dim3(ss.total_items / 256), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items,
+    if (!ss.total_wgs) return;
+    k_huff_decode<<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items,
                                                                    codes, (uint8_t*)side, err);
 }
 

@@ -25,14 +25,18 @@ struct StreamDesc {
     uint32_t tsize;          // bytes per raw value (4 or 8)
     uint32_t pad_;
     uint64_t n_unpred;       // unpredictable values stored in the stream
+    uint32_t ipw;            // decode work items per workgroup (<= 256)
+    uint32_t wg_base;        // first decode workgroup of this stream
 };
 
 struct StreamSet {
     StreamDesc s[kStreams];
     uint32_t total_chunks;
     uint32_t total_sub;
-    uint32_t total_items;    // padded to 256 per stream
+    uint32_t total_items;    // padded to ipw per stream
     uint32_t nstreams;
+    uint32_t total_wgs;      // decode workgroups
+    uint32_t pad_[3];
 };
 
 // decode table of one stream (device memory)
@@ -51,7 +55,7 @@ struct DecItem {
     uint32_t p0;       // stream position of the first symbol
     uint32_t cnt;      // symbols
     uint32_t stream;
-    uint32_t pad_;
+    uint32_t bits;     // encoded bits of the sub-chunk
 };
 
 // ---- k_huff.hip ----

@@ -611,13 +611,18 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* w
     const size_t ts = tsize_of((int)m.fh.dtype);
     out.codes = DBuf(c->pool, L.total_codes * 2);
     out.side = DBuf(c->pool, L.total_codes * ts);
-    // work items: per stream, 256-aligned ranges
+    // work items: per stream, ranges padded to its items-per-workgroup
     std::vector<uint32_t> list;  // (stream, chunk, item) triples for partial streams
     bool all_full = true;
-    uint32_t item = 0;
+    uint32_t item = 0, wg = 0;
     for (int s = 0; s < kStreams; ++s) {
         StreamDesc& d = ss.s[s];
         d.item_base = item;
+        // sub-chunks per workgroup so that their expected bits fit the 48 KiB LDS stage
+        const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
+        d.ipw = 256;
+        while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > 12288.0 * 32.0) d.ipw >>= 1;
+        d.wg_base = wg;
         if (want_all[s]) {
             d.item_count = L.n_sub[s];
             for (uint32_t ch = 0; ch < L.g[s].n_chunks; ++ch) {
@@ -636,9 +641,12 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* w
             }
             d.item_count = k;
         }
-        item += (d.item_count + 255u) & ~255u;
+        const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
+        item += nwg * d.ipw;
+        wg += nwg;
     }
     ss.total_items = item;
+    ss.total_wgs = wg;
     if (!item) return;
     DBuf items(c->pool, (size_t)item * sizeof(DecItem)), err(c->pool, 64);
     uint8_t* pay = ds->payload.as<uint8_t>();
