@@ -154,10 +154,19 @@ __global__ __launch_bounds__(256) void k_huff_len(StreamSet ss, const uint16_t* 
         const uint64_t p0 = pstart + (uint64_t)j * kSub;
         const uint64_t p1 = p0 + kSub < pend ? p0 + kSub : pend;
         uint32_t bits = 0, un = 0;
-        for (uint64_t p = p0 + lane; p < p1; p += kWave) {
-            const uint32_t q = cs[p];
-            bits += es[q] >> 24;
-            un += q == 0u;
+        // all 16 codes of the lane first, then 16 independent table reads
+        uint32_t q[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint64_t p = p0 + lane + (uint64_t)k * kWave;
+            q[k] = p < p1 ? (uint32_t)cs[p] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (q[k] != 0xFFFFFFFFu) {
+                bits += es[q[k]] >> 24;
+                un += q[k] == 0u;
+            }
         }
         bits = wave_sum_u32(bits);
         un = wave_sum_u32(un);

@@ -47,11 +47,14 @@ struct PassPoint {
     int dim;
 };
 
-__device__ __forceinline__ PassPoint pass_point(const InterpPass& p, uint64_t k)
+__device__ __forceinline__ PassPoint pass_point(const InterpPass& p, uint64_t k64)
 {
-    const int ix = (int)(k % p.cnt[0]);
-    const int iy = (int)((k / p.cnt[0]) % p.cnt[1]);
-    const int iz = (int)(k / ((uint64_t)p.cnt[0] * p.cnt[1]));
+    const uint32_t k = (uint32_t)k64;  // a pass has < 2^31 points
+    const uint32_t c0 = (uint32_t)p.cnt[0], c1 = (uint32_t)p.cnt[1];
+    const uint32_t t = k / c0;
+    const int ix = (int)(k - t * c0);
+    const int iz = (int)(t / c1);
+    const int iy = (int)(t - (uint32_t)iz * c1);
     const int cx = p.start[0] + ix * p.step[0];
     const int cy = p.start[1] + iy * p.step[1];
     const int cz = p.start[2] + iz * p.step[2];
@@ -141,6 +144,88 @@ __global__ __launch_bounds__(256) void k_interp_decode(InterpPass p, T* __restri
     }
 }
 
+// Coarse passes back to back in one workgroup of 1024 lanes, on the
+// stride-S0 lattice held in LDS (its predicates in lattice units equal the
+// base-grid ones: pos + s < dim  <=>  pos/S0 + s/S0 < (dim-1)/S0 + 1).
+// The barrier between passes orders them.
+template <typename T>
+__device__ __forceinline__ void lattice_io(const InterpPassList& l, T* __restrict__ L, T* __restrict__ v, bool load)
+{
+    const uint32_t n = (uint32_t)l.lx * l.ly * l.lz;
+    for (uint32_t k = threadIdx.x; k < n; k += 1024u) {
+        const uint32_t t = k / (uint32_t)l.lx;
+        const int x = (int)(k - t * (uint32_t)l.lx);
+        const int z = (int)(t / (uint32_t)l.ly);
+        const int y = (int)(t - (uint32_t)z * (uint32_t)l.ly);
+        const size_t g = ((size_t)(z * l.S0) * l.by + (size_t)y * l.S0) * l.bx + (size_t)x * l.S0;
+        if (load) L[k] = v[g];
+        else v[g] = L[k];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_encode_fused(InterpPassList l, T* __restrict__ v,
+                                                              uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                              uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t sh[kHW];
+    __shared__ T L[kLatticeBytes / sizeof(T)];
+    for (int k = threadIdx.x; k < kHW; k += 1024) sh[k] = 0u;
+    lattice_io(l, L, v, true);
+    __syncthreads();
+    for (int pi = 0; pi < l.n; ++pi) {
+        const InterpPass& p = l.p[pi];
+        const uint64_t kend = (p.n + 1023u) / 1024u * 1024u;
+        for (uint64_t k = threadIdx.x; k < kend; k += 1024u) {
+            const bool ok = k < p.n;
+            uint32_t q = 0;
+            if (ok) {
+                T dec;
+                if (p.axis < 0) {
+                    q = quantize(L[0], p.eb, p.recip, dec);
+                    codes[p.pos + k] = (uint16_t)q;
+                    if (q) L[0] = dec; else side[p.pos + k] = L[0];
+                } else {
+                    const PassPoint pp = pass_point(p, k);
+                    const T pred = interp_pred(L, pp.c, pp.st, pp.pos_axis, p.s, pp.dim, p.interp);
+                    const T orig = L[pp.c];
+                    q = quantize(orig - pred, p.eb, p.recip, dec);
+                    codes[p.pos + k] = (uint16_t)q;
+                    if (q) L[pp.c] = pred + dec; else side[p.pos + k] = orig;
+                }
+            }
+            hist_add(sh, hist, q, ok);
+        }
+        __syncthreads();
+    }
+    lattice_io(l, L, v, false);
+    for (int k = threadIdx.x; k < kHW; k += 1024)
+        if (sh[k]) atomicAdd(&hist[kHLo + k], sh[k]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_decode_fused(InterpPassList l, T* __restrict__ v,
+                                                              const uint16_t* __restrict__ codes,
+                                                              const T* __restrict__ side)
+{
+    __shared__ T L[kLatticeBytes / sizeof(T)];
+    for (int pi = 0; pi < l.n; ++pi) {
+        const InterpPass& p = l.p[pi];
+        for (uint64_t k = threadIdx.x; k < p.n; k += 1024u) {
+            const uint32_t q = codes[p.pos + k];
+            if (p.axis < 0) {
+                L[0] = q ? dequantize<T>(q, p.eb) : side[p.pos + k];
+            } else {
+                const PassPoint pp = pass_point(p, k);
+                const T pred = interp_pred(L, pp.c, pp.st, pp.pos_axis, p.s, pp.dim, p.interp);
+                L[pp.c] = q ? pred + dequantize<T>(q, p.eb) : side[p.pos + k];
+            }
+        }
+        __syncthreads();
+    }
+    lattice_io(l, L, v, false);
+}
+
 inline unsigned pass_grid(uint64_t n)
 {
     uint64_t b = (n + 255) / 256;
@@ -163,6 +248,24 @@ void launch_interp_decode(hipStream_t s, const InterpPass& p, T* v, const uint16
     k_interp_decode<T><<<dim3(pass_grid(p.n)), dim3(256), 0, s>>>(p, v, codes, side);
 }
 
+template <typename T>
+void launch_interp_encode_fused(hipStream_t s, const InterpPassList& l, T* v, uint16_t* codes, T* side, uint32_t* hist)
+{
+    if (l.n <= 0) return;
+    k_interp_encode_fused<T><<<dim3(1), dim3(1024), 0, s>>>(l, v, codes, side, hist);
+}
+
+template <typename T>
+void launch_interp_decode_fused(hipStream_t s, const InterpPassList& l, T* v, const uint16_t* codes, const T* side)
+{
+    if (l.n <= 0) return;
+    k_interp_decode_fused<T><<<dim3(1), dim3(1024), 0, s>>>(l, v, codes, side);
+}
+
+template void launch_interp_encode_fused<float>(hipStream_t, const InterpPassList&, float*, uint16_t*, float*, uint32_t*);
+template void launch_interp_encode_fused<double>(hipStream_t, const InterpPassList&, double*, uint16_t*, double*, uint32_t*);
+template void launch_interp_decode_fused<float>(hipStream_t, const InterpPassList&, float*, const uint16_t*, const float*);
+template void launch_interp_decode_fused<double>(hipStream_t, const InterpPassList&, double*, const uint16_t*, const double*);
 template void launch_interp_encode<float>(hipStream_t, const InterpPass&, float*, uint16_t*, float*, uint32_t*);
 template void launch_interp_encode<double>(hipStream_t, const InterpPass&, double*, uint16_t*, double*, uint32_t*);
 template void launch_interp_decode<float>(hipStream_t, const InterpPass&, float*, const uint16_t*, const float*);

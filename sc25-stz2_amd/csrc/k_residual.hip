@@ -69,27 +69,174 @@ __device__ __forceinline__ void hist_flush(uint32_t (*sh)[kHW], int nstreams, ui
     }
 }
 
+// ---------------------------------------------------------------- predictors
+
+// Interior predictions of all 7 blocks at x on the row `row` (the block grid
+// row of the point, so the point is row[x]); requires 1 <= x <= dx-3 and the
+// same on y and z.  The 34 neighbours sit on 12 rows whose base pointers are
+// wave-uniform (scalar), so every load is scalar base + lane offset + an
+// immediate.  Same expressions, in the same order, as predict<B>'s interior.
 template <typename T>
-struct Tile {
-    int x, y, z;
-    bool ok;
+__device__ __forceinline__ void predict7_interior(const T* __restrict__ row, int x, int sx, int sxy, T p[7])
+{
+    const T* R00 = row;
+    const T* Rm0 = row - sx;             // (dy, dz) = (-1, 0)
+    const T* R10 = row + sx;
+    const T* R20 = row + 2 * sx;
+    const T* R0m = row - sxy;            // (0, -1)
+    const T* R01 = row + sxy;
+    const T* R02 = row + 2 * sxy;
+    const T* Rmm = row - sxy - sx;       // (-1, -1)
+    const T* R11 = row + sxy + sx;
+    const T* R22 = row + 2 * sxy + 2 * sx;
+    const T* R2m = row - sxy + 2 * sx;   // (2, -1)
+    const T* Rm2 = row + 2 * sxy - sx;   // (-1, 2)
+    const T c = R00[x];
+    p[0] = cub(R00[x - 1], c, R00[x + 1], R00[x + 2]);
+    p[1] = cub(Rm0[x], c, R10[x], R20[x]);
+    p[3] = cub(R0m[x], c, R01[x], R02[x]);
+    T p0, v0, p1, v1, p2, v2, p3, v3;
+    pv(Rm0[x - 1], c, R10[x + 1], R20[x + 2], p0, v0);
+    pv(Rm0[x + 2], R00[x + 1], R10[x], R20[x - 1], p1, v1);
+    p[2] = blend2(p0, v0, p1, v1);
+    pv(R0m[x - 1], c, R01[x + 1], R02[x + 2], p0, v0);
+    pv(R0m[x + 2], R00[x + 1], R01[x], R02[x - 1], p1, v1);
+    p[4] = blend2(p0, v0, p1, v1);
+    pv(Rmm[x], c, R11[x], R22[x], p0, v0);
+    pv(R2m[x], R10[x], R01[x], Rm2[x], p1, v1);
+    p[5] = blend2(p0, v0, p1, v1);
+    pv(Rmm[x - 1], c, R11[x + 1], R22[x + 2], p0, v0);
+    pv(Rmm[x + 2], R00[x + 1], R11[x], R22[x - 1], p1, v1);
+    pv(R2m[x - 1], R10[x], R01[x + 1], Rm2[x + 2], p2, v2);
+    pv(Rm2[x - 1], R01[x], R10[x + 1], R2m[x + 2], p3, v3);
+    p[6] = blend4(p0, v0, p1, v1, p2, v2, p3, v3);
+}
+
+// all 7 predictions at (x,y,z): the interior form when `inner` (wave-uniform), else checked
+template <typename T>
+__device__ __forceinline__ void predict7(bool inner, const T* __restrict__ r, int x, int y, int z, int dx, int dy,
+                                         int dz, T p[7])
+{
+    if (inner) {
+        predict7_interior(r + ((size_t)z * dy + y) * dx, x, dx, dx * dy, p);
+    } else {
+        const int i = (z * dy + y) * dx + x;
+        p[0] = predict<0>(r, i, x, y, z, dx, dy, dz);
+        p[1] = predict<1>(r, i, x, y, z, dx, dy, dz);
+        p[2] = predict<2>(r, i, x, y, z, dx, dy, dz);
+        p[3] = predict<3>(r, i, x, y, z, dx, dy, dz);
+        p[4] = predict<4>(r, i, x, y, z, dx, dy, dz);
+        p[5] = predict<5>(r, i, x, y, z, dx, dy, dz);
+        p[6] = predict<6>(r, i, x, y, z, dx, dy, dz);
+    }
+}
+
+// Work split of a block grid: the main kernels cover x in [1, dx-3] in
+// 64 x 4 tiles (a wave is one row, so y and z are wave-uniform and rows away
+// from the y/z faces take the interior form); the edge kernels cover the
+// columns x in {0, dx-2, dx-1} (every x when dx < 4) with checked predictors.
+struct EdgeMap {
+    int ncol;       // columns handled by the edge kernel
+    int cols[4];
 };
+
+__host__ __device__ inline EdgeMap edge_map(int dx)
+{
+    EdgeMap e{};
+    if (dx < 4) {
+        e.ncol = dx;
+        for (int k = 0; k < dx; ++k) e.cols[k] = k;
+    } else {
+        e.ncol = 3;
+        e.cols[0] = 0;
+        e.cols[1] = dx - 2;
+        e.cols[2] = dx - 1;
+    }
+    return e;
+}
+
+__device__ __forceinline__ bool row_inner(int y, int z, int dy, int dz)
+{
+    return y >= 1 && y <= dy - 3 && z >= 1 && z <= dz - 3;
+}
 
 // ---------------------------------------------------------------- encode
 
-template <int B, typename T>
-__device__ __forceinline__ void enc_one(const T* __restrict__ ref, int i, int x, int y, int z, int dx, int dy,
-                                        int dz, T tgt, const LevelArgs& a, uint32_t pos,
-                                        uint16_t* __restrict__ codes, T* __restrict__ side, uint32_t& q, T& rec)
+template <typename T>
+__device__ __forceinline__ uint32_t enc1(T tgt, T p, const LevelArgs& a, int b, uint32_t pos,
+                                         uint16_t* __restrict__ codes, T* __restrict__ side, T& rec)
 {
-    const T p = predict<B>(ref, i, x, y, z, dx, dy, dz);
     const T d = tgt - p;
     T dec;
-    q = quantize(d, a.eb, a.recip, dec);
-    codes[a.code_off[B] + pos] = (uint16_t)q;
-    if (!q) side[a.code_off[B] + pos] = d;
+    const uint32_t q = quantize(d, a.eb, a.recip, dec);
+    codes[a.code_off[b] + pos] = (uint16_t)q;
+    if (!q) side[a.code_off[b] + pos] = d;
     rec = dec + p;  // decoded diff + prediction (F5:1381-1385 then F5:1077-1083)
+    return q;
 }
+
+// high level: targets of the 2x2x2 cell of mid point (x,y,z) straight from the field
+template <typename T>
+__device__ __forceinline__ void high_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ mid,
+                                               const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
+                                               T* __restrict__ side, int x, int y, int z, bool inner, uint32_t q[7])
+{
+    typedef typename V2<T>::type v2;
+    T p[7];
+    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
+    const uint32_t pos = stream_pos(g, x, y, z);
+    const size_t fy = (size_t)px, fz = (size_t)px * py;
+    const size_t f = ((size_t)(2 * z) * py + 2 * y) * px + 2 * x;
+    const v2 r00 = *(const v2*)(full + f);
+    const v2 r01 = *(const v2*)(full + f + fy);
+    const v2 r10 = *(const v2*)(full + f + fz);
+    const v2 r11 = *(const v2*)(full + f + fz + fy);
+    const T tg[7] = {r00.y, r01.x, r01.y, r10.x, r10.y, r11.x, r11.y};
+    T rec;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) q[b] = enc1(tg[b], p[b], a, b, pos, codes, side, rec);
+}
+
+// low level: targets from the field at stride 4, reconstruction written into the mid grid
+template <typename T>
+__device__ __forceinline__ void low_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ base,
+                                              const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
+                                              T* __restrict__ side, T* __restrict__ mid, int x, int y, int z,
+                                              bool inner, bool vec, uint32_t q[7])
+{
+    typedef typename V2<T>::type v2;
+    const int lx = g.dx, ly = g.dy;
+    const int mx = 2 * lx, my = 2 * ly;
+    T p[7];
+    predict7(inner, base, x, y, z, lx, ly, g.dz, p);
+    const uint32_t pos = stream_pos(g, x, y, z);
+    // low block k at (x,y,z) = mid[2z+pz][2y+py][2x+px] = full[4z+2pz][4y+2py][4x+2px]
+    const size_t fy = (size_t)2 * px, fz = (size_t)2 * px * py;
+    const size_t f = ((size_t)(4 * z) * py + 4 * y) * px + 4 * x;
+    const T tg[7] = {full[f + 2], full[f + fy], full[f + fy + 2], full[f + fz], full[f + fz + 2],
+                     full[f + fz + fy], full[f + fz + fy + 2]};
+    T r[8];
+    r[0] = base[((size_t)z * ly + y) * lx + x];
+#pragma unroll
+    for (int b = 0; b < 7; ++b) q[b] = enc1(tg[b], p[b], a, b, pos, codes, side, r[b + 1]);
+    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
+    const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
+    if (vec) {
+        v2 w;
+        w.x = r[0]; w.y = r[1]; *(v2*)(mid + m) = w;
+        w.x = r[2]; w.y = r[3]; *(v2*)(mid + m + my_) = w;
+        w.x = r[4]; w.y = r[5]; *(v2*)(mid + m + mz_) = w;
+        w.x = r[6]; w.y = r[7]; *(v2*)(mid + m + mz_ + my_) = w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) mid[m + (k >> 2) * mz_ + ((k >> 1) & 1) * my_ + (k & 1)] = r[k];
+    }
+}
+
+#define STZ_HIST_ADD7(q, ok)                                                    \
+    do {                                                                        \
+        _Pragma("unroll") for (int b_ = 0; b_ < 7; ++b_) hist_add(sh[b_], hist + b_ * 65536, q[b_], ok); \
+    } while (0)
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_high_encode(const T* __restrict__ full, int px, int py,
@@ -97,44 +244,45 @@ __global__ __launch_bounds__(256) void k_high_encode(const T* __restrict__ full,
                                                      uint16_t* __restrict__ codes, T* __restrict__ side,
                                                      uint32_t* __restrict__ hist)
 {
-    typedef typename V2<T>::type v2;
     __shared__ uint32_t sh[7][kHW];
     for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
     __syncthreads();
     const int mx = g.dx, my = g.dy, mz = g.dz;
-    const int ntx = (mx + 63) >> 6, nty = (my + 3) >> 2;
-    const uint64_t ntiles = (uint64_t)ntx * nty * mz;
+    const int nxi = mx - 3;  // interior columns 1 .. mx-3
+    const int ntx = (nxi + 63) >> 6, nty = (my + 3) >> 2;
+    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * mz : 0;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const size_t fy = (size_t)px, fz = (size_t)px * py;
     for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
         const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = tx * 64 + lane, y = ty * 4 + row;
-        const bool ok = x < mx && y < my;
-        uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0;
+        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
+        const bool ok = x <= mx - 3 && y < my;
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (ok) high_enc_point(full, px, py, mid, g, a, codes, side, x, y, z, row_inner(y, z, my, mz), q);
+        STZ_HIST_ADD7(q, ok);
+    }
+    hist_flush(sh, 7, hist);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_high_encode_edge(const T* __restrict__ full, int px, int py,
+                                                          const T* __restrict__ mid, Geom g, LevelArgs a,
+                                                          uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                          uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t sh[7][kHW];
+    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
+    __syncthreads();
+    const EdgeMap e = edge_map(g.dx);
+    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
+    const uint64_t kend = (n + 255) / 256 * 256;
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < kend; k += (uint64_t)gridDim.x * 256ull) {
+        const bool ok = k < n;
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
         if (ok) {
-            const int i = (z * my + y) * mx + x;
-            const uint32_t pos = stream_pos(g, x, y, z);
-            const size_t f = ((size_t)(2 * z) * py + 2 * y) * px + 2 * x;
-            const v2 r00 = *(const v2*)(full + f);
-            const v2 r01 = *(const v2*)(full + f + fy);
-            const v2 r10 = *(const v2*)(full + f + fz);
-            const v2 r11 = *(const v2*)(full + f + fz + fy);
-            T rec;
-            enc_one<0>(mid, i, x, y, z, mx, my, mz, r00.y, a, pos, codes, side, q0, rec);
-            enc_one<1>(mid, i, x, y, z, mx, my, mz, r01.x, a, pos, codes, side, q1, rec);
-            enc_one<2>(mid, i, x, y, z, mx, my, mz, r01.y, a, pos, codes, side, q2, rec);
-            enc_one<3>(mid, i, x, y, z, mx, my, mz, r10.x, a, pos, codes, side, q3, rec);
-            enc_one<4>(mid, i, x, y, z, mx, my, mz, r10.y, a, pos, codes, side, q4, rec);
-            enc_one<5>(mid, i, x, y, z, mx, my, mz, r11.x, a, pos, codes, side, q5, rec);
-            enc_one<6>(mid, i, x, y, z, mx, my, mz, r11.y, a, pos, codes, side, q6, rec);
+            const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
+            high_enc_point(full, px, py, mid, g, a, codes, side, e.cols[c], y, z, false, q);
         }
-        hist_add(sh[0], hist + 0 * 65536, q0, ok);
-        hist_add(sh[1], hist + 1 * 65536, q1, ok);
-        hist_add(sh[2], hist + 2 * 65536, q2, ok);
-        hist_add(sh[3], hist + 3 * 65536, q3, ok);
-        hist_add(sh[4], hist + 4 * 65536, q4, ok);
-        hist_add(sh[5], hist + 5 * 65536, q5, ok);
-        hist_add(sh[6], hist + 6 * 65536, q6, ok);
+        STZ_HIST_ADD7(q, ok);
     }
     hist_flush(sh, 7, hist);
 }
@@ -145,66 +293,124 @@ __global__ __launch_bounds__(256) void k_low_encode(const T* __restrict__ full, 
                                                     uint16_t* __restrict__ codes, T* __restrict__ side,
                                                     uint32_t* __restrict__ hist, T* __restrict__ mid)
 {
-    typedef typename V2<T>::type v2;
     __shared__ uint32_t sh[7][kHW];
     for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
     __syncthreads();
     const int lx = g.dx, ly = g.dy, lz = g.dz;
-    const int mx = 2 * lx, my = 2 * ly;
-    const int ntx = (lx + 63) >> 6, nty = (ly + 3) >> 2;
-    const uint64_t ntiles = (uint64_t)ntx * nty * lz;
+    const int nxi = lx - 3;
+    const int ntx = (nxi + 63) >> 6, nty = (ly + 3) >> 2;
+    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * lz : 0;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const size_t fy = (size_t)2 * px, fz = (size_t)2 * px * py;  // one mid step = two full steps
-    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
     for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
         const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = tx * 64 + lane, y = ty * 4 + row;
-        const bool ok = x < lx && y < ly;
-        uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0, q6 = 0;
+        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
+        const bool ok = x <= lx - 3 && y < ly;
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (ok) low_enc_point(full, px, py, base, g, a, codes, side, mid, x, y, z, row_inner(y, z, ly, lz), true, q);
+        STZ_HIST_ADD7(q, ok);
+    }
+    hist_flush(sh, 7, hist);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_low_encode_edge(const T* __restrict__ full, int px, int py,
+                                                         const T* __restrict__ base, Geom g, LevelArgs a,
+                                                         uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                         uint32_t* __restrict__ hist, T* __restrict__ mid)
+{
+    __shared__ uint32_t sh[7][kHW];
+    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
+    __syncthreads();
+    const EdgeMap e = edge_map(g.dx);
+    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
+    const uint64_t kend = (n + 255) / 256 * 256;
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < kend; k += (uint64_t)gridDim.x * 256ull) {
+        const bool ok = k < n;
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
         if (ok) {
-            const int i = (z * ly + y) * lx + x;
-            const uint32_t pos = stream_pos(g, x, y, z);
-            // low block k at (x,y,z) = mid[2z+pz][2y+py][2x+px] = full[4z+2pz][4y+2py][4x+2px]
-            const size_t f = ((size_t)(4 * z) * py + 4 * y) * px + 4 * x;
-            const T t1 = full[f + 2], t2 = full[f + fy], t3 = full[f + fy + 2];
-            const T t4 = full[f + fz], t5 = full[f + fz + 2], t6 = full[f + fz + fy], t7 = full[f + fz + fy + 2];
-            T r1, r2, r3, r4, r5, r6, r7;
-            enc_one<0>(base, i, x, y, z, lx, ly, lz, t1, a, pos, codes, side, q0, r1);
-            enc_one<1>(base, i, x, y, z, lx, ly, lz, t2, a, pos, codes, side, q1, r2);
-            enc_one<2>(base, i, x, y, z, lx, ly, lz, t3, a, pos, codes, side, q2, r3);
-            enc_one<3>(base, i, x, y, z, lx, ly, lz, t4, a, pos, codes, side, q3, r4);
-            enc_one<4>(base, i, x, y, z, lx, ly, lz, t5, a, pos, codes, side, q4, r5);
-            enc_one<5>(base, i, x, y, z, lx, ly, lz, t6, a, pos, codes, side, q5, r6);
-            enc_one<6>(base, i, x, y, z, lx, ly, lz, t7, a, pos, codes, side, q6, r7);
-            // unpredictable points reconstruct as raw residual + prediction
-            const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
-            v2 w;
-            w.x = base[i]; w.y = r1; *(v2*)(mid + m) = w;
-            w.x = r2; w.y = r3; *(v2*)(mid + m + my_) = w;
-            w.x = r4; w.y = r5; *(v2*)(mid + m + mz_) = w;
-            w.x = r6; w.y = r7; *(v2*)(mid + m + mz_ + my_) = w;
+            const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
+            low_enc_point(full, px, py, base, g, a, codes, side, mid, e.cols[c], y, z, false, false, q);
         }
-        hist_add(sh[0], hist + 0 * 65536, q0, ok);
-        hist_add(sh[1], hist + 1 * 65536, q1, ok);
-        hist_add(sh[2], hist + 2 * 65536, q2, ok);
-        hist_add(sh[3], hist + 3 * 65536, q3, ok);
-        hist_add(sh[4], hist + 4 * 65536, q4, ok);
-        hist_add(sh[5], hist + 5 * 65536, q5, ok);
-        hist_add(sh[6], hist + 6 * 65536, q6, ok);
+        STZ_HIST_ADD7(q, ok);
     }
     hist_flush(sh, 7, hist);
 }
 
 // ---------------------------------------------------------------- decode
 
-template <int B, typename T>
-__device__ __forceinline__ T dec_one(const T* __restrict__ ref, int i, int x, int y, int z, int dx, int dy,
-                                     int dz, const LevelArgs& a, uint32_t pos,
-                                     const uint16_t* __restrict__ codes, const T* __restrict__ side)
+template <typename T>
+__device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, const uint16_t* __restrict__ codes,
+                                  const T* __restrict__ side)
 {
-    const uint32_t q = codes[a.code_off[B] + pos];
-    const T v = q ? dequantize<T>(q, a.eb) : side[a.code_off[B] + pos];
-    return v + predict<B>(ref, i, x, y, z, dx, dy, dz);  // F5:1080 de_sub = deData + pred
+    const uint32_t q = codes[a.code_off[b] + pos];
+    const T v = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
+    return v + p;  // F5:1080 de_sub = deData + pred
+}
+
+template <typename T>
+__device__ __forceinline__ void low_dec_point(const T* __restrict__ base, const Geom& g, const LevelArgs& a,
+                                              const uint16_t* __restrict__ codes, const T* __restrict__ side,
+                                              T* __restrict__ mid, int x, int y, int z, bool inner, bool vec)
+{
+    typedef typename V2<T>::type v2;
+    const int lx = g.dx, ly = g.dy;
+    const int mx = 2 * lx, my = 2 * ly;
+    T p[7];
+    predict7(inner, base, x, y, z, lx, ly, g.dz, p);
+    const uint32_t pos = stream_pos(g, x, y, z);
+    T r[8];
+    r[0] = base[((size_t)z * ly + y) * lx + x];
+#pragma unroll
+    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, codes, side);
+    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
+    const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
+    if (vec) {
+        v2 w;
+        w.x = r[0]; w.y = r[1]; *(v2*)(mid + m) = w;
+        w.x = r[2]; w.y = r[3]; *(v2*)(mid + m + my_) = w;
+        w.x = r[4]; w.y = r[5]; *(v2*)(mid + m + mz_) = w;
+        w.x = r[6]; w.y = r[7]; *(v2*)(mid + m + mz_ + my_) = w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) mid[m + (k >> 2) * mz_ + ((k >> 1) & 1) * my_ + (k & 1)] = r[k];
+    }
+}
+
+// the 2x2x2 cell of mid point (x,y,z) into the caller's (nx,ny,nz) field;
+// vec = 2-wide stores (nx even and the output 2-element aligned)
+template <typename T>
+__device__ __forceinline__ void high_dec_point(const T* __restrict__ mid, const Geom& g, const LevelArgs& a,
+                                               const uint16_t* __restrict__ codes, const T* __restrict__ side,
+                                               T* __restrict__ out, int nx, int ny, int nz, int x, int y, int z,
+                                               bool inner, bool vec)
+{
+    typedef typename V2<T>::type v2;
+    const int X = 2 * x, Y = 2 * y, Z = 2 * z;
+    if (X >= nx || Y >= ny || Z >= nz) return;  // cell entirely in the padding
+    T p[7];
+    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
+    const uint32_t pos = stream_pos(g, x, y, z);
+    T v[8];
+    v[0] = mid[((size_t)z * g.dy + y) * g.dx + x];
+#pragma unroll
+    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, codes, side);
+    const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
+    const size_t o = (size_t)Z * oz + (size_t)Y * oy + X;
+    if (vec) {
+        v2 w;
+        w.x = v[0]; w.y = v[1]; *(v2*)(out + o) = w;
+        if (Y + 1 < ny) { w.x = v[2]; w.y = v[3]; *(v2*)(out + o + oy) = w; }
+        if (Z + 1 < nz) {
+            w.x = v[4]; w.y = v[5]; *(v2*)(out + o + oz) = w;
+            if (Y + 1 < ny) { w.x = v[6]; w.y = v[7]; *(v2*)(out + o + oz + oy) = w; }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int kx = k & 1, ky = (k >> 1) & 1, kz = k >> 2;
+            if (X + kx < nx && Y + ky < ny && Z + kz < nz) out[o + (size_t)kz * oz + (size_t)ky * oy + kx] = v[k];
+        }
+    }
 }
 
 template <typename T>
@@ -212,84 +418,62 @@ __global__ __launch_bounds__(256) void k_low_decode(const T* __restrict__ base, 
                                                     const uint16_t* __restrict__ codes,
                                                     const T* __restrict__ side, T* __restrict__ mid)
 {
-    typedef typename V2<T>::type v2;
     const int lx = g.dx, ly = g.dy, lz = g.dz;
-    const int mx = 2 * lx, my = 2 * ly;
-    const int ntx = (lx + 63) >> 6, nty = (ly + 3) >> 2;
-    const uint64_t ntiles = (uint64_t)ntx * nty * lz;
+    const int nxi = lx - 3;
+    const int ntx = (nxi + 63) >> 6, nty = (ly + 3) >> 2;
+    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * lz : 0;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
     for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
         const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = tx * 64 + lane, y = ty * 4 + row;
-        if (x >= lx || y >= ly) continue;
-        const int i = (z * ly + y) * lx + x;
-        const uint32_t pos = stream_pos(g, x, y, z);
-        const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
-        v2 w;
-        w.x = base[i];
-        w.y = dec_one<0>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        *(v2*)(mid + m) = w;
-        w.x = dec_one<1>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        w.y = dec_one<2>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        *(v2*)(mid + m + my_) = w;
-        w.x = dec_one<3>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        w.y = dec_one<4>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        *(v2*)(mid + m + mz_) = w;
-        w.x = dec_one<5>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        w.y = dec_one<6>(base, i, x, y, z, lx, ly, lz, a, pos, codes, side);
-        *(v2*)(mid + m + mz_ + my_) = w;
+        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
+        if (x > lx - 3 || y >= ly) continue;
+        low_dec_point(base, g, a, codes, side, mid, x, y, z, row_inner(y, z, ly, lz), true);
     }
 }
 
-// writes the 2x2x2 cell of mid point (x,y,z) into the caller's (nx,ny,nz)
-// field; vec = 1 when nx is even and the output is 2-element aligned
+template <typename T>
+__global__ __launch_bounds__(256) void k_low_decode_edge(const T* __restrict__ base, Geom g, LevelArgs a,
+                                                         const uint16_t* __restrict__ codes,
+                                                         const T* __restrict__ side, T* __restrict__ mid)
+{
+    const EdgeMap e = edge_map(g.dx);
+    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256ull) {
+        const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
+        low_dec_point(base, g, a, codes, side, mid, e.cols[c], y, z, false, false);
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_high_decode(const T* __restrict__ mid, Geom g, LevelArgs a,
                                                      const uint16_t* __restrict__ codes,
                                                      const T* __restrict__ side, T* __restrict__ out,
                                                      int nx, int ny, int nz, int vec)
 {
-    typedef typename V2<T>::type v2;
     const int mx = g.dx, my = g.dy, mz = g.dz;
-    const int ntx = (mx + 63) >> 6, nty = (my + 3) >> 2;
-    const uint64_t ntiles = (uint64_t)ntx * nty * mz;
+    const int nxi = mx - 3;
+    const int ntx = (nxi + 63) >> 6, nty = (my + 3) >> 2;
+    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * mz : 0;
     const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
     for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
         const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = tx * 64 + lane, y = ty * 4 + row;
-        if (x >= mx || y >= my) continue;
-        const int X = 2 * x, Y = 2 * y, Z = 2 * z;
-        if (X >= nx || Y >= ny || Z >= nz) continue;  // cell entirely in the padding
-        const int i = (z * my + y) * mx + x;
-        const uint32_t pos = stream_pos(g, x, y, z);
-        T v[8];
-        v[0] = mid[i];
-        v[1] = dec_one<0>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[2] = dec_one<1>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[3] = dec_one<2>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[4] = dec_one<3>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[5] = dec_one<4>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[6] = dec_one<5>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        v[7] = dec_one<6>(mid, i, x, y, z, mx, my, mz, a, pos, codes, side);
-        const size_t o = (size_t)Z * oz + (size_t)Y * oy + X;
-        if (vec) {
-            v2 w;
-            w.x = v[0]; w.y = v[1]; *(v2*)(out + o) = w;
-            if (Y + 1 < ny) { w.x = v[2]; w.y = v[3]; *(v2*)(out + o + oy) = w; }
-            if (Z + 1 < nz) {
-                w.x = v[4]; w.y = v[5]; *(v2*)(out + o + oz) = w;
-                if (Y + 1 < ny) { w.x = v[6]; w.y = v[7]; *(v2*)(out + o + oz + oy) = w; }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int kx = k & 1, ky = (k >> 1) & 1, kz = k >> 2;
-                if (X + kx < nx && Y + ky < ny && Z + kz < nz)
-                    out[o + (size_t)kz * oz + (size_t)ky * oy + kx] = v[k];
-            }
-        }
+        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
+        if (x > mx - 3 || y >= my) continue;
+        high_dec_point(mid, g, a, codes, side, out, nx, ny, nz, x, y, z, row_inner(y, z, my, mz), vec != 0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_high_decode_edge(const T* __restrict__ mid, Geom g, LevelArgs a,
+                                                          const uint16_t* __restrict__ codes,
+                                                          const T* __restrict__ side, T* __restrict__ out,
+                                                          int nx, int ny, int nz)
+{
+    const EdgeMap e = edge_map(g.dx);
+    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256ull) {
+        const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
+        high_dec_point(mid, g, a, codes, side, out, nx, ny, nz, e.cols[c], y, z, false, false);
     }
 }
 
@@ -351,7 +535,8 @@ __global__ void k_high_decode_box(const T* __restrict__ mid, Geom g, LevelArgs a
 
 inline unsigned tile_grid(int dx, int dy, int dz)
 {
-    const uint64_t nt = (uint64_t)((dx + 63) / 64) * ((dy + 3) / 4) * dz;
+    const int nxi = dx - 3 > 0 ? dx - 3 : 1;
+    const uint64_t nt = (uint64_t)((nxi + 63) / 64) * ((dy + 3) / 4) * dz;
     uint64_t g = nt < 2048 ? nt : 2048;
     if (g >= 64) g &= ~7ull;  // multiple of 8 for the XCD mapping
     return (unsigned)(g ? g : 1);
@@ -363,6 +548,8 @@ inline unsigned flat_grid(uint64_t n)
     return (unsigned)(b < 1 ? 1 : b > 8192 ? 8192 : b);
 }
 
+inline uint64_t edge_points(const Geom& g) { return (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz; }
+
 }  // namespace
 
 template <typename T>
@@ -370,8 +557,11 @@ void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, con
                        uint16_t* codes, T* side, uint32_t* hist, T* mid)
 {
     (void)pz;
-    k_low_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, base, g, a, codes, side,
-                                                                           hist, mid);
+    if (g.dx >= 4)
+        k_low_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, base, g, a, codes,
+                                                                               side, hist, mid);
+    k_low_encode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(full, px, py, base, g, a, codes, side,
+                                                                             hist, mid);
 }
 
 template <typename T>
@@ -379,15 +569,20 @@ void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, co
                         uint16_t* codes, T* side, uint32_t* hist)
 {
     (void)pz;
-    k_high_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, mid, g, a, codes, side,
-                                                                            hist);
+    if (g.dx >= 4)
+        k_high_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, mid, g, a, codes,
+                                                                                side, hist);
+    k_high_encode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(full, px, py, mid, g, a, codes,
+                                                                              side, hist);
 }
 
 template <typename T>
 void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
                        T* mid)
 {
-    k_low_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(base, g, a, codes, side, mid);
+    if (g.dx >= 4)
+        k_low_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(base, g, a, codes, side, mid);
+    k_low_decode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(base, g, a, codes, side, mid);
 }
 
 template <typename T>
@@ -395,8 +590,11 @@ void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const 
                         T* out, int nx, int ny, int nz)
 {
     const int vec = ((nx & 1) == 0) && (((uintptr_t)out % (2 * sizeof(T))) == 0);
-    k_high_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(mid, g, a, codes, side, out, nx, ny,
-                                                                            nz, vec);
+    if (g.dx >= 4)
+        k_high_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(mid, g, a, codes, side, out, nx,
+                                                                                ny, nz, vec);
+    k_high_decode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(mid, g, a, codes, side, out, nx,
+                                                                             ny, nz);
 }
 
 template <typename T>

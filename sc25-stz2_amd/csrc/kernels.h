@@ -140,6 +140,23 @@ struct InterpPass {
     uint64_t pos;    // first code position of the pass in the base stream
     uint64_t n;      // points in the pass
 };
+constexpr int kMaxFusedPasses = 24;
+// Coarse passes of the base codec in lattice units: every pass with stride
+// s >= S0 lives on the stride-S0 sub-grid, which one workgroup keeps in LDS.
+struct InterpPassList {
+    int n;
+    int S0;                    // lattice step in base-grid points
+    int bx, by, bz;            // base grid dims
+    int lx, ly, lz;            // lattice dims
+    InterpPass p[kMaxFusedPasses];  // start/step/s/dims in lattice units
+};
+constexpr int kLatticeBytes = 128 * 1024;
+template <typename T>
+void launch_interp_encode_fused(hipStream_t s, const InterpPassList& l, T* v, uint16_t* codes, T* side,
+                                uint32_t* hist);
+template <typename T>
+void launch_interp_decode_fused(hipStream_t s, const InterpPassList& l, T* v, const uint16_t* codes,
+                                const T* side);
 template <typename T>
 void launch_interp_encode(hipStream_t s, const InterpPass& p, T* v, uint16_t* codes, T* side,
                           uint32_t* hist);

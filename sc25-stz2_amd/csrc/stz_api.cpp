@@ -364,6 +364,41 @@ StreamSet make_streamset(const HostLayout& L, const Meta& m, int dtype)
     return ss;
 }
 
+// base-codec schedule: the coarse passes (stride >= S0, S0 the smallest
+// power of two whose lattice fits the LDS budget) run in the fused
+// single-workgroup kernel on an LDS copy of the lattice; every finer pass gets
+// a launch of its own.  Stream order is unchanged.
+template <typename FF, typename FP>
+void run_passes(const std::vector<InterpPass>& passes, size_t tsize, FF&& fused, FP&& single)
+{
+    if (passes.empty()) return;
+    const int bx = passes[0].bx, by = passes[0].by, bz = passes[0].bz;
+    const size_t cap = kLatticeBytes / tsize;
+    int S0 = 1;
+    auto lat = [&](int S, int d) { return (d - 1) / S + 1; };
+    while ((size_t)lat(S0, bx) * lat(S0, by) * lat(S0, bz) > cap) S0 <<= 1;
+    InterpPassList fl{};
+    fl.S0 = S0;
+    fl.bx = bx; fl.by = by; fl.bz = bz;
+    fl.lx = lat(S0, bx); fl.ly = lat(S0, by); fl.lz = lat(S0, bz);
+    size_t i = 0;
+    for (; i < passes.size(); ++i) {
+        const InterpPass& p = passes[i];
+        if (p.axis >= 0 && p.s < S0) break;
+        if (fl.n == kMaxFusedPasses) break;
+        InterpPass q = p;
+        q.bx = fl.lx; q.by = fl.ly; q.bz = fl.lz;
+        for (int a = 0; a < 3; ++a) {
+            q.start[a] = p.start[a] / S0;
+            q.step[a] = p.step[a] >= S0 ? p.step[a] / S0 : 1;
+        }
+        q.s = p.axis >= 0 ? p.s / S0 : 0;
+        fl.p[fl.n++] = q;
+    }
+    fused(fl);
+    for (; i < passes.size(); ++i) single(passes[i]);
+}
+
 // ------------------------------------------------------------ compress
 template <typename T>
 void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
@@ -425,11 +460,17 @@ void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, 
     // base level (F5:2132-2145): gather low block 0, then the pass schedule
     K(c, "gather_low0", [&] { launch_gather_low0<T>(st, full, (int)px, (int)py, (int)pz, vbase.as<T>()); });
     const auto passes = make_passes(lx, ly, lz, ebl[0], prm.interp, prm.order, prm.alpha, prm.beta, 0);
-    for (const auto& p : passes)
+    run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
+        K(c, "interp_encode_fused", [&] {
+            launch_interp_encode_fused<T>(st, fl, vbase.as<T>(), codes.as<uint16_t>() + L.code_off[0],
+                                          side.as<T>() + L.code_off[0], hist.as<uint32_t>());
+        });
+    }, [&](const InterpPass& p) {
         K(c, "interp_encode", [&] {
             launch_interp_encode<T>(st, p, vbase.as<T>(), codes.as<uint16_t>() + L.code_off[0],
                                     side.as<T>() + L.code_off[0], hist.as<uint32_t>());
         });
+    });
     // low level (F5:2180-2230) fused with the mid merge
     LevelArgs la{};
     la.eb = ebl[1];
@@ -676,11 +717,17 @@ void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& 
 {
     const auto passes = make_passes(L.g[0].dx, L.g[0].dy, L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
                                     (int)m.fh.order, m.fh.alpha, m.fh.beta, stop_level);
-    for (const auto& p : passes)
+    run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
+        K(c, "interp_decode_fused", [&] {
+            launch_interp_decode_fused<T>(c->stream, fl, v, dd.codes.as<uint16_t>() + L.code_off[0],
+                                          dd.side.as<T>() + L.code_off[0]);
+        });
+    }, [&](const InterpPass& p) {
         K(c, "interp_decode", [&] {
             launch_interp_decode<T>(c->stream, p, v, dd.codes.as<uint16_t>() + L.code_off[0],
                                     dd.side.as<T>() + L.code_off[0]);
         });
+    });
 }
 
 LevelArgs level_args(const Meta& m, const HostLayout& L, int level)
