@@ -1,0 +1,491 @@
+"""CPU checks that pin the oracle (oracle/stz_oracle.c via oracle/pyoracle.py).
+
+1. Golden vectors written by the reference's own functions (tests/golden/
+   ref_vectors.npz, made by tests/golden/make_golden.py from
+   oracle/_ref/libstz_ref.so): residual predictors for all 7 blocks
+   (F5:141-882, F5:1145-1191), the quantizer (F5:1204-1253), the parity split
+   (F5:107-122) and the ROI planner (F5:1602-1830).
+2. The live reference build, when oracle/_ref is present, on more shapes.
+3. What the mathematics fixes, for the parts with no reference counterpart
+   here (the base codec and the Huffman code, whose reference lives in the
+   absent SZ3-ncp): polynomial reproduction, an independent level-by-level
+   evaluator of the base codec, Huffman optimality against a textbook heap
+   construction and brute force, Kraft equality, error bounds, ROI == crop.
+"""
+import ctypes as C
+import heapq
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD_PATH = os.path.join(ROOT, "tests", "golden", "ref_vectors.npz")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libstz_ref.so")
+N_SHAPES = 5
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD_PATH)
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+# ---------------------------------------------------------------- golden vectors
+@pytest.mark.parametrize("suf", ["f32", "f64"])
+def test_golden_residual_predictors(oracle, gold, suf):
+    for si in range(N_SHAPES):
+        sub = gold["res_%s_%d_sub" % (suf, si)]
+        ref = gold["res_%s_%d_ref" % (suf, si)]
+        de = gold["res_%s_%d_de" % (suf, si)]
+        for b in range(7):
+            got = oracle.preprocess(b, sub, ref)
+            assert np.array_equal(_bits(got), _bits(gold["res_%s_%d_pre%d" % (suf, si, b)])), (suf, si, b)
+            got = oracle.depreprocess(b, de, ref)
+            assert np.array_equal(_bits(got), _bits(gold["res_%s_%d_depre%d" % (suf, si, b)])), (suf, si, b)
+
+
+@pytest.mark.parametrize("suf", ["f32", "f64"])
+def test_golden_quantizer(oracle, gold, suf):
+    d = gold["q_%s_in" % suf]
+    for ei in range(3):
+        eb = float(gold["q_%s_%d_eb" % (suf, ei)])
+        codes, unpred, _ = oracle.quantize(d, eb)
+        assert np.array_equal(codes.astype(np.int32), gold["q_%s_%d_codes" % (suf, ei)]), (suf, eb)
+        assert np.array_equal(_bits(unpred), _bits(gold["q_%s_%d_unpred" % (suf, ei)])), (suf, eb)
+
+
+def test_golden_split_merge(oracle, gold):
+    for si in range(2):
+        full = gold["split_%d_full" % si]
+        subs = oracle.split(full)
+        assert np.array_equal(np.stack(subs), gold["split_%d_subs" % si])
+        assert np.array_equal(oracle.merge(subs), full)
+
+
+def test_golden_planner(oracle, gold):
+    pin = gold["plan_in"]
+    hoff = loff = 0
+    hall, lall = gold["plan_high_chunks"], gold["plan_low_chunks"]
+    for k in range(pin.shape[0]):
+        dims = tuple(int(v) for v in pin[k, :3])
+        ch, cl = int(pin[k, 3]), int(pin[k, 4])
+        q = [int(v) for v in pin[k, 5:11]]
+        qa, is_slice, high, low = oracle.plan_query(dims, q, ch, cl, 4096)
+        gq = gold["plan_query"][k]
+        assert qa == tuple(int(v) for v in gq[:6]) and is_slice == bool(gq[6]), k
+        for plans, table, allc, off in ((high, gold["plan_high"][k], hall, hoff), (low, gold["plan_low"][k], lall, loff)):
+            for b in range(7):
+                row = table[b]
+                assert bool(row[0]) == (b in plans), (k, b)
+                if row[0]:
+                    assert plans[b][0] == tuple(int(v) for v in row[1:7]), (k, b)
+                    n = int(row[7])
+                    assert np.array_equal(plans[b][1], allc[off:off + n]), (k, b)
+                    off += n
+            if plans is high:
+                hoff = off
+            else:
+                loff = off
+
+
+# ---------------------------------------------------------------- live reference build
+@pytest.fixture(scope="module")
+def ref():
+    if not os.path.exists(REF_PATH):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    R = C.CDLL(REF_PATH)
+    R.ref_plan_query.argtypes = [C.c_int] * 11 + [C.c_void_p] * 4 + [C.c_void_p, C.c_int64]
+    return R
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_reference_predictors_live(oracle, ref, dt):
+    suf = "f32" if dt == np.float32 else "f64"
+    rng = np.random.default_rng(77)
+    for shape in [(17, 9, 33), (5, 31, 8), (20, 20, 20), (1, 3, 4)]:
+        dz, dy, dx = shape
+        r = np.cumsum(rng.normal(size=shape), axis=1).astype(dt)
+        s = (r + rng.normal(scale=0.3, size=shape)).astype(dt)
+        out = np.empty_like(s)
+        for b in range(7):
+            getattr(ref, "ref_preprocess_" + suf)(b, _p(s), _p(r), _p(out), dx, dy, dz)
+            assert np.array_equal(_bits(oracle.preprocess(b, s, r)), _bits(out)), (shape, b)
+            getattr(ref, "ref_depreprocess_" + suf)(b, _p(s), _p(r), _p(out), dx, dy, dz)
+            assert np.array_equal(_bits(oracle.depreprocess(b, s, r)), _bits(out)), (shape, b)
+
+
+def test_reference_planner_live(oracle, ref):
+    rng = np.random.default_rng(3)
+    cap = 4096
+    for it in range(400):
+        ch, cl = [(32, 16), (8, 4), (4, 4), (16, 8)][it % 4]
+        lcm = 4 * cl * 2 * ch // math.gcd(4 * cl, 2 * ch)
+        dims = [lcm * int(rng.integers(1, 4)) for _ in range(3)]
+        q = [int(rng.integers(-3, d + 3)) for d in dims] * 1
+        q += [int(rng.integers(-3, d + 3)) for d in dims]
+        if it % 5 == 0:
+            q[3] = q[0]
+        qa, sl, high, low = oracle.plan_query(tuple(dims), q, ch, cl, cap)
+        query = np.zeros(7, np.int32)
+        hp = np.zeros((7, 8), np.int32)
+        lp = np.zeros((7, 8), np.int32)
+        hc = np.zeros((7, cap), np.int64)
+        lc = np.zeros((7, cap), np.int64)
+        ref.ref_plan_query(*dims, ch, cl, *q, _p(query), _p(hp), _p(hc), _p(lp), _p(lc), cap)
+        assert qa == tuple(int(v) for v in query[:6]) and sl == bool(query[6])
+        for plans, P, Cc in ((high, hp, hc), (low, lp, lc)):
+            assert set(plans) == {b for b in range(7) if P[b, 0]}
+            for b in plans:
+                assert plans[b][0] == tuple(int(v) for v in P[b, 1:7])
+                assert np.array_equal(plans[b][1], Cc[b, :P[b, 7]])
+
+
+# ---------------------------------------------------------------- quantizer properties
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_quantizer_properties(oracle, dt):
+    rng = np.random.default_rng(11)
+    eb = 1e-2
+    r = 32768
+    d = np.concatenate([rng.normal(size=20000) * 10, [0.0, np.nan, np.inf, -np.inf, (2 * r - 1) * eb * 1.01]]).astype(dt)
+    codes, unpred, dec = oracle.quantize(d, eb)
+    ok = codes != 0
+    assert codes[ok].min() >= 1 and codes[ok].max() <= 2 * r - 1
+    assert np.all(np.abs(dec[ok].astype(np.float64) - d[ok].astype(np.float64)) <= eb)
+    assert codes[-5] == r                                   # 0 -> the centre code
+    assert np.all(codes[-4:] == 0)                          # NaN, +-Inf, out of range -> unpredictable
+    assert np.array_equal(_bits(unpred), _bits(d[~ok]))    # stored raw, in element order
+    back = oracle.dequantize(codes, unpred, eb, dt)
+    # equal as values; the encoder's dec keeps the residual's sign at code r (-0.0)
+    assert np.array_equal(back[ok], dec[ok])
+    q = codes[ok].astype(np.int64)                           # dequantized value is the closed form 2(q-r)eb
+    assert np.array_equal(back[ok], (2.0 * (q - r) * eb).astype(dt))
+
+
+# ---------------------------------------------------------------- Huffman code lengths
+def _textbook_huffman_cost(freq):
+    """Total bits of an optimal (unrestricted) prefix code, by the classic
+    two-smallest merge: the sum of all internal node weights."""
+    h = [int(f) for f in freq if f > 0]
+    if len(h) == 1:
+        return h[0]
+    heapq.heapify(h)
+    cost = 0
+    while len(h) > 1:
+        a = heapq.heappop(h)
+        b = heapq.heappop(h)
+        cost += a + b
+        heapq.heappush(h, a + b)
+    return cost
+
+
+def _cost(freq, lens):
+    return int(np.sum(freq.astype(np.uint64) * lens.astype(np.uint64)))
+
+
+def _kraft(lens, lmax=24):
+    nz = lens[lens > 0].astype(np.int64)
+    return int(np.sum(np.int64(1) << (lmax - nz)))
+
+
+def test_huffman_optimal_and_complete(oracle):
+    rng = np.random.default_rng(5)
+    for n in [2, 3, 17, 300, 4096]:
+        f = rng.integers(1, 1000, size=n).astype(np.uint64)
+        f[rng.random(n) < 0.2] = 0
+        if np.count_nonzero(f) < 2:
+            f[:2] = 1
+        L = oracle.huff_lengths(f)
+        assert np.all((L == 0) == (f == 0))
+        assert _cost(f, L) == _textbook_huffman_cost(f)
+        assert _kraft(L) == 1 << 24
+
+
+def test_huffman_single_symbol(oracle):
+    f = np.zeros(100, np.uint64)
+    f[42] = 9
+    L = oracle.huff_lengths(f)
+    assert L[42] == 1 and np.count_nonzero(L) == 1
+
+
+def test_huffman_length_limit(oracle):
+    # Fibonacci weights make the unrestricted tree 39 deep
+    fib = [1, 1]
+    while len(fib) < 40:
+        fib.append(fib[-1] + fib[-2])
+    f = np.array(fib, np.uint64)
+    L = oracle.huff_lengths(f, 24)
+    assert L.max() <= 24
+    assert _kraft(L) <= 1 << 24
+    assert _cost(f, L) >= _textbook_huffman_cost(f)
+
+
+def _brute_force_best(f, lmax):
+    best = None
+    for lens in itertools.product(range(1, lmax + 1), repeat=len(f)):
+        if sum(2.0 ** -l for l in lens) <= 1.0:
+            c = sum(int(a) * b for a, b in zip(f, lens))
+            best = c if best is None or c < best else best
+    return best
+
+
+def test_huffman_brute_force_optimal(oracle):
+    """With no binding limit the lengths reach the minimum over every prefix code."""
+    rng = np.random.default_rng(9)
+    for n in [2, 3, 5, 6, 7]:
+        for _ in range(4):
+            f = rng.integers(1, 60, size=n).astype(np.uint64)
+            L = oracle.huff_lengths(f, 24)
+            assert _cost(f, L) == _brute_force_best(f, n - 1), (f, L)
+
+
+def _annex_k3(bits, lmax):
+    """ITU-T T.81 Annex K.3 (Figure K.3, Adjust_BITS) on per-length counts:
+    while a code longer than lmax exists, take two symbols of the longest
+    length i, give one prefix to the next shorter length and put the pair
+    under a leaf taken from the longest length j < i - 1 that has one."""
+    bits = list(bits)
+    for i in range(len(bits) - 1, lmax, -1):
+        while bits[i] > 0:
+            j = i - 2
+            while bits[j] == 0:
+                j -= 1
+            bits[i] -= 2
+            bits[i - 1] += 1
+            bits[j + 1] += 2
+            bits[j] -= 1
+    return bits
+
+
+def test_huffman_length_limit_annex_k3(oracle):
+    """A binding limit: the per-length counts are Annex K.3's adjustment of the
+    unlimited code's, Kraft equality still holds, and lengths never grow as
+    frequency rises (new lengths go out in old-length order)."""
+    rng = np.random.default_rng(21)
+    cases = []
+    fib = [1, 1]
+    while len(fib) < 40:
+        fib.append(fib[-1] + fib[-2])
+    cases.append((np.array(fib, np.uint64), 24))
+    for n, lmax in [(6, 3), (8, 4), (12, 5), (300, 9)]:
+        f = (rng.pareto(0.7, size=n) * 10 + 1).astype(np.uint64)
+        cases.append((f, lmax))
+    for f, lmax in cases:
+        free = oracle.huff_lengths(f, 64)
+        if free.max() <= lmax:
+            continue
+        L = oracle.huff_lengths(f, lmax)
+        want = _annex_k3(np.bincount(free, minlength=int(free.max()) + 1), lmax)
+        got = np.bincount(L, minlength=len(want))
+        assert list(got[1:]) == list(want[1:len(got)]) and not any(want[len(got):])
+        assert L.max() <= lmax
+        assert sum(2.0 ** -int(l) for l in L) == 1.0
+        assert _monotone_by_old_length(free, L)
+        if len(f) <= 8:
+            assert _cost(f, L) >= _brute_force_best(f, lmax)
+
+
+def _monotone_by_old_length(free, L):
+    o = np.lexsort((np.arange(len(free)), free))
+    return bool(np.all(np.diff(L[o].astype(int)) >= 0))
+
+
+# ---------------------------------------------------------------- base codec
+ORDERS = [(2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 0, 2), (0, 2, 1), (0, 1, 2)]  # axis order, 0 = x
+
+
+def _base_codec_levelwise(oracle, v, eb, interp, order):
+    """Dyadic multilevel interpolation codec, written level by level with
+    numpy slices: at level l (stride s = 2^(l-1)) each axis in turn predicts
+    the points at odd multiples of s along it from neighbours at +-s and +-3s
+    (cubic (-1, 9, 9, -1)/16 inside, 3-point or 2-point rules near the ends,
+    or linear), quantizes the residual with the oracle's quantizer and
+    writes the reconstruction back before the next axis.  Codes are emitted
+    pass by pass, x fastest.  Also returns, per code, the rule that predicted
+    it: 0 origin, 1 copy of the left neighbour, 2 two-point mean, 3 linear
+    extrapolation, 4 four-point cubic, 5/6 three-point end rules."""
+    T = v.dtype.type
+    w = v.copy()
+    shape = w.shape                      # (z, y, x)
+    dims = [shape[2], shape[1], shape[0]]
+    L = 0
+    while (1 << L) < max(dims):
+        L += 1
+    codes_all, side_all, rules_all = [], [], []
+
+    def q(vals, ebl):
+        c, _, dec = oracle.quantize(np.ascontiguousarray(vals).ravel(), ebl)
+        return c, dec
+
+    c, dec = q(np.array([w[0, 0, 0]], dtype=v.dtype), eb)
+    codes_all.append(c)
+    rules_all.append(np.zeros(1, np.int8))
+    if c[0]:
+        w[0, 0, 0] = dec[0]
+    else:
+        side_all.append(np.array([v[0, 0, 0]]))
+    for lev in range(L, 0, -1):
+        s = 1 << (lev - 1)
+        ordv = ORDERS[order]
+        for pi, a in enumerate(ordv):
+            idx = []
+            for ax in range(3):
+                rank = ordv.index(ax)
+                if rank == pi:
+                    idx.append(np.arange(s, dims[ax], 2 * s))
+                elif rank < pi:
+                    idx.append(np.arange(0, dims[ax], s))
+                else:
+                    idx.append(np.arange(0, dims[ax], 2 * s))
+            ix, iy, iz = idx
+            if min(len(ix), len(iy), len(iz)) == 0:
+                continue
+            ax_np = 2 - a                  # numpy axis of x/y/z
+            dim = dims[a]
+
+            def take(pos):
+                sel = [iz, iy, ix]
+                sel[ax_np] = np.array([pos])
+                return w[np.ix_(*sel)]
+
+            pred = np.empty((len(iz), len(iy), len(ix)), dtype=v.dtype)
+            rule = np.empty(pred.shape, np.int8)
+            for k, pos in enumerate(idx[a]):
+                b = take(pos - s)
+                r1, l3, r3 = pos + s < dim, pos >= 3 * s, pos + 3 * s < dim
+                if interp == 1:
+                    if r1:
+                        cc = take(pos + s)
+                        if l3 and r3:
+                            p, rl = T(-0.0625) * take(pos - 3 * s) + T(0.5625) * b + T(0.5625) * cc - T(0.0625) * take(pos + 3 * s), 4
+                        elif l3:
+                            p, rl = T(-0.125) * take(pos - 3 * s) + T(0.75) * b + T(0.375) * cc, 5
+                        elif r3:
+                            p, rl = T(0.375) * b + T(0.75) * cc - T(0.125) * take(pos + 3 * s), 6
+                        else:
+                            p, rl = T(0.5) * (b + cc), 2
+                    elif l3:
+                        p, rl = T(1.5) * b - T(0.5) * take(pos - 3 * s), 3
+                    else:
+                        p, rl = b, 1
+                else:
+                    if r1:
+                        p, rl = T(0.5) * (b + take(pos + s)), 2
+                    elif l3:
+                        p, rl = T(1.5) * b - T(0.5) * take(pos - 3 * s), 3
+                    else:
+                        p, rl = b, 1
+                sl = [slice(None)] * 3
+                sl[ax_np] = slice(k, k + 1)
+                pred[tuple(sl)] = p
+                rule[tuple(sl)] = rl
+            orig = w[np.ix_(iz, iy, ix)]
+            c, dec = q(orig - pred, eb)
+            c = c.reshape(pred.shape)
+            rec = np.where(c != 0, pred + dec.reshape(pred.shape), orig)
+            side_all.append(orig[c == 0])
+            codes_all.append(c.ravel())
+            rules_all.append(rule.ravel())
+            w[np.ix_(iz, iy, ix)] = rec
+    side = np.concatenate(side_all) if side_all else np.zeros(0, v.dtype)
+    return w, np.concatenate(codes_all), side, np.concatenate(rules_all)
+
+
+@pytest.mark.parametrize("shape,dt,interp,order", [
+    ((9, 7, 13), np.float64, 1, 0), ((8, 8, 8), np.float32, 1, 3), ((5, 12, 6), np.float32, 0, 5),
+    ((17, 3, 10), np.float64, 0, 1), ((1, 6, 9), np.float32, 1, 2), ((16, 16, 16), np.float32, 1, 4),
+])
+def test_base_codec_levelwise(oracle, shape, dt, interp, order):
+    rng = np.random.default_rng(sum(shape))
+    v = (np.cumsum(rng.normal(size=shape), axis=0) + 3 * np.sin(np.arange(shape[2]) / 2.0)).astype(dt)
+    eb = 0.05
+    want_rec, want_codes, want_side, _ = _base_codec_levelwise(oracle, v, eb, interp, order)
+    rec, codes, side = oracle.interp_compress(v, eb, interp, order)
+    assert np.array_equal(codes, want_codes)
+    assert np.array_equal(_bits(side), _bits(want_side))
+    assert np.array_equal(_bits(rec), _bits(want_rec))
+    assert np.abs(rec.astype(np.float64) - v).max() <= eb
+
+
+@pytest.mark.parametrize("interp,order", [(0, 0), (1, 0), (1, 5), (0, 3)])
+def test_base_codec_reproduces_linear(oracle, interp, order):
+    """Every rule but the left-neighbour copy is exact on linear data."""
+    z, y, x = np.meshgrid(np.arange(11), np.arange(13), np.arange(19), indexing="ij")
+    v = (x + 2 * y - 3 * z).astype(np.float32)
+    rec, codes, side = oracle.interp_compress(v, 0.25, interp, order)
+    _, want, _, rules = _base_codec_levelwise(oracle, v, 0.25, interp, order)
+    assert np.array_equal(codes, want)
+    assert np.all(codes[rules != 1] == 32768)
+    assert np.count_nonzero(rules == 1) > 0 and side.size == 0
+    assert np.array_equal(rec, v)
+
+
+def test_base_codec_cubic_reproduces_cubics(oracle):
+    """On a cubic along x (constant along y, z) the four-point rule is exact."""
+    x = np.arange(37, dtype=np.float64)
+    v = np.broadcast_to(((x - 20) ** 3 / 512.0)[None, None, :], (5, 6, 37)).copy()
+    _, codes, _ = oracle.interp_compress(v, 1e-9, 1, 0)
+    _, want, _, rules = _base_codec_levelwise(oracle, v, 1e-9, 1, 0)
+    assert np.array_equal(codes, want)
+    assert np.count_nonzero(rules == 4) > 100
+    assert np.all(codes[rules == 4] == 32768)
+    assert np.any(codes[rules == 5] != 32768) or np.any(codes[rules == 6] != 32768)
+
+
+# ---------------------------------------------------------------- whole oracle path
+def _field(shape, dt, seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    f = np.zeros(shape)
+    for _ in range(8):
+        k = rng.integers(1, 4, size=3)
+        f += rng.uniform(0.1, 1) * np.sin(2 * np.pi * (k[0] * x / shape[2] + k[1] * y / shape[1] + k[2] * z / shape[0])
+                                          + rng.uniform(0, 6.28))
+    return (f + rng.normal(0, 1e-3, size=shape)).astype(dt)
+
+
+@pytest.mark.parametrize("shape,dt", [((32, 32, 32), np.float32), ((24, 40, 16), np.float64),
+                                      ((13, 22, 9), np.float32), ((4, 4, 4), np.float32)])
+def test_oracle_round_trip(oracle, shape, dt):
+    f = _field(shape, dt, 1)
+    p = oracle.compress(f, 1e-3, oracle.EB_REL)
+    assert np.array_equal(p, oracle.compress(f, 1e-3, oracle.EB_REL))          # deterministic
+    h = oracle.header(p)
+    assert (h["nz"], h["ny"], h["nx"]) == shape
+    r = oracle.decompress(p)
+    E = 1e-3 * (float(f.max()) - float(f.min()))
+    assert np.abs(r.astype(np.float64) - f.astype(np.float64)).max() <= E * (1 + 1e-9)
+    assert h["eb_high"] <= E * (1 + 1e-12)
+    # ROI and slices are crops of the full decode (F5:2609-2640)
+    nz, ny, nx = shape
+    for (x0, y0, z0, x1, y1, z1) in [(1, 2, 3, nx - 2, ny - 1, nz - 3), (0, 0, 0, nx - 1, ny - 1, nz - 1),
+                                     (nx - 1, 0, 2, 0, ny - 1, 2), (0, ny // 2, 0, nx - 1, ny // 2, nz - 1)]:
+        roi = oracle.decompress_roi(p, x0, y0, z0, x1, y1, z1)
+        want = r[min(z0, z1):max(z0, z1) + 1, min(y0, y1):max(y0, y1) + 1, min(x0, x1):max(x0, x1) + 1]
+        assert np.array_equal(_bits(roi), _bits(want))
+
+
+def test_oracle_levels_are_subsamples(oracle):
+    f = _field((32, 24, 40), np.float32, 2)
+    p = oracle.compress(f, 1e-3, oracle.EB_REL)
+    r = oracle.decompress(p)
+    assert np.array_equal(oracle.decompress_level(p, 2), r[::2, ::2, ::2])
+    assert np.array_equal(oracle.decompress_level(p, 1), r[::4, ::4, ::4])
+    assert np.array_equal(oracle.decompress_level(p, 3), r)
+
+
+def test_oracle_abs_mode_and_params(oracle):
+    f = _field((16, 16, 32), np.float64, 3)
+    p = oracle.compress(f, 1e-4, oracle.EB_ABS, interp=0, order=4, mult_low=2.0, chunk_high=8, chunk_low=4)
+    r = oracle.decompress(p)
+    assert np.abs(r - f).max() <= 1e-4 * (1 + 1e-12)
