@@ -122,6 +122,26 @@ int stz_dstream_decompress_roi(stz_ctx* ctx, stz_dstream* ds, int x0, int y0, in
                                int y1, int z1, void* out, size_t out_dims[3]);
 int stz_dstream_decompress_level(stz_ctx* ctx, stz_dstream* ds, int level, void* out,
                                  size_t out_dims[3]);
+/* ---- random access in batches (serving) --------------------------------
+ * A request is a 3D box (a = x0 y0 z0 x1 y1 z1, inclusive, normalised like
+ * make_query F5:1637-1644), an axis slice (a = axis, index) or a level
+ * (a = level as in stz_decompress_level).  One call decodes the union of the
+ * requests' chunks once (SZ_decompress_nopred_marked_chunks, F5:1841-1911,
+ * over all of them) and runs every request's region decode in one launch per
+ * level; results are bit-identical to the requests run one by one. */
+#define STZ_REQ_BOX 0
+#define STZ_REQ_SLICE 1
+#define STZ_REQ_LEVEL 2
+typedef struct {
+    int kind;
+    int a[6];
+} stz_request;
+/* decode tables, and (keep_base != 0) the decoded base level kept on the
+ * device so random-access requests skip it (F5:2339 decodes it per query) */
+int stz_dstream_prepare(stz_ctx* ctx, stz_dstream* ds, int keep_base);
+/* outs[i]: host or device buffer of the request's size; out_dims: 3 per request */
+int stz_dstream_decompress_batch(stz_ctx* ctx, stz_dstream* ds, int n, const stz_request* reqs,
+                                 void* const* outs, size_t* out_dims);
 /* device payload pointer and size (pre-zstd layout) */
 int stz_dstream_payload(const stz_dstream* ds, const void** d_payload, size_t* bytes);
 void stz_dstream_free(stz_dstream* ds);

@@ -60,15 +60,6 @@ __device__ __forceinline__ int find_stream_by_chunk(const StreamSet& ss, uint32_
     return s;
 }
 
-__device__ __forceinline__ uint64_t chunk_start(const Geom& g, uint32_t c)
-{
-    if (g.chunk <= 0) return (uint64_t)c * kBaseChunk;
-    const int C = g.chunk;
-    const int cx = (int)(c % g.ncx), cy = (int)((c / g.ncx) % g.ncy), cz = (int)(c / ((uint32_t)g.ncx * g.ncy));
-    const int wy = imin_(C, g.dy - cy * C), wz = imin_(C, g.dz - cz * C);
-    return (uint64_t)cz * C * g.dx * g.dy + (uint64_t)cy * C * g.dx * wz + (uint64_t)cx * C * wy * wz;
-}
-
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -367,20 +358,24 @@ __device__ __forceinline__ int find_stream_by_wg(const StreamSet& ss, uint32_t w
 // expand chunks into 1024-symbol decode work items
 __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
                            const uint32_t* __restrict__ list, uint32_t nlist,
-                           DecItem* __restrict__ items)
+                           DecItem* __restrict__ items, uint32_t* __restrict__ cmap)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t s, c, ib;
+    uint64_t pstart;
     if (list) {
         if (i >= nlist) return;
-        s = list[3 * i];
-        c = list[3 * i + 1];
-        ib = list[3 * i + 2];
+        s = list[4 * i];
+        c = list[4 * i + 1];
+        ib = list[4 * i + 2];
+        pstart = list[4 * i + 3];
+        cmap[ss.s[s].cm_base + c] = (uint32_t)pstart;
     } else {
         if (i >= ss.total_chunks) return;
         s = (uint32_t)find_stream_by_chunk(ss, i);
         c = i - ss.s[s].chunk_base;
         ib = ss.s[s].item_base + ((const uint32_t*)(payload + ss.s[s].off_chunk_sub))[c];
+        pstart = chunk_start(ss.s[s].g, c);
     }
     const StreamDesc& d = ss.s[s];
     const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
@@ -389,7 +384,6 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
     uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
     uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
     const uint32_t sb = csub[c], se = csub[c + 1];
-    const uint64_t pstart = chunk_start(d.g, c);
     const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
     for (uint32_t j = 0; j < se - sb; ++j) {
         DecItem it;
@@ -632,11 +626,11 @@ void launch_huff_pack(hipStream_t s, const StreamSet& ss, const uint16_t* codes,
 }
 
 void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const uint32_t* list,
-                     uint32_t nlist, DecItem* items)
+                     uint32_t nlist, DecItem* items, uint32_t* cmap)
 {
     const uint32_t n = list ? nlist : ss.total_chunks;
     if (!n) return;
-    k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items);
+    k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items, cmap);
 }
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,

@@ -477,18 +477,52 @@ __global__ __launch_bounds__(256) void k_high_decode_edge(const T* __restrict__ 
     }
 }
 
-// ROI: reference values of sub-block 0 on a box of the mid grid
-// (F5:2508-2538 low depreprocess restricted to the regions + gather)
-template <typename T>
-__global__ void k_low_decode_box(const T* __restrict__ base, Geom g, LevelArgs a,
-                                 const uint16_t* __restrict__ codes, const T* __restrict__ side,
-                                 T* __restrict__ mid, int mx, int my, int x0, int x1, int y0, int y1, int z0,
-                                 int z1)
+// Random access reads codes through the chunk map: the element of block
+// point (x,y,z) is cmap[cm_off[b] + chunk] + its chunk-local row-major index
+// (the stream order of F5:1884-1905).  Without a map it is the stream
+// position in the full layout.
+__device__ __forceinline__ uint64_t ra_index(const LevelArgs& a, const Geom& g, int b, int x, int y, int z)
 {
-    const int bx = x1 - x0 + 1, by = y1 - y0 + 1, bz = z1 - z0 + 1;
+    if (!a.cmap) return a.code_off[b] + stream_pos(g, x, y, z);
+    const int C = g.chunk;
+    const int cx = x / C, cy = y / C, cz = z / C;
+    const uint32_t m = a.cmap[a.cm_off[b] + ((uint32_t)cz * g.ncy + cy) * g.ncx + cx];
+    if (m == kNoChunk) {
+        atomicOr(a.err, 2u);
+        return 0;
+    }
+    const int wx = imin_(C, g.dx - cx * C), wy = imin_(C, g.dy - cy * C);
+    return (uint64_t)m + ((uint32_t)(z - cz * C) * wy + (uint32_t)(y - cy * C)) * wx + (uint32_t)(x - cx * C);
+}
+
+// the job a workgroup serves (jobs sorted by blk0, jobs[0].blk0 == 0)
+__device__ __forceinline__ uint32_t find_job(const BoxJob* __restrict__ jobs, uint32_t njobs, uint32_t blk)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].blk0 <= blk) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+// Random access: reference values of sub-block 0 on boxes of the mid grid
+// (F5:2508-2538, low depreprocess restricted to the regions + gather)
+template <typename T>
+__global__ __launch_bounds__(256) void k_low_decode_boxes(const T* __restrict__ base, Geom g, LevelArgs a,
+                                                          const uint16_t* __restrict__ codes,
+                                                          const T* __restrict__ side, T* __restrict__ mid,
+                                                          int mx, int my, const BoxJob* __restrict__ jobs,
+                                                          uint32_t njobs, uint32_t nblocks)
+{
+    const uint32_t j = find_job(jobs, njobs, blockIdx.x);
+    const BoxJob J = jobs[j];
+    const uint32_t nb = (j + 1 < njobs ? jobs[j + 1].blk0 : nblocks) - J.blk0;
+    const int bx = J.x1 - J.x0 + 1, by = J.y1 - J.y0 + 1, bz = J.z1 - J.z0 + 1;
     const uint64_t n = (uint64_t)bx * by * bz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int X = x0 + (int)(k % bx), Y = y0 + (int)((k / bx) % by), Z = z0 + (int)(k / ((uint64_t)bx * by));
+    for (uint64_t k = (blockIdx.x - J.blk0) * 256ull + threadIdx.x; k < n; k += (uint64_t)nb * 256u) {
+        const uint64_t t = k / (uint64_t)bx;
+        const int X = J.x0 + (int)(k - t * bx), Y = J.y0 + (int)(t % (uint64_t)by), Z = J.z0 + (int)(t / (uint64_t)by);
         const int par = ((Z & 1) << 2) | ((Y & 1) << 1) | (X & 1);
         const int x = X >> 1, y = Y >> 1, z = Z >> 1;
         const int i = (z * g.dy + y) * g.dx + x;
@@ -497,25 +531,32 @@ __global__ void k_low_decode_box(const T* __restrict__ base, Geom g, LevelArgs a
             v = base[i];
         } else {
             const int b = par - 1;
-            const uint32_t pos = stream_pos(g, x, y, z);
-            const uint32_t q = codes[a.code_off[b] + pos];
-            const T r = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
+            const uint64_t e = ra_index(a, g, b, x, y, z);
+            const uint32_t q = codes[e];
+            const T r = q ? dequantize<T>(q, a.eb) : side[e];
             v = r + predict_rt(b, base, i, x, y, z, g.dx, g.dy, g.dz);
         }
         mid[((size_t)Z * my + Y) * mx + X] = v;
     }
 }
 
-// ROI: full-resolution values of the query box (F5:2574-2605)
+// Random access: full-resolution values of query boxes (F5:2574-2605)
 template <typename T>
-__global__ void k_high_decode_box(const T* __restrict__ mid, Geom g, LevelArgs a,
-                                  const uint16_t* __restrict__ codes, const T* __restrict__ side,
-                                  T* __restrict__ out, int x0, int x1, int y0, int y1, int z0, int z1)
+__global__ __launch_bounds__(256) void k_high_decode_boxes(const T* __restrict__ mid, Geom g, LevelArgs a,
+                                                           const uint16_t* __restrict__ codes,
+                                                           const T* __restrict__ side,
+                                                           const BoxJob* __restrict__ jobs, uint32_t njobs,
+                                                           uint32_t nblocks)
 {
-    const int bx = x1 - x0 + 1, by = y1 - y0 + 1, bz = z1 - z0 + 1;
+    const uint32_t j = find_job(jobs, njobs, blockIdx.x);
+    const BoxJob J = jobs[j];
+    T* __restrict__ out = (T*)J.out;
+    const uint32_t nb = (j + 1 < njobs ? jobs[j + 1].blk0 : nblocks) - J.blk0;
+    const int bx = J.x1 - J.x0 + 1, by = J.y1 - J.y0 + 1, bz = J.z1 - J.z0 + 1;
     const uint64_t n = (uint64_t)bx * by * bz;
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
-        const int X = x0 + (int)(k % bx), Y = y0 + (int)((k / bx) % by), Z = z0 + (int)(k / ((uint64_t)bx * by));
+    for (uint64_t k = (blockIdx.x - J.blk0) * 256ull + threadIdx.x; k < n; k += (uint64_t)nb * 256u) {
+        const uint64_t t = k / (uint64_t)bx;
+        const int X = J.x0 + (int)(k - t * bx), Y = J.y0 + (int)(t % (uint64_t)by), Z = J.z0 + (int)(t / (uint64_t)by);
         const int par = ((Z & 1) << 2) | ((Y & 1) << 1) | (X & 1);
         const int x = X >> 1, y = Y >> 1, z = Z >> 1;
         const int i = (z * g.dy + y) * g.dx + x;
@@ -524,9 +565,9 @@ __global__ void k_high_decode_box(const T* __restrict__ mid, Geom g, LevelArgs a
             v = mid[i];
         } else {
             const int b = par - 1;
-            const uint32_t pos = stream_pos(g, x, y, z);
-            const uint32_t q = codes[a.code_off[b] + pos];
-            const T r = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
+            const uint64_t e = ra_index(a, g, b, x, y, z);
+            const uint32_t q = codes[e];
+            const T r = q ? dequantize<T>(q, a.eb) : side[e];
             v = r + predict_rt(b, mid, i, x, y, z, g.dx, g.dy, g.dz);
         }
         out[k] = v;
@@ -598,21 +639,21 @@ void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const 
 }
 
 template <typename T>
-void launch_low_decode_box(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
-                           const T* side, T* mid, int mx, int my, int x0, int x1, int y0, int y1, int z0, int z1)
+void launch_low_decode_boxes(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
+                             const T* side, T* mid, int mx, int my, const BoxJob* jobs, uint32_t njobs,
+                             uint32_t nblocks)
 {
-    const uint64_t n = (uint64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-    k_low_decode_box<T><<<dim3(flat_grid(n)), dim3(256), 0, s>>>(base, g, a, codes, side, mid, mx, my, x0, x1, y0,
-                                                                y1, z0, z1);
+    if (!njobs || !nblocks) return;
+    k_low_decode_boxes<T><<<dim3(nblocks), dim3(256), 0, s>>>(base, g, a, codes, side, mid, mx, my, jobs, njobs,
+                                                              nblocks);
 }
 
 template <typename T>
-void launch_high_decode_box(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
-                            const T* side, T* out, int x0, int x1, int y0, int y1, int z0, int z1)
+void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
+                              const T* side, const BoxJob* jobs, uint32_t njobs, uint32_t nblocks)
 {
-    const uint64_t n = (uint64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-    k_high_decode_box<T><<<dim3(flat_grid(n)), dim3(256), 0, s>>>(mid, g, a, codes, side, out, x0, x1, y0, y1,
-                                                                 z0, z1);
+    if (!njobs || !nblocks) return;
+    k_high_decode_boxes<T><<<dim3(nblocks), dim3(256), 0, s>>>(mid, g, a, codes, side, jobs, njobs, nblocks);
 }
 
 #define STZ_INST(T)                                                                                              \
@@ -623,10 +664,10 @@ void launch_high_decode_box(hipStream_t s, const T* mid, Geom g, LevelArgs a, co
     template void launch_low_decode<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, T*);  \
     template void launch_high_decode<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, T*,  \
                                         int, int, int);                                                          \
-    template void launch_low_decode_box<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*,   \
-                                           T*, int, int, int, int, int, int, int, int);                          \
-    template void launch_high_decode_box<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*,  \
-                                            T*, int, int, int, int, int, int);
+    template void launch_low_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, \
+                                             T*, int, int, const BoxJob*, uint32_t, uint32_t);                  \
+    template void launch_high_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*,         \
+                                              const T*, const BoxJob*, uint32_t, uint32_t);
 STZ_INST(float)
 STZ_INST(double)
 

@@ -27,6 +27,8 @@ struct StreamDesc {
     uint64_t n_unpred;       // unpredictable values stored in the stream
     uint32_t ipw;            // decode work items per workgroup (<= 256)
     uint32_t wg_base;        // first decode workgroup of this stream
+    uint32_t cm_base;        // first entry of this stream in a chunk map
+    uint32_t pad2_;
 };
 
 struct StreamSet {
@@ -72,9 +74,11 @@ void launch_huff_pack(hipStream_t s, const StreamSet& ss, const uint16_t* codes,
                       const uint32_t* enc, const uint32_t* sub_pre_bits, const uint32_t* sub_pre_un,
                       uint8_t* payload);
 void launch_zero_bits(hipStream_t s, const StreamSet& ss, uint8_t* payload);
-// chunk list: nullptr = every chunk of every stream; else (stream, chunk, item_base) triples
+// chunk list: nullptr = every chunk of every stream at its stream position;
+// else (stream, chunk, item_base, out_pos) quads: the chunk's symbols go to
+// codes[out_pos ..] and cmap[cm_base + chunk] = out_pos
 void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
-                     const uint32_t* list, uint32_t nlist, DecItem* items);
+                     const uint32_t* list, uint32_t nlist, DecItem* items, uint32_t* cmap);
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                         const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
                         uint16_t* codes, void* side, uint32_t* err);
@@ -106,6 +110,22 @@ void launch_crop(hipStream_t s, const T* in, int px, int py, int ox, int oy, int
 struct LevelArgs {
     double eb, recip;
     uint64_t code_off[7];   // element offsets of the 7 block streams in codes/side
+    // random access into compact decode buffers: cmap[cm_off[b] + chunk] is the
+    // element offset of the chunk's first symbol, kNoChunk when it was not
+    // decoded (then *err is set and the point reads element 0)
+    const uint32_t* cmap;
+    uint32_t cm_off[7];
+    uint32_t* err;
+};
+constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
+
+// one box of a batched random-access decode (inclusive bounds); workgroups
+// blk0 .. next job's blk0 - 1 work on it
+struct BoxJob {
+    int x0, x1, y0, y1, z0, z1;
+    uint32_t blk0;
+    uint32_t pad_;
+    void* out;              // dense output of a full-resolution box
 };
 template <typename T>
 void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, const T* base,
@@ -119,15 +139,15 @@ void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const 
 template <typename T>
 void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
                         const T* side, T* out, int nx, int ny, int nz);
-// ROI: mid values on a box (mid coords, inclusive) -> mid array (full mid dims)
+// random access: mid values on boxes of the mid grid -> mid array (full mid dims)
 template <typename T>
-void launch_low_decode_box(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
-                           const T* side, T* mid, int mx, int my, int x0, int x1, int y0, int y1,
-                           int z0, int z1);
-// ROI: full-resolution box (inclusive) -> dense output
+void launch_low_decode_boxes(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
+                             const T* side, T* mid, int mx, int my, const BoxJob* jobs, uint32_t njobs,
+                             uint32_t nblocks);
+// random access: full-resolution boxes -> each job's dense output
 template <typename T>
-void launch_high_decode_box(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
-                            const T* side, T* out, int x0, int x1, int y0, int y1, int z0, int z1);
+void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
+                              const T* side, const BoxJob* jobs, uint32_t njobs, uint32_t nblocks);
 
 // ---- k_interp.hip ----
 struct InterpPass {

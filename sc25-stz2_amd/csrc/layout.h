@@ -80,6 +80,16 @@ __host__ __device__ inline uint32_t stream_pos(const Geom& g, int x, int y, int 
     return start + ((uint32_t)(z - cz * C) * wy + (uint32_t)(y - cy * C)) * wx + (uint32_t)(x - cx * C);
 }
 
+// first stream position of chunk c
+__host__ __device__ inline uint64_t chunk_start(const Geom& g, uint32_t c)
+{
+    if (g.chunk <= 0) return (uint64_t)c * kBaseChunk;
+    const int C = g.chunk;
+    const int cx = (int)(c % g.ncx), cy = (int)((c / g.ncx) % g.ncy), cz = (int)(c / ((uint32_t)g.ncx * g.ncy));
+    const int wy = imin_(C, g.dy - cy * C), wz = imin_(C, g.dz - cz * C);
+    return (uint64_t)cz * C * g.dx * g.dy + (uint64_t)cy * C * g.dx * wz + (uint64_t)cx * C * wy * wz;
+}
+
 __host__ __device__ inline uint64_t chunk_volume(const Geom& g, uint32_t c, uint64_t n)
 {
     if (g.chunk <= 0) {

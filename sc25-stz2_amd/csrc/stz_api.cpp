@@ -147,17 +147,35 @@ struct stz_ctx {
     std::vector<ProfRec> recs;
     std::map<std::string, std::pair<long, double>> agg;
 
+    size_t stage_off = 0;
+    std::vector<void*> retired;  // outgrown staging buffers, freed at the next call
+
+    // Pinned staging for async copies: each request gets a region of its own
+    // for the rest of the API call (a bump arena reset when a call starts).
     void* host_stage(size_t n)
     {
-        if (n > pinned_cap) {
-            if (pinned) (void)hipHostFree(pinned);
+        n = align_up(std::max<size_t>(n, 1), 256);
+        if (stage_off + n > pinned_cap) {
+            if (pinned) retired.push_back(pinned);
             pinned = nullptr;
             pinned_cap = 0;
-            const size_t cap = align_up(std::max<size_t>(n, 4u << 20), 1u << 20);
+            stage_off = 0;
+            const size_t cap = align_up(std::max<size_t>(2 * n, 8u << 20), 1u << 20);
             HIPCHK(hipHostMalloc(&pinned, cap, hipHostMallocDefault));
             pinned_cap = cap;
         }
-        return pinned;
+        void* r = (uint8_t*)pinned + stage_off;
+        stage_off += n;
+        return r;
+    }
+    // start of an API call: copies of the previous call have finished
+    void stage_reset()
+    {
+        if (stage_off == 0 && retired.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (void* p : retired) (void)hipHostFree(p);
+        retired.clear();
+        stage_off = 0;
     }
 };
 
@@ -177,6 +195,9 @@ struct stz_dstream {
     // decode tables (built on first decode)
     DBuf tabs, sorted;
     bool have_tabs = false;
+    // decoded base level kept for random access (stz_dstream_prepare)
+    DBuf base;
+    bool have_base = false;
 };
 
 namespace {
@@ -628,7 +649,6 @@ void ensure_tabs(stz_ctx* c, stz_dstream* ds)
     std::memcpy(stage + tb, all.data(), all.size() * 2);
     HIPCHK(hipMemcpyAsync(ds->tabs.p, stage, tb, hipMemcpyHostToDevice, c->stream));
     if (!all.empty()) HIPCHK(hipMemcpyAsync(ds->sorted.p, stage + tb, all.size() * 2, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     ds->have_tabs = true;
 }
 
@@ -637,25 +657,53 @@ void layout_of(const Meta& m, HostLayout& L)
     make_layout(L, m.fh.nx, m.fh.ny, m.fh.nz, m.fh.px, m.fh.py, m.fh.pz, (int)m.fh.chunk_high, (int)m.fh.chunk_low);
 }
 
-// decoded symbols of the selected chunks (lists[s] empty + all[s]: every chunk)
+// Decoded symbols.  Full layout (every chunk of every stream): stream s
+// starts at code_base[s] = its code_off.  Compact layout (random access):
+// streams decoded in full keep their stream order from code_base[s]; listed
+// chunks of the other streams are packed 32-symbol aligned, located through
+// cmap[cm_base[s] + chunk].
 struct Decoded {
-    DBuf codes, side;
+    DBuf codes, side, cmap;
+    uint64_t code_base[kStreams] = {};
+    uint32_t cm_base[kStreams] = {};
+    bool compact = false;
 };
 
-void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* want_all,
-                 const std::vector<int64_t>* lists, Decoded& out)
+// device error word: bit 0 corrupt bitstream, bit 1 a read outside the plan
+struct ErrFlag {
+    DBuf d;
+    uint32_t* h = nullptr;
+    explicit ErrFlag(stz_ctx* c) : d(c->pool, 64)
+    {
+        h = (uint32_t*)c->host_stage(64);
+        *h = 0;
+        K(c, "err_zero", [&] { launch_fill_u32(c->stream, d.as<uint32_t>(), 1, 0u); });
+    }
+    uint32_t* ptr() { return d.as<uint32_t>(); }
+    void fetch(stz_ctx* c) { HIPCHK(hipMemcpyAsync(h, d.p, 4, hipMemcpyDeviceToHost, c->stream)); }
+    void check() const
+    {
+        if (*h & 1u) throw StzError(STZ_E_STREAM, "corrupt Huffman bitstream");
+        if (*h & 2u) throw StzError(STZ_E_STREAM, "random access read a chunk outside its plan");
+    }
+};
+
+// Enqueue the Huffman decode of full[s] streams (every chunk) and of the
+// chunks lists[s] of the others.
+void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* full,
+                 const std::vector<int64_t>* lists, Decoded& out, uint32_t* err)
 {
     ensure_tabs(c, ds);
     hipStream_t st = c->stream;
     const Meta& m = ds->meta;
     StreamSet ss = make_streamset(L, m, (int)m.fh.dtype);
     const size_t ts = tsize_of((int)m.fh.dtype);
-    out.codes = DBuf(c->pool, L.total_codes * 2);
-    out.side = DBuf(c->pool, L.total_codes * ts);
-    // work items: per stream, ranges padded to its items-per-workgroup
-    std::vector<uint32_t> list;  // (stream, chunk, item) triples for partial streams
     bool all_full = true;
-    uint32_t item = 0, wg = 0;
+    for (int s = 0; s < kStreams; ++s) all_full = all_full && full[s];
+    out.compact = !all_full;
+    std::vector<uint32_t> quads;
+    uint64_t pos = 0;
+    uint32_t cm = 0, item = 0, wg = 0;
     for (int s = 0; s < kStreams; ++s) {
         StreamDesc& d = ss.s[s];
         d.item_base = item;
@@ -664,23 +712,38 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* w
         d.ipw = 256;
         while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > 12288.0 * 32.0) d.ipw >>= 1;
         d.wg_base = wg;
-        if (want_all[s]) {
+        d.cm_base = cm;
+        out.cm_base[s] = cm;
+        cm += L.g[s].n_chunks;
+        if (all_full) {
+            out.code_base[s] = L.code_off[s];
             d.item_count = L.n_sub[s];
-            for (uint32_t ch = 0; ch < L.g[s].n_chunks; ++ch) {
-                list.push_back((uint32_t)s);
-                list.push_back(ch);
-                list.push_back(item + L.chunk_sub[s][ch]);
-            }
         } else {
-            all_full = false;
-            uint32_t k = 0;
-            for (int64_t ch : lists[s]) {
-                list.push_back((uint32_t)s);
-                list.push_back((uint32_t)ch);
-                list.push_back(item + k);
-                k += L.chunk_sub[s][ch + 1] - L.chunk_sub[s][ch];
+            pos = align_up(pos, 64);
+            out.code_base[s] = pos;
+            d.code_off = 0;
+            if (full[s]) {
+                for (uint32_t ch = 0; ch < L.g[s].n_chunks; ++ch) {
+                    quads.push_back((uint32_t)s);
+                    quads.push_back(ch);
+                    quads.push_back(item + L.chunk_sub[s][ch]);
+                    quads.push_back((uint32_t)(pos + chunk_start(L.g[s], ch)));
+                }
+                pos += L.n[s];
+                d.item_count = L.n_sub[s];
+            } else {
+                uint32_t k = 0;
+                for (int64_t ch : lists[s]) {
+                    quads.push_back((uint32_t)s);
+                    quads.push_back((uint32_t)ch);
+                    quads.push_back(item + k);
+                    quads.push_back((uint32_t)pos);
+                    k += L.chunk_sub[s][ch + 1] - L.chunk_sub[s][ch];
+                    pos += align_up(chunk_volume(L.g[s], (uint32_t)ch, L.n[s]), 32);
+                }
+                d.item_count = k;
             }
-            d.item_count = k;
+            if (pos >= (1ull << 32)) throw StzError(STZ_E_DIMS, "random-access decode larger than 2^32 symbols");
         }
         const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
         item += nwg * d.ipw;
@@ -688,28 +751,30 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* w
     }
     ss.total_items = item;
     ss.total_wgs = wg;
+    const uint64_t ncodes = all_full ? L.total_codes : pos;
+    out.codes = DBuf(c->pool, ncodes * 2 + 64);
+    out.side = DBuf(c->pool, ncodes * ts + 64);
+    out.cmap = DBuf(c->pool, (size_t)cm * 4 + 16);
+    if (!all_full) K(c, "cmap_fill", [&] { launch_fill_u32(st, out.cmap.as<uint32_t>(), cm, kNoChunk); });
     if (!item) return;
-    DBuf items(c->pool, (size_t)item * sizeof(DecItem)), err(c->pool, 64);
+    DBuf items(c->pool, (size_t)item * sizeof(DecItem));
     uint8_t* pay = ds->payload.as<uint8_t>();
     if (all_full) {
-        K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, nullptr, 0, items.as<DecItem>()); });
+        K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, nullptr, 0, items.as<DecItem>(), nullptr); });
     } else {
-        DBuf ld(c->pool, list.size() * 4 + 16);
-        uint32_t* stage = (uint32_t*)c->host_stage(list.size() * 4 + 16);
-        std::memcpy(stage, list.data(), list.size() * 4);
-        HIPCHK(hipMemcpyAsync(ld.p, stage, list.size() * 4, hipMemcpyHostToDevice, st));
-        K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, ld.as<uint32_t>(), (uint32_t)(list.size() / 3), items.as<DecItem>()); });
-        HIPCHK(hipStreamSynchronize(st));  // staging reuse
+        DBuf ld(c->pool, quads.size() * 4 + 16);
+        uint32_t* stage = (uint32_t*)c->host_stage(quads.size() * 4 + 16);
+        std::memcpy(stage, quads.data(), quads.size() * 4);
+        HIPCHK(hipMemcpyAsync(ld.p, stage, quads.size() * 4, hipMemcpyHostToDevice, st));
+        K(c, "dec_prep", [&] {
+            launch_dec_prep(st, ss, pay, ld.as<uint32_t>(), (uint32_t)(quads.size() / 4), items.as<DecItem>(),
+                            out.cmap.as<uint32_t>());
+        });
     }
-    K(c, "err_zero", [&] { launch_fill_u32(st, err.as<uint32_t>(), 1, 0u); });
     K(c, "huff_decode", [&] {
         launch_huff_decode(st, ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
-                           out.codes.as<uint16_t>(), out.side.p, err.as<uint32_t>());
+                           out.codes.as<uint16_t>(), out.side.p, err);
     });
-    uint32_t* herr = (uint32_t*)c->host_stage(64);
-    HIPCHK(hipMemcpyAsync(herr, err.p, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (*herr) throw StzError(STZ_E_STREAM, "corrupt Huffman bitstream");
 }
 
 template <typename T>
@@ -719,42 +784,49 @@ void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& 
                                     (int)m.fh.order, m.fh.alpha, m.fh.beta, stop_level);
     run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
         K(c, "interp_decode_fused", [&] {
-            launch_interp_decode_fused<T>(c->stream, fl, v, dd.codes.as<uint16_t>() + L.code_off[0],
-                                          dd.side.as<T>() + L.code_off[0]);
+            launch_interp_decode_fused<T>(c->stream, fl, v, dd.codes.as<uint16_t>() + dd.code_base[0],
+                                          dd.side.as<T>() + dd.code_base[0]);
         });
     }, [&](const InterpPass& p) {
         K(c, "interp_decode", [&] {
-            launch_interp_decode<T>(c->stream, p, v, dd.codes.as<uint16_t>() + L.code_off[0],
-                                    dd.side.as<T>() + L.code_off[0]);
+            launch_interp_decode<T>(c->stream, p, v, dd.codes.as<uint16_t>() + dd.code_base[0],
+                                    dd.side.as<T>() + dd.code_base[0]);
         });
     });
 }
 
-LevelArgs level_args(const Meta& m, const HostLayout& L, int level)
+// level 1 (low) or 2 (high) arguments; random access goes through the chunk map
+LevelArgs level_args(const Meta& m, const Decoded& dd, int level, uint32_t* err)
 {
     LevelArgs a{};
     a.eb = level == 1 ? m.fh.eb_low : m.fh.eb_high;
     a.recip = 1.0 / a.eb;
-    for (int b = 0; b < 7; ++b) a.code_off[b] = L.code_off[(level == 1 ? 1 : 8) + b];
+    const int s0 = level == 1 ? 1 : 8;
+    for (int b = 0; b < 7; ++b) {
+        a.code_off[b] = dd.code_base[s0 + b];
+        a.cm_off[b] = dd.cm_base[s0 + b];
+    }
+    a.cmap = dd.compact ? dd.cmap.as<uint32_t>() : nullptr;
+    a.err = err;
     return a;
 }
 
 // output buffer: the caller's device pointer, or a pool buffer copied back
 struct OutBuf {
-    void* user;
-    bool dev;
+    void* user = nullptr;
+    bool dev = false;
     DBuf tmp;
-    void* ptr;
-    size_t bytes;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    OutBuf() = default;
     OutBuf(stz_ctx* c, void* u, size_t n) : user(u), dev(is_device_ptr(u)), bytes(n)
     {
         if (dev) ptr = u;
-        else { tmp = DBuf(c->pool, n); ptr = tmp.p; }
+        else { tmp = DBuf(c->pool, n + 16); ptr = tmp.p; }
     }
-    void finish(stz_ctx* c)
+    void copy_back(stz_ctx* c)
     {
-        if (!dev) HIPCHK(hipMemcpyAsync(user, ptr, bytes, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!dev && bytes) HIPCHK(hipMemcpyAsync(user, ptr, bytes, hipMemcpyDeviceToHost, c->stream));
     }
 };
 
@@ -766,103 +838,205 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     layout_of(m, L);
     bool all[kStreams];
     for (int s = 0; s < kStreams; ++s) all[s] = true;
+    ErrFlag err(c);
     Decoded dd;
-    huff_decode(c, ds, L, all, nullptr, dd);
+    huff_decode(c, ds, L, all, nullptr, dd, err.ptr());
     hipStream_t st = c->stream;
     const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
     DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));
     decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
-    const LevelArgs la = level_args(m, L, 1), ha = level_args(m, L, 2);
+    const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
     K(c, "low_decode", [&] { launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
     OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
     K(c, "high_decode", [&] {
         launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
                               (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz);
     });
-    ob.finish(c);
+    ob.copy_back(c);
+    err.fetch(c);
+    HIPCHK(hipStreamSynchronize(st));
+    err.check();
 }
 
-template <typename T>
-void decompress_level(stz_ctx* c, stz_dstream* ds, int level, void* out, size_t od[3])
+// the normalised box of a box or slice request
+Box request_box(const Meta& m, const stz_request& r)
 {
-    const Meta& m = ds->meta;
-    if (level == 3) {
-        od[0] = m.fh.nx; od[1] = m.fh.ny; od[2] = m.fh.nz;
-        decompress_full<T>(c, ds, out);
-        return;
-    }
-    HostLayout L;
-    layout_of(m, L);
-    bool all[kStreams];
-    std::vector<int64_t> lists[kStreams];
-    for (int s = 0; s < kStreams; ++s) all[s] = (s == 0) || (level == 2 && s < 8);
-    Decoded dd;
-    huff_decode(c, ds, L, all, lists, dd);
-    hipStream_t st = c->stream;
-    const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
-    DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T));
-    decode_base<T>(c, m, L, dd, vb.as<T>(), level == 0 ? 2 : 0);
-    const int sh = level == 0 ? 3 : level == 1 ? 2 : 1;
-    od[0] = (m.fh.nx + (1u << sh) - 1) >> sh;
-    od[1] = (m.fh.ny + (1u << sh) - 1) >> sh;
-    od[2] = (m.fh.nz + (1u << sh) - 1) >> sh;
-    OutBuf ob(c, out, od[0] * od[1] * od[2] * sizeof(T));
-    if (level == 2) {
-        const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
-        DBuf mid(c->pool, (size_t)mx * my * mz * sizeof(T));
-        const LevelArgs la = level_args(m, L, 1);
-        K(c, "low_decode", [&] { launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
-        K(c, "crop", [&] { launch_crop<T>(st, mid.as<T>(), mx, my, (int)od[0], (int)od[1], (int)od[2], 0, 0, 0, 1, (T*)ob.ptr); });
-        ob.finish(c);
+    int a[3], b[3];
+    if (r.kind == STZ_REQ_BOX) {
+        for (int k = 0; k < 3; ++k) { a[k] = r.a[k]; b[k] = r.a[3 + k]; }
     } else {
-        const int step = level == 0 ? 2 : 1;
-        K(c, "crop", [&] { launch_crop<T>(st, vb.as<T>(), lx, ly, (int)od[0], (int)od[1], (int)od[2], 0, 0, 0, step, (T*)ob.ptr); });
-        ob.finish(c);
+        const int axis = r.a[0], index = r.a[1];
+        if (axis < 0 || axis > 2) throw StzError(STZ_E_ARG, "slice axis");
+        const uint64_t d[3] = {m.fh.nx, m.fh.ny, m.fh.nz};
+        if (index < 0 || (uint64_t)index >= d[axis]) throw StzError(STZ_E_ARG, "slice index");
+        for (int k = 0; k < 3; ++k) { a[k] = 0; b[k] = (int)d[k] - 1; }
+        a[axis] = b[axis] = index;
     }
+    return Box{a[0], b[0], a[1], b[1], a[2], b[2]};
 }
 
+// Decode of the kept base level into ds->base (stz_dstream_prepare).
 template <typename T>
-void decompress_roi(stz_ctx* c, stz_dstream* ds, int ax, int ay, int az, int bx, int by, int bz, void* out, size_t od[3])
+void prepare_base(stz_ctx* c, stz_dstream* ds)
+{
+    if (ds->have_base) return;
+    const Meta& m = ds->meta;
+    HostLayout L;
+    layout_of(m, L);
+    bool full[kStreams] = {};
+    std::vector<int64_t> lists[kStreams];
+    full[0] = true;
+    ErrFlag err(c);
+    Decoded dd;
+    huff_decode(c, ds, L, full, lists, dd, err.ptr());
+    DBuf b(ds->pool, (size_t)L.g[0].dx * L.g[0].dy * L.g[0].dz * sizeof(T));
+    decode_base<T>(c, m, L, dd, b.as<T>(), 0);
+    err.fetch(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    err.check();
+    ds->base = std::move(b);
+    ds->have_base = true;
+}
+
+// Random-access requests served together: one Huffman decode of the union
+// of their chunks, one launch for all low regions, one for all high boxes.
+// Full-field requests run after, through the full pipeline.
+template <typename T>
+void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq, void* const* outs, size_t* od)
 {
     const Meta& m = ds->meta;
     HostLayout L;
     layout_of(m, L);
-    Plan pl;
-    make_plan(pl, (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, (int)m.fh.px, (int)m.fh.py, (int)m.fh.pz,
-              (int)m.fh.chunk_high, (int)m.fh.chunk_low, ax, ay, az, bx, by, bz);
-    bool all[kStreams];
-    std::vector<int64_t> lists[kStreams];
-    all[0] = true;  // the base is decoded in full (F5:2339)
-    for (int b = 0; b < 7; ++b) {
-        all[1 + b] = false;
-        all[8 + b] = false;
-        if (pl.low[b].valid) lists[1 + b] = pl.low[b].chunks;
-        if (pl.high[b].valid) lists[8 + b] = pl.high[b].chunks;
-    }
-    Decoded dd;
-    huff_decode(c, ds, L, all, lists, dd);
     hipStream_t st = c->stream;
-    const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
-    const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
-    DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)mx * my * mz * sizeof(T));
-    decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
-    const LevelArgs la = level_args(m, L, 1), ha = level_args(m, L, 2);
-    for (const Box& R : pl.regions) {
-        if (R.x0 > R.x1 || R.y0 > R.y1 || R.z0 > R.z1) continue;
-        K(c, "low_decode_box", [&] {
-            launch_low_decode_box<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
-                                     mx, my, R.x0, R.x1, R.y0, R.y1, R.z0, R.z1);
-        });
+    std::vector<Plan> plans(n);
+    std::vector<std::vector<uint8_t>> want(kStreams);
+    bool full[kStreams] = {};
+    bool need_base = false, need_mid = false, need_lowfull = false;
+    std::vector<int> full_reqs;
+    for (int i = 0; i < n; ++i) {
+        const stz_request& r = rq[i];
+        size_t* o = od + 3 * i;
+        if (r.kind == STZ_REQ_BOX || r.kind == STZ_REQ_SLICE) {
+            const Box q = request_box(m, r);
+            Plan& pl = plans[i];
+            make_plan(pl, (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, (int)m.fh.px, (int)m.fh.py, (int)m.fh.pz,
+                      (int)m.fh.chunk_high, (int)m.fh.chunk_low, q.x0, q.y0, q.z0, q.x1, q.y1, q.z1);
+            // make_query keeps lo = min(a, b) when both corners lie past the
+            // far end (F5:1629-1633), a box outside the field: refuse it
+            if (pl.q.x0 >= (int)m.fh.nx || pl.q.y0 >= (int)m.fh.ny || pl.q.z0 >= (int)m.fh.nz)
+                throw StzError(STZ_E_ARG, "query box outside the field");
+            for (int b = 0; b < 7; ++b) {
+                for (int lv = 0; lv < 2; ++lv) {
+                    const BlockPlan& pb = lv == 0 ? pl.low[b] : pl.high[b];
+                    if (!pb.valid) continue;
+                    auto& w = want[(lv == 0 ? 1 : 8) + b];
+                    if (w.empty()) w.assign(L.g[(lv == 0 ? 1 : 8) + b].n_chunks, 0);
+                    for (int64_t ch : pb.chunks) w[(size_t)ch] = 1;
+                }
+            }
+            o[0] = (size_t)(pl.q.x1 - pl.q.x0 + 1);
+            o[1] = (size_t)(pl.q.y1 - pl.q.y0 + 1);
+            o[2] = (size_t)(pl.q.z1 - pl.q.z0 + 1);
+            need_base = need_mid = true;
+        } else if (r.kind == STZ_REQ_LEVEL) {
+            const int level = r.a[0];
+            if (level < 0 || level > 3) throw StzError(STZ_E_ARG, "level");
+            const int sh = 3 - level;
+            o[0] = (m.fh.nx + (1u << sh) - 1) >> sh;
+            o[1] = (m.fh.ny + (1u << sh) - 1) >> sh;
+            o[2] = (m.fh.nz + (1u << sh) - 1) >> sh;
+            if (level == 3) { full_reqs.push_back(i); continue; }
+            need_base = true;
+            if (level == 2) need_mid = need_lowfull = true;
+        } else {
+            throw StzError(STZ_E_ARG, "request kind");
+        }
     }
-    od[0] = (size_t)(pl.q.x1 - pl.q.x0 + 1);
-    od[1] = (size_t)(pl.q.y1 - pl.q.y0 + 1);
-    od[2] = (size_t)(pl.q.z1 - pl.q.z0 + 1);
-    OutBuf ob(c, out, od[0] * od[1] * od[2] * sizeof(T));
-    K(c, "high_decode_box", [&] {
-        launch_high_decode_box<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
-                                  pl.q.x0, pl.q.x1, pl.q.y0, pl.q.y1, pl.q.z0, pl.q.z1);
-    });
-    ob.finish(c);
+    std::vector<OutBuf> ob(n);
+    for (int i = 0; i < n; ++i) {
+        if (!outs[i]) throw StzError(STZ_E_ARG, "null output");
+        ob[i] = OutBuf(c, outs[i], od[3 * i] * od[3 * i + 1] * od[3 * i + 2] * sizeof(T));
+    }
+    ErrFlag err(c);
+    if (need_base) {
+        full[0] = !ds->have_base;
+        for (int b = 0; b < 7; ++b) full[1 + b] = need_lowfull;
+        std::vector<int64_t> lists[kStreams];
+        for (int s = 1; s < kStreams; ++s)
+            if (!full[s] && !want[s].empty())
+                for (uint32_t ch = 0; ch < want[s].size(); ++ch)
+                    if (want[s][ch]) lists[s].push_back(ch);
+        Decoded dd;
+        huff_decode(c, ds, L, full, lists, dd, err.ptr());
+        const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
+        const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
+        DBuf vbuf;
+        const T* vb;
+        if (ds->have_base) {
+            vb = ds->base.as<T>();
+        } else {
+            vbuf = DBuf(c->pool, (size_t)lx * ly * lz * sizeof(T));
+            decode_base<T>(c, m, L, dd, vbuf.as<T>(), 0);
+            vb = vbuf.as<T>();
+        }
+        DBuf mid;
+        if (need_mid) mid = DBuf(c->pool, (size_t)mx * my * mz * sizeof(T));
+        const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
+        if (need_lowfull)
+            K(c, "low_decode", [&] { launch_low_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
+        // box jobs: low regions first, then the high boxes
+        std::vector<BoxJob> lowj, highj;
+        uint32_t lowb = 0, highb = 0;
+        auto add = [](std::vector<BoxJob>& v, uint32_t& blocks, const Box& R, void* out) {
+            const uint64_t vol = (uint64_t)(R.x1 - R.x0 + 1) * (R.y1 - R.y0 + 1) * (R.z1 - R.z0 + 1);
+            BoxJob j{};
+            j.x0 = R.x0; j.x1 = R.x1; j.y0 = R.y0; j.y1 = R.y1; j.z0 = R.z0; j.z1 = R.z1;
+            j.blk0 = blocks;
+            j.out = out;
+            blocks += (uint32_t)std::min<uint64_t>(std::max<uint64_t>((vol + 1023) / 1024, 1), 2048);
+            v.push_back(j);
+        };
+        for (int i = 0; i < n; ++i) {
+            if (rq[i].kind != STZ_REQ_BOX && rq[i].kind != STZ_REQ_SLICE) continue;
+            for (const Box& R : plans[i].regions)
+                if (R.x0 <= R.x1 && R.y0 <= R.y1 && R.z0 <= R.z1 && !need_lowfull) add(lowj, lowb, R, nullptr);
+            add(highj, highb, plans[i].q, ob[i].ptr);
+        }
+        const size_t jb = (lowj.size() + highj.size()) * sizeof(BoxJob);
+        DBuf jobs(c->pool, jb + 16);
+        if (jb) {
+            uint8_t* stage = (uint8_t*)c->host_stage(jb);
+            if (!lowj.empty()) std::memcpy(stage, lowj.data(), lowj.size() * sizeof(BoxJob));
+            if (!highj.empty()) std::memcpy(stage + lowj.size() * sizeof(BoxJob), highj.data(), highj.size() * sizeof(BoxJob));
+            HIPCHK(hipMemcpyAsync(jobs.p, stage, jb, hipMemcpyHostToDevice, st));
+        }
+        const BoxJob* jl = jobs.as<BoxJob>();
+        if (!lowj.empty())
+            K(c, "low_decode_box", [&] {
+                launch_low_decode_boxes<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                                           mx, my, jl, (uint32_t)lowj.size(), lowb);
+            });
+        if (!highj.empty())
+            K(c, "high_decode_box", [&] {
+                launch_high_decode_boxes<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(),
+                                            jl + lowj.size(), (uint32_t)highj.size(), highb);
+            });
+        for (int i = 0; i < n; ++i) {
+            if (rq[i].kind != STZ_REQ_LEVEL || rq[i].a[0] == 3) continue;
+            const size_t* o = od + 3 * i;
+            const int level = rq[i].a[0];
+            if (level == 2)
+                K(c, "crop", [&] { launch_crop<T>(st, mid.as<T>(), mx, my, (int)o[0], (int)o[1], (int)o[2], 0, 0, 0, 1, (T*)ob[i].ptr); });
+            else
+                K(c, "crop", [&] { launch_crop<T>(st, vb, lx, ly, (int)o[0], (int)o[1], (int)o[2], 0, 0, 0, level == 0 ? 2 : 1, (T*)ob[i].ptr); });
+        }
+        for (int i = 0; i < n; ++i)
+            if (!(rq[i].kind == STZ_REQ_LEVEL && rq[i].a[0] == 3)) ob[i].copy_back(c);
+        err.fetch(c);
+        HIPCHK(hipStreamSynchronize(st));
+        err.check();
+    }
+    for (int i : full_reqs) decompress_full<T>(c, ds, outs[i]);
 }
 
 // ------------------------------------------------------------ payload parsing
@@ -1044,7 +1218,10 @@ template <typename F>
 int guard(stz_ctx* c, F&& f)
 {
     try {
-        if (c) HIPCHK(hipSetDevice(c->device));
+        if (c) {
+            HIPCHK(hipSetDevice(c->device));
+            c->stage_reset();
+        }
         f();
         return STZ_OK;
     } catch (const StzError& e) {
@@ -1131,6 +1308,7 @@ void stz_destroy(stz_ctx* c)
         (void)hipEventDestroy(r.b);
     }
     if (c->pinned) (void)hipHostFree(c->pinned);
+    for (void* p : c->retired) (void)hipHostFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -1262,18 +1440,42 @@ int stz_dstream_decompress_roi(stz_ctx* c, stz_dstream* ds, int x0, int y0, int 
                                size_t od[3])
 {
     if (!c || !ds || !out || !od) return STZ_E_ARG;
-    return guard(c, [&] {
-        if (ds->meta.fh.dtype == STZ_F32) decompress_roi<float>(c, ds, x0, y0, z0, x1, y1, z1, out, od);
-        else decompress_roi<double>(c, ds, x0, y0, z0, x1, y1, z1, out, od);
-    });
+    stz_request r{};
+    r.kind = STZ_REQ_BOX;
+    r.a[0] = x0; r.a[1] = y0; r.a[2] = z0; r.a[3] = x1; r.a[4] = y1; r.a[5] = z1;
+    return stz_dstream_decompress_batch(c, ds, 1, &r, &out, od);
 }
 
 int stz_dstream_decompress_level(stz_ctx* c, stz_dstream* ds, int level, void* out, size_t od[3])
 {
     if (!c || !ds || !out || !od || level < 0 || level > 3) return STZ_E_ARG;
+    stz_request r{};
+    r.kind = STZ_REQ_LEVEL;
+    r.a[0] = level;
+    return stz_dstream_decompress_batch(c, ds, 1, &r, &out, od);
+}
+
+int stz_dstream_prepare(stz_ctx* c, stz_dstream* ds, int keep_base)
+{
+    if (!c || !ds) return STZ_E_ARG;
     return guard(c, [&] {
-        if (ds->meta.fh.dtype == STZ_F32) decompress_level<float>(c, ds, level, out, od);
-        else decompress_level<double>(c, ds, level, out, od);
+        ensure_tabs(c, ds);
+        if (keep_base) {
+            if (ds->meta.fh.dtype == STZ_F32) prepare_base<float>(c, ds);
+            else prepare_base<double>(c, ds);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stz_dstream_decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* reqs, void* const* outs,
+                                 size_t* out_dims)
+{
+    if (!c || !ds || n < 0 || (n > 0 && (!reqs || !outs || !out_dims))) return STZ_E_ARG;
+    if (n == 0) return STZ_OK;
+    return guard(c, [&] {
+        if (ds->meta.fh.dtype == STZ_F32) decompress_batch<float>(c, ds, n, reqs, outs, out_dims);
+        else decompress_batch<double>(c, ds, n, reqs, outs, out_dims);
     });
 }
 

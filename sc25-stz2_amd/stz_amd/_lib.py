@@ -21,7 +21,7 @@ EXPORTS = [
     "stz_query_info",
     "stz_compress_device", "stz_dstream_upload", "stz_dstream_download", "stz_dstream_info",
     "stz_dstream_decompress", "stz_dstream_decompress_roi", "stz_dstream_decompress_level",
-    "stz_dstream_payload", "stz_dstream_free",
+    "stz_dstream_payload", "stz_dstream_free", "stz_dstream_prepare", "stz_dstream_decompress_batch",
     "stz_split", "stz_merge", "stz_residual_predict", "stz_residual_reconstruct", "stz_quantize",
     "stz_dequantize", "stz_huff_lengths", "stz_plan_query",
 ]
@@ -41,6 +41,13 @@ class Params(C.Structure):
         ("alpha", C.c_double), ("beta", C.c_double),
         ("zstd_level", C.c_int), ("tuner", C.c_int),
     ]
+
+
+REQ_BOX, REQ_SLICE, REQ_LEVEL = 0, 1, 2
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("a", C.c_int * 6)]
 
 
 class Info(C.Structure):
@@ -101,6 +108,8 @@ def lib():
     L.stz_dstream_decompress_roi.argtypes = [vp, vp, i32, i32, i32, i32, i32, i32, vp, P(sz)]
     L.stz_dstream_decompress_level.argtypes = [vp, vp, i32, vp, P(sz)]
     L.stz_dstream_payload.argtypes = [vp, P(vp), P(sz)]
+    L.stz_dstream_prepare.argtypes = [vp, vp, i32]
+    L.stz_dstream_decompress_batch.argtypes = [vp, vp, i32, P(Request), P(vp), P(sz)]
     L.stz_dstream_free.argtypes = [vp]
     L.stz_dstream_free.restype = None
     L.stz_split.argtypes = [vp, i32, vp, sz, sz, sz, P(vp)]

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import Info, Params, StzError, check, lib
+from ._lib import REQ_BOX, REQ_LEVEL, REQ_SLICE, Info, Params, Request, StzError, check, lib
 
 F32, F64 = 0, 1
 EB_MODES = {"abs": 0, "rel": 1, "ref": 2}
@@ -143,6 +143,56 @@ class DStream:
         check(lib().stz_dstream_decompress_level(self.ctx.h, self.h, level, _ptr(out), od),
               "stz_dstream_decompress_level")
         return out[: od[0] * od[1] * od[2]].reshape(od[2], od[1], od[0])
+
+
+    # -- random access in batches
+    def prepare(self, keep_base=True, ctx=None):
+        """Build the decode tables and (keep_base) keep the decoded base level on
+        the device, so random-access requests skip it."""
+        c = ctx if ctx is not None else self.ctx
+        check(lib().stz_dstream_prepare(c.h, self.h, 1 if keep_base else 0), "stz_dstream_prepare")
+
+    def request_capacity(self, req):
+        """Elements a request's output needs at most."""
+        i = self.info
+        kind = req[0]
+        if kind == "box":
+            x0, y0, z0, x1, y1, z1 = req[1:7]
+            return (abs(x1 - x0) + 1) * (abs(y1 - y0) + 1) * (abs(z1 - z0) + 1)
+        if kind == "slice":
+            d = [i.nx, i.ny, i.nz]
+            d[req[1]] = 1
+            return d[0] * d[1] * d[2]
+        if kind == "level":
+            sh = 3 - req[1]
+            return (((i.nx + (1 << sh) - 1) >> sh) * ((i.ny + (1 << sh) - 1) >> sh) *
+                    ((i.nz + (1 << sh) - 1) >> sh))
+        raise ValueError("request kind %r" % (kind,))
+
+    def decompress_batch(self, requests, outs=None, like=None, ctx=None):
+        """Serve several random-access requests in one decode.  requests:
+        ("box", x0, y0, z0, x1, y1, z1) | ("slice", axis, index) | ("level", level).
+        Returns one (z, y, x) array per request."""
+        c = ctx if ctx is not None else self.ctx
+        n = len(requests)
+        R = (Request * n)()
+        kinds = {"box": REQ_BOX, "slice": REQ_SLICE, "level": REQ_LEVEL}
+        for k, r in enumerate(requests):
+            R[k].kind = kinds[r[0]]
+            for j, v in enumerate(r[1:]):
+                R[k].a[j] = int(v)
+        i = self.info
+        if outs is None:
+            base = like if like is not None else np.empty(0)
+            outs = [_empty_like_kind(base, (self.request_capacity(r),), i.dtype) for r in requests]
+        ptrs = (C.c_void_p * n)(*[_ptr(o) for o in outs])
+        od = (C.c_size_t * (3 * n))()
+        check(lib().stz_dstream_decompress_batch(c.h, self.h, n, R, ptrs, od), "stz_dstream_decompress_batch")
+        res = []
+        for k in range(n):
+            a, b, z = od[3 * k], od[3 * k + 1], od[3 * k + 2]
+            res.append(outs[k][: a * b * z].reshape(z, b, a))
+        return res
 
 
 class Context:
