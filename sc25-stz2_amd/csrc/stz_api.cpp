@@ -1290,7 +1290,9 @@ int stz_create(stz_ctx** out, int device)
     auto c = std::unique_ptr<stz_ctx>(new stz_ctx());
     c->device = device;
     return guard(c.get(), [&] {
-        HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        // a blocking stream: it orders after work on the legacy default
+        // stream, where torch puts the caller's tensors by default
+        HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
         c->stream = c->own;
         c->pool = std::make_shared<Pool>();
         c->pool->device = device;

@@ -67,6 +67,30 @@ def grf(shape, slope, seed, device=None):
     return g
 
 
+def grf_device(shape, slope, seed, device, dtype=None):
+    """grf() computed on the GPU and returned as a device tensor (torch dtype,
+    float64 by default).  The white noise is the same numpy PCG64 stream,
+    drawn plane by plane so no full host copy is held."""
+    import torch
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    t = torch.empty(shape, dtype=torch.float64, device=device)
+    for z in range(nz):
+        t[z].copy_(torch.from_numpy(rng.standard_normal(size=(ny, nx))))
+    F = torch.fft.rfftn(t)
+    del t
+    kz = torch.fft.fftfreq(nz, device=device, dtype=torch.float64) * nz
+    ky = torch.fft.fftfreq(ny, device=device, dtype=torch.float64) * ny
+    kx = torch.fft.rfftfreq(nx, device=device, dtype=torch.float64) * nx
+    for z in range(nz):   # k^(slope/2) amplitude, plane by plane to bound memory
+        k = torch.sqrt(kz[z] ** 2 + ky[:, None] ** 2 + kx[None, :] ** 2)
+        F[z] *= torch.where(k > 0, k.clamp(min=1e-30) ** (slope / 2.0), torch.zeros_like(k))
+    g = torch.fft.irfftn(F, s=shape)
+    del F
+    g = (g - g.mean()) / g.std()
+    return g if dtype is None else g.to(dtype)
+
+
 def lognormal(n=512, sigma=1.5, slope=-3.0, seed=1002, dtype=np.float32, device=None, shape=None):
     """exp(sigma * g), g a unit GRF with P(k) ~ k^-3 (density-like field)."""
     shape = shape if shape is not None else (n, n, n)

@@ -4,8 +4,9 @@ Streams stay compressed in HBM (stz_dstream, decode tables and the decoded
 base level prepared once).  Requests -- 3D boxes, axis slices and
 progressive levels -- queue on arrival.  Each worker thread owns a context
 (its own HIP stream and memory pool).  When a worker frees up it admits the
-queued requests that fit its slot limit and the server's memory budget and
-serves them with ONE stz_dstream_decompress_batch call: one Huffman decode of
+queued requests of its class (worker 0: full-resolution decodes; the
+others: boxes, slices and the 1/8, 1/4, 1/2 levels) that fit its slot limit and the server's
+memory budget and serves them with ONE stz_dstream_decompress_batch call: one Huffman decode of
 the union of their chunks, one launch for all their low regions and one for
 all their high boxes.  Requests that arrive while batches are on the GPU wait
 only for the next free worker; batches of different workers overlap on the
@@ -22,7 +23,7 @@ __all__ = ["RoiServer", "poisson_trace", "summarize"]
 
 
 class _Pending:
-    __slots__ = ("idx", "req", "t_arrival", "nbytes", "t_done", "out", "batch")
+    __slots__ = ("idx", "req", "t_arrival", "nbytes", "t_done", "out", "batch", "large")
 
     def __init__(self, idx, req, t_arrival, nbytes):
         self.idx = idx
@@ -32,6 +33,8 @@ class _Pending:
         self.t_done = None
         self.out = None
         self.batch = None
+        # a full-resolution level decodes every stream: tens of ms at 1024^3
+        self.large = req[0] == "level" and req[1] == 3
 
 
 class RoiServer:
@@ -71,19 +74,29 @@ class RoiServer:
             self.cv.notify()
         return p
 
-    def _take_batch(self):
-        """FIFO admission: the head always goes (alone if it exceeds the
-        budget by itself); then queued requests while slots and memory last."""
+    def _take_batch(self, w):
+        """Admission for worker w.  With more than one worker, worker 0 takes
+        the full-resolution decodes (tens of ms at 1024^3) and the others
+        everything else, so a box never queues behind a full decode; worker 0
+        falls back to the other requests when no full decode waits.  FIFO within a class, up to max_batch requests
+        and the memory budget (a request larger than the budget goes alone
+        once nothing else is in flight)."""
+        if len(self.ctxs) == 1:
+            picks = list(self.queue)
+        elif w == 0:
+            picks = [p for p in self.queue if p.large] or list(self.queue)
+        else:
+            picks = [p for p in self.queue if not p.large]
         batch = []
-        while self.queue and len(batch) < self.max_batch:
-            p = self.queue[0]
-            if batch and self.inflight_bytes + p.nbytes > self.mem_budget:
+        for p in picks:
+            if len(batch) >= self.max_batch:
                 break
-            if not batch and self.inflight_bytes + p.nbytes > self.mem_budget and self.inflight_bytes > 0:
+            if self.inflight_bytes + p.nbytes > self.mem_budget and (batch or self.inflight_bytes > 0):
                 break
-            self.queue.popleft()
             self.inflight_bytes += p.nbytes
             batch.append(p)
+        for p in batch:
+            self.queue.remove(p)
         return batch
 
     def _worker(self, w):
@@ -95,7 +108,7 @@ class RoiServer:
                 while not batch:
                     if self.closed and not self.queue:
                         return
-                    batch = self._take_batch()
+                    batch = self._take_batch(w)
                     if not batch:
                         self.cv.wait(0.01)
             t0 = time.perf_counter()
