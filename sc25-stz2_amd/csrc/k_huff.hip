@@ -367,7 +367,7 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
         if (i >= nlist) return;
         s = list[4 * i];
         c = list[4 * i + 1];
-        ib = list[4 * i + 2];
+        ib = ss.s[s].item_base + list[4 * i + 2];
         pstart = list[4 * i + 3];
         cmap[ss.s[s].cm_base + c] = (uint32_t)pstart;
     } else {
@@ -402,8 +402,7 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
 
 // Sequential Huffman decode of one lane's sub-chunk.  Bits come from `W`
 // (the workgroup's LDS stage or global memory; word w of the stream is
-// W[w - wbase]).  The refill is branch-free and its word is fetched one
-// symbol ahead, so the only serial latency per symbol is the table lookup.
+// W[w - wbase]).  Each refill word is fetched one refill ahead.
 // Codes leave in 64-byte bursts (32 codes held in registers) when the
 // destination is 64-byte aligned, in 8-byte groups otherwise.
 template <int kGroup>
@@ -426,13 +425,13 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     const uint32_t cnt = it.cnt;
     uint32_t i = 0;
     auto step = [&]() -> uint32_t {
-        const bool need = nb < 32;
-        const int sh = need ? 32 - nb : 0;
-        buf |= need ? ((uint64_t)nextw << sh) : 0ull;
-        bad |= (need && wpos >= nwords) ? 1u : 0u;
-        wpos += need ? 1u : 0u;
-        nb += need ? 32 : 0;
-        nextw = wpos < nwords ? W[wpos] : 0u;
+        if (nb < 32) {
+            buf |= (uint64_t)nextw << (32 - nb);
+            bad |= wpos >= nwords ? 1u : 0u;
+            ++wpos;
+            nb += 32;
+            nextw = wpos < nwords ? W[wpos] : 0u;
+        }
         const uint32_t e = lut[(uint32_t)(buf >> (64 - kLutBits))];
         uint32_t sym;
         int l;
@@ -500,10 +499,13 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
 }
 
 // One workgroup = `ipw` consecutive sub-chunks of one stream, one per lane.
-// Their bits form one contiguous range; when it fits, the workgroup first
-// copies it into LDS with coalesced loads, so the lanes' data-dependent
-// refills are LDS reads.  ipw is chosen per stream on the host so that the
-// expected range fits the stage.
+// Their bits form one contiguous range.  kStage: the workgroup first copies
+// that range into LDS with coalesced loads when it fits, so the lanes'
+// data-dependent refills are LDS reads (low-entropy payloads; ipw is chosen
+// per stream so the range fits).  Without the stage (dense payloads) the
+// refills read global memory, every workgroup decodes 256 sub-chunks, and the
+// smaller LDS footprint lets four times as many workgroups share a CU.
+template <bool kStage>
 __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
                                                      const DecTab* __restrict__ tabs,
                                                      const uint16_t* __restrict__ sorted,
@@ -514,8 +516,9 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
 {
     __shared__ uint32_t lut[1 << kLutBits];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
-    __shared__ uint32_t stage[kDecStageWords];
-    const int s = find_stream_by_wg(ss, blockIdx.x);
+    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    const uint32_t wgid = blockIdx.x;
+    const int s = find_stream_by_wg(ss, wgid);
     if (s < 0) return;  // uniform across the workgroup
     const StreamDesc& d = ss.s[s];
     const DecTab* t = tabs + s;
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
         cn[threadIdx.x] = t->count[threadIdx.x];
         fi[threadIdx.x] = t->first_index[threadIdx.x];
     }
-    const uint32_t first = d.item_base + (blockIdx.x - d.wg_base) * d.ipw;
+    const uint32_t first = d.item_base + (wgid - d.wg_base) * d.ipw;
     const uint32_t end = d.item_base + d.item_count;
     const uint32_t lim = first + d.ipw < end ? first + d.ipw : end;
     const uint32_t last = lim - 1u;
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
     const uint64_t w0 = items[first].bit0 >> 5;
     const uint64_t w1r = ((items[last].bit0 + items[last].bits + 31) >> 5) + 2;
     const uint64_t w1 = w1r < d.bits_words ? w1r : d.bits_words;
-    const bool use_lds = w1 > w0 && (w1 - w0) <= (uint64_t)kDecStageWords;
+    const bool use_lds = kStage && w1 > w0 && (w1 - w0) <= (uint64_t)kDecStageWords;
     if (use_lds) {
         const uint64_t nw = w1 - w0;
         const uint64_t a0 = (w0 + 3) & ~3ull;  // 16-byte aligned body
@@ -635,11 +638,15 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,
                         const uint16_t* sorted, const DecItem* items, uint16_t* codes, void* side,
-                        uint32_t* err)
+                        uint32_t* err, bool stage)
 {
     if (!ss.total_wgs) return;
-    k_huff_decode<<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items,
-                                                                   codes, (uint8_t*)side, err);
+    if (stage)
+        k_huff_decode<true><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
+                                                                     (uint8_t*)side, err);
+    else
+        k_huff_decode<false><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
+                                                                      (uint8_t*)side, err);
 }
 
 }  // namespace stz

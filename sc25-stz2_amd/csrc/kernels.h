@@ -75,13 +75,14 @@ void launch_huff_pack(hipStream_t s, const StreamSet& ss, const uint16_t* codes,
                       uint8_t* payload);
 void launch_zero_bits(hipStream_t s, const StreamSet& ss, uint8_t* payload);
 // chunk list: nullptr = every chunk of every stream at its stream position;
-// else (stream, chunk, item_base, out_pos) quads: the chunk's symbols go to
-// codes[out_pos ..] and cmap[cm_base + chunk] = out_pos
+// else (stream, chunk, first item relative to the stream's item_base,
+// out_pos) quads: the chunk's symbols go to codes[out_pos ..] and
+// cmap[cm_base + chunk] = out_pos
 void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                      const uint32_t* list, uint32_t nlist, DecItem* items, uint32_t* cmap);
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                         const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
-                        uint16_t* codes, void* side, uint32_t* err);
+                        uint16_t* codes, void* side, uint32_t* err, bool stage);
 
 // ---- k_field.hip ----
 template <typename T>
