@@ -13,6 +13,11 @@ events around each launch on the library's stream, in extra profiled steps
 after the timed ones.  cpu_baseline: the CPU oracle (C/OpenMP restatement of
 the reference) on the same field, compress + decompress, about 10 s of work.
 
+Also reported: "learned" (the same steps with the learned tuner choosing each
+compress's configuration), "tuner_c3" (learned tuner vs exhaustive trial on a
+256x384x384 fp64 turbulence field at rel eb 1e-6) and "serving" (random
+access on a resident 1024^3 stream under a Poisson request load; roi_p99_ms).
+
     python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--n 512] [--no-cpu]
 """
 import argparse
@@ -65,6 +70,7 @@ def main():
     ap.add_argument("--calib", action="store_true",
                     help="run a 1 GiB device-to-device copy first (known-size reference for HBM counters)")
     ap.add_argument("--no-serve", action="store_true", help="skip the random-access serving run")
+    ap.add_argument("--no-tuner", action="store_true", help="skip the learned-tuner runs")
     ap.add_argument("--serve-n", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
@@ -207,6 +213,10 @@ def main():
                             float(np.median(tds)))}
         log("cpu baseline", cpu)
 
+    learned = tuner_c3 = None
+    if not args.no_tuner:
+        learned, tuner_c3 = tuner_bench(args, ctx, stream, field, out, N, s, payload, dev)
+
     serving = None
     if not args.no_serve:
         serving = serve_bench(args, dev)
@@ -227,12 +237,87 @@ def main():
         "roofline": roof, "whole_step": whole,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
+        "learned": learned,
+        "tuner_c3": tuner_c3,
         "roi_p99_ms": serving["latency"]["box"]["p99_ms"] if serving else None,
         "serving": serving,
     }
     print(json.dumps(rec), flush=True)
     last.free()
     ctx.close()
+
+
+def tuner_bench(args, ctx, stream, field, out, N, s, payload_plain, dev):
+    """(1) The same comp+decomp steps with the learned tuner choosing the
+    configuration of every compress (its statistics + MLP run inside the
+    timed step).  (2) The tuner workload: 256x384x384 fp64 turbulence-like
+    field (GRF k^-11/3 + 8 tanh fronts of width 2, seed 1003), rel eb 1e-6:
+    learned choice vs the exhaustive trial over all 48 configurations --
+    tuning time and payload bytes -- and vs the untuned configuration."""
+    import numpy as np
+    import torch
+    from stz_amd import fields, tuner
+    params, meta = tuner.load_params()
+    tuner.set_model(ctx, params, min_confidence=0.0)
+
+    def step():
+        ds = ctx.compress_device(field, args.eb, "rel", tuner=1)
+        ds.decompress(out)
+        return ds
+
+    for _ in range(args.warmup):
+        step().free()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    last = None
+    for _ in range(args.steps):
+        if last is not None:
+            last.free()
+        last = step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    info = last.info
+    err = float((out.double() - field.double()).abs().max())
+    learned = {"value": round(2.0 * N * s / (ms * 1e-3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
+               "payload_bytes": int(last.payload_bytes), "cr_payload": round(N * s / last.payload_bytes, 3),
+               "cr_vs_plain": round(payload_plain / last.payload_bytes, 4),
+               "choice": {"interp": info.interp, "order": info.order, "mult": [info.mult_base, info.mult_low,
+                                                                                 info.mult_high],
+                          "alpha": info.alpha, "beta": info.beta},
+               "max_err": err, "eb_abs": info.eb_high, "error_bound_ok": bool(err <= info.eb_high * (1 + 1e-12) +
+                                                                              2 * float(np.spacing(np.float32(
+                                                                                  field.max().item()))))}
+    last.free()
+    log("learned", json.dumps(learned))
+
+    t0 = time.time()
+    f3 = fields.turbulence(shape=(256, 384, 384), seed=1003, device=dev)
+    f3t = torch.from_numpy(f3).to(dev)
+    torch.cuda.synchronize()
+    log("tuner field 256x384x384 fp64 generated in %.1fs" % (time.time() - t0))
+    tuner.choose(ctx, f3t, 1e-6)   # warm the pool
+    t0 = time.perf_counter()
+    k, conf = tuner.choose(ctx, f3t, 1e-6)
+    t_learned = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sizes = tuner.trial_sizes(ctx, f3t, 1e-6).astype(np.float64)
+    t_trial = time.perf_counter() - t0
+    nbytes = f3.nbytes
+    c3 = {"workload": "256x384x384 fp64 turbulence-like field (GRF k^-11/3 + 8 tanh fronts, seed 1003), rel eb 1e-6",
+          "configs": 48, "tuning_ms_learned": round(t_learned * 1e3, 3), "tuning_ms_trial": round(t_trial * 1e3, 1),
+          "tuning_speedup": round(t_trial / t_learned, 1), "choice": int(k), "trial_best": int(sizes.argmin()),
+          "confidence": round(conf, 4), "cr_learned": round(nbytes / sizes[k], 4),
+          "cr_trial": round(nbytes / sizes.min(), 4), "cr_untuned": round(nbytes / sizes[tuner.DEFAULT_CONFIG], 4),
+          "cr_learned_vs_trial": round(sizes.min() / sizes[k], 5),
+          "gain_vs_untuned": round(sizes[tuner.DEFAULT_CONFIG] / sizes[k] - 1.0, 4),
+          "model": {"holdout_mean_regret": meta.get("holdout_eval", {}).get("mean_regret"),
+                    "holdout_gain_vs_untuned": meta.get("holdout_eval", {}).get("mean_gain_vs_default")}}
+    log("tuner_c3", json.dumps(c3))
+    del f3t
+    return learned, c3
 
 
 def serve_bench(args, dev):

@@ -142,6 +142,37 @@ int stz_dstream_prepare(stz_ctx* ctx, stz_dstream* ds, int keep_base);
 /* outs[i]: host or device buffer of the request's size; out_dims: 3 per request */
 int stz_dstream_decompress_batch(stz_ctx* ctx, stz_dstream* ds, int n, const stz_request* reqs,
                                  void* const* outs, size_t* out_dims);
+/* ---- learned tuner -------------------------------------------------------
+ * Chooses the base interpolator, base axis order and level-eb scaling (the
+ * tuning targets of F5:2120-2129 and SZ3's sample-and-trial, F5:1439) among
+ * STZ_TUNE_CONFIGS configurations: 2 interpolators x 6 orders x 4 scalings.
+ * The choice lands in the stream header; decoding and the error bound are
+ * unchanged.  params.tuner = 1 compresses with the model's choice (falling
+ * back to the exhaustive trial below the model's confidence threshold),
+ * params.tuner = 2 with the trial's.  Tuning needs an ABS or REL bound. */
+#define STZ_TUNE_CONFIGS 48
+#define STZ_TUNE_FEATURES 16
+#define STZ_MLP_PARAMS 8368 /* 16x64 + 64 + 64x64 + 64 + 64x48 + 48 fp32 */
+int stz_tune_config(int index, stz_params* p);
+/* 16 statistics per 64^3 sub-block (x fastest block order) */
+int stz_tune_features(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,
+                      double eb_abs, float* feats, size_t cap_blocks, size_t* n_blocks);
+/* host fp32 parameters; confidence below min_confidence falls back to the trial */
+int stz_tune_set_model(stz_ctx* ctx, const float* params, size_t n, float min_confidence);
+/* the model's configuration: fills interp / order / mult_* / alpha / beta */
+int stz_tune(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
+             stz_eb_mode mode, stz_params* inout, float* confidence, int* config);
+/* exhaustive trial: payload bytes of every configuration (sizes may be NULL) */
+int stz_tune_trial(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,
+                   double eb, stz_eb_mode mode, const stz_params* base, uint64_t* sizes, int* best);
+/* MLP kernels (bf16 MFMA, fp32 accumulation; device pointers): logits of n
+ * feature rows; mean soft-target cross entropy and its gradient (overwrites
+ * grad and loss); one Adam step (beta 0.9 / 0.999, eps 1e-8) that clears grad */
+int stz_mlp_forward(stz_ctx* ctx, const float* params, const float* x, int n, float* logits);
+int stz_mlp_grad(stz_ctx* ctx, const float* params, const float* x, const float* target, int n,
+                 float* grad, float* loss);
+int stz_mlp_adam(stz_ctx* ctx, float* params, float* grad, float* m, float* v, int step, float lr,
+                 float weight_decay);
 /* device payload pointer and size (pre-zstd layout) */
 int stz_dstream_payload(const stz_dstream* ds, const void** d_payload, size_t* bytes);
 void stz_dstream_free(stz_dstream* ds);

@@ -185,4 +185,16 @@ template <typename T>
 void launch_interp_decode(hipStream_t s, const InterpPass& p, T* v, const uint16_t* codes,
                           const T* side);
 
+// ---- k_tuner.hip ----
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 48;
+constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid * kMlpHid + kMlpHid + kMlpHid * kMlpOut + kMlpOut;
+constexpr int kFeatSums = 18;   // per-sub-block sums of k_block_stats
+void launch_mlp_forward(hipStream_t s, const float* P, const float* X, uint32_t n, float* Y);
+void launch_mlp_grad(hipStream_t s, const float* P, const float* X, const float* T, uint32_t n, float* G,
+                     float* loss);
+void launch_adam(hipStream_t s, float* P, float* G, float* M, float* V, uint32_t n, float lr, float b1, float b2,
+                 float eps, int step, float wd);
+template <typename T>
+void launch_block_stats(hipStream_t s, const T* f, int nx, int ny, int nz, double eb, double* out);
+
 }  // namespace stz

@@ -152,6 +152,10 @@ struct stz_ctx {
 
     size_t stage_off = 0;
     std::vector<void*> retired;  // outgrown staging buffers, freed at the next call
+    // learned tuner model (fp32 MLP parameters on the device)
+    DBuf tuner_w;
+    bool have_tuner = false;
+    float tune_min_conf = 0.f;
 
     // Pinned staging for async copies: each request gets a region of its own
     // for the rest of the API call (a bump arena reset when a call starts).
@@ -1260,6 +1264,202 @@ stz_params defaults()
     return p;
 }
 
+// ---------------------------------------------------------------- tuner
+// The configurations the tuner chooses among: base interpolator (2) x base
+// axis order (6) x level-eb scaling (4).  Scalings: F5's multipliers
+// (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
+// every level; F5's multipliers with the base codec's level bounds shrinking
+// by 1.5 per level up to 4x (SZ3-style).  In ABS / REL modes the largest
+// multiplier maps to the requested bound, so no choice changes it.
+struct TuneScale { double mb, ml, mh, alpha, beta; };
+const TuneScale kScales[4] = {
+    {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
+constexpr int kTuneConfigs = 48;
+
+void tune_config(int k, stz_params& p)
+{
+    p.interp = k / 24;
+    p.order = (k / 4) % 6;
+    const TuneScale& t = kScales[k % 4];
+    p.mult_base = t.mb;
+    p.mult_low = t.ml;
+    p.mult_high = t.mh;
+    p.alpha = t.alpha;
+    p.beta = t.beta;
+}
+
+// 16 features of one 64^3 sub-block from its k_block_stats sums, relative to
+// the absolute bound; logs in units of 8 octaves
+void block_features(const double* S, double eb, float* f)
+{
+    auto lg = [](double v) { return (float)(std::log2(v + 1.0 / 1024.0) / 8.0); };
+    const double n = S[0] > 0 ? S[0] : 1.0;
+    const double mean = S[1] / n, var = std::max(S[2] / n - mean * mean, 0.0);
+    f[0] = lg(std::sqrt(var) / eb);
+    f[1] = lg((S[5] - S[4]) / eb);
+    for (int a = 0; a < 3; ++a) {
+        f[2 + a] = lg(S[6 + a] / n / eb);
+        f[5 + a] = lg(S[9 + a] / n / eb);
+        f[8 + a] = lg(S[12 + a] / n / eb);
+    }
+    f[11] = (float)(S[0] > 0 ? S[15] / n : 0.0);
+    f[12] = (float)(S[0] > 0 ? S[16] / n : 0.0);
+    f[13] = lg(S[17] / n / eb);
+    const double m2 = S[2] / n, m4 = S[3] / n;
+    f[14] = (float)(std::log2(m4 / (m2 * m2 + 1e-300) + 1e-30) / 8.0);
+    const double lmax = std::max(S[6], std::max(S[7], S[8])), lmin = std::min(S[6], std::min(S[7], S[8]));
+    f[15] = (float)(std::log2((lmax + 1e-300) / (lmin + 1e-300)) / 8.0);
+    if (S[0] <= 0)
+        for (int k = 2; k < 16; ++k) f[k] = 0.f;
+}
+
+// device view of a field (copied when it is host memory)
+struct FieldView {
+    DBuf copy;
+    const void* p;
+    FieldView(stz_ctx* c, const void* f, size_t bytes)
+    {
+        if (is_device_ptr(f)) {
+            p = f;
+        } else {
+            copy = DBuf(c->pool, bytes);
+            HIPCHK(hipMemcpyAsync(copy.p, f, bytes, hipMemcpyHostToDevice, c->stream));
+            p = copy.p;
+        }
+    }
+};
+
+template <typename T>
+double abs_bound(stz_ctx* c, const T* d, size_t n, double eb, int mode)
+{
+    if (mode != STZ_EB_REL) return eb;
+    DBuf part(c->pool, 2 * 2048 * sizeof(double) + 64);
+    double* outp = part.as<double>() + 2 * 2048;
+    K(c, "minmax", [&] { launch_minmax<T>(c->stream, d, n, part.as<double>(), outp); });
+    double* h = (double*)c->host_stage(64);
+    HIPCHK(hipMemcpyAsync(h, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return eb * (h[1] - h[0]);
+}
+
+template <typename T>
+std::vector<float> tune_features(stz_ctx* c, const T* d, size_t nx, size_t ny, size_t nz, double eb_abs,
+                                 std::vector<double>* counts = nullptr)
+{
+    const size_t nb = ((nx + 63) / 64) * ((ny + 63) / 64) * ((nz + 63) / 64);
+    DBuf sums(c->pool, nb * kFeatSums * sizeof(double));
+    K(c, "block_stats", [&] { launch_block_stats<T>(c->stream, d, (int)nx, (int)ny, (int)nz, eb_abs, sums.as<double>()); });
+    double* h = (double*)c->host_stage(nb * kFeatSums * sizeof(double));
+    HIPCHK(hipMemcpyAsync(h, sums.p, nb * kFeatSums * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<float> f(nb * kMlpIn);
+    const double e = eb_abs > 0 ? eb_abs : 1e-300;
+    for (size_t b = 0; b < nb; ++b) block_features(h + b * kFeatSums, e, &f[b * kMlpIn]);
+    if (counts) {
+        counts->resize(nb);
+        for (size_t b = 0; b < nb; ++b) (*counts)[b] = h[b * kFeatSums];
+    }
+    return f;
+}
+
+// MLP over the sub-blocks; the field's choice maximises the mean log
+// probability over blocks, weighted by each block's sampled points (blocks
+// too thin for the stencils carry no vote); confidence = the weighted mean
+// probability of that choice.
+template <typename T>
+int tune_learned(stz_ctx* c, const T* d, size_t nx, size_t ny, size_t nz, double eb, int mode, float& conf)
+{
+    if (!c->have_tuner) throw StzError(STZ_E_ARG, "no tuner model set (stz_tune_set_model)");
+    const size_t n = nx * ny * nz;
+    const double eb_abs = abs_bound<T>(c, d, n, eb, mode);
+    std::vector<double> cnt;
+    const std::vector<float> f = tune_features<T>(c, d, nx, ny, nz, eb_abs, &cnt);
+    const size_t nb = f.size() / kMlpIn;
+    double wsum = 0;
+    for (double v : cnt) wsum += v;
+    if (wsum <= 0) {
+        for (double& v : cnt) v = 1.0;
+        wsum = (double)nb;
+    }
+    DBuf x(c->pool, f.size() * 4), y(c->pool, nb * kMlpOut * 4);
+    float* hs = (float*)c->host_stage(f.size() * 4 + nb * kMlpOut * 4);
+    std::memcpy(hs, f.data(), f.size() * 4);
+    HIPCHK(hipMemcpyAsync(x.p, hs, f.size() * 4, hipMemcpyHostToDevice, c->stream));
+    K(c, "mlp_forward", [&] { launch_mlp_forward(c->stream, c->tuner_w.as<float>(), x.as<float>(), (uint32_t)nb, y.as<float>()); });
+    float* hy = hs + f.size();
+    HIPCHK(hipMemcpyAsync(hy, y.p, nb * kMlpOut * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> lp(kMlpOut, 0.0), pr(kMlpOut, 0.0);
+    for (size_t b = 0; b < nb; ++b) {
+        const float* r = hy + b * kMlpOut;
+        double mx = r[0];
+        for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
+        double se = 0;
+        for (int k = 0; k < kMlpOut; ++k) se += std::exp(r[k] - mx);
+        const double lse = mx + std::log(se);
+        const double w = cnt[b] / wsum;
+        for (int k = 0; k < kMlpOut; ++k) {
+            lp[k] += w * (r[k] - lse);
+            pr[k] += w * std::exp(r[k] - lse);
+        }
+    }
+    int best = 0;
+    for (int k = 1; k < kMlpOut; ++k)
+        if (lp[k] > lp[best]) best = k;
+    conf = (float)pr[best];
+    return best;
+}
+
+// exhaustive trial: every configuration on the whole field; the smallest
+// payload wins (kept in *keep when given); sizes[k] = payload bytes of k
+template <typename T>
+int tune_trial(stz_ctx* c, int dtype, const T* d, size_t nx, size_t ny, size_t nz, double eb, int mode,
+               const stz_params& base, stz_dstream* keep, uint64_t* sizes)
+{
+    int best = -1;
+    uint64_t bb = ~0ull;
+    for (int k = 0; k < kTuneConfigs; ++k) {
+        stz_params p = base;
+        tune_config(k, p);
+        stz_dstream tmp;
+        compress_impl<T>(c, dtype, d, nx, ny, nz, eb, mode, p, &tmp);
+        if (sizes) sizes[k] = tmp.bytes;
+        if (tmp.bytes < bb) {
+            bb = tmp.bytes;
+            best = k;
+            if (keep) std::swap(*keep, tmp);
+        }
+    }
+    return best;
+}
+
+// compress with the parameters the tuner picks (params.tuner 1: learned,
+// falling back to the trial below the context's confidence threshold; 2: trial)
+template <typename T>
+void compress_tuned(stz_ctx* c, int dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
+                    stz_params p, stz_dstream* ds)
+{
+    if (p.tuner == 0) {
+        compress_impl<T>(c, dtype, (const T*)field, nx, ny, nz, eb, mode, p, ds);
+        return;
+    }
+    if (mode == STZ_EB_REF || p.tuner < 0 || p.tuner > 2)
+        throw StzError(STZ_E_ARG, "tuning needs an ABS or REL bound");
+    check_dims(nx, ny, nz, p.chunk_high, p.chunk_low);
+    FieldView fv(c, field, nx * ny * nz * sizeof(T));
+    const T* d = (const T*)fv.p;
+    if (p.tuner == 1) {
+        float conf = 0.f;
+        const int k = tune_learned<T>(c, d, nx, ny, nz, eb, mode, conf);
+        if (conf >= c->tune_min_conf) {
+            tune_config(k, p);
+            compress_impl<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds);
+            return;
+        }
+    }
+    tune_trial<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds, nullptr);
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1390,9 +1590,117 @@ int stz_compress_device(stz_ctx* c, stz_dtype dtype, const void* field, size_t n
     const stz_params p = params ? *params : defaults();
     return guard(c, [&] {
         auto ds = std::unique_ptr<stz_dstream>(new stz_dstream());
-        if (dtype == STZ_F32) compress_impl<float>(c, dtype, (const float*)field, nx, ny, nz, eb, mode, p, ds.get());
-        else compress_impl<double>(c, dtype, (const double*)field, nx, ny, nz, eb, mode, p, ds.get());
+        if (dtype == STZ_F32) compress_tuned<float>(c, dtype, field, nx, ny, nz, eb, mode, p, ds.get());
+        else compress_tuned<double>(c, dtype, field, nx, ny, nz, eb, mode, p, ds.get());
         *out = ds.release();
+    });
+}
+
+// ---- learned tuner ------------------------------------------------------
+int stz_tune_config(int index, stz_params* p)
+{
+    if (!p || index < 0 || index >= kTuneConfigs) return STZ_E_ARG;
+    tune_config(index, *p);
+    return STZ_OK;
+}
+
+int stz_tune_features(stz_ctx* c, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,
+                      double eb_abs, float* feats, size_t cap_blocks, size_t* n_blocks)
+{
+    if (!c || !field || !feats || !n_blocks || !(eb_abs > 0) || (dtype != STZ_F32 && dtype != STZ_F64))
+        return STZ_E_ARG;
+    return guard(c, [&] {
+        if (!nx || !ny || !nz) throw StzError(STZ_E_DIMS, "empty field");
+        const size_t ts = dtype == STZ_F32 ? 4 : 8;
+        FieldView fv(c, field, nx * ny * nz * ts);
+        const std::vector<float> f = dtype == STZ_F32
+                                         ? tune_features<float>(c, (const float*)fv.p, nx, ny, nz, eb_abs)
+                                         : tune_features<double>(c, (const double*)fv.p, nx, ny, nz, eb_abs);
+        *n_blocks = f.size() / kMlpIn;
+        if (*n_blocks > cap_blocks) throw StzError(STZ_E_ARG, "feature buffer too small");
+        std::memcpy(feats, f.data(), f.size() * 4);
+    });
+}
+
+int stz_tune_set_model(stz_ctx* c, const float* params, size_t n, float min_confidence)
+{
+    if (!c || !params || n != (size_t)kMlpParams) return STZ_E_ARG;
+    return guard(c, [&] {
+        c->tuner_w = DBuf(c->pool, n * 4);
+        float* h = (float*)c->host_stage(n * 4);
+        std::memcpy(h, params, n * 4);
+        HIPCHK(hipMemcpyAsync(c->tuner_w.p, h, n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_tuner = true;
+        c->tune_min_conf = min_confidence;
+    });
+}
+
+int stz_tune(stz_ctx* c, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
+             stz_eb_mode mode, stz_params* inout, float* confidence, int* config)
+{
+    if (!c || !field || !inout || (dtype != STZ_F32 && dtype != STZ_F64) || mode == STZ_EB_REF) return STZ_E_ARG;
+    return guard(c, [&] {
+        const size_t ts = dtype == STZ_F32 ? 4 : 8;
+        FieldView fv(c, field, nx * ny * nz * ts);
+        float conf = 0.f;
+        const int k = dtype == STZ_F32 ? tune_learned<float>(c, (const float*)fv.p, nx, ny, nz, eb, mode, conf)
+                                       : tune_learned<double>(c, (const double*)fv.p, nx, ny, nz, eb, mode, conf);
+        tune_config(k, *inout);
+        if (confidence) *confidence = conf;
+        if (config) *config = k;
+    });
+}
+
+int stz_tune_trial(stz_ctx* c, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
+                   stz_eb_mode mode, const stz_params* base, uint64_t* sizes, int* best)
+{
+    if (!c || !field || (dtype != STZ_F32 && dtype != STZ_F64) || mode == STZ_EB_REF) return STZ_E_ARG;
+    const stz_params p = base ? *base : defaults();
+    return guard(c, [&] {
+        check_dims(nx, ny, nz, p.chunk_high, p.chunk_low);
+        const size_t ts = dtype == STZ_F32 ? 4 : 8;
+        FieldView fv(c, field, nx * ny * nz * ts);
+        const int k = dtype == STZ_F32
+                          ? tune_trial<float>(c, dtype, (const float*)fv.p, nx, ny, nz, eb, mode, p, nullptr, sizes)
+                          : tune_trial<double>(c, dtype, (const double*)fv.p, nx, ny, nz, eb, mode, p, nullptr, sizes);
+        if (best) *best = k;
+    });
+}
+
+int stz_mlp_forward(stz_ctx* c, const float* params, const float* x, int n, float* logits)
+{
+    if (!c || !params || !x || !logits || n < 0) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (!is_device_ptr(params) || !is_device_ptr(x) || !is_device_ptr(logits))
+            throw StzError(STZ_E_ARG, "device pointers expected");
+        K(c, "mlp_forward", [&] { launch_mlp_forward(c->stream, params, x, (uint32_t)n, logits); });
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stz_mlp_grad(stz_ctx* c, const float* params, const float* x, const float* target, int n, float* grad,
+                 float* loss)
+{
+    if (!c || !params || !x || !target || !grad || !loss || n <= 0) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (!is_device_ptr(params) || !is_device_ptr(x) || !is_device_ptr(target) || !is_device_ptr(grad) ||
+            !is_device_ptr(loss))
+            throw StzError(STZ_E_ARG, "device pointers expected");
+        K(c, "grad_zero", [&] {
+            launch_fill_u32(c->stream, (uint32_t*)grad, kMlpParams, 0u);
+            launch_fill_u32(c->stream, (uint32_t*)loss, 1, 0u);
+        });
+        K(c, "mlp_grad", [&] { launch_mlp_grad(c->stream, params, x, target, (uint32_t)n, grad, loss); });
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stz_mlp_adam(stz_ctx* c, float* params, float* grad, float* m, float* v, int step, float lr, float weight_decay)
+{
+    if (!c || !params || !grad || !m || !v || step < 1) return STZ_E_ARG;
+    return guard(c, [&] {
+        K(c, "adam", [&] { launch_adam(c->stream, params, grad, m, v, kMlpParams, lr, 0.9f, 0.999f, 1e-8f, step, weight_decay); });
     });
 }
 

@@ -24,7 +24,10 @@ EXPORTS = [
     "stz_dstream_payload", "stz_dstream_free", "stz_dstream_prepare", "stz_dstream_decompress_batch",
     "stz_split", "stz_merge", "stz_residual_predict", "stz_residual_reconstruct", "stz_quantize",
     "stz_dequantize", "stz_huff_lengths", "stz_plan_query",
+    "stz_tune_config", "stz_tune_features", "stz_tune_set_model", "stz_tune", "stz_tune_trial",
+    "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
 ]
+TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS = 48, 16, 8368
 
 
 class StzError(RuntimeError):
@@ -120,6 +123,15 @@ def lib():
     L.stz_dequantize.argtypes = [vp, i32, vp, sz, dbl, vp, vp]
     L.stz_huff_lengths.argtypes = [vp, i32, i32, vp]
     L.stz_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
+    f32 = C.c_float
+    L.stz_tune_config.argtypes = [i32, P(Params)]
+    L.stz_tune_features.argtypes = [vp, i32, vp, sz, sz, sz, dbl, vp, sz, P(sz)]
+    L.stz_tune_set_model.argtypes = [vp, vp, sz, f32]
+    L.stz_tune.argtypes = [vp, i32, vp, sz, sz, sz, dbl, i32, P(Params), P(f32), P(i32)]
+    L.stz_tune_trial.argtypes = [vp, i32, vp, sz, sz, sz, dbl, i32, P(Params), vp, P(i32)]
+    L.stz_mlp_forward.argtypes = [vp, vp, vp, i32, vp]
+    L.stz_mlp_grad.argtypes = [vp, vp, vp, vp, i32, vp, vp]
+    L.stz_mlp_adam.argtypes = [vp, vp, vp, vp, vp, i32, f32, f32]
     _lib = L
     return L
 

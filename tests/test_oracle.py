@@ -489,3 +489,39 @@ def test_oracle_abs_mode_and_params(oracle):
     p = oracle.compress(f, 1e-4, oracle.EB_ABS, interp=0, order=4, mult_low=2.0, chunk_high=8, chunk_low=4)
     r = oracle.decompress(p)
     assert np.abs(r - f).max() <= 1e-4 * (1 + 1e-12)
+
+
+# ---------------------------------------------------------------- tuner MLP reference
+def test_mlp_reference_gradient_by_finite_differences():
+    """The float64 reference of the tuner's MLP, whose bf16-rounded variant
+    checks the device kernels: its analytic gradient against central
+    differences of its own loss."""
+    import mlp_ref
+    rng = np.random.default_rng(0)
+    P = rng.normal(0, 0.3, size=mlp_ref.N_PARAMS)
+    X = rng.normal(size=(24, 16))
+    T = rng.dirichlet(np.ones(48), size=24)
+    _, g = mlp_ref.loss_grad(P, X, T, bf=False)
+    eps = 1e-6
+    blocks = np.cumsum([0, 16 * 64, 64, 64 * 64, 64, 64 * 48, 48])
+    for lo, hi in zip(blocks[:-1], blocks[1:]):     # a few parameters of every block
+        for i in rng.choice(np.arange(lo, hi), 6, replace=False):
+            Pp, Pm = P.copy(), P.copy()
+            Pp[i] += eps
+            Pm[i] -= eps
+            fd = (mlp_ref.loss_grad(Pp, X, T, False)[0] - mlp_ref.loss_grad(Pm, X, T, False)[0]) / (2 * eps)
+            assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_bf16_rounding_reference():
+    import mlp_ref
+    one = 1.0
+    ulp = 2.0 ** -7
+    cases = {one: one, one + ulp / 2: one, one + 3 * ulp / 2: one + 2 * ulp, one + ulp / 2 + 2 ** -20: one + ulp,
+             -2.5: -2.5, 3.0e-39: 3.0e-39 - (3.0e-39 % 2 ** -133) if False else None}
+    for x, want in cases.items():
+        if want is None:
+            continue
+        assert mlp_ref.bf16(np.float32(x)) == want, x
+    v = np.float32(np.pi)
+    assert mlp_ref.bf16(v) == 3.140625          # 0x4049 (pi rounds down)
