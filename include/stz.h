@@ -142,6 +142,23 @@ int stz_dstream_prepare(stz_ctx* ctx, stz_dstream* ds, int keep_base);
 /* outs[i]: host or device buffer of the request's size; out_dims: 3 per request */
 int stz_dstream_decompress_batch(stz_ctx* ctx, stz_dstream* ds, int n, const stz_request* reqs,
                                  void* const* outs, size_t* out_dims);
+/* ---- many fields in one call ---------------------------------------------
+ * Uneven fields of one dtype.  The residual levels of all fields run as one
+ * persistent launch per level fed by a device work queue; its launch
+ * geometry (waves per workgroup, workgroups per CU) is timed per dtype,
+ * level and size class the first time and kept in the context.  Streams and
+ * decoded fields are bit-identical to one call per field.  The batch decode
+ * is captured as a HIP graph and replayed while a call names the same
+ * streams and outputs (device memory).  Replaces per-field runs of the
+ * compress / decompress halves of run_typed (F5:2102-2451). */
+int stz_compress_batch(stz_ctx* ctx, stz_dtype dtype, int n, const void* const* fields,
+                       const size_t* dims /* 3 per field: nx ny nz */, double eb, stz_eb_mode mode,
+                       const stz_params* params, stz_dstream** out);
+int stz_decompress_batch(stz_ctx* ctx, int n, stz_dstream* const* ds, void* const* outs);
+/* enable (1) / disable (0) the geometry timing (-1: leave); JSON of the
+ * choices and their trial times into report when given */
+int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
+
 /* ---- learned tuner -------------------------------------------------------
  * Chooses the base interpolator, base axis order and level-eb scaling (the
  * tuning targets of F5:2120-2129 and SZ3's sample-and-trial, F5:1439) among

@@ -574,6 +574,101 @@ __global__ __launch_bounds__(256) void k_high_decode_boxes(const T* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- many fields
+// Persistent multi-field kernel: each workgroup takes work items from the
+// queue head (one atomic per item) until the queue is empty.  Items are
+// handed out in increasing order, so a workgroup meets each field once;
+// it flushes its LDS histograms when the field changes.
+__device__ __forceinline__ uint32_t find_level_job(const LevelJob* __restrict__ jobs, uint32_t njobs, uint32_t it)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].item0 <= it) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+template <typename T, int kMode, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                             uint32_t total, uint32_t* __restrict__ counter)
+{
+    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr int kThreads = 64 * kWaves;
+    __shared__ uint32_t sh[kEnc ? 7 : 1][kHW];
+    __shared__ uint32_t item_s;
+    if constexpr (kEnc) {
+        for (int k = threadIdx.x; k < 7 * kHW; k += kThreads) sh[k / kHW][k % kHW] = 0u;
+    }
+    int cur = -1;
+    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t it = item_s;
+        __syncthreads();
+        if (it >= total) break;  // every wave of the workgroup sees the same item
+        const int j = (int)find_level_job(jobs, njobs, it);
+        const LevelJob& J = jobs[j];
+        if constexpr (kEnc) {
+            if (j != cur && cur >= 0) {
+                hist_flush(sh, 7, jobs[cur].hist);
+                __syncthreads();
+                for (int k = threadIdx.x; k < 7 * kHW; k += kThreads) sh[k / kHW][k % kHW] = 0u;
+            }
+            if (j != cur) __syncthreads();
+            cur = j;
+        }
+        const Geom g = J.g;
+        const uint32_t local = it - J.item0;
+        int x, y, z;
+        bool ok, inner, vec;
+        if (local < J.ntiles) {
+            const int nxi = g.dx - 3;
+            const int ntx = (nxi + 63) >> 6, nty = (g.dy + kWaves - 1) / kWaves;
+            const int tx = (int)(local % ntx), ty = (int)((local / ntx) % nty), tz = (int)(local / ((uint32_t)ntx * nty));
+            x = 1 + tx * 64 + lane;
+            y = ty * kWaves + row;
+            z = tz;
+            ok = x <= g.dx - 3 && y < g.dy;
+            inner = row_inner(y, z, g.dy, g.dz);
+            vec = true;
+        } else {
+            const EdgeMap e = edge_map(g.dx);
+            const uint64_t k = (uint64_t)(local - J.ntiles) * kThreads + threadIdx.x;
+            ok = k < (uint64_t)e.ncol * g.dy * g.dz;
+            const int c = (int)(k % e.ncol);
+            y = (int)((k / e.ncol) % g.dy);
+            z = (int)(k / ((uint64_t)e.ncol * g.dy));
+            x = e.cols[c];
+            inner = false;
+            vec = false;
+        }
+        if constexpr (kEnc) {
+            uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (ok) {
+                if (kMode == kHighEnc)
+                    high_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side, x, y,
+                                   z, inner, q);
+                else
+                    low_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side,
+                                  (T*)J.mid, x, y, z, inner, vec, q);
+            }
+            uint32_t* hist = J.hist;
+            STZ_HIST_ADD7(q, ok);
+        } else if (ok) {
+            if (kMode == kHighDec)
+                high_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.out, J.nx, J.ny, J.nz, x, y,
+                               z, inner, vec && J.vec != 0);
+            else
+                low_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.mid, x, y, z, inner, vec);
+        }
+    }
+    if constexpr (kEnc) {
+        if (cur >= 0) hist_flush(sh, 7, jobs[cur].hist);
+    }
+}
+
 inline unsigned tile_grid(int dx, int dy, int dz)
 {
     const int nxi = dx - 3 > 0 ? dx - 3 : 1;
@@ -656,6 +751,41 @@ void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, 
     k_high_decode_boxes<T><<<dim3(nblocks), dim3(256), 0, s>>>(mid, g, a, codes, side, jobs, njobs, nblocks);
 }
 
+uint32_t level_tiles(const Geom& g, int waves)
+{
+    const int nxi = g.dx - 3;
+    if (nxi <= 0) return 0;
+    return (uint32_t)(((nxi + 63) / 64) * ((g.dy + waves - 1) / waves) * g.dz);
+}
+
+uint32_t level_edge_groups(const Geom& g, int waves)
+{
+    const uint64_t n = edge_points(g);
+    return (uint32_t)((n + 64u * waves - 1) / (64u * waves));
+}
+
+template <typename T, int kW>
+void level_multi_w(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total, uint32_t* counter,
+                   uint32_t grid)
+{
+    const dim3 G(grid), B(64 * kW);
+    switch (mode) {
+    case kLowEnc: k_level_multi<T, kLowEnc, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    case kHighEnc: k_level_multi<T, kHighEnc, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    case kLowDec: k_level_multi<T, kLowDec, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    default: k_level_multi<T, kHighDec, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    }
+}
+
+template <typename T>
+void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total_items,
+                        uint32_t* counter, int waves, uint32_t grid)
+{
+    if (!njobs || !total_items || !grid) return;
+    if (waves == 8) level_multi_w<T, 8>(s, mode, jobs, njobs, total_items, counter, grid);
+    else level_multi_w<T, 4>(s, mode, jobs, njobs, total_items, counter, grid);
+}
+
 #define STZ_INST(T)                                                                                              \
     template void launch_low_encode<T>(hipStream_t, const T*, int, int, int, const T*, Geom, LevelArgs,         \
                                        uint16_t*, T*, uint32_t*, T*);                                            \
@@ -667,7 +797,9 @@ void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, 
     template void launch_low_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, \
                                              T*, int, int, const BoxJob*, uint32_t, uint32_t);                  \
     template void launch_high_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*,         \
-                                              const T*, const BoxJob*, uint32_t, uint32_t);
+                                              const T*, const BoxJob*, uint32_t, uint32_t);                      \
+    template void launch_level_multi<T>(hipStream_t, int, const LevelJob*, uint32_t, uint32_t, uint32_t*, int,  \
+                                        uint32_t);
 STZ_INST(float)
 STZ_INST(double)
 

@@ -150,6 +150,33 @@ template <typename T>
 void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
                               const T* side, const BoxJob* jobs, uint32_t njobs, uint32_t nblocks);
 
+// One field's level in a multi-field persistent launch: work items
+// item0 .. item0 + ntiles - 1 are its main 64 x waves tiles, then nedge
+// groups of one edge point per thread.
+enum LevelMode { kLowEnc = 0, kHighEnc = 1, kLowDec = 2, kHighDec = 3 };
+struct LevelJob {
+    const void* full;     // encode: padded field
+    const void* ref;      // base grid (low) or mid grid (high)
+    void* mid;            // low level: mid grid written
+    void* out;            // high decode: the caller's field
+    uint16_t* codes;
+    void* side;
+    uint32_t* hist;       // encode: this field's 7 level histograms
+    int px, py;           // padded dims of `full`
+    int nx, ny, nz;       // high decode: output dims
+    int vec;              // high decode: 2-wide output stores allowed
+    Geom g;
+    LevelArgs a;
+    uint32_t item0, ntiles, nedge, pad_;
+};
+// all jobs of one mode; counter: a zeroed device word (the work queue head)
+template <typename T>
+void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total_items,
+                        uint32_t* counter, int waves, uint32_t grid);
+// work items of one field's level for a given number of waves per block
+uint32_t level_tiles(const Geom& g, int waves);
+uint32_t level_edge_groups(const Geom& g, int waves);
+
 // ---- k_interp.hip ----
 struct InterpPass {
     int start[3], step[3], cnt[3];

@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <atomic>
 #include <unordered_map>
 #include <vector>
 
@@ -134,6 +135,7 @@ struct DBuf {
 };
 
 struct ProfRec { const char* name; hipEvent_t a, b; };
+struct BatchPlan;
 
 }  // namespace
 
@@ -152,6 +154,14 @@ struct stz_ctx {
 
     size_t stage_off = 0;
     std::vector<void*> retired;  // outgrown staging buffers, freed at the next call
+    // launch geometry of the multi-field level kernels, per (dtype, mode,
+    // size class), chosen by timing the candidates once
+    struct LaunchCfg { int waves = 4; uint32_t grid = 0; float ms = 0.f; std::string trials; };
+    std::map<std::string, LaunchCfg> tuned;
+    bool autotune = true;
+    int n_cu = 0;
+    // captured batch decode (graph + the buffers it reads and writes)
+    std::unique_ptr<BatchPlan> plan;
     // learned tuner model (fp32 MLP parameters on the device)
     DBuf tuner_w;
     bool have_tuner = false;
@@ -205,6 +215,13 @@ struct stz_dstream {
     // decoded base level kept for random access (stz_dstream_prepare)
     DBuf base;
     bool have_base = false;
+    // identity for cached launch plans (addresses can be reused)
+    uint64_t gen = next_gen();
+    static uint64_t next_gen()
+    {
+        static std::atomic<uint64_t> g{1};
+        return g.fetch_add(1);
+    }
 };
 
 namespace {
@@ -428,131 +445,194 @@ void run_passes(const std::vector<InterpPass>& passes, size_t tsize, FF&& fused,
 }
 
 // ------------------------------------------------------------ compress
+// One field's compress state through the phases.  A single compress runs
+// the phases back to back; a batch runs each phase over every field, with
+// one host synchronisation between phases and the two residual levels of all
+// fields in one persistent launch each.  Every phase enqueues on the
+// context's stream without waiting, except where marked.
 template <typename T>
-void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
-                   const stz_params& prm, stz_dstream* ds)
+struct FieldEnc {
+    int dtype = 0, mode = 0;
+    double eb = 0;
+    stz_params prm{};
+    size_t nx = 0, ny = 0, nz = 0, px = 0, py = 0, pz = 0;
+    uint64_t nin = 0;
+    HostLayout L;
+    DBuf in_copy, padded, mm_part;
+    const T* din = nullptr;
+    const T* full = nullptr;
+    double* mm_host = nullptr;  // pinned: min, max
+    double ebl[3] = {0, 0, 0}, unit = 0, vmin = 0, vmax = 0;
+    DBuf codes, side, hist, vbase, mid, compact, counts;
+    uint8_t* hs = nullptr;      // pinned: stream symbol counts + first entries
+    std::vector<std::vector<uint64_t>> lists;
+    size_t enc_n = 0;
+    stz_dstream* ds = nullptr;
+    static constexpr uint32_t kFirst = 4096;
+};
+
+// checks, device input, padding; the value range is requested (no wait)
+template <typename T>
+void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
 {
-    check_dims(nx, ny, nz, prm.chunk_high, prm.chunk_low);
-    if (!(eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
+    const stz_params& prm = fe.prm;
+    check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
+    if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
         prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0))
         throw StzError(STZ_E_ARG, "bad parameters");
     hipStream_t st = c->stream;
     auto pool = c->pool;
-    const size_t px = pad4(nx), py = pad4(ny), pz = pad4(nz);
-    const uint64_t nin = (uint64_t)nx * ny * nz;
-    HostLayout L;
-    make_layout(L, nx, ny, nz, px, py, pz, prm.chunk_high, prm.chunk_low);
-    const int lx = (int)(px / 4), ly = (int)(py / 4), lz = (int)(pz / 4);
-    const int mx = (int)(px / 2), my = (int)(py / 2), mz = (int)(pz / 2);
-
+    fe.px = pad4(fe.nx);
+    fe.py = pad4(fe.ny);
+    fe.pz = pad4(fe.nz);
+    fe.nin = (uint64_t)fe.nx * fe.ny * fe.nz;
+    make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low);
     // input on device, 2-element aligned, padded to multiples of 4
-    DBuf in_copy, padded;
-    const T* din = field;
+    fe.din = field;
+    const bool padded = fe.px != fe.nx || fe.py != fe.ny || fe.pz != fe.nz;
     if (!is_device_ptr(field)) {
-        in_copy = DBuf(pool, nin * sizeof(T));
-        HIPCHK(hipMemcpyAsync(in_copy.p, field, nin * sizeof(T), hipMemcpyHostToDevice, st));
-        din = in_copy.as<T>();
-    } else if (((uintptr_t)field % (2 * sizeof(T))) != 0 && px == nx && py == ny && pz == nz) {
-        in_copy = DBuf(pool, nin * sizeof(T));
-        HIPCHK(hipMemcpyAsync(in_copy.p, field, nin * sizeof(T), hipMemcpyDeviceToDevice, st));
-        din = in_copy.as<T>();
+        fe.in_copy = DBuf(pool, fe.nin * sizeof(T));
+        HIPCHK(hipMemcpyAsync(fe.in_copy.p, field, fe.nin * sizeof(T), hipMemcpyHostToDevice, st));
+        fe.din = fe.in_copy.template as<T>();
+    } else if (((uintptr_t)field % (2 * sizeof(T))) != 0 && !padded) {
+        fe.in_copy = DBuf(pool, fe.nin * sizeof(T));
+        HIPCHK(hipMemcpyAsync(fe.in_copy.p, field, fe.nin * sizeof(T), hipMemcpyDeviceToDevice, st));
+        fe.din = fe.in_copy.template as<T>();
     }
-    const T* full = din;
-    if (px != nx || py != ny || pz != nz) {
-        padded = DBuf(pool, px * py * pz * sizeof(T));
-        K(c, "pad", [&] { launch_pad<T>(st, din, (int)nx, (int)ny, (int)nz, padded.as<T>(), (int)px, (int)py, (int)pz); });
-        full = padded.as<T>();
+    fe.full = fe.din;
+    if (padded) {
+        fe.padded = DBuf(pool, fe.px * fe.py * fe.pz * sizeof(T));
+        K(c, "pad", [&] {
+            launch_pad<T>(st, fe.din, (int)fe.nx, (int)fe.ny, (int)fe.nz, fe.padded.template as<T>(), (int)fe.px, (int)fe.py,
+                          (int)fe.pz);
+        });
+        fe.full = fe.padded.template as<T>();
     }
-
     // value range (computeRange, F5:92-104) -> level error bounds
-    double vmin = 0, vmax = 0;
-    {
-        DBuf part(pool, 2 * 2048 * sizeof(double) + 64);
-        double* outp = part.as<double>() + 2 * 2048;
-        K(c, "minmax", [&] { launch_minmax<T>(st, din, nin, part.as<double>(), outp); });
-        double* h = (double*)c->host_stage(64);
-        HIPCHK(hipMemcpyAsync(h, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        vmin = h[0];
-        vmax = h[1];
-    }
-    double ebl[3], unit;
-    level_ebs(eb, mode, vmin, vmax, prm, ebl, unit);
+    fe.mm_part = DBuf(pool, 2 * 2048 * sizeof(double) + 64);
+    double* outp = fe.mm_part.template as<double>() + 2 * 2048;
+    K(c, "minmax", [&] { launch_minmax<T>(st, fe.din, fe.nin, fe.mm_part.template as<double>(), outp); });
+    fe.mm_host = (double*)c->host_stage(64);
+    HIPCHK(hipMemcpyAsync(fe.mm_host, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+}
 
-    // scratch
-    DBuf codes(pool, L.total_codes * 2), side(pool, L.total_codes * sizeof(T));
-    DBuf hist(pool, (size_t)kStreams * 65536 * 4);
-    DBuf vbase(pool, (size_t)lx * ly * lz * sizeof(T)), mid(pool, (size_t)mx * my * mz * sizeof(T));
-    K(c, "hist_zero", [&] { launch_fill_u32(st, hist.as<uint32_t>(), (uint64_t)kStreams * 65536, 0u); });
-
+// after a wait: level bounds, base level; the residual levels when `levels`
+template <typename T>
+void enc_phase1(stz_ctx* c, FieldEnc<T>& fe, bool levels)
+{
+    hipStream_t st = c->stream;
+    auto pool = c->pool;
+    const HostLayout& L = fe.L;
+    fe.vmin = fe.mm_host[0];
+    fe.vmax = fe.mm_host[1];
+    level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.prm, fe.ebl, fe.unit);
+    const int lx = (int)(fe.px / 4), ly = (int)(fe.py / 4), lz = (int)(fe.pz / 4);
+    const int mx = (int)(fe.px / 2), my = (int)(fe.py / 2), mz = (int)(fe.pz / 2);
+    fe.codes = DBuf(pool, L.total_codes * 2);
+    fe.side = DBuf(pool, L.total_codes * sizeof(T));
+    fe.hist = DBuf(pool, (size_t)kStreams * 65536 * 4);
+    fe.vbase = DBuf(pool, (size_t)lx * ly * lz * sizeof(T));
+    fe.mid = DBuf(pool, (size_t)mx * my * mz * sizeof(T));
+    K(c, "hist_zero", [&] { launch_fill_u32(st, fe.hist.template as<uint32_t>(), (uint64_t)kStreams * 65536, 0u); });
     // base level (F5:2132-2145): gather low block 0, then the pass schedule
-    K(c, "gather_low0", [&] { launch_gather_low0<T>(st, full, (int)px, (int)py, (int)pz, vbase.as<T>()); });
-    const auto passes = make_passes(lx, ly, lz, ebl[0], prm.interp, prm.order, prm.alpha, prm.beta, 0);
+    K(c, "gather_low0", [&] {
+        launch_gather_low0<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>());
+    });
+    const auto passes = make_passes(lx, ly, lz, fe.ebl[0], fe.prm.interp, fe.prm.order, fe.prm.alpha, fe.prm.beta, 0);
+    uint16_t* c0 = fe.codes.template as<uint16_t>() + L.code_off[0];
+    T* s0 = fe.side.template as<T>() + L.code_off[0];
     run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
         K(c, "interp_encode_fused", [&] {
-            launch_interp_encode_fused<T>(st, fl, vbase.as<T>(), codes.as<uint16_t>() + L.code_off[0],
-                                          side.as<T>() + L.code_off[0], hist.as<uint32_t>());
+            launch_interp_encode_fused<T>(st, fl, fe.vbase.template as<T>(), c0, s0, fe.hist.template as<uint32_t>());
         });
     }, [&](const InterpPass& p) {
         K(c, "interp_encode", [&] {
-            launch_interp_encode<T>(st, p, vbase.as<T>(), codes.as<uint16_t>() + L.code_off[0],
-                                    side.as<T>() + L.code_off[0], hist.as<uint32_t>());
+            launch_interp_encode<T>(st, p, fe.vbase.template as<T>(), c0, s0, fe.hist.template as<uint32_t>());
         });
     });
+    if (!levels) return;
     // low level (F5:2180-2230) fused with the mid merge
-    LevelArgs la{};
-    la.eb = ebl[1];
-    la.recip = 1.0 / ebl[1];
-    for (int b = 0; b < 7; ++b) la.code_off[b] = L.code_off[1 + b];
+    const LevelArgs la = enc_level_args(fe, 1);
     K(c, "low_encode", [&] {
-        launch_low_encode<T>(st, full, (int)px, (int)py, (int)pz, vbase.as<T>(), L.g[1], la, codes.as<uint16_t>(),
-                             side.as<T>(), hist.as<uint32_t>() + 65536, mid.as<T>());
+        launch_low_encode<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>(), L.g[1], la,
+                             fe.codes.template as<uint16_t>(), fe.side.template as<T>(),
+                             fe.hist.template as<uint32_t>() + 65536, fe.mid.template as<T>());
     });
     // high level (F5:2252-2296)
-    LevelArgs ha{};
-    ha.eb = ebl[2];
-    ha.recip = 1.0 / ebl[2];
-    for (int b = 0; b < 7; ++b) ha.code_off[b] = L.code_off[8 + b];
+    const LevelArgs ha = enc_level_args(fe, 2);
     K(c, "high_encode", [&] {
-        launch_high_encode<T>(st, full, (int)px, (int)py, (int)pz, mid.as<T>(), L.g[8], ha, codes.as<uint16_t>(),
-                              side.as<T>(), hist.as<uint32_t>() + 8 * 65536);
+        launch_high_encode<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.mid.template as<T>(), L.g[8], ha,
+                              fe.codes.template as<uint16_t>(), fe.side.template as<T>(),
+                              fe.hist.template as<uint32_t>() + 8 * 65536);
     });
+}
 
-    // histograms -> host, codebooks (F5:1271 / F5:1402 preprocess_encode_fast)
-    constexpr uint32_t kFirst = 4096;
-    DBuf compact(pool, (size_t)kStreams * 65536 * 8), counts(pool, 64 * 4);
-    K(c, "hist_compact", [&] { launch_hist_compact(st, hist.as<uint32_t>(), kStreams, compact.as<uint64_t>(), counts.as<uint32_t>()); });
-    uint8_t* hs = (uint8_t*)c->host_stage(64 * 4 + (size_t)kStreams * kFirst * 8);
-    uint32_t* hcounts = (uint32_t*)hs;
-    uint64_t* hfirst = (uint64_t*)(hs + 256);
-    HIPCHK(hipMemcpyAsync(hcounts, counts.p, kStreams * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpy2DAsync(hfirst, kFirst * 8, compact.p, 65536 * 8, kFirst * 8, kStreams, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<std::vector<uint64_t>> lists(kStreams);
+template <typename T>
+LevelArgs enc_level_args(const FieldEnc<T>& fe, int level)
+{
+    LevelArgs a{};
+    a.eb = fe.ebl[level];
+    a.recip = 1.0 / fe.ebl[level];
+    for (int b = 0; b < 7; ++b) a.code_off[b] = fe.L.code_off[(level == 1 ? 1 : 8) + b];
+    return a;
+}
+
+// histograms -> host (F5:1271 / F5:1402 preprocess_encode_fast): requested, no wait
+template <typename T>
+void enc_phase2(stz_ctx* c, FieldEnc<T>& fe)
+{
+    hipStream_t st = c->stream;
+    fe.compact = DBuf(c->pool, (size_t)kStreams * 65536 * 8);
+    fe.counts = DBuf(c->pool, 64 * 4);
+    K(c, "hist_compact", [&] {
+        launch_hist_compact(st, fe.hist.template as<uint32_t>(), kStreams, fe.compact.template as<uint64_t>(),
+                            fe.counts.template as<uint32_t>());
+    });
+    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
+    fe.hs = (uint8_t*)c->host_stage(64 * 4 + (size_t)kStreams * kFirst * 8);
+    HIPCHK(hipMemcpyAsync(fe.hs, fe.counts.p, kStreams * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpy2DAsync(fe.hs + 256, kFirst * 8, fe.compact.p, 65536 * 8, kFirst * 8, kStreams,
+                            hipMemcpyDeviceToHost, st));
+}
+
+// after a wait: the rest of the symbol lists (waits when a stream uses more
+// than kFirst symbols), then codebooks and headers (host only)
+template <typename T>
+void enc_phase3_fetch(stz_ctx* c, FieldEnc<T>& fe)
+{
+    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
+    const uint32_t* hcounts = (const uint32_t*)fe.hs;
+    const uint64_t* hfirst = (const uint64_t*)(fe.hs + 256);
+    fe.lists.assign(kStreams, {});
     for (int s = 0; s < kStreams; ++s) {
         const uint32_t n = hcounts[s];
-        lists[s].resize(n);
-        if (n <= kFirst) std::memcpy(lists[s].data(), hfirst + (size_t)s * kFirst, n * 8);
-    }
-    for (int s = 0; s < kStreams; ++s)
-        if (hcounts[s] > kFirst)
-            HIPCHK(hipMemcpy(lists[s].data(), compact.as<uint64_t>() + (size_t)s * 65536, (size_t)hcounts[s] * 8,
+        fe.lists[s].resize(n);
+        if (n <= kFirst) std::memcpy(fe.lists[s].data(), hfirst + (size_t)s * kFirst, n * 8);
+        else
+            HIPCHK(hipMemcpy(fe.lists[s].data(), fe.compact.template as<uint64_t>() + (size_t)s * 65536, (size_t)n * 8,
                              hipMemcpyDeviceToHost));
+    }
+    (void)c;
+}
 
-    Meta& m = ds->meta;
+template <typename T>
+void enc_phase3_books(FieldEnc<T>& fe)
+{
+    Meta& m = fe.ds->meta;
     std::memset(&m.fh, 0, sizeof(m.fh));
     FileHeader& fh = m.fh;
+    const stz_params& prm = fe.prm;
     fh.magic = kMagic;
     fh.version = kVersion;
-    fh.dtype = (uint32_t)dtype;
-    fh.eb_mode = (uint32_t)mode;
-    fh.nx = nx; fh.ny = ny; fh.nz = nz;
-    fh.px = px; fh.py = py; fh.pz = pz;
-    fh.eb_user = eb;
-    fh.eb_unit = unit;
+    fh.dtype = (uint32_t)fe.dtype;
+    fh.eb_mode = (uint32_t)fe.mode;
+    fh.nx = fe.nx; fh.ny = fe.ny; fh.nz = fe.nz;
+    fh.px = fe.px; fh.py = fe.py; fh.pz = fe.pz;
+    fh.eb_user = fe.eb;
+    fh.eb_unit = fe.unit;
     fh.mult_base = prm.mult_base; fh.mult_low = prm.mult_low; fh.mult_high = prm.mult_high;
-    fh.eb_base = ebl[0]; fh.eb_low = ebl[1]; fh.eb_high = ebl[2];
-    fh.vmin = vmin; fh.vmax = vmax;
+    fh.eb_base = fe.ebl[0]; fh.eb_low = fe.ebl[1]; fh.eb_high = fe.ebl[2];
+    fh.vmin = fe.vmin; fh.vmax = fe.vmax;
     fh.radius = kRadius;
     fh.chunk_high = (uint32_t)prm.chunk_high;
     fh.chunk_low = (uint32_t)prm.chunk_low;
@@ -564,28 +644,41 @@ void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, 
     fh.alpha = prm.alpha;
     fh.beta = prm.beta;
     uint64_t total = sizeof(FileHeader);
-    size_t enc_n = 0;
+    fe.enc_n = 0;
     for (int s = 0; s < kStreams; ++s) {
-        const uint32_t n = hcounts[s];
-        build_codebook(lists[s].data(), n, kLmax, m.cb[s]);
-        const uint64_t n_unpred = (n && (lists[s][0] >> 32) == 0) ? (lists[s][0] & 0xFFFFFFFFull) : 0;
-        const double ebs = s == 0 ? ebl[0] : s < 8 ? ebl[1] : ebl[2];
-        fill_stream_header(m.sh[s], s, L, dtype, ebs, n, n_unpred, m.cb[s].total_bits);
+        const uint32_t n = (uint32_t)fe.lists[s].size();
+        build_codebook(fe.lists[s].data(), n, kLmax, m.cb[s]);
+        const uint64_t n_unpred = (n && (fe.lists[s][0] >> 32) == 0) ? (fe.lists[s][0] & 0xFFFFFFFFull) : 0;
+        const double ebs = s == 0 ? fe.ebl[0] : s < 8 ? fe.ebl[1] : fe.ebl[2];
+        fill_stream_header(m.sh[s], s, fe.L, fe.dtype, ebs, n, n_unpred, m.cb[s].total_bits);
         fh.stream_off[s] = total;
         fh.stream_bytes[s] = m.sh[s].stream_bytes;
         total += m.sh[s].stream_bytes;
-        enc_n += n;
+        fe.enc_n += n;
     }
     fh.total_bytes = total;
+}
 
-    // payload: zeroed, then host-written sections (headers, codebooks, chunk->sub tables)
-    ds->payload = DBuf(pool, align_up(total, 4));
+// payload: zeroed, then host-written sections (headers, codebooks, chunk->sub
+// tables), then the marked encode (F5:1412 encode_marked_fast): lengths ->
+// chunk scan -> pack.  No wait; the staging regions live until the call ends.
+template <typename T>
+void enc_phase4(stz_ctx* c, FieldEnc<T>& fe)
+{
+    hipStream_t st = c->stream;
+    auto pool = c->pool;
+    stz_dstream* ds = fe.ds;
+    Meta& m = ds->meta;
+    const FileHeader& fh = m.fh;
+    const HostLayout& L = fe.L;
+    const uint64_t total = fh.total_bytes;
+    ds->payload = DBuf(pool, align_up(total, 4) + 64);
     ds->bytes = total;
     ds->pool = pool;
     ds->device = c->device;
     uint8_t* pay = ds->payload.as<uint8_t>();
     K(c, "payload_zero", [&] { launch_fill_u32(st, (uint32_t*)pay, align_up(total, 4) / 4, 0u); });
-    size_t host_bytes = sizeof(FileHeader) + enc_n * 8;
+    size_t host_bytes = sizeof(FileHeader) + fe.enc_n * 8;
     for (int s = 0; s < kStreams; ++s) host_bytes += m.sh[s].off_sub_bits;
     uint8_t* stage = (uint8_t*)c->host_stage(host_bytes + 64);
     std::memcpy(stage, &fh, sizeof(fh));
@@ -613,26 +706,45 @@ void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, 
                 k++;
             }
     }
-    DBuf encl_d(pool, enc_n * 8 + 8), enc(pool, (size_t)kStreams * 65536 * 4);
-    HIPCHK(hipMemcpyAsync(encl_d.p, encl, enc_n * 8, hipMemcpyHostToDevice, st));
-    K(c, "enc_scatter", [&] { launch_enc_scatter(st, encl_d.as<uint32_t>(), (uint32_t)enc_n, enc.as<uint32_t>()); });
-
-    // marked encode (F5:1412 encode_marked_fast): lengths -> chunk scan -> pack
-    const StreamSet ss = make_streamset(L, m, dtype);
+    DBuf encl_d(pool, fe.enc_n * 8 + 8), enc(pool, (size_t)kStreams * 65536 * 4);
+    HIPCHK(hipMemcpyAsync(encl_d.p, encl, fe.enc_n * 8, hipMemcpyHostToDevice, st));
+    K(c, "enc_scatter", [&] { launch_enc_scatter(st, encl_d.as<uint32_t>(), (uint32_t)fe.enc_n, enc.as<uint32_t>()); });
+    const StreamSet ss = make_streamset(L, m, fe.dtype);
     DBuf pre_b(pool, (size_t)ss.total_sub * 4 + 16), pre_u(pool, (size_t)ss.total_sub * 4 + 16);
     DBuf tot_b(pool, (size_t)ss.total_chunks * 8 + 16), tot_u(pool, (size_t)ss.total_chunks * 8 + 16);
     K(c, "huff_len", [&] {
-        launch_huff_len(st, ss, codes.as<uint16_t>(), enc.as<uint32_t>(), pay, pre_b.as<uint32_t>(),
+        launch_huff_len(st, ss, fe.codes.template as<uint16_t>(), enc.as<uint32_t>(), pay, pre_b.as<uint32_t>(),
                         pre_u.as<uint32_t>(), tot_b.as<uint64_t>(), tot_u.as<uint64_t>());
     });
     K(c, "huff_scan", [&] { launch_huff_scan(st, ss, tot_b.as<uint64_t>(), tot_u.as<uint64_t>(), pay); });
     K(c, "huff_pack", [&] {
-        launch_huff_pack(st, ss, codes.as<uint16_t>(), side.p, enc.as<uint32_t>(), pre_b.as<uint32_t>(),
+        launch_huff_pack(st, ss, fe.codes.template as<uint16_t>(), fe.side.p, enc.as<uint32_t>(), pre_b.as<uint32_t>(),
                          pre_u.as<uint32_t>(), pay);
     });
-    // the staging buffer feeds async copies: finish before it can be reused
-    HIPCHK(hipStreamSynchronize(st));
     ds->have_tabs = false;
+}
+
+template <typename T>
+void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
+                   const stz_params& prm, stz_dstream* ds)
+{
+    FieldEnc<T> fe;
+    fe.dtype = dtype;
+    fe.mode = mode;
+    fe.eb = eb;
+    fe.prm = prm;
+    fe.nx = nx; fe.ny = ny; fe.nz = nz;
+    fe.ds = ds;
+    enc_phase0(c, fe, field);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    enc_phase1(c, fe, true);
+    enc_phase2(c, fe);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    enc_phase3_fetch(c, fe);
+    enc_phase3_books(fe);
+    enc_phase4(c, fe);
+    // the staging buffers feed async copies: finish before they can be reused
+    HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 // ------------------------------------------------------------ decode helpers
@@ -1061,6 +1173,356 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
         err.check();
     }
     for (int i : full_reqs) decompress_full<T>(c, ds, outs[i]);
+}
+
+// ------------------------------------------------------------ batches
+// Several fields in one call.  Their residual levels run as one persistent
+// launch per level fed by a device work queue (k_level_multi); its launch
+// geometry (waves per workgroup: the tile height; workgroups per CU) is
+// chosen per dtype, level mode and size class by timing every candidate once
+// on the live batch (the runs are idempotent or their histograms are
+// cleared), and kept in the context.  Results are bit-identical to
+// single-field calls: every point runs the same arithmetic.
+struct JobSet {
+    std::vector<LevelJob> h[2];  // waves 4, waves 8
+    uint32_t total[2] = {0, 0};
+    DBuf d[2];
+    DBuf counter;
+};
+
+void jobs_finish(stz_ctx* c, JobSet& js)
+{
+    for (int v = 0; v < 2; ++v) {
+        const int waves = v == 0 ? 4 : 8;
+        uint32_t item = 0;
+        for (LevelJob& j : js.h[v]) {
+            j.item0 = item;
+            j.ntiles = j.g.dx >= 4 ? level_tiles(j.g, waves) : 0;
+            j.nedge = level_edge_groups(j.g, waves);
+            item += j.ntiles + j.nedge;
+        }
+        js.total[v] = item;
+        const size_t bytes = js.h[v].size() * sizeof(LevelJob);
+        js.d[v] = DBuf(c->pool, bytes + 64);
+        void* hs = c->host_stage(bytes);
+        std::memcpy(hs, js.h[v].data(), bytes);
+        HIPCHK(hipMemcpyAsync(js.d[v].p, hs, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    js.counter = DBuf(c->pool, 64);
+}
+
+template <typename T>
+void jobs_launch(stz_ctx* c, int mode, JobSet& js, const stz_ctx::LaunchCfg& cfg)
+{
+    const int v = cfg.waves == 8 ? 1 : 0;
+    K(c, "counter_zero", [&] { launch_fill_u32(c->stream, js.counter.as<uint32_t>(), 1, 0u); });
+    K(c, mode == kLowEnc ? "low_encode_multi" : mode == kHighEnc ? "high_encode_multi"
+         : mode == kLowDec ? "low_decode_multi" : "high_decode_multi", [&] {
+        launch_level_multi<T>(c->stream, mode, js.d[v].as<LevelJob>(), (uint32_t)js.h[v].size(), js.total[v],
+                              js.counter.as<uint32_t>(), cfg.waves, cfg.grid);
+    });
+}
+
+// the tuned geometry for this launch; times the candidates the first time
+// (reset() restores the launch's inputs before each candidate)
+template <typename T, typename R>
+stz_ctx::LaunchCfg level_cfg(stz_ctx* c, int mode, JobSet& js, R&& reset)
+{
+    uint64_t pts = 0;
+    for (const LevelJob& j : js.h[0]) pts += (uint64_t)j.g.dx * j.g.dy * j.g.dz;
+    int lg = 0;
+    while ((1ull << (lg + 1)) <= pts) ++lg;
+    char key[64];
+    std::snprintf(key, sizeof(key), "%s/%s/2^%d", sizeof(T) == 4 ? "f32" : "f64",
+                  mode == kLowEnc ? "low_encode" : mode == kHighEnc ? "high_encode" : mode == kLowDec ? "low_decode"
+                                                                                           : "high_decode", lg);
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) return it->second;
+    stz_ctx::LaunchCfg best;
+    best.waves = 4;
+    best.grid = (uint32_t)std::max(1, c->n_cu) * 2;
+    if (!c->autotune) return best;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    best.ms = 1e30f;
+    std::string trials;
+    const bool prof = c->prof;
+    c->prof = false;
+    for (int waves : {4, 8})
+        for (int per_cu : {1, 2, 4}) {
+            stz_ctx::LaunchCfg cand;
+            cand.waves = waves;
+            cand.grid = (uint32_t)(std::max(1, c->n_cu) * per_cu * 4 / waves);
+            reset();
+            HIPCHK(hipEventRecord(e0, c->stream));
+            jobs_launch<T>(c, mode, js, cand);
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            char t[96];
+            std::snprintf(t, sizeof(t), "%s{\"waves\": %d, \"grid\": %u, \"ms\": %.4f}", trials.empty() ? "" : ", ",
+                          waves, cand.grid, ms);
+            trials += t;
+            if (ms < best.ms) {
+                best = cand;
+                best.ms = ms;
+            }
+        }
+    c->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    best.trials = "[" + trials + "]";
+    reset();
+    c->tuned[key] = best;
+    return best;
+}
+
+template <typename T>
+void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
+                         int mode, const stz_params& prm, stz_dstream** out)
+{
+    std::vector<FieldEnc<T>> fe(n);
+    std::vector<std::unique_ptr<stz_dstream>> dss(n);
+    for (int i = 0; i < n; ++i) {
+        if (!fields[i]) throw StzError(STZ_E_ARG, "null field");
+        dss[i].reset(new stz_dstream());
+        FieldEnc<T>& f = fe[i];
+        f.dtype = dtype;
+        f.mode = mode;
+        f.eb = eb;
+        f.prm = prm;
+        f.nx = dims[3 * i];
+        f.ny = dims[3 * i + 1];
+        f.nz = dims[3 * i + 2];
+        f.ds = dss[i].get();
+        enc_phase0(c, f, (const T*)fields[i]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) enc_phase1(c, fe[i], false);
+    for (int level = 1; level <= 2; ++level) {
+        const int md = level == 1 ? kLowEnc : kHighEnc;
+        JobSet js;
+        for (int v = 0; v < 2; ++v)
+            for (int i = 0; i < n; ++i) {
+                FieldEnc<T>& f = fe[i];
+                LevelJob j{};
+                j.full = f.full;
+                j.px = (int)f.px;
+                j.py = (int)f.py;
+                j.ref = level == 1 ? f.vbase.p : f.mid.p;
+                j.mid = level == 1 ? f.mid.p : nullptr;
+                j.codes = f.codes.template as<uint16_t>();
+                j.side = f.side.p;
+                j.hist = f.hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536;
+                j.g = f.L.g[level == 1 ? 1 : 8];
+                j.a = enc_level_args(f, level);
+                js.h[v].push_back(j);
+            }
+        jobs_finish(c, js);
+        const stz_ctx::LaunchCfg cfg = level_cfg<T>(c, md, js, [&] {
+            for (int i = 0; i < n; ++i)
+                launch_fill_u32(c->stream, fe[i].hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536,
+                                7 * 65536, 0u);
+        });
+        jobs_launch<T>(c, md, js, cfg);  // the pool hands the job tables out again only to later work
+    }
+    for (int i = 0; i < n; ++i) enc_phase2(c, fe[i]);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) enc_phase3_fetch(c, fe[i]);
+    {   // codebooks and headers on host threads
+        const int nt = std::max(1, std::min(n, 8));
+        std::vector<std::thread> th;
+        std::vector<std::exception_ptr> errs(nt);
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                try {
+                    for (int i = t; i < n; i += nt) enc_phase3_books(fe[i]);
+                } catch (...) {
+                    errs[t] = std::current_exception();
+                }
+            });
+        for (auto& x : th) x.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
+    }
+    for (int i = 0; i < n; ++i) enc_phase4(c, fe[i]);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) out[i] = dss[i].release();
+}
+
+// Batch decode: per field the Huffman decode and the base passes, then the
+// low and high levels of all fields as one persistent launch each.  The
+// launch sequence is captured as a HIP graph on the first call and replayed
+// while the call names the same streams and outputs.
+struct BatchPlan {
+    std::vector<stz_dstream*> ds;
+    std::vector<uint64_t> gens;
+    std::vector<void*> outs;
+    int dtype = 0;
+    struct Field {
+        HostLayout L;
+        StreamSet ss;
+        bool stage = false;
+        Decoded dd;
+        DBuf items, vb, mid;
+    };
+    std::vector<Field> f;
+    DBuf err;
+    JobSet low, high;
+    stz_ctx::LaunchCfg cfg_low, cfg_high;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchPlan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+// the full Huffman decode of one stream set into preallocated buffers
+void plan_huff_setup(stz_ctx* c, stz_dstream* ds, BatchPlan::Field& F)
+{
+    ensure_tabs(c, ds);
+    const Meta& m = ds->meta;
+    layout_of(m, F.L);
+    const HostLayout& L = F.L;
+    F.ss = make_streamset(L, m, (int)m.fh.dtype);
+    double wbits = 0, wsym = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        F.ss.s[s].item_count = L.n_sub[s];
+        F.dd.code_base[s] = L.code_off[s];
+        const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
+        wbits += bps * L.n_sub[s];
+        wsym += L.n_sub[s];
+    }
+    F.stage = wsym > 0 && wbits / wsym <= 2.0;
+    uint32_t item = 0, wg = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        StreamDesc& d = F.ss.s[s];
+        const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
+        d.ipw = 256;
+        if (F.stage)
+            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > 12288.0 * 32.0) d.ipw >>= 1;
+        d.item_base = item;
+        d.wg_base = wg;
+        const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
+        item += nwg * d.ipw;
+        wg += nwg;
+    }
+    F.ss.total_items = item;
+    F.ss.total_wgs = wg;
+    const size_t ts = tsize_of((int)m.fh.dtype);
+    F.dd.codes = DBuf(c->pool, L.total_codes * 2 + 64);
+    F.dd.side = DBuf(c->pool, L.total_codes * ts + 64);
+    F.dd.compact = false;
+    F.items = DBuf(c->pool, (size_t)std::max<uint32_t>(item, 1) * sizeof(DecItem));
+    F.vb = DBuf(c->pool, (size_t)L.g[0].dx * L.g[0].dy * L.g[0].dz * ts);
+    F.mid = DBuf(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * ts);
+}
+
+template <typename T>
+void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
+{
+    hipStream_t st = c->stream;
+    const int n = (int)P.ds.size();
+    K(c, "err_zero", [&] { launch_fill_u32(st, P.err.as<uint32_t>(), (uint64_t)n, 0u); });
+    for (int i = 0; i < n; ++i) {
+        BatchPlan::Field& F = P.f[i];
+        stz_dstream* ds = P.ds[i];
+        uint8_t* pay = ds->payload.as<uint8_t>();
+        if (F.ss.total_items) {
+            K(c, "dec_prep", [&] { launch_dec_prep(st, F.ss, pay, nullptr, 0, F.items.as<DecItem>(), nullptr); });
+            K(c, "huff_decode", [&] {
+                launch_huff_decode(st, F.ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(),
+                                   F.items.as<DecItem>(), F.dd.codes.as<uint16_t>(), F.dd.side.p,
+                                   P.err.as<uint32_t>() + i, F.stage);
+            });
+        }
+        decode_base<T>(c, ds->meta, F.L, F.dd, F.vb.as<T>(), 0);
+    }
+    if (!levels) return;
+    jobs_launch<T>(c, kLowDec, P.low, P.cfg_low);
+    jobs_launch<T>(c, kHighDec, P.high, P.cfg_high);
+}
+
+template <typename T>
+void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* const* outs)
+{
+    bool same = c->plan && (int)c->plan->ds.size() == n;
+    for (int i = 0; same && i < n; ++i)
+        same = c->plan->ds[i] == ds[i] && c->plan->gens[i] == ds[i]->gen && c->plan->outs[i] == outs[i];
+    if (!same) {
+        c->plan.reset();
+        std::unique_ptr<BatchPlan> P(new BatchPlan());
+        P->ds.assign(ds, ds + n);
+        for (int i = 0; i < n; ++i) P->gens.push_back(ds[i]->gen);
+        P->outs.assign(outs, outs + n);
+        P->f.resize(n);
+        P->err = DBuf(c->pool, (size_t)n * 4 + 64);
+        for (int i = 0; i < n; ++i) {
+            if (!is_device_ptr(outs[i])) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
+            plan_huff_setup(c, ds[i], P->f[i]);
+        }
+        for (int i = 0; i < n; ++i) {
+            BatchPlan::Field& F = P->f[i];
+            const Meta& m = ds[i]->meta;
+            for (int v = 0; v < 2; ++v) {
+                LevelJob lo{};
+                lo.ref = F.vb.p;
+                lo.mid = F.mid.p;
+                lo.codes = F.dd.codes.as<uint16_t>();
+                lo.side = F.dd.side.p;
+                lo.g = F.L.g[1];
+                lo.a = level_args(m, F.dd, 1, P->err.as<uint32_t>() + i);
+                P->low.h[v].push_back(lo);
+                LevelJob hi{};
+                hi.ref = F.mid.p;
+                hi.out = outs[i];
+                hi.codes = F.dd.codes.as<uint16_t>();
+                hi.side = F.dd.side.p;
+                hi.nx = (int)m.fh.nx;
+                hi.ny = (int)m.fh.ny;
+                hi.nz = (int)m.fh.nz;
+                hi.vec = ((m.fh.nx & 1) == 0) && (((uintptr_t)outs[i] % (2 * sizeof(T))) == 0);
+                hi.g = F.L.g[8];
+                hi.a = level_args(m, F.dd, 2, P->err.as<uint32_t>() + i);
+                P->high.h[v].push_back(hi);
+            }
+        }
+        jobs_finish(c, P->low);
+        jobs_finish(c, P->high);
+        // eager run (also the inputs of the timing runs), then the geometry
+        plan_enqueue<T>(c, *P, false);
+        P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
+        jobs_launch<T>(c, kLowDec, P->low, P->cfg_low);
+        P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // capture the whole sequence
+        const bool prof = c->prof;
+        c->prof = false;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        try {
+            plan_enqueue<T>(c, *P, true);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            c->prof = prof;
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(c->stream, &P->graph));
+        c->prof = prof;
+        HIPCHK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+        c->plan = std::move(P);
+    }
+    HIPCHK(hipGraphLaunch(c->plan->exec, c->stream));
+    uint32_t* herr = (uint32_t*)c->host_stage((size_t)n * 4);
+    HIPCHK(hipMemcpyAsync(herr, c->plan->err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i)
+        if (herr[i]) throw StzError(STZ_E_STREAM, "corrupt Huffman bitstream");
 }
 
 // ------------------------------------------------------------ payload parsing
@@ -1519,6 +1981,9 @@ int stz_create(stz_ctx** out, int device)
         c->stream = c->own;
         c->pool = std::make_shared<Pool>();
         c->pool->device = device;
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        c->n_cu = prop.multiProcessorCount;
         *out = c.release();
     });
 }
@@ -1532,6 +1997,7 @@ void stz_destroy(stz_ctx* c)
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
+    c->plan.reset();
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (void* p : c->retired) (void)hipHostFree(p);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
@@ -1593,6 +2059,60 @@ int stz_compress_device(stz_ctx* c, stz_dtype dtype, const void* field, size_t n
         if (dtype == STZ_F32) compress_tuned<float>(c, dtype, field, nx, ny, nz, eb, mode, p, ds.get());
         else compress_tuned<double>(c, dtype, field, nx, ny, nz, eb, mode, p, ds.get());
         *out = ds.release();
+    });
+}
+
+// ---- batches ------------------------------------------------------------
+int stz_compress_batch(stz_ctx* c, stz_dtype dtype, int n, const void* const* fields, const size_t* dims,
+                       double eb, stz_eb_mode mode, const stz_params* params, stz_dstream** out)
+{
+    if (!c || n < 0 || (n > 0 && (!fields || !dims || !out)) || (dtype != STZ_F32 && dtype != STZ_F64) ||
+        mode < STZ_EB_ABS || mode > STZ_EB_REF)
+        return STZ_E_ARG;
+    const stz_params p = params ? *params : defaults();
+    if (p.tuner != 0) return STZ_E_ARG;
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) compress_batch_impl<float>(c, dtype, n, fields, dims, eb, mode, p, out);
+        else compress_batch_impl<double>(c, dtype, n, fields, dims, eb, mode, p, out);
+    });
+}
+
+int stz_decompress_batch(stz_ctx* c, int n, stz_dstream* const* ds, void* const* outs)
+{
+    if (!c || n < 0 || (n > 0 && (!ds || !outs))) return STZ_E_ARG;
+    if (n == 0) return STZ_OK;
+    return guard(c, [&] {
+        const uint32_t dt = ds[0] ? ds[0]->meta.fh.dtype : 0;
+        for (int i = 0; i < n; ++i) {
+            if (!ds[i] || !outs[i]) throw StzError(STZ_E_ARG, "null stream or output");
+            if (ds[i]->meta.fh.dtype != dt) throw StzError(STZ_E_ARG, "mixed dtypes in one batch");
+        }
+        if (dt == STZ_F32) decompress_batch_impl<float>(c, n, ds, outs);
+        else decompress_batch_impl<double>(c, n, ds, outs);
+    });
+}
+
+int stz_autotune(stz_ctx* c, int enable, char* report, size_t cap)
+{
+    if (!c) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (enable >= 0) c->autotune = enable != 0;
+        if (!report) return;
+        std::string s = "{";
+        bool first = true;
+        for (auto& kv : c->tuned) {
+            char tmp[160];
+            std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"waves\": %d, \"grid\": %u, \"ms\": %.4f, \"trials\": ",
+                          first ? "" : ", ", kv.first.c_str(), kv.second.waves, kv.second.grid, kv.second.ms);
+            s += tmp;
+            s += kv.second.trials.empty() ? "[]" : kv.second.trials;
+            s += "}";
+            first = false;
+        }
+        s += "}";
+        if (s.size() + 1 > cap) throw StzError(STZ_E_ARG, "report buffer too small");
+        std::memcpy(report, s.c_str(), s.size() + 1);
     });
 }
 

@@ -26,6 +26,7 @@ EXPORTS = [
     "stz_dequantize", "stz_huff_lengths", "stz_plan_query",
     "stz_tune_config", "stz_tune_features", "stz_tune_set_model", "stz_tune", "stz_tune_trial",
     "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
+    "stz_compress_batch", "stz_decompress_batch", "stz_autotune",
 ]
 TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS = 48, 16, 8368
 
@@ -123,6 +124,9 @@ def lib():
     L.stz_dequantize.argtypes = [vp, i32, vp, sz, dbl, vp, vp]
     L.stz_huff_lengths.argtypes = [vp, i32, i32, vp]
     L.stz_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
+    L.stz_compress_batch.argtypes = [vp, i32, i32, P(vp), P(sz), dbl, i32, P(Params), P(vp)]
+    L.stz_decompress_batch.argtypes = [vp, i32, P(vp), P(vp)]
+    L.stz_autotune.argtypes = [vp, i32, C.c_char_p, sz]
     f32 = C.c_float
     L.stz_tune_config.argtypes = [i32, P(Params)]
     L.stz_tune_features.argtypes = [vp, i32, vp, sz, sz, sz, dbl, vp, sz, P(sz)]

@@ -256,6 +256,43 @@ class Context:
                                         EB_MODES[mode], C.byref(p), C.byref(h)), "stz_compress_device")
         return DStream(self, h.value)
 
+    # -- many fields
+    def compress_batch(self, fields, eb, mode="rel", params=None, **kw):
+        """Uneven fields of one dtype in one call (bit-identical to one call each)."""
+        n = len(fields)
+        if n == 0:
+            return []
+        dt = _dtype_id(fields[0])
+        ptrs = (C.c_void_p * n)(*[_ptr(f) for f in fields])
+        dims = (C.c_size_t * (3 * n))()
+        for i, f in enumerate(fields):
+            if _dtype_id(f) != dt:
+                raise TypeError("one dtype per batch")
+            nz, ny, nx = f.shape
+            dims[3 * i], dims[3 * i + 1], dims[3 * i + 2] = nx, ny, nz
+        p = _params(params, kw)
+        out = (C.c_void_p * n)()
+        check(lib().stz_compress_batch(self.h, dt, n, ptrs, dims, float(eb), EB_MODES[mode], C.byref(p), out),
+              "stz_compress_batch")
+        return [DStream(self, out[i]) for i in range(n)]
+
+    def decompress_batch(self, dstreams, outs):
+        """Decode several device-resident streams into device tensors (one
+        captured launch sequence, replayed while the streams and outputs stay)."""
+        n = len(dstreams)
+        hs = (C.c_void_p * n)(*[d.h.value for d in dstreams])
+        ps = (C.c_void_p * n)(*[_ptr(o) for o in outs])
+        check(lib().stz_decompress_batch(self.h, n, hs, ps), "stz_decompress_batch")
+        return outs
+
+    def autotune(self, enable=None):
+        """Launch geometries chosen so far (and switch the timing on/off)."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().stz_autotune(self.h, -1 if enable is None else (1 if enable else 0), buf, len(buf)),
+              "stz_autotune")
+        return json.loads(buf.value.decode())
+
     def upload(self, stream):
         buf = np.ascontiguousarray(np.frombuffer(stream, dtype=np.uint8))
         h = C.c_void_p()
