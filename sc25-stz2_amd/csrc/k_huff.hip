@@ -506,18 +506,13 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
 // refills read global memory, every workgroup decodes 256 sub-chunks, and the
 // smaller LDS footprint lets four times as many workgroups share a CU.
 template <bool kStage>
-__global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
-                                                     const DecTab* __restrict__ tabs,
-                                                     const uint16_t* __restrict__ sorted,
-                                                     const DecItem* __restrict__ items,
-                                                     uint16_t* __restrict__ codes,
-                                                     uint8_t* __restrict__ side,
-                                                     uint32_t* __restrict__ err)
+__device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_t* __restrict__ payload,
+                                               const DecTab* __restrict__ tabs, const uint16_t* __restrict__ sorted,
+                                               const DecItem* __restrict__ items, uint16_t* __restrict__ codes,
+                                               uint8_t* __restrict__ side, uint32_t* __restrict__ err,
+                                               uint32_t wgid, uint32_t* lut, uint32_t* fc, uint32_t* cn,
+                                               uint32_t* fi, uint32_t* stage)
 {
-    __shared__ uint32_t lut[1 << kLutBits];
-    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
-    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
-    const uint32_t wgid = blockIdx.x;
     const int s = find_stream_by_wg(ss, wgid);
     if (s < 0) return;  // uniform across the workgroup
     const StreamDesc& d = ss.s[s];
@@ -573,6 +568,85 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
     else
         bad = decode_run<1>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
     if (bad) atomicOr(err, 1u);
+}
+
+
+template <bool kStage>
+__global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
+                                                     const DecTab* __restrict__ tabs,
+                                                     const uint16_t* __restrict__ sorted,
+                                                     const DecItem* __restrict__ items,
+                                                     uint16_t* __restrict__ codes,
+                                                     uint8_t* __restrict__ side,
+                                                     uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
+    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage);
+}
+
+// many fields: workgroups wg0 .. of job j decode field j
+__device__ __forceinline__ uint32_t find_dec_job(const DecJob* __restrict__ jobs, uint32_t njobs, uint32_t wg)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].wg0 <= wg) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+template <bool kStage>
+__global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restrict__ jobs, uint32_t njobs)
+{
+    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
+    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];
+    huff_decode_wg<kStage>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
+                           blockIdx.x - J.wg0, lut, fc, cn, fi, stage);
+}
+
+// many fields, every chunk: thread -> (field, chunk)
+__global__ void k_dec_prep_multi(const DecJob* __restrict__ jobs, uint32_t njobs, uint32_t total)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].chunk0 <= i) lo = m; else hi = m;
+    }
+    const DecJob& J = jobs[lo];
+    const StreamSet& ss = J.ss;
+    const uint32_t gc = i - J.chunk0;
+    const uint32_t s = (uint32_t)find_stream_by_chunk(ss, gc);
+    const uint32_t c = gc - ss.s[s].chunk_base;
+    const StreamDesc& d = ss.s[s];
+    const uint8_t* payload = J.payload;
+    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
+    const uint16_t* sbits = (const uint16_t*)(payload + d.off_sub_bits);
+    const uint16_t* sun = (const uint16_t*)(payload + d.off_sub_unpred);
+    uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
+    uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
+    const uint32_t sb = csub[c], se = csub[c + 1];
+    const uint32_t ib = d.item_base + sb;
+    const uint64_t pstart = chunk_start(d.g, c);
+    const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
+    for (uint32_t j = 0; j < se - sb; ++j) {
+        DecItem it;
+        it.bit0 = bit;
+        it.u0 = u;
+        it.p0 = (uint32_t)(pstart + (uint64_t)j * kSub);
+        const uint64_t rem = vol - (uint64_t)j * kSub;
+        it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
+        it.stream = s;
+        it.bits = sbits[sb + j];
+        J.items[ib + j] = it;
+        bit += sbits[sb + j];
+        u += sun[sb + j];
+    }
 }
 
 }  // namespace
@@ -634,6 +708,19 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
     const uint32_t n = list ? nlist : ss.total_chunks;
     if (!n) return;
     k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items, cmap);
+}
+
+void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage)
+{
+    if (!njobs || !wgs) return;
+    if (stage) k_huff_decode_multi<true><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+    else k_huff_decode_multi<false><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+}
+
+void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks)
+{
+    if (!njobs || !total_chunks) return;
+    k_dec_prep_multi<<<dim3((total_chunks + 255) / 256), dim3(256), 0, s>>>(jobs, njobs, total_chunks);
 }
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,

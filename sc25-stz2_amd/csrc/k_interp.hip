@@ -204,11 +204,10 @@ __global__ __launch_bounds__(1024) void k_interp_encode_fused(InterpPassList l, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_interp_decode_fused(InterpPassList l, T* __restrict__ v,
-                                                              const uint16_t* __restrict__ codes,
-                                                              const T* __restrict__ side)
+__device__ __forceinline__ void interp_decode_fused_wg(const InterpPassList& l, T* __restrict__ v,
+                                                       const uint16_t* __restrict__ codes,
+                                                       const T* __restrict__ side, T* L)
 {
-    __shared__ T L[kLatticeBytes / sizeof(T)];
     for (int pi = 0; pi < l.n; ++pi) {
         const InterpPass& p = l.p[pi];
         for (uint64_t k = threadIdx.x; k < p.n; k += 1024u) {
@@ -224,6 +223,49 @@ __global__ __launch_bounds__(1024) void k_interp_decode_fused(InterpPassList l, 
         __syncthreads();
     }
     lattice_io(l, L, v, false);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_decode_fused(InterpPassList l, T* __restrict__ v,
+                                                              const uint16_t* __restrict__ codes,
+                                                              const T* __restrict__ side)
+{
+    __shared__ T L[kLatticeBytes / sizeof(T)];
+    interp_decode_fused_wg(l, v, codes, side, L);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_decode_fused_multi(const FusedJob* __restrict__ jobs)
+{
+    __shared__ T L[kLatticeBytes / sizeof(T)];
+    const FusedJob& J = jobs[blockIdx.x];
+    interp_decode_fused_wg(J.l, (T*)J.v, J.codes, (const T*)J.side, L);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_decode_multi(const PassJob* __restrict__ jobs, uint32_t njobs)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].wg0 <= blockIdx.x) lo = m; else hi = m;
+    }
+    const PassJob& J = jobs[lo];
+    const InterpPass& p = J.p;
+    T* __restrict__ v = (T*)J.v;
+    const T* __restrict__ side = (const T*)J.side;
+    const uint64_t stride = (uint64_t)J.nwg * 256u;
+    for (uint64_t k = (blockIdx.x - J.wg0) * 256ull + threadIdx.x; k < p.n; k += stride) {
+        const uint32_t q = J.codes[p.pos + k];
+        if (p.axis < 0) {
+            const long c = (long)p.start[0] + (long)p.start[1] * p.bx + (long)p.start[2] * p.bx * p.by;
+            v[c] = q ? dequantize<T>(q, p.eb) : side[p.pos + k];
+        } else {
+            const PassPoint pp = pass_point(p, k);
+            const T pred = interp_pred(v, pp.c, pp.st, pp.pos_axis, p.s, pp.dim, p.interp);
+            v[pp.c] = q ? pred + dequantize<T>(q, p.eb) : side[p.pos + k];
+        }
+    }
 }
 
 inline unsigned pass_grid(uint64_t n)
@@ -262,6 +304,24 @@ void launch_interp_decode_fused(hipStream_t s, const InterpPassList& l, T* v, co
     k_interp_decode_fused<T><<<dim3(1), dim3(1024), 0, s>>>(l, v, codes, side);
 }
 
+template <typename T>
+void launch_interp_decode_fused_multi(hipStream_t s, const FusedJob* jobs, uint32_t njobs)
+{
+    if (!njobs) return;
+    k_interp_decode_fused_multi<T><<<dim3(njobs), dim3(1024), 0, s>>>(jobs);
+}
+
+template <typename T>
+void launch_interp_decode_multi(hipStream_t s, const PassJob* jobs, uint32_t njobs, uint32_t wgs)
+{
+    if (!njobs || !wgs) return;
+    k_interp_decode_multi<T><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+}
+
+template void launch_interp_decode_fused_multi<float>(hipStream_t, const FusedJob*, uint32_t);
+template void launch_interp_decode_fused_multi<double>(hipStream_t, const FusedJob*, uint32_t);
+template void launch_interp_decode_multi<float>(hipStream_t, const PassJob*, uint32_t, uint32_t);
+template void launch_interp_decode_multi<double>(hipStream_t, const PassJob*, uint32_t, uint32_t);
 template void launch_interp_encode_fused<float>(hipStream_t, const InterpPassList&, float*, uint16_t*, float*, uint32_t*);
 template void launch_interp_encode_fused<double>(hipStream_t, const InterpPassList&, double*, uint16_t*, double*, uint32_t*);
 template void launch_interp_decode_fused<float>(hipStream_t, const InterpPassList&, float*, const uint16_t*, const float*);

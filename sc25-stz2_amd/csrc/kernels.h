@@ -61,6 +61,22 @@ struct DecItem {
 };
 
 // ---- k_huff.hip ----
+// one field of a multi-field decode launch (every chunk of every stream)
+struct DecJob {
+    StreamSet ss;
+    const uint8_t* payload;
+    const DecTab* tabs;
+    const uint16_t* sorted;
+    DecItem* items;
+    uint16_t* codes;
+    void* side;
+    uint32_t* err;
+    uint32_t wg0;      // first workgroup of this field in its decode launch
+    uint32_t chunk0;   // first chunk of this field in the prep launch
+};
+void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks);
+void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage);
+
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 void launch_hist_compact(hipStream_t s, const uint32_t* hist, int nstreams, uint64_t* out,
                          uint32_t* counts);
@@ -199,6 +215,25 @@ struct InterpPassList {
     InterpPass p[kMaxFusedPasses];  // start/step/s/dims in lattice units
 };
 constexpr int kLatticeBytes = 128 * 1024;
+// many fields: the fused coarse passes one workgroup per field; a fine pass
+// of every field in one launch (workgroups wg0 .. wg0 + nwg - 1 per field)
+struct FusedJob {
+    InterpPassList l;
+    void* v;
+    const uint16_t* codes;
+    const void* side;
+};
+struct PassJob {
+    InterpPass p;
+    void* v;
+    const uint16_t* codes;
+    const void* side;
+    uint32_t wg0, nwg;
+};
+template <typename T>
+void launch_interp_decode_fused_multi(hipStream_t s, const FusedJob* jobs, uint32_t njobs);
+template <typename T>
+void launch_interp_decode_multi(hipStream_t s, const PassJob* jobs, uint32_t njobs, uint32_t wgs);
 template <typename T>
 void launch_interp_encode_fused(hipStream_t s, const InterpPassList& l, T* v, uint16_t* codes, T* side,
                                 uint32_t* hist);

@@ -1370,6 +1370,15 @@ struct BatchPlan {
     };
     std::vector<Field> f;
     DBuf err;
+    // Huffman decode of all fields: prep over every chunk, then the staged and
+    // the unstaged fields as one launch each; base passes: the fused coarse
+    // passes one workgroup per field, then one launch per fine-pass round
+    DBuf dec_all, dec_st, dec_un;
+    uint32_t n_st = 0, n_un = 0, wg_st = 0, wg_un = 0, chunks = 0;
+    DBuf fused;
+    uint32_t n_fused = 0;
+    std::vector<DBuf> rounds;
+    std::vector<std::pair<uint32_t, uint32_t>> round_n;  // jobs, workgroups
     JobSet low, high;
     stz_ctx::LaunchCfg cfg_low, cfg_high;
     hipGraph_t graph = nullptr;
@@ -1428,23 +1437,109 @@ void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
     hipStream_t st = c->stream;
     const int n = (int)P.ds.size();
     K(c, "err_zero", [&] { launch_fill_u32(st, P.err.as<uint32_t>(), (uint64_t)n, 0u); });
-    for (int i = 0; i < n; ++i) {
-        BatchPlan::Field& F = P.f[i];
-        stz_dstream* ds = P.ds[i];
-        uint8_t* pay = ds->payload.as<uint8_t>();
-        if (F.ss.total_items) {
-            K(c, "dec_prep", [&] { launch_dec_prep(st, F.ss, pay, nullptr, 0, F.items.as<DecItem>(), nullptr); });
-            K(c, "huff_decode", [&] {
-                launch_huff_decode(st, F.ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(),
-                                   F.items.as<DecItem>(), F.dd.codes.as<uint16_t>(), F.dd.side.p,
-                                   P.err.as<uint32_t>() + i, F.stage);
-            });
-        }
-        decode_base<T>(c, ds->meta, F.L, F.dd, F.vb.as<T>(), 0);
-    }
+    K(c, "dec_prep", [&] { launch_dec_prep_multi(st, P.dec_all.as<DecJob>(), (uint32_t)n, P.chunks); });
+    K(c, "huff_decode", [&] {
+        launch_huff_decode_multi(st, P.dec_st.as<DecJob>(), P.n_st, P.wg_st, true);
+        launch_huff_decode_multi(st, P.dec_un.as<DecJob>(), P.n_un, P.wg_un, false);
+    });
+    K(c, "interp_decode_fused", [&] { launch_interp_decode_fused_multi<T>(st, P.fused.as<FusedJob>(), P.n_fused); });
+    for (size_t r = 0; r < P.rounds.size(); ++r)
+        K(c, "interp_decode", [&] {
+            launch_interp_decode_multi<T>(st, P.rounds[r].as<PassJob>(), P.round_n[r].first, P.round_n[r].second);
+        });
     if (!levels) return;
     jobs_launch<T>(c, kLowDec, P.low, P.cfg_low);
     jobs_launch<T>(c, kHighDec, P.high, P.cfg_high);
+}
+
+// upload a host vector into a pool buffer (staged copy on the context stream)
+template <typename V>
+DBuf upload_vec(stz_ctx* c, const std::vector<V>& v)
+{
+    const size_t bytes = v.size() * sizeof(V);
+    DBuf d(c->pool, bytes + 64);
+    if (bytes) {
+        void* hs = c->host_stage(bytes);
+        std::memcpy(hs, v.data(), bytes);
+        HIPCHK(hipMemcpyAsync(d.p, hs, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    return d;
+}
+
+// the decode launches of all fields: Huffman jobs and base-pass jobs
+template <typename T>
+void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
+{
+    const int n = (int)P.ds.size();
+    std::vector<DecJob> all, stj, unj;
+    std::vector<FusedJob> fj;
+    std::vector<std::vector<PassJob>> rounds;
+    uint32_t chunk = 0;
+    for (int i = 0; i < n; ++i) {
+        BatchPlan::Field& F = P.f[i];
+        stz_dstream* ds = P.ds[i];
+        DecJob j{};
+        j.ss = F.ss;
+        j.payload = ds->payload.as<uint8_t>();
+        j.tabs = ds->tabs.as<DecTab>();
+        j.sorted = ds->sorted.as<uint16_t>();
+        j.items = F.items.as<DecItem>();
+        j.codes = F.dd.codes.as<uint16_t>();
+        j.side = F.dd.side.p;
+        j.err = P.err.as<uint32_t>() + i;
+        j.chunk0 = chunk;
+        chunk += F.ss.total_chunks;
+        all.push_back(j);
+        if (F.ss.total_wgs) {
+            j.wg0 = F.stage ? P.wg_st : P.wg_un;
+            (F.stage ? P.wg_st : P.wg_un) += F.ss.total_wgs;
+            (F.stage ? stj : unj).push_back(j);
+        }
+        // base passes
+        const Meta& m = ds->meta;
+        const auto passes = make_passes(F.L.g[0].dx, F.L.g[0].dy, F.L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
+                                        (int)m.fh.order, m.fh.alpha, m.fh.beta, 0);
+        const uint16_t* c0 = F.dd.codes.as<uint16_t>() + F.dd.code_base[0];
+        const T* s0 = F.dd.side.as<T>() + F.dd.code_base[0];
+        size_t r = 0;
+        run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
+            if (fl.n <= 0) return;
+            FusedJob f{};
+            f.l = fl;
+            f.v = F.vb.p;
+            f.codes = c0;
+            f.side = s0;
+            fj.push_back(f);
+        }, [&](const InterpPass& p) {
+            if (!p.n) return;
+            if (rounds.size() <= r) rounds.resize(r + 1);
+            PassJob q{};
+            q.p = p;
+            q.v = F.vb.p;
+            q.codes = c0;
+            q.side = s0;
+            q.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, 1024);
+            rounds[r].push_back(q);
+            ++r;
+        });
+    }
+    P.chunks = chunk;
+    P.n_st = (uint32_t)stj.size();
+    P.n_un = (uint32_t)unj.size();
+    P.dec_all = upload_vec(c, all);
+    P.dec_st = upload_vec(c, stj);
+    P.dec_un = upload_vec(c, unj);
+    P.n_fused = (uint32_t)fj.size();
+    P.fused = upload_vec(c, fj);
+    for (auto& rj : rounds) {
+        uint32_t wg = 0;
+        for (PassJob& q : rj) {
+            q.wg0 = wg;
+            wg += q.nwg;
+        }
+        P.round_n.push_back({(uint32_t)rj.size(), wg});
+        P.rounds.push_back(upload_vec(c, rj));
+    }
 }
 
 template <typename T>
@@ -1491,6 +1586,7 @@ void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* cons
                 P->high.h[v].push_back(hi);
             }
         }
+        plan_decode_jobs<T>(c, *P);
         jobs_finish(c, P->low);
         jobs_finish(c, P->high);
         // eager run (also the inputs of the timing runs), then the geometry

@@ -4,6 +4,8 @@ All randomness comes from numpy PCG64 with the stated seed; spectra are
 applied in float64 with numpy's FFT, or torch's when a device is given
 (faster at 512^3 and above; the white noise is still drawn by numpy).
 """
+import os
+
 import numpy as np
 
 
@@ -55,13 +57,19 @@ def grf(shape, slope, seed, device=None):
         del F
         g = (g - g.mean()) / g.std()
         return g.cpu().numpy()
-    F = np.fft.rfftn(w)
+    try:  # multithreaded pocketfft when scipy is there (any length, no plan compilation)
+        import scipy.fft as sfft
+        workers = min(16, os.cpu_count() or 1)
+        F = sfft.rfftn(w, workers=workers)
+    except ImportError:
+        sfft = None
+        F = np.fft.rfftn(w)
     del w
     k = _kmag(shape)
     with np.errstate(divide="ignore"):
         amp = np.where(k > 0, k ** (slope / 2.0), 0.0)
     F *= amp
-    g = np.fft.irfftn(F, s=shape)
+    g = sfft.irfftn(F, s=shape, workers=workers) if sfft is not None else np.fft.irfftn(F, s=shape)
     g -= g.mean()
     g /= g.std()
     return g
