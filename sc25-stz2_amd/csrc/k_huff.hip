@@ -77,12 +77,11 @@ __global__ void k_zero_bits(StreamSet ss, uint8_t* __restrict__ payload)
 }
 
 // one workgroup per stream: nonzero histogram bins -> (symbol << 32 | freq), symbol order
-__global__ __launch_bounds__(1024) void k_hist_compact(const uint32_t* __restrict__ hist,
-                                                       uint64_t* __restrict__ out,
-                                                       uint32_t* __restrict__ counts)
+__device__ __forceinline__ void hist_compact_body(const uint32_t* __restrict__ hist, uint64_t* __restrict__ out,
+                                                  uint32_t* __restrict__ counts, int s)
 {
     __shared__ uint32_t wsum[16];
-    const int s = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
+    const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
     const uint32_t* h = hist + (size_t)s * 65536 + tid * 64;
     uint32_t v[64];
     uint32_t cnt = 0;
@@ -110,6 +109,13 @@ __global__ __launch_bounds__(1024) void k_hist_compact(const uint32_t* __restric
     if (tid == 0) counts[s] = wsum[15];
 }
 
+__global__ __launch_bounds__(1024) void k_hist_compact(const uint32_t* __restrict__ hist,
+                                                       uint64_t* __restrict__ out,
+                                                       uint32_t* __restrict__ counts)
+{
+    hist_compact_body(hist, out, counts, blockIdx.x);
+}
+
 __global__ void k_enc_scatter(const uint32_t* __restrict__ list, uint32_t n, uint32_t* __restrict__ enc)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,16 +125,13 @@ __global__ void k_enc_scatter(const uint32_t* __restrict__ list, uint32_t n, uin
 }
 
 // per chunk: bits and unpredictable count of each sub-chunk, in-chunk prefixes, chunk totals
-__global__ __launch_bounds__(256) void k_huff_len(StreamSet ss, const uint16_t* __restrict__ codes,
-                                                  const uint32_t* __restrict__ enc,
-                                                  uint8_t* __restrict__ payload,
-                                                  uint32_t* __restrict__ sub_pre_bits,
-                                                  uint32_t* __restrict__ sub_pre_un,
-                                                  uint64_t* __restrict__ chunk_tot_bits,
-                                                  uint64_t* __restrict__ chunk_tot_un)
+__device__ __forceinline__ void huff_len_body(const StreamSet& ss, const uint16_t* __restrict__ codes,
+                                              const uint32_t* __restrict__ enc, uint8_t* __restrict__ payload,
+                                              uint32_t* __restrict__ sub_pre_bits, uint32_t* __restrict__ sub_pre_un,
+                                              uint64_t* __restrict__ chunk_tot_bits,
+                                              uint64_t* __restrict__ chunk_tot_un, uint32_t gc)
 {
     __shared__ uint32_t lb[256], lu[256];
-    const uint32_t gc = blockIdx.x;
     const int s = find_stream_by_chunk(ss, gc);
     const StreamDesc& d = ss.s[s];
     const uint32_t c = gc - d.chunk_base;
@@ -182,13 +185,24 @@ __global__ __launch_bounds__(256) void k_huff_len(StreamSet ss, const uint16_t* 
     }
 }
 
+__global__ __launch_bounds__(256) void k_huff_len(StreamSet ss, const uint16_t* __restrict__ codes,
+                                                  const uint32_t* __restrict__ enc,
+                                                  uint8_t* __restrict__ payload,
+                                                  uint32_t* __restrict__ sub_pre_bits,
+                                                  uint32_t* __restrict__ sub_pre_un,
+                                                  uint64_t* __restrict__ chunk_tot_bits,
+                                                  uint64_t* __restrict__ chunk_tot_un)
+{
+    huff_len_body(ss, codes, enc, payload, sub_pre_bits, sub_pre_un, chunk_tot_bits, chunk_tot_un, blockIdx.x);
+}
+
 // per stream: exclusive scan of chunk totals -> marker table (chunk bit offsets)
-__global__ __launch_bounds__(1024) void k_huff_scan(StreamSet ss, const uint64_t* __restrict__ tb,
-                                                    const uint64_t* __restrict__ tu,
-                                                    uint8_t* __restrict__ payload)
+__device__ __forceinline__ void huff_scan_body(const StreamSet& ss, const uint64_t* __restrict__ tb,
+                                               const uint64_t* __restrict__ tu, uint8_t* __restrict__ payload,
+                                               int sidx)
 {
     __shared__ uint64_t wsb[16], wsu[16];
-    const StreamDesc& d = ss.s[blockIdx.x];
+    const StreamDesc& d = ss.s[sidx];
     const uint32_t n = d.n_chunks;
     const int tid = threadIdx.x, lane = __lane_id(), wid = tid >> 6;
     const uint32_t per = (n + 1023) / 1024;
@@ -224,17 +238,22 @@ __global__ __launch_bounds__(1024) void k_huff_scan(StreamSet ss, const uint64_t
     }
 }
 
+__global__ __launch_bounds__(1024) void k_huff_scan(StreamSet ss, const uint64_t* __restrict__ tb,
+                                                    const uint64_t* __restrict__ tu,
+                                                    uint8_t* __restrict__ payload)
+{
+    huff_scan_body(ss, tb, tu, payload, blockIdx.x);
+}
+
 // per chunk: wavefront bit-pack of each sub-chunk + its unpredictable values
-__global__ __launch_bounds__(256) void k_huff_pack(StreamSet ss, const uint16_t* __restrict__ codes,
-                                                   const uint8_t* __restrict__ side,
-                                                   const uint32_t* __restrict__ enc,
-                                                   const uint32_t* __restrict__ sub_pre_bits,
-                                                   const uint32_t* __restrict__ sub_pre_un,
-                                                   uint8_t* __restrict__ payload)
+__device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16_t* __restrict__ codes,
+                                               const uint8_t* __restrict__ side, const uint32_t* __restrict__ enc,
+                                               const uint32_t* __restrict__ sub_pre_bits,
+                                               const uint32_t* __restrict__ sub_pre_un,
+                                               uint8_t* __restrict__ payload, uint32_t gc)
 {
     constexpr int kWords = (31 + kSub * kLmax + 31) / 32 + 2;  // 772
     __shared__ uint32_t wbuf[4][kWords];
-    const uint32_t gc = blockIdx.x;
     const int s = find_stream_by_chunk(ss, gc);
     const StreamDesc& d = ss.s[s];
     const uint32_t c = gc - d.chunk_base;
@@ -342,6 +361,76 @@ __global__ __launch_bounds__(256) void k_huff_pack(StreamSet ss, const uint16_t*
             }
         }
         __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_huff_pack(StreamSet ss, const uint16_t* __restrict__ codes,
+                                                   const uint8_t* __restrict__ side,
+                                                   const uint32_t* __restrict__ enc,
+                                                   const uint32_t* __restrict__ sub_pre_bits,
+                                                   const uint32_t* __restrict__ sub_pre_un,
+                                                   uint8_t* __restrict__ payload)
+{
+    huff_pack_body(ss, codes, side, enc, sub_pre_bits, sub_pre_un, payload, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- many fields (encode)
+__device__ __forceinline__ uint32_t find_enc_job(const EncJob* __restrict__ jobs, uint32_t njobs, uint32_t gc)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].chunk0 <= gc) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(1024) void k_hist_compact_multi(const EncJob* __restrict__ jobs)
+{
+    const EncJob& J = jobs[blockIdx.x / kStreams];
+    hist_compact_body(J.hist, J.compact, J.counts, (int)(blockIdx.x % kStreams));
+}
+
+__global__ void k_enc_scatter_multi(const uint32_t* __restrict__ list, uint32_t n, const EncJob* __restrict__ jobs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = list[3 * i], key = list[3 * i + 1], val = list[3 * i + 2];
+    jobs[f].enc[(size_t)(key >> 16) * 65536 + (key & 0xFFFFu)] = val;
+}
+
+__global__ __launch_bounds__(256) void k_huff_len_multi(const EncJob* __restrict__ jobs, uint32_t njobs)
+{
+    const EncJob& J = jobs[find_enc_job(jobs, njobs, blockIdx.x)];
+    huff_len_body(J.ss, J.codes, J.enc, J.payload, J.sub_pre_bits, J.sub_pre_un, J.chunk_tot_bits, J.chunk_tot_un,
+                  blockIdx.x - J.chunk0);
+}
+
+__global__ __launch_bounds__(1024) void k_huff_scan_multi(const EncJob* __restrict__ jobs)
+{
+    const EncJob& J = jobs[blockIdx.x / kStreams];
+    huff_scan_body(J.ss, J.chunk_tot_bits, J.chunk_tot_un, J.payload, (int)(blockIdx.x % kStreams));
+}
+
+__global__ __launch_bounds__(256) void k_huff_pack_multi(const EncJob* __restrict__ jobs, uint32_t njobs)
+{
+    const EncJob& J = jobs[find_enc_job(jobs, njobs, blockIdx.x)];
+    huff_pack_body(J.ss, J.codes, (const uint8_t*)J.side, J.enc, J.sub_pre_bits, J.sub_pre_un, J.payload,
+                   blockIdx.x - J.chunk0);
+}
+
+// one workgroup per segment: src[off .. off+bytes) -> dst
+__global__ __launch_bounds__(256) void k_copy_segments(const uint8_t* __restrict__ src, const Segment* __restrict__ segs)
+{
+    const Segment g = segs[blockIdx.x];
+    const uint8_t* a = src + g.src_off;
+    uint8_t* b = (uint8_t*)g.dst;
+    if (((g.src_off | (uint64_t)(uintptr_t)b | g.bytes) & 3u) == 0) {
+        const uint32_t* a4 = (const uint32_t*)a;
+        uint32_t* b4 = (uint32_t*)b;
+        for (uint64_t k = threadIdx.x; k < g.bytes / 4; k += 256) b4[k] = a4[k];
+    } else {
+        for (uint64_t k = threadIdx.x; k < g.bytes; k += 256) b[k] = a[k];
     }
 }
 
@@ -708,6 +797,29 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
     const uint32_t n = list ? nlist : ss.total_chunks;
     if (!n) return;
     k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items, cmap);
+}
+
+void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs)
+{
+    if (njobs) k_hist_compact_multi<<<dim3(njobs * kStreams), dim3(1024), 0, s>>>(jobs);
+}
+
+void launch_enc_scatter_multi(hipStream_t s, const uint32_t* list, uint32_t n, const EncJob* jobs)
+{
+    if (n) k_enc_scatter_multi<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(list, n, jobs);
+}
+
+void launch_huff_encode_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, uint32_t total_chunks)
+{
+    if (!njobs || !total_chunks) return;
+    k_huff_len_multi<<<dim3(total_chunks), dim3(256), 0, s>>>(jobs, njobs);
+    k_huff_scan_multi<<<dim3(njobs * kStreams), dim3(1024), 0, s>>>(jobs);
+    k_huff_pack_multi<<<dim3(total_chunks), dim3(256), 0, s>>>(jobs, njobs);
+}
+
+void launch_copy_segments(hipStream_t s, const uint8_t* src, const Segment* segs, uint32_t n)
+{
+    if (n) k_copy_segments<<<dim3(n), dim3(256), 0, s>>>(src, segs);
 }
 
 void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage)

@@ -90,16 +90,16 @@ __device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t* gh, uint32_t q,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_encode(InterpPass p, T* __restrict__ v,
-                                                       uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                       uint32_t* __restrict__ hist)
+__device__ __forceinline__ void interp_encode_wg(const InterpPass& p, T* __restrict__ v, uint16_t* __restrict__ codes,
+                                                 T* __restrict__ side, uint32_t* __restrict__ hist, uint32_t wg,
+                                                 uint32_t nwg)
 {
     __shared__ uint32_t sh[kHW];
     for (int k = threadIdx.x; k < kHW; k += 256) sh[k] = 0u;
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    const uint64_t stride = (uint64_t)nwg * 256u;
     const uint64_t kend = (p.n + 255u) / 256u * 256u;  // every lane of a wave runs the same trip count
-    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < kend; k += stride) {
+    for (uint64_t k = wg * 256ull + threadIdx.x; k < kend; k += stride) {
         const bool ok = k < p.n;
         uint32_t q = 0;
         if (ok) {
@@ -123,6 +123,26 @@ __global__ __launch_bounds__(256) void k_interp_encode(InterpPass p, T* __restri
     __syncthreads();
     for (int k = threadIdx.x; k < kHW; k += 256)
         if (sh[k]) atomicAdd(&hist[kHLo + k], sh[k]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_encode(InterpPass p, T* __restrict__ v,
+                                                       uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                       uint32_t* __restrict__ hist)
+{
+    interp_encode_wg(p, v, codes, side, hist, blockIdx.x, gridDim.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp_encode_multi(const PassEncJob* __restrict__ jobs, uint32_t njobs)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (jobs[m].wg0 <= blockIdx.x) lo = m; else hi = m;
+    }
+    const PassEncJob& J = jobs[lo];
+    interp_encode_wg(J.p, (T*)J.v, J.codes, (T*)J.side, J.hist, blockIdx.x - J.wg0, J.nwg);
 }
 
 template <typename T>
@@ -164,9 +184,9 @@ __device__ __forceinline__ void lattice_io(const InterpPassList& l, T* __restric
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_interp_encode_fused(InterpPassList l, T* __restrict__ v,
-                                                              uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                              uint32_t* __restrict__ hist)
+__device__ __forceinline__ void interp_encode_fused_wg(const InterpPassList& l, T* __restrict__ v,
+                                                       uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                       uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t sh[kHW];
     __shared__ T L[kLatticeBytes / sizeof(T)];
@@ -201,6 +221,21 @@ __global__ __launch_bounds__(1024) void k_interp_encode_fused(InterpPassList l, 
     lattice_io(l, L, v, false);
     for (int k = threadIdx.x; k < kHW; k += 1024)
         if (sh[k]) atomicAdd(&hist[kHLo + k], sh[k]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_encode_fused(InterpPassList l, T* __restrict__ v,
+                                                              uint16_t* __restrict__ codes, T* __restrict__ side,
+                                                              uint32_t* __restrict__ hist)
+{
+    interp_encode_fused_wg(l, v, codes, side, hist);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_interp_encode_fused_multi(const FusedEncJob* __restrict__ jobs)
+{
+    const FusedEncJob& J = jobs[blockIdx.x];
+    interp_encode_fused_wg(J.l, (T*)J.v, J.codes, (T*)J.side, J.hist);
 }
 
 template <typename T>
@@ -318,6 +353,24 @@ void launch_interp_decode_multi(hipStream_t s, const PassJob* jobs, uint32_t njo
     k_interp_decode_multi<T><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
 }
 
+template <typename T>
+void launch_interp_encode_fused_multi(hipStream_t s, const FusedEncJob* jobs, uint32_t njobs)
+{
+    if (!njobs) return;
+    k_interp_encode_fused_multi<T><<<dim3(njobs), dim3(1024), 0, s>>>(jobs);
+}
+
+template <typename T>
+void launch_interp_encode_multi(hipStream_t s, const PassEncJob* jobs, uint32_t njobs, uint32_t wgs)
+{
+    if (!njobs || !wgs) return;
+    k_interp_encode_multi<T><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+}
+
+template void launch_interp_encode_fused_multi<float>(hipStream_t, const FusedEncJob*, uint32_t);
+template void launch_interp_encode_fused_multi<double>(hipStream_t, const FusedEncJob*, uint32_t);
+template void launch_interp_encode_multi<float>(hipStream_t, const PassEncJob*, uint32_t, uint32_t);
+template void launch_interp_encode_multi<double>(hipStream_t, const PassEncJob*, uint32_t, uint32_t);
 template void launch_interp_decode_fused_multi<float>(hipStream_t, const FusedJob*, uint32_t);
 template void launch_interp_decode_fused_multi<double>(hipStream_t, const FusedJob*, uint32_t);
 template void launch_interp_decode_multi<float>(hipStream_t, const PassJob*, uint32_t, uint32_t);

@@ -75,6 +75,33 @@ struct DecJob {
     uint32_t chunk0;   // first chunk of this field in the prep launch
 };
 void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks);
+// one field of a multi-field encode launch
+struct EncJob {
+    StreamSet ss;
+    const uint16_t* codes;
+    const void* side;
+    uint32_t* enc;             // dense per-stream encoder tables (15 x 65536)
+    uint8_t* payload;
+    uint32_t* sub_pre_bits;
+    uint32_t* sub_pre_un;
+    uint64_t* chunk_tot_bits;
+    uint64_t* chunk_tot_un;
+    const uint32_t* hist;      // 15 x 65536 histograms
+    uint64_t* compact;         // 15 x 65536 (symbol << 32 | freq)
+    uint32_t* counts;          // symbols used per stream
+    uint32_t chunk0;           // first chunk of this field in the per-chunk launches
+};
+void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs);
+// list: (field, stream << 16 | symbol, code | len << 24) triples
+void launch_enc_scatter_multi(hipStream_t s, const uint32_t* list, uint32_t n, const EncJob* jobs);
+// huff_len, huff_scan, huff_pack of every field (chunk prefix in chunk0)
+void launch_huff_encode_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, uint32_t total_chunks);
+struct Segment {
+    uint64_t src_off;
+    void* dst;
+    uint64_t bytes;
+};
+void launch_copy_segments(hipStream_t s, const uint8_t* src, const Segment* segs, uint32_t n);
 void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage);
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
@@ -230,6 +257,25 @@ struct PassJob {
     const void* side;
     uint32_t wg0, nwg;
 };
+struct FusedEncJob {
+    InterpPassList l;
+    void* v;
+    uint16_t* codes;
+    void* side;
+    uint32_t* hist;
+};
+struct PassEncJob {
+    InterpPass p;
+    void* v;
+    uint16_t* codes;
+    void* side;
+    uint32_t* hist;
+    uint32_t wg0, nwg;
+};
+template <typename T>
+void launch_interp_encode_fused_multi(hipStream_t s, const FusedEncJob* jobs, uint32_t njobs);
+template <typename T>
+void launch_interp_encode_multi(hipStream_t s, const PassEncJob* jobs, uint32_t njobs, uint32_t wgs);
 template <typename T>
 void launch_interp_decode_fused_multi(hipStream_t s, const FusedJob* jobs, uint32_t njobs);
 template <typename T>

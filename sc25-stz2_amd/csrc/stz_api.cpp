@@ -516,9 +516,9 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     HIPCHK(hipMemcpyAsync(fe.mm_host, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
 }
 
-// after a wait: level bounds, base level; the residual levels when `levels`
+// after a wait: level bounds, scratch, histograms cleared, base grid gathered
 template <typename T>
-void enc_phase1(stz_ctx* c, FieldEnc<T>& fe, bool levels)
+void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
 {
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -538,33 +538,6 @@ void enc_phase1(stz_ctx* c, FieldEnc<T>& fe, bool levels)
     K(c, "gather_low0", [&] {
         launch_gather_low0<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>());
     });
-    const auto passes = make_passes(lx, ly, lz, fe.ebl[0], fe.prm.interp, fe.prm.order, fe.prm.alpha, fe.prm.beta, 0);
-    uint16_t* c0 = fe.codes.template as<uint16_t>() + L.code_off[0];
-    T* s0 = fe.side.template as<T>() + L.code_off[0];
-    run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
-        K(c, "interp_encode_fused", [&] {
-            launch_interp_encode_fused<T>(st, fl, fe.vbase.template as<T>(), c0, s0, fe.hist.template as<uint32_t>());
-        });
-    }, [&](const InterpPass& p) {
-        K(c, "interp_encode", [&] {
-            launch_interp_encode<T>(st, p, fe.vbase.template as<T>(), c0, s0, fe.hist.template as<uint32_t>());
-        });
-    });
-    if (!levels) return;
-    // low level (F5:2180-2230) fused with the mid merge
-    const LevelArgs la = enc_level_args(fe, 1);
-    K(c, "low_encode", [&] {
-        launch_low_encode<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>(), L.g[1], la,
-                             fe.codes.template as<uint16_t>(), fe.side.template as<T>(),
-                             fe.hist.template as<uint32_t>() + 65536, fe.mid.template as<T>());
-    });
-    // high level (F5:2252-2296)
-    const LevelArgs ha = enc_level_args(fe, 2);
-    K(c, "high_encode", [&] {
-        launch_high_encode<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.mid.template as<T>(), L.g[8], ha,
-                              fe.codes.template as<uint16_t>(), fe.side.template as<T>(),
-                              fe.hist.template as<uint32_t>() + 8 * 65536);
-    });
 }
 
 template <typename T>
@@ -575,44 +548,6 @@ LevelArgs enc_level_args(const FieldEnc<T>& fe, int level)
     a.recip = 1.0 / fe.ebl[level];
     for (int b = 0; b < 7; ++b) a.code_off[b] = fe.L.code_off[(level == 1 ? 1 : 8) + b];
     return a;
-}
-
-// histograms -> host (F5:1271 / F5:1402 preprocess_encode_fast): requested, no wait
-template <typename T>
-void enc_phase2(stz_ctx* c, FieldEnc<T>& fe)
-{
-    hipStream_t st = c->stream;
-    fe.compact = DBuf(c->pool, (size_t)kStreams * 65536 * 8);
-    fe.counts = DBuf(c->pool, 64 * 4);
-    K(c, "hist_compact", [&] {
-        launch_hist_compact(st, fe.hist.template as<uint32_t>(), kStreams, fe.compact.template as<uint64_t>(),
-                            fe.counts.template as<uint32_t>());
-    });
-    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
-    fe.hs = (uint8_t*)c->host_stage(64 * 4 + (size_t)kStreams * kFirst * 8);
-    HIPCHK(hipMemcpyAsync(fe.hs, fe.counts.p, kStreams * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpy2DAsync(fe.hs + 256, kFirst * 8, fe.compact.p, 65536 * 8, kFirst * 8, kStreams,
-                            hipMemcpyDeviceToHost, st));
-}
-
-// after a wait: the rest of the symbol lists (waits when a stream uses more
-// than kFirst symbols), then codebooks and headers (host only)
-template <typename T>
-void enc_phase3_fetch(stz_ctx* c, FieldEnc<T>& fe)
-{
-    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
-    const uint32_t* hcounts = (const uint32_t*)fe.hs;
-    const uint64_t* hfirst = (const uint64_t*)(fe.hs + 256);
-    fe.lists.assign(kStreams, {});
-    for (int s = 0; s < kStreams; ++s) {
-        const uint32_t n = hcounts[s];
-        fe.lists[s].resize(n);
-        if (n <= kFirst) std::memcpy(fe.lists[s].data(), hfirst + (size_t)s * kFirst, n * 8);
-        else
-            HIPCHK(hipMemcpy(fe.lists[s].data(), fe.compact.template as<uint64_t>() + (size_t)s * 65536, (size_t)n * 8,
-                             hipMemcpyDeviceToHost));
-    }
-    (void)c;
 }
 
 template <typename T>
@@ -659,92 +594,20 @@ void enc_phase3_books(FieldEnc<T>& fe)
     fh.total_bytes = total;
 }
 
-// payload: zeroed, then host-written sections (headers, codebooks, chunk->sub
-// tables), then the marked encode (F5:1412 encode_marked_fast): lengths ->
-// chunk scan -> pack.  No wait; the staging regions live until the call ends.
 template <typename T>
-void enc_phase4(stz_ctx* c, FieldEnc<T>& fe)
-{
-    hipStream_t st = c->stream;
-    auto pool = c->pool;
-    stz_dstream* ds = fe.ds;
-    Meta& m = ds->meta;
-    const FileHeader& fh = m.fh;
-    const HostLayout& L = fe.L;
-    const uint64_t total = fh.total_bytes;
-    ds->payload = DBuf(pool, align_up(total, 4) + 64);
-    ds->bytes = total;
-    ds->pool = pool;
-    ds->device = c->device;
-    uint8_t* pay = ds->payload.as<uint8_t>();
-    K(c, "payload_zero", [&] { launch_fill_u32(st, (uint32_t*)pay, align_up(total, 4) / 4, 0u); });
-    size_t host_bytes = sizeof(FileHeader) + fe.enc_n * 8;
-    for (int s = 0; s < kStreams; ++s) host_bytes += m.sh[s].off_sub_bits;
-    uint8_t* stage = (uint8_t*)c->host_stage(host_bytes + 64);
-    std::memcpy(stage, &fh, sizeof(fh));
-    HIPCHK(hipMemcpyAsync(pay, stage, sizeof(fh), hipMemcpyHostToDevice, st));
-    size_t so = sizeof(fh);
-    for (int s = 0; s < kStreams; ++s) {
-        const StreamHeader& sh = m.sh[s];
-        uint8_t* b = stage + so;
-        std::memset(b, 0, sh.off_sub_bits);
-        std::memcpy(b, &sh, sizeof(sh));
-        uint32_t* cbw = (uint32_t*)(b + sh.off_codebook);
-        for (size_t i = 0; i < m.cb[s].sym.size(); ++i) cbw[i] = (uint32_t)m.cb[s].sym[i] | ((uint32_t)m.cb[s].len[i] << 16);
-        std::memcpy(b + sh.off_chunk_sub, L.chunk_sub[s].data(), 4 * L.chunk_sub[s].size());
-        HIPCHK(hipMemcpyAsync(pay + fh.stream_off[s], b, sh.off_sub_bits, hipMemcpyHostToDevice, st));
-        so += sh.off_sub_bits;
-    }
-    // encoder tables: (stream << 16 | symbol, code | len << 24) scattered into dense per-stream tables
-    uint32_t* encl = (uint32_t*)(stage + so);
-    {
-        size_t k = 0;
-        for (int s = 0; s < kStreams; ++s)
-            for (size_t i = 0; i < m.cb[s].sym.size(); ++i) {
-                encl[2 * k] = ((uint32_t)s << 16) | m.cb[s].sym[i];
-                encl[2 * k + 1] = m.cb[s].code[i] | ((uint32_t)m.cb[s].len[i] << 24);
-                k++;
-            }
-    }
-    DBuf encl_d(pool, fe.enc_n * 8 + 8), enc(pool, (size_t)kStreams * 65536 * 4);
-    HIPCHK(hipMemcpyAsync(encl_d.p, encl, fe.enc_n * 8, hipMemcpyHostToDevice, st));
-    K(c, "enc_scatter", [&] { launch_enc_scatter(st, encl_d.as<uint32_t>(), (uint32_t)fe.enc_n, enc.as<uint32_t>()); });
-    const StreamSet ss = make_streamset(L, m, fe.dtype);
-    DBuf pre_b(pool, (size_t)ss.total_sub * 4 + 16), pre_u(pool, (size_t)ss.total_sub * 4 + 16);
-    DBuf tot_b(pool, (size_t)ss.total_chunks * 8 + 16), tot_u(pool, (size_t)ss.total_chunks * 8 + 16);
-    K(c, "huff_len", [&] {
-        launch_huff_len(st, ss, fe.codes.template as<uint16_t>(), enc.as<uint32_t>(), pay, pre_b.as<uint32_t>(),
-                        pre_u.as<uint32_t>(), tot_b.as<uint64_t>(), tot_u.as<uint64_t>());
-    });
-    K(c, "huff_scan", [&] { launch_huff_scan(st, ss, tot_b.as<uint64_t>(), tot_u.as<uint64_t>(), pay); });
-    K(c, "huff_pack", [&] {
-        launch_huff_pack(st, ss, fe.codes.template as<uint16_t>(), fe.side.p, enc.as<uint32_t>(), pre_b.as<uint32_t>(),
-                         pre_u.as<uint32_t>(), pay);
-    });
-    ds->have_tabs = false;
-}
+LevelArgs enc_level_args(const FieldEnc<T>& fe, int level);
+
+template <typename T>
+void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
+                     int mode, const stz_params& prm, stz_dstream* const* out);
 
 template <typename T>
 void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
                    const stz_params& prm, stz_dstream* ds)
 {
-    FieldEnc<T> fe;
-    fe.dtype = dtype;
-    fe.mode = mode;
-    fe.eb = eb;
-    fe.prm = prm;
-    fe.nx = nx; fe.ny = ny; fe.nz = nz;
-    fe.ds = ds;
-    enc_phase0(c, fe, field);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    enc_phase1(c, fe, true);
-    enc_phase2(c, fe);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    enc_phase3_fetch(c, fe);
-    enc_phase3_books(fe);
-    enc_phase4(c, fe);
-    // the staging buffers feed async copies: finish before they can be reused
-    HIPCHK(hipStreamSynchronize(c->stream));
+    const void* f = field;
+    const size_t dims[3] = {nx, ny, nz};
+    compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, prm, &ds);
 }
 
 // ------------------------------------------------------------ decode helpers
@@ -1279,15 +1142,36 @@ stz_ctx::LaunchCfg level_cfg(stz_ctx* c, int mode, JobSet& js, R&& reset)
     return best;
 }
 
-template <typename T>
-void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
-                         int mode, const stz_params& prm, stz_dstream** out)
+// upload a host vector into a pool buffer (staged copy on the context stream)
+template <typename V>
+DBuf upload_vec(stz_ctx* c, const std::vector<V>& v)
 {
+    const size_t bytes = v.size() * sizeof(V);
+    DBuf d(c->pool, bytes + 64);
+    if (bytes) {
+        void* hs = c->host_stage(bytes);
+        std::memcpy(hs, v.data(), bytes);
+        HIPCHK(hipMemcpyAsync(d.p, hs, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    return d;
+}
+
+// Compress n fields of one dtype.  Phases over all fields, with one wait
+// between them; within a phase each step is one launch for every field: the
+// base-codec passes (coarse passes one workgroup per field, then one launch
+// per fine-pass round), the two residual levels (one field: the tiled
+// kernels; many: one persistent launch per level), the histogram compaction,
+// the header/codebook upload (one copy + one segment-scatter launch) and the
+// marked Huffman encode (lengths, chunk scan, pack).
+template <typename T>
+void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
+                     int mode, const stz_params& prm, stz_dstream* const* out)
+{
+    hipStream_t st = c->stream;
+    auto pool = c->pool;
     std::vector<FieldEnc<T>> fe(n);
-    std::vector<std::unique_ptr<stz_dstream>> dss(n);
     for (int i = 0; i < n; ++i) {
         if (!fields[i]) throw StzError(STZ_E_ARG, "null field");
-        dss[i].reset(new stz_dstream());
         FieldEnc<T>& f = fe[i];
         f.dtype = dtype;
         f.mode = mode;
@@ -1296,12 +1180,77 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
         f.nx = dims[3 * i];
         f.ny = dims[3 * i + 1];
         f.nz = dims[3 * i + 2];
-        f.ds = dss[i].get();
+        f.ds = out[i];
         enc_phase0(c, f, (const T*)fields[i]);
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n; ++i) enc_phase1(c, fe[i], false);
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) enc_phase1(c, fe[i]);
+    // base level: coarse passes of every field, then the fine-pass rounds
+    {
+        std::vector<FusedEncJob> fj;
+        std::vector<std::vector<PassEncJob>> rounds;
+        for (int i = 0; i < n; ++i) {
+            FieldEnc<T>& f = fe[i];
+            const HostLayout& L = f.L;
+            const auto passes = make_passes(L.g[0].dx, L.g[0].dy, L.g[0].dz, f.ebl[0], f.prm.interp, f.prm.order,
+                                            f.prm.alpha, f.prm.beta, 0);
+            uint16_t* c0 = f.codes.template as<uint16_t>() + L.code_off[0];
+            T* s0 = f.side.template as<T>() + L.code_off[0];
+            size_t r = 0;
+            run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
+                if (fl.n <= 0) return;
+                FusedEncJob j{};
+                j.l = fl;
+                j.v = f.vbase.p;
+                j.codes = c0;
+                j.side = s0;
+                j.hist = f.hist.template as<uint32_t>();
+                fj.push_back(j);
+            }, [&](const InterpPass& p) {
+                if (!p.n) return;
+                if (rounds.size() <= r) rounds.resize(r + 1);
+                PassEncJob j{};
+                j.p = p;
+                j.v = f.vbase.p;
+                j.codes = c0;
+                j.side = s0;
+                j.hist = f.hist.template as<uint32_t>();
+                j.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, 4096);
+                rounds[r].push_back(j);
+                ++r;
+            });
+        }
+        DBuf dfj = upload_vec(c, fj);
+        K(c, "interp_encode_fused", [&] { launch_interp_encode_fused_multi<T>(st, dfj.as<FusedEncJob>(), (uint32_t)fj.size()); });
+        for (auto& rj : rounds) {
+            uint32_t wg = 0;
+            for (PassEncJob& q : rj) {
+                q.wg0 = wg;
+                wg += q.nwg;
+            }
+            DBuf dr = upload_vec(c, rj);
+            K(c, "interp_encode", [&] { launch_interp_encode_multi<T>(st, dr.as<PassEncJob>(), (uint32_t)rj.size(), wg); });
+        }
+    }
+    // residual levels
     for (int level = 1; level <= 2; ++level) {
+        if (n == 1) {
+            FieldEnc<T>& f = fe[0];
+            const LevelArgs a = enc_level_args(f, level);
+            if (level == 1)
+                K(c, "low_encode", [&] {
+                    launch_low_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.vbase.template as<T>(),
+                                         f.L.g[1], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
+                                         f.hist.template as<uint32_t>() + 65536, f.mid.template as<T>());
+                });
+            else
+                K(c, "high_encode", [&] {
+                    launch_high_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.mid.template as<T>(),
+                                          f.L.g[8], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
+                                          f.hist.template as<uint32_t>() + 8 * 65536);
+                });
+            continue;
+        }
         const int md = level == 1 ? kLowEnc : kHighEnc;
         JobSet js;
         for (int v = 0; v < 2; ++v)
@@ -1323,32 +1272,154 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
         jobs_finish(c, js);
         const stz_ctx::LaunchCfg cfg = level_cfg<T>(c, md, js, [&] {
             for (int i = 0; i < n; ++i)
-                launch_fill_u32(c->stream, fe[i].hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536,
-                                7 * 65536, 0u);
+                launch_fill_u32(st, fe[i].hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, 7 * 65536, 0u);
         });
-        jobs_launch<T>(c, md, js, cfg);  // the pool hands the job tables out again only to later work
+        jobs_launch<T>(c, md, js, cfg);
     }
-    for (int i = 0; i < n; ++i) enc_phase2(c, fe[i]);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n; ++i) enc_phase3_fetch(c, fe[i]);
-    {   // codebooks and headers on host threads
+    // histograms -> host (F5:1271 / F5:1402 preprocess_encode_fast)
+    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
+    DBuf compact(pool, (size_t)n * kStreams * 65536 * 8), counts(pool, (size_t)n * 64 * 4);
+    {
+        std::vector<EncJob> hj(n);
+        for (int i = 0; i < n; ++i) {
+            hj[i].hist = fe[i].hist.template as<uint32_t>();
+            hj[i].compact = compact.as<uint64_t>() + (size_t)i * kStreams * 65536;
+            hj[i].counts = counts.as<uint32_t>() + (size_t)i * 64;
+        }
+        DBuf dh = upload_vec(c, hj);
+        K(c, "hist_compact", [&] { launch_hist_compact_multi(st, dh.as<EncJob>(), (uint32_t)n); });
+        uint8_t* hs = (uint8_t*)c->host_stage((size_t)n * 256 + (size_t)n * kStreams * kFirst * 8);
+        HIPCHK(hipMemcpyAsync(hs, counts.p, (size_t)n * 256, hipMemcpyDeviceToHost, st));
+        uint64_t* hf = (uint64_t*)(hs + (size_t)n * 256);
+        HIPCHK(hipMemcpy2DAsync(hf, kFirst * 8, compact.p, 65536 * 8, kFirst * 8, (size_t)n * kStreams,
+                                hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < n; ++i) {
+            FieldEnc<T>& f = fe[i];
+            const uint32_t* hc = (const uint32_t*)(hs + (size_t)i * 256);
+            f.lists.assign(kStreams, {});
+            for (int s2 = 0; s2 < kStreams; ++s2) {
+                const uint32_t m = hc[s2];
+                f.lists[s2].resize(m);
+                const uint64_t* src = hf + ((size_t)i * kStreams + s2) * kFirst;
+                if (m <= kFirst) std::memcpy(f.lists[s2].data(), src, (size_t)m * 8);
+                else
+                    HIPCHK(hipMemcpy(f.lists[s2].data(), compact.as<uint64_t>() + ((size_t)i * kStreams + s2) * 65536,
+                                     (size_t)m * 8, hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    // codebooks and headers on host threads
+    {
         const int nt = std::max(1, std::min(n, 8));
-        std::vector<std::thread> th;
-        std::vector<std::exception_ptr> errs(nt);
-        for (int t = 0; t < nt; ++t)
-            th.emplace_back([&, t] {
-                try {
-                    for (int i = t; i < n; i += nt) enc_phase3_books(fe[i]);
-                } catch (...) {
-                    errs[t] = std::current_exception();
-                }
-            });
-        for (auto& x : th) x.join();
-        for (auto& e : errs)
-            if (e) std::rethrow_exception(e);
+        if (nt == 1) {
+            enc_phase3_books(fe[0]);
+        } else {
+            std::vector<std::thread> th;
+            std::vector<std::exception_ptr> errs(nt);
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back([&, t] {
+                    try {
+                        for (int i = t; i < n; i += nt) enc_phase3_books(fe[i]);
+                    } catch (...) {
+                        errs[t] = std::current_exception();
+                    }
+                });
+            for (auto& x : th) x.join();
+            for (auto& e : errs)
+                if (e) std::rethrow_exception(e);
+        }
     }
-    for (int i = 0; i < n; ++i) enc_phase4(c, fe[i]);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // payloads: zeroed, host-written sections in one upload + scatter, encoder tables, marked encode
+    std::vector<DBuf> enc(n), pre_b(n), pre_u(n), tot_b(n), tot_u(n);
+    std::vector<EncJob> ej(n);
+    std::vector<Segment> segs;
+    std::vector<uint32_t> encl;
+    size_t img_bytes = 0;
+    for (int i = 0; i < n; ++i) {
+        const Meta& m = fe[i].ds->meta;
+        img_bytes += align_up(sizeof(FileHeader), 16);
+        for (int s2 = 0; s2 < kStreams; ++s2) img_bytes += align_up(m.sh[s2].off_sub_bits, 16);
+    }
+    uint8_t* img = (uint8_t*)c->host_stage(img_bytes + 64);
+    size_t io = 0;
+    uint32_t chunk0 = 0;
+    for (int i = 0; i < n; ++i) {
+        FieldEnc<T>& f = fe[i];
+        stz_dstream* ds = f.ds;
+        Meta& m = ds->meta;
+        const FileHeader& fh = m.fh;
+        const uint64_t total = fh.total_bytes;
+        ds->payload = DBuf(pool, align_up(total, 4) + 64);
+        ds->bytes = total;
+        ds->pool = pool;
+        ds->device = c->device;
+        ds->have_tabs = false;
+        uint8_t* pay = ds->payload.as<uint8_t>();
+        K(c, "payload_zero", [&] { launch_fill_u32(st, (uint32_t*)pay, align_up(total, 4) / 4, 0u); });
+        std::memcpy(img + io, &fh, sizeof(fh));
+        segs.push_back(Segment{io, pay, sizeof(fh)});
+        io += align_up(sizeof(FileHeader), 16);
+        for (int s2 = 0; s2 < kStreams; ++s2) {
+            const StreamHeader& sh = m.sh[s2];
+            uint8_t* b = img + io;
+            std::memset(b, 0, sh.off_sub_bits);
+            std::memcpy(b, &sh, sizeof(sh));
+            uint32_t* cbw = (uint32_t*)(b + sh.off_codebook);
+            for (size_t k = 0; k < m.cb[s2].sym.size(); ++k)
+                cbw[k] = (uint32_t)m.cb[s2].sym[k] | ((uint32_t)m.cb[s2].len[k] << 16);
+            std::memcpy(b + sh.off_chunk_sub, f.L.chunk_sub[s2].data(), 4 * f.L.chunk_sub[s2].size());
+            segs.push_back(Segment{io, pay + fh.stream_off[s2], sh.off_sub_bits});
+            io += align_up(sh.off_sub_bits, 16);
+            for (size_t k = 0; k < m.cb[s2].sym.size(); ++k) {
+                encl.push_back((uint32_t)i);
+                encl.push_back(((uint32_t)s2 << 16) | m.cb[s2].sym[k]);
+                encl.push_back(m.cb[s2].code[k] | ((uint32_t)m.cb[s2].len[k] << 24));
+            }
+        }
+        const StreamSet ss = make_streamset(f.L, m, f.dtype);
+        enc[i] = DBuf(pool, (size_t)kStreams * 65536 * 4);
+        pre_b[i] = DBuf(pool, (size_t)ss.total_sub * 4 + 16);
+        pre_u[i] = DBuf(pool, (size_t)ss.total_sub * 4 + 16);
+        tot_b[i] = DBuf(pool, (size_t)ss.total_chunks * 8 + 16);
+        tot_u[i] = DBuf(pool, (size_t)ss.total_chunks * 8 + 16);
+        EncJob& j = ej[i];
+        j.ss = ss;
+        j.codes = f.codes.template as<uint16_t>();
+        j.side = f.side.p;
+        j.enc = enc[i].as<uint32_t>();
+        j.payload = pay;
+        j.sub_pre_bits = pre_b[i].as<uint32_t>();
+        j.sub_pre_un = pre_u[i].as<uint32_t>();
+        j.chunk_tot_bits = tot_b[i].as<uint64_t>();
+        j.chunk_tot_un = tot_u[i].as<uint64_t>();
+        j.chunk0 = chunk0;
+        chunk0 += ss.total_chunks;
+    }
+    DBuf dimg(pool, img_bytes + 64);
+    HIPCHK(hipMemcpyAsync(dimg.p, img, img_bytes, hipMemcpyHostToDevice, st));
+    DBuf dsegs = upload_vec(c, segs);
+    K(c, "copy_segments", [&] { launch_copy_segments(st, dimg.as<uint8_t>(), dsegs.as<Segment>(), (uint32_t)segs.size()); });
+    DBuf dej = upload_vec(c, ej);
+    DBuf dencl = upload_vec(c, encl);
+    K(c, "enc_scatter", [&] { launch_enc_scatter_multi(st, dencl.as<uint32_t>(), (uint32_t)(encl.size() / 3), dej.as<EncJob>()); });
+    // marked encode (F5:1412 encode_marked_fast): lengths -> chunk scan -> pack
+    K(c, "huff_encode", [&] { launch_huff_encode_multi(st, dej.as<EncJob>(), (uint32_t)n, chunk0); });
+    // the staging buffers feed async copies: finish before they can be reused
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+template <typename T>
+void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
+                         int mode, const stz_params& prm, stz_dstream** out)
+{
+    std::vector<std::unique_ptr<stz_dstream>> dss(n);
+    std::vector<stz_dstream*> raw(n);
+    for (int i = 0; i < n; ++i) {
+        dss[i].reset(new stz_dstream());
+        raw[i] = dss[i].get();
+    }
+    compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data());
     for (int i = 0; i < n; ++i) out[i] = dss[i].release();
 }
 
@@ -1453,18 +1524,6 @@ void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
 }
 
 // upload a host vector into a pool buffer (staged copy on the context stream)
-template <typename V>
-DBuf upload_vec(stz_ctx* c, const std::vector<V>& v)
-{
-    const size_t bytes = v.size() * sizeof(V);
-    DBuf d(c->pool, bytes + 64);
-    if (bytes) {
-        void* hs = c->host_stage(bytes);
-        std::memcpy(hs, v.data(), bytes);
-        HIPCHK(hipMemcpyAsync(d.p, hs, bytes, hipMemcpyHostToDevice, c->stream));
-    }
-    return d;
-}
 
 // the decode launches of all fields: Huffman jobs and base-pass jobs
 template <typename T>
