@@ -398,6 +398,41 @@ def batch_bench(args, dev):
     t_single = time.perf_counter() - a
     nbytes = pts * 4
     ms = (tc + td) / steps * 1e3
+    # learned tuner on the same batch (one statistics pass + one MLP launch for
+    # all fields inside the compress call), and the exhaustive trial once
+    from stz_amd import tuner
+    tparams, _ = tuner.load_params()
+    tuner.set_model(ctx, tparams, min_confidence=0.0)
+    ld = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
+    for d in ld:
+        d.free()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    ld = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
+    t_lc = time.perf_counter() - a
+    louts = [torch.empty_like(f) for f in fs]
+    ctx.decompress_batch(ld, louts)
+    a = time.perf_counter()
+    ctx.decompress_batch(ld, louts)
+    t_ld = time.perf_counter() - a
+    lpay = sum(d.payload_bytes for d in ld)
+    l_ok = all(float((o.double() - f.double()).abs().max()) <= d.info.eb_high * (1 + 1e-12) +
+               2 * float(np.spacing(np.float32(f.abs().max().item()))) for o, f, d in zip(louts, fs, ld))
+    for d in ld:
+        d.free()
+    del louts
+    a = time.perf_counter()
+    td_ = ctx.compress_batch(fs, 1e-3, "rel", tuner=2)
+    t_trial = time.perf_counter() - a
+    tpay = sum(d.payload_bytes for d in td_)
+    for d in td_:
+        d.free()
+    learned = {"compress_ms": round(t_lc * 1e3, 3), "decompress_ms_graph": round(t_ld * 1e3, 3),
+               "value": round(2.0 * nbytes / (t_lc + t_ld) / 1e9, 2), "unit": "GB/s",
+               "payload_bytes": int(lpay), "cr_payload": round(nbytes / lpay, 3),
+               "cr_vs_plain": round(payload / lpay, 4), "error_bound_ok": bool(l_ok),
+               "trial_compress_ms": round(t_trial * 1e3, 1), "trial_payload_bytes": int(tpay),
+               "cr_learned_vs_trial": round(tpay / lpay, 5)}
     out = {"workload": "%d fp32 GRF fields, dims U{32..384}, slopes U[-4,-2] (seeds 1004+i), rel eb 1e-3" % len(fs),
            "points": int(pts), "bytes": int(nbytes), "payload_bytes": int(payload),
            "cr_payload": round(nbytes / payload, 3),
@@ -407,7 +442,7 @@ def batch_bench(args, dev):
            "one_call_per_field_ms": round(t_single * 1e3, 3),
            "speedup_vs_one_call_per_field": round(t_single * 1e3 / ms, 3),
            "first_step_ms_incl_autotune_and_capture": round(t_first * 1e3, 1),
-           "error_bound_ok": bool(err_ok), "autotune": ctx.autotune()}
+           "error_bound_ok": bool(err_ok), "learned": learned, "autotune": ctx.autotune()}
     log("batch", json.dumps(out))
     for d in ds:
         d.free()

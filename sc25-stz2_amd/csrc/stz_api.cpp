@@ -1038,6 +1038,55 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
     for (int i : full_reqs) decompress_full<T>(c, ds, outs[i]);
 }
 
+// ---------------------------------------------------------------- tuner
+// The configurations the tuner chooses among: base interpolator (2) x base
+// axis order (6) x level-eb scaling (4).  Scalings: F5's multipliers
+// (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
+// every level; F5's multipliers with the base codec's level bounds shrinking
+// by 1.5 per level up to 4x (SZ3-style).  In ABS / REL modes the largest
+// multiplier maps to the requested bound, so no choice changes it.
+struct TuneScale { double mb, ml, mh, alpha, beta; };
+const TuneScale kScales[4] = {
+    {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
+constexpr int kTuneConfigs = 48;
+
+void tune_config(int k, stz_params& p)
+{
+    p.interp = k / 24;
+    p.order = (k / 4) % 6;
+    const TuneScale& t = kScales[k % 4];
+    p.mult_base = t.mb;
+    p.mult_low = t.ml;
+    p.mult_high = t.mh;
+    p.alpha = t.alpha;
+    p.beta = t.beta;
+}
+
+// 16 features of one 64^3 sub-block from its k_block_stats sums, relative to
+// the absolute bound; logs in units of 8 octaves
+void block_features(const double* S, double eb, float* f)
+{
+    auto lg = [](double v) { return (float)(std::log2(v + 1.0 / 1024.0) / 8.0); };
+    const double n = S[0] > 0 ? S[0] : 1.0;
+    const double mean = S[1] / n, var = std::max(S[2] / n - mean * mean, 0.0);
+    f[0] = lg(std::sqrt(var) / eb);
+    f[1] = lg((S[5] - S[4]) / eb);
+    for (int a = 0; a < 3; ++a) {
+        f[2 + a] = lg(S[6 + a] / n / eb);
+        f[5 + a] = lg(S[9 + a] / n / eb);
+        f[8 + a] = lg(S[12 + a] / n / eb);
+    }
+    f[11] = (float)(S[0] > 0 ? S[15] / n : 0.0);
+    f[12] = (float)(S[0] > 0 ? S[16] / n : 0.0);
+    f[13] = lg(S[17] / n / eb);
+    const double m2 = S[2] / n, m4 = S[3] / n;
+    f[14] = (float)(std::log2(m4 / (m2 * m2 + 1e-300) + 1e-30) / 8.0);
+    const double lmax = std::max(S[6], std::max(S[7], S[8])), lmin = std::min(S[6], std::min(S[7], S[8]));
+    f[15] = (float)(std::log2((lmax + 1e-300) / (lmin + 1e-300)) / 8.0);
+    if (S[0] <= 0)
+        for (int k = 2; k < 16; ++k) f[k] = 0.f;
+}
+
 // ------------------------------------------------------------ batches
 // Several fields in one call.  Their residual levels run as one persistent
 // launch per level fed by a device work queue (k_level_multi); its launch
@@ -1156,6 +1205,65 @@ DBuf upload_vec(stz_ctx* c, const std::vector<V>& v)
     return d;
 }
 
+// The learned tuner's choice for every field of a batch: the sub-block
+// statistics of all fields, one MLP launch over all their blocks, then per
+// field the same weighted vote as a single-field call.
+template <typename T>
+void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe)
+{
+    if (!c->have_tuner) throw StzError(STZ_E_ARG, "no tuner model set (stz_tune_set_model)");
+    hipStream_t st = c->stream;
+    const size_t n = fe.size();
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t i = 0; i < n; ++i)
+        off[i + 1] = off[i] + ((fe[i].nx + 63) / 64) * ((fe[i].ny + 63) / 64) * ((fe[i].nz + 63) / 64);
+    const size_t nb = off[n];
+    std::vector<double> ebabs(n);
+    DBuf sums(c->pool, nb * kFeatSums * sizeof(double));
+    for (size_t i = 0; i < n; ++i) {
+        const FieldEnc<T>& f = fe[i];
+        ebabs[i] = f.mode == STZ_EB_REL ? f.eb * (f.mm_host[1] - f.mm_host[0]) : f.eb;
+        if (!(ebabs[i] > 0)) ebabs[i] = 1e-300;
+        K(c, "block_stats", [&] {
+            launch_block_stats<T>(st, f.din, (int)f.nx, (int)f.ny, (int)f.nz, ebabs[i],
+                                  sums.as<double>() + off[i] * kFeatSums);
+        });
+    }
+    double* hs = (double*)c->host_stage(nb * kFeatSums * sizeof(double));
+    HIPCHK(hipMemcpyAsync(hs, sums.p, nb * kFeatSums * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<float> X(nb * kMlpIn);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t b = off[i]; b < off[i + 1]; ++b) block_features(hs + b * kFeatSums, ebabs[i], &X[b * kMlpIn]);
+    DBuf x(c->pool, X.size() * 4), y(c->pool, nb * kMlpOut * 4);
+    float* hx = (float*)c->host_stage(X.size() * 4 + nb * kMlpOut * 4);
+    std::memcpy(hx, X.data(), X.size() * 4);
+    HIPCHK(hipMemcpyAsync(x.p, hx, X.size() * 4, hipMemcpyHostToDevice, st));
+    K(c, "mlp_forward", [&] { launch_mlp_forward(st, c->tuner_w.as<float>(), x.as<float>(), (uint32_t)nb, y.as<float>()); });
+    float* hy = hx + X.size();
+    HIPCHK(hipMemcpyAsync(hy, y.p, nb * kMlpOut * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) {
+        double wsum = 0;
+        for (size_t b = off[i]; b < off[i + 1]; ++b) wsum += hs[b * kFeatSums];
+        std::vector<double> lp(kMlpOut, 0.0);
+        for (size_t b = off[i]; b < off[i + 1]; ++b) {
+            const double w = wsum > 0 ? hs[b * kFeatSums] / wsum : 1.0 / (double)(off[i + 1] - off[i]);
+            const float* r = hy + b * kMlpOut;
+            double mx = r[0];
+            for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
+            double se = 0;
+            for (int k = 0; k < kMlpOut; ++k) se += std::exp(r[k] - mx);
+            const double lse = mx + std::log(se);
+            for (int k = 0; k < kMlpOut; ++k) lp[k] += w * (r[k] - lse);
+        }
+        int best = 0;
+        for (int k = 1; k < kMlpOut; ++k)
+            if (lp[k] > lp[best]) best = k;
+        tune_config(best, fe[i].prm);
+    }
+}
+
 // Compress n fields of one dtype.  Phases over all fields, with one wait
 // between them; within a phase each step is one launch for every field: the
 // base-codec passes (coarse passes one workgroup per field, then one launch
@@ -1184,6 +1292,7 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         enc_phase0(c, f, (const T*)fields[i]);
     }
     HIPCHK(hipStreamSynchronize(st));
+    if (prm.tuner == 1) learned_choice_batch<T>(c, fe);
     for (int i = 0; i < n; ++i) enc_phase1(c, fe[i]);
     // base level: coarse passes of every field, then the fine-pass rounds
     {
@@ -1419,7 +1528,26 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
         dss[i].reset(new stz_dstream());
         raw[i] = dss[i].get();
     }
-    compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data());
+    if (prm.tuner == 2) {
+        // exhaustive trial over the whole batch: every configuration, each
+        // field keeps its smallest payload
+        std::vector<std::unique_ptr<stz_dstream>> tmp(n);
+        std::vector<stz_dstream*> traw(n);
+        for (int k = 0; k < kTuneConfigs; ++k) {
+            stz_params pk = prm;
+            pk.tuner = 0;
+            tune_config(k, pk);
+            for (int i = 0; i < n; ++i) {
+                tmp[i].reset(new stz_dstream());
+                traw[i] = tmp[i].get();
+            }
+            compress_fields<T>(c, dtype, n, fields, dims, eb, mode, pk, traw.data());
+            for (int i = 0; i < n; ++i)
+                if (k == 0 || traw[i]->bytes < raw[i]->bytes) std::swap(dss[i], tmp[i]), raw[i] = dss[i].get();
+        }
+    } else {
+        compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data());
+    }
     for (int i = 0; i < n; ++i) out[i] = dss[i].release();
 }
 
@@ -1881,55 +2009,6 @@ stz_params defaults()
     return p;
 }
 
-// ---------------------------------------------------------------- tuner
-// The configurations the tuner chooses among: base interpolator (2) x base
-// axis order (6) x level-eb scaling (4).  Scalings: F5's multipliers
-// (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
-// every level; F5's multipliers with the base codec's level bounds shrinking
-// by 1.5 per level up to 4x (SZ3-style).  In ABS / REL modes the largest
-// multiplier maps to the requested bound, so no choice changes it.
-struct TuneScale { double mb, ml, mh, alpha, beta; };
-const TuneScale kScales[4] = {
-    {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
-constexpr int kTuneConfigs = 48;
-
-void tune_config(int k, stz_params& p)
-{
-    p.interp = k / 24;
-    p.order = (k / 4) % 6;
-    const TuneScale& t = kScales[k % 4];
-    p.mult_base = t.mb;
-    p.mult_low = t.ml;
-    p.mult_high = t.mh;
-    p.alpha = t.alpha;
-    p.beta = t.beta;
-}
-
-// 16 features of one 64^3 sub-block from its k_block_stats sums, relative to
-// the absolute bound; logs in units of 8 octaves
-void block_features(const double* S, double eb, float* f)
-{
-    auto lg = [](double v) { return (float)(std::log2(v + 1.0 / 1024.0) / 8.0); };
-    const double n = S[0] > 0 ? S[0] : 1.0;
-    const double mean = S[1] / n, var = std::max(S[2] / n - mean * mean, 0.0);
-    f[0] = lg(std::sqrt(var) / eb);
-    f[1] = lg((S[5] - S[4]) / eb);
-    for (int a = 0; a < 3; ++a) {
-        f[2 + a] = lg(S[6 + a] / n / eb);
-        f[5 + a] = lg(S[9 + a] / n / eb);
-        f[8 + a] = lg(S[12 + a] / n / eb);
-    }
-    f[11] = (float)(S[0] > 0 ? S[15] / n : 0.0);
-    f[12] = (float)(S[0] > 0 ? S[16] / n : 0.0);
-    f[13] = lg(S[17] / n / eb);
-    const double m2 = S[2] / n, m4 = S[3] / n;
-    f[14] = (float)(std::log2(m4 / (m2 * m2 + 1e-300) + 1e-30) / 8.0);
-    const double lmax = std::max(S[6], std::max(S[7], S[8])), lmin = std::min(S[6], std::min(S[7], S[8]));
-    f[15] = (float)(std::log2((lmax + 1e-300) / (lmin + 1e-300)) / 8.0);
-    if (S[0] <= 0)
-        for (int k = 2; k < 16; ++k) f[k] = 0.f;
-}
-
 // device view of a field (copied when it is host memory)
 struct FieldView {
     DBuf copy;
@@ -2037,6 +2116,7 @@ int tune_trial(stz_ctx* c, int dtype, const T* d, size_t nx, size_t ny, size_t n
     uint64_t bb = ~0ull;
     for (int k = 0; k < kTuneConfigs; ++k) {
         stz_params p = base;
+        p.tuner = 0;
         tune_config(k, p);
         stz_dstream tmp;
         compress_impl<T>(c, dtype, d, nx, ny, nz, eb, mode, p, &tmp);
@@ -2070,6 +2150,7 @@ void compress_tuned(stz_ctx* c, int dtype, const void* field, size_t nx, size_t 
         const int k = tune_learned<T>(c, d, nx, ny, nz, eb, mode, conf);
         if (conf >= c->tune_min_conf) {
             tune_config(k, p);
+            p.tuner = 0;
             compress_impl<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds);
             return;
         }
@@ -2225,7 +2306,7 @@ int stz_compress_batch(stz_ctx* c, stz_dtype dtype, int n, const void* const* fi
         mode < STZ_EB_ABS || mode > STZ_EB_REF)
         return STZ_E_ARG;
     const stz_params p = params ? *params : defaults();
-    if (p.tuner != 0) return STZ_E_ARG;
+    if (p.tuner < 0 || p.tuner > 2 || (p.tuner != 0 && mode == STZ_EB_REF)) return STZ_E_ARG;
     for (int i = 0; i < n; ++i) out[i] = nullptr;
     return guard(c, [&] {
         if (dtype == STZ_F32) compress_batch_impl<float>(c, dtype, n, fields, dims, eb, mode, p, out);

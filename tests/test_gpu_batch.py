@@ -90,3 +90,25 @@ def test_batch_argument_checks(stz, ctx):
     assert ctx.compress_batch([], 1e-3) == []
     d32.free()
     d64.free()
+
+
+def test_batch_learned_and_trial_choices(stz, ctx):
+    """tuner = 1 in a batch picks what single calls pick (same model, no
+    fallback); tuner = 2 keeps each field's smallest payload over the 48
+    configurations."""
+    import torch
+    from stz_amd import tuner
+    dev = torch.device("cuda:0")
+    fs = _fields(np.float32, dev, seed=40)[:4]
+    params, _ = tuner.load_params()
+    tuner.set_model(ctx, params, min_confidence=0.0)
+    batch = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
+    for b, f in zip(batch, fs):
+        single = ctx.compress_device(f, 1e-3, "rel", tuner=1)
+        assert np.array_equal(b.download(-1), single.download(-1))
+        single.free()
+    trial = ctx.compress_batch(fs, 1e-3, "rel", tuner=2)
+    for t, f in zip(trial, fs):
+        assert t.payload_bytes == tuner.trial_sizes(ctx, f, 1e-3).min()
+    for d in batch + trial:
+        d.free()
