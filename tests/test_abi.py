@@ -97,3 +97,23 @@ def test_plan_query_matches_golden(stz):
             for b in plans:
                 assert plans[b][0] == tuple(int(v) for v in table[b, 1:7])
                 assert len(plans[b][1]) == table[b, 7]
+
+
+def _cli():
+    return os.path.join(ROOT, "sc25-stz2_amd", "lib", "stz_cli")
+
+
+def test_cli_argument_errors(tmp_path):
+    import subprocess
+    if not os.path.exists(_cli()):
+        pytest.skip("stz_cli not built")
+    r = subprocess.run([_cli()], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    f = tmp_path / "t.f32"
+    np.zeros((4, 5, 6), np.float32).tofile(f)
+    r = subprocess.run([_cli(), "1e-3", str(f), "6", "5", "5", "-f"], capture_output=True, text=True)
+    assert r.returncode == 1 and "size mismatch" in r.stderr
+    r = subprocess.run([_cli(), "1e-3", str(f), "6", "5", "4", "-f", "1", "2"], capture_output=True, text=True)
+    assert r.returncode == 1 and "exactly six integers" in r.stderr
+    r = subprocess.run([_cli(), "1e-3", str(f), "6", "5", "4", "-f", "--trial"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--trial needs" in r.stderr
