@@ -58,6 +58,25 @@ def kernel_bytes(name, n, s, payload):
     return None
 
 
+TRAFFIC_FILE = "profiles/hbm_traffic_512cubed.json"
+
+
+def measured_traffic(name, n, s):
+    """HBM bytes per launch of kernel `name` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE on the 512^3 fp32 workload (calibrated against a 1 GiB copy), or
+    None for another workload or a kernel the profile does not cover."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if n != 512 or s != 4 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d["kernels"].get(name)
+    if k is None:
+        return None
+    return {"bytes_per_launch": int(k["bytes_per_launch"]), "fetch_factor": d["fetch_factor"],
+            "write_factor": d["write_factor"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +203,12 @@ def main():
         roof = {"kernel": dom_name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(dom_avg_ms, 4), "algorithmic_bytes": int(kb)}
+        tr = measured_traffic(dom_name, args.n, s)
+        if tr is not None:
+            roof["traffic"] = tr["bytes_per_launch"]
+            roof["traffic_over_algorithmic"] = round(tr["bytes_per_launch"] / kb, 3)
+            roof["traffic_source"] = TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE x %.3f + WRITE_SIZE x %.3f, " \
+                "factors from a 1 GiB device copy)" % (tr["fetch_factor"], tr["write_factor"])
     step_alg_bytes = 2.0 * (N * s + payload)
     whole = {"algorithmic_bytes_per_step": int(step_alg_bytes),
              "achieved_GBps": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
