@@ -133,7 +133,8 @@ int huff_lengths_dense(const uint64_t* freq, int nsym, int lmax, uint8_t* len)
 void build_dectab(const uint16_t* sym, const uint8_t* len, uint32_t n, DecTab& t, std::vector<uint16_t>& sorted)
 {
     std::memset(&t, 0, sizeof(t));
-    for (int k = 0; k < (1 << kLutBits); ++k) t.lut[k] = 0xFFFFFFFFu;
+    constexpr uint32_t kEsc = 0xFFFFFFFFu, kMask = (1u << kLutBits) - 1u;
+    std::vector<uint32_t> one(1u << kLutBits, kEsc);  // (len << 16) | sym of the window's first code
     uint32_t count[kLmax + 2] = {0};
     for (uint32_t i = 0; i < n; ++i) count[len[i]]++;
     uint32_t c = 0, idx = 0;
@@ -154,8 +155,29 @@ void build_dectab(const uint16_t* sym, const uint8_t* len, uint32_t n, DecTab& t
         sorted[fill[l]++] = sym[i];
         if (l <= kLutBits) {
             const uint32_t lo = cd << (kLutBits - l), hi = (cd + 1) << (kLutBits - l);
-            for (uint32_t k = lo; k < hi; ++k) t.lut[k] = ((uint32_t)l << 16) | sym[i];
+            for (uint32_t k = lo; k < hi; ++k) one[k] = ((uint32_t)l << 16) | sym[i];
         }
+    }
+    // chain the codes that fit in each window: the window shifted left by the
+    // bits already used (zeros below) looks up the next code, which is valid
+    // when its length fits the bits that remain
+    for (uint32_t w = 0; w < (1u << kLutBits); ++w) {
+        const uint32_t e1 = one[w];
+        if (e1 == kEsc) continue;  // n = 0
+        const uint32_t l1 = e1 >> 16, s1 = e1 & 0xFFFFu;
+        uint64_t e = s1;
+        uint32_t cnt = 1, used = l1;
+        while (s1 != 0 && cnt < 3 && used < (uint32_t)kLutBits) {
+            const uint32_t ek = one[(w << used) & kMask];
+            if (ek == kEsc) break;
+            const uint32_t lk = ek >> 16, sk = ek & 0xFFFFu;
+            if (used + lk > (uint32_t)kLutBits || sk == 0) break;
+            e |= (uint64_t)sk << (16 * cnt);
+            ++cnt;
+            used += lk;
+        }
+        e |= (uint64_t)cnt << kDecNShift | (uint64_t)l1 << kDecL1Shift | (uint64_t)used << kDecLtShift;
+        t.lut[w] = e;
     }
 }
 

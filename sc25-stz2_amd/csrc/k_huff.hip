@@ -6,12 +6,14 @@
 // decode_marked_transform / decode_marked_chunks_fast (F5:1536, F5:1872).
 // The tree itself is built on the host from the compacted histogram; all
 // per-symbol work runs here:
-//   histogram   fused into the quantize kernels (wave-aggregated LDS bins)
+//   histogram   fused into the quantize kernels (LDS bins; the centre code
+//               counted per wave with one ballot)
 //   encode      per-sub-chunk code-length sums -> per-stream chunk scan ->
 //               wavefront bit-pack (16 symbols per lane, LDS word assembly,
 //               atomics only on the two words a sub-chunk may share)
-//   decode      one lane per 1024-symbol sub-chunk, 12-bit LDS lookup table,
-//               canonical fallback for longer codes
+//   decode      one lane per 1024-symbol sub-chunk, 12-bit LDS lookup table
+//               giving up to three symbols per lookup, canonical fallback
+//               for longer codes
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -21,7 +23,6 @@ namespace stz {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kDecStageWords = 12288;  // 48 KiB decode stage
 
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
 {
@@ -491,16 +492,18 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
 
 // Sequential Huffman decode of one lane's sub-chunk.  Bits come from `W`
 // (the workgroup's LDS stage or global memory; word w of the stream is
-// W[w - wbase]).  Each refill word is fetched one refill ahead.
-// Codes leave in 64-byte bursts (32 codes held in registers) when the
-// destination is 64-byte aligned, in 8-byte groups otherwise.
-template <int kGroup>
+// W[w - wbase]); each refill word is fetched one refill ahead.  One table
+// lookup takes up to three symbols (the codes that lie wholly in the next
+// kLutBits bits).  Codes leave in 8-byte groups once the destination is
+// 8-byte aligned (`al` = element offset of co[0]); the first symbols up to
+// that alignment and the last ones go one at a time.
 __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
-                                               const DecItem& it, const uint32_t* __restrict__ lut,
+                                               const DecItem& it, const uint64_t* __restrict__ lut,
                                                const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
                                                const uint32_t* __restrict__ fi, const uint16_t* __restrict__ srt,
                                                uint16_t* __restrict__ co, uint8_t* __restrict__ so,
-                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred)
+                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
+                                               uint64_t al)
 {
     uint64_t wpos = (it.bit0 >> 5) - wbase;
     const int skip = (int)(it.bit0 & 31u);
@@ -512,8 +515,7 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     uint32_t nextw = wpos < nwords ? W[wpos] : 0u;
     uint64_t u = it.u0;
     const uint32_t cnt = it.cnt;
-    uint32_t i = 0;
-    auto step = [&]() -> uint32_t {
+    auto refill = [&]() {
         if (nb < 32) {
             buf |= (uint64_t)nextw << (32 - nb);
             bad |= wpos >= nwords ? 1u : 0u;
@@ -521,85 +523,96 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
             nb += 32;
             nextw = wpos < nwords ? W[wpos] : 0u;
         }
-        const uint32_t e = lut[(uint32_t)(buf >> (64 - kLutBits))];
+    };
+    // a code longer than the table window: canonical search on the top 32 bits
+    auto slow = [&](int& l) -> uint32_t {
+        const uint32_t win = (uint32_t)(buf >> 32);
+        for (l = kLutBits + 1; l <= kLmax; ++l)
+            if ((win >> (32 - l)) - fc[l] < cn[l]) break;
+        if (l > kLmax) {
+            bad = 1;
+            l = 1;
+            return 0u;
+        }
+        return srt[fi[l] + (win >> (32 - l)) - fc[l]];
+    };
+    // symbol 0 at stream position `pos` of the sub-chunk: the next raw value
+    auto unpred = [&](uint32_t pos) {
+        if (u < n_unpred) {
+            if (tsize == 4) ((uint32_t*)so)[pos] = ((const uint32_t*)un)[u];
+            else ((uint64_t*)so)[pos] = ((const uint64_t*)un)[u];
+        } else {
+            bad = 1;
+        }
+        u++;
+    };
+    auto single = [&](uint32_t pos) -> uint32_t {
+        refill();
+        const uint64_t e = lut[buf >> (64 - kLutBits)];
         uint32_t sym;
         int l;
-        if (e != 0xFFFFFFFFu) {
-            sym = e & 0xFFFFu;
-            l = (int)(e >> 16);
+        if ((e >> kDecNShift) & 3u) {
+            sym = (uint32_t)e & 0xFFFFu;
+            l = (int)((e >> kDecL1Shift) & 15u);
         } else {
-            const uint32_t win = (uint32_t)(buf >> 32);
-            for (l = kLutBits + 1; l <= kLmax; ++l)
-                if ((win >> (32 - l)) - fc[l] < cn[l]) break;
-            if (l > kLmax) {
-                bad = 1;
-                l = 1;
-                sym = 0;
-            } else {
-                sym = srt[fi[l] + (win >> (32 - l)) - fc[l]];
-            }
+            sym = slow(l);
         }
         buf <<= l;
         nb -= l;
-        if (sym == 0u) {
-            if (u < n_unpred) {
-                if (tsize == 4) ((uint32_t*)so)[i] = ((const uint32_t*)un)[u];
-                else ((uint64_t*)so)[i] = ((const uint64_t*)un)[u];
-            } else {
-                bad = 1;
-            }
-            u++;
-        }
+        if (sym == 0u) unpred(pos);
         return sym;
     };
-    if (kGroup == 32) {
-        // 32 codes -> 4 x 16-byte stores
-        for (; i + 32 <= cnt;) {
-            uint32_t r[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const uint32_t a = step();
-                ++i;
-                const uint32_t b = step();
-                ++i;
-                r[k] = a | (b << 16);
-            }
-            uint4* d = (uint4*)(co + i - 32);
-            d[0] = make_uint4(r[0], r[1], r[2], r[3]);
-            d[1] = make_uint4(r[4], r[5], r[6], r[7]);
-            d[2] = make_uint4(r[8], r[9], r[10], r[11]);
-            d[3] = make_uint4(r[12], r[13], r[14], r[15]);
+    uint32_t i = 0;
+    const uint32_t head = (uint32_t)((4u - (uint32_t)(al & 3u)) & 3u);
+    for (; i < head && i < cnt; ++i) co[i] = (uint16_t)single(i);
+    // co + i is 8-byte aligned; `lo` holds pc decoded codes not yet stored
+    uint64_t lo = 0;
+    uint32_t pc = 0;
+    while (i + pc + 3 <= cnt) {
+        refill();
+        const uint64_t e = lut[buf >> (64 - kLutBits)];
+        uint32_t n = (uint32_t)(e >> kDecNShift) & 3u;
+        int l;
+        uint64_t v;
+        if (n) {
+            l = (int)((e >> kDecLtShift) & 15u);
+            v = e & ((1ull << (16 * n)) - 1ull);
+        } else {
+            v = slow(l);
+            n = 1;
         }
-    } else if (kGroup == 4) {
-        for (; i + 4 <= cnt;) {
-            const uint32_t a = step();
-            ++i;
-            const uint32_t b = step();
-            ++i;
-            const uint32_t c = step();
-            ++i;
-            const uint32_t d = step();
-            ++i;
-            *(uint2*)(co + i - 4) = make_uint2(a | (b << 16), c | (d << 16));
+        if (v == 0ull) unpred(i + pc);  // symbol 0 comes alone
+        buf <<= l;
+        nb -= l;
+        const uint64_t spill = pc ? v >> (64 - 16 * pc) : 0ull;
+        lo |= v << (16 * pc);
+        pc += n;
+        if (pc >= 4) {
+            *(uint2*)(co + i) = make_uint2((uint32_t)lo, (uint32_t)(lo >> 32));
+            i += 4;
+            pc -= 4;
+            lo = spill;
         }
     }
-    for (; i < cnt; ++i) co[i] = (uint16_t)step();
+    for (uint32_t k = 0; k < pc; ++k) co[i + k] = (uint16_t)(lo >> (16 * k));
+    i += pc;
+    for (; i < cnt; ++i) co[i] = (uint16_t)single(i);
     return bad;
 }
 
 // One workgroup = `ipw` consecutive sub-chunks of one stream, one per lane.
 // Their bits form one contiguous range.  kStage: the workgroup first copies
-// that range into LDS with coalesced loads when it fits, so the lanes'
-// data-dependent refills are LDS reads (low-entropy payloads; ipw is chosen
-// per stream so the range fits).  Without the stage (dense payloads) the
-// refills read global memory, every workgroup decodes 256 sub-chunks, and the
-// smaller LDS footprint lets four times as many workgroups share a CU.
+// that range (up to kDecStageWords words of it) into LDS with coalesced
+// loads, so the lanes' data-dependent refills are LDS reads (low-entropy
+// payloads); lanes past the staged words read global memory.  Without the
+// stage (dense payloads) every refill reads global memory, and the smaller
+// LDS footprint lets more workgroups share a CU.
 template <bool kStage>
 __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_t* __restrict__ payload,
                                                const DecTab* __restrict__ tabs, const uint16_t* __restrict__ sorted,
                                                const DecItem* __restrict__ items, uint16_t* __restrict__ codes,
                                                uint8_t* __restrict__ side, uint32_t* __restrict__ err,
-                                               uint32_t wgid, uint32_t* lut, uint32_t* fc, uint32_t* cn,
+                                               uint32_t wgid, uint64_t* lut, uint32_t* fc, uint32_t* cn,
                                                uint32_t* fi, uint32_t* stage)
 {
     const int s = find_stream_by_wg(ss, wgid);
@@ -620,8 +633,11 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     // items of a stream are in increasing bit order, so the range is [first.bit0, last.bit0 + last.bits)
     const uint64_t w0 = items[first].bit0 >> 5;
     const uint64_t w1r = ((items[last].bit0 + items[last].bits + 31) >> 5) + 2;
-    const uint64_t w1 = w1r < d.bits_words ? w1r : d.bits_words;
-    const bool use_lds = kStage && w1 > w0 && (w1 - w0) <= (uint64_t)kDecStageWords;
+    const uint64_t w1a = w1r < d.bits_words ? w1r : d.bits_words;
+    // the stage holds the first kDecStageWords words of the range; lanes
+    // whose bits lie past it read global memory
+    const uint64_t w1 = w1a > w0 + kDecStageWords ? w0 + kDecStageWords : w1a;
+    const bool use_lds = kStage && w1 > w0;
     if (use_lds) {
         const uint64_t nw = w1 - w0;
         const uint64_t a0 = (w0 + 3) & ~3ull;  // 16-byte aligned body
@@ -649,13 +665,7 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     const uint32_t* W = in_stage ? stage : bw;
     const uint64_t wb = in_stage ? w0 : 0;
     const uint64_t nw = in_stage ? w1 - w0 : d.bits_words;
-    uint32_t bad;
-    if ((al & 31u) == 0)
-        bad = decode_run<32>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
-    else if ((al & 3u) == 0)
-        bad = decode_run<4>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
-    else
-        bad = decode_run<1>(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+    const uint32_t bad = decode_run(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
     if (bad) atomicOr(err, 1u);
 }
 
@@ -669,7 +679,7 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
                                                      uint8_t* __restrict__ side,
                                                      uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint64_t lut[1 << kLutBits];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage);
@@ -689,7 +699,7 @@ __device__ __forceinline__ uint32_t find_dec_job(const DecJob* __restrict__ jobs
 template <bool kStage>
 __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restrict__ jobs, uint32_t njobs)
 {
-    __shared__ uint32_t lut[1 << kLutBits];
+    __shared__ uint64_t lut[1 << kLutBits];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];

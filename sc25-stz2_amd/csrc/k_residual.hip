@@ -12,6 +12,9 @@
 // pass (F5:1096-1159 fused depreprocess+merge, F5:2424-2451).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "device_math.h"
 #include "kernels.h"
 
@@ -26,8 +29,15 @@ template <typename T> struct V2;
 template <> struct V2<float> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct V2<double> { typedef double type __attribute__((ext_vector_type(2))); };
 
+// index of the calling wave in its workgroup, as a wave-uniform (scalar) value
+// so that row-dependent addresses stay in scalar registers
+__device__ __forceinline__ int wave_row()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // blockIdx -> logical block so that each XCD (round-robin dispatch over 8)
-// walks a contiguous range of tiles and keeps its halo rows in its own L2
+// walks a contiguous range of work items and keeps its halo rows in its own L2
 __device__ __forceinline__ uint32_t xcd_block()
 {
     const uint32_t G = gridDim.x;
@@ -41,23 +51,18 @@ __device__ __forceinline__ void hist_put(uint32_t* sh, uint32_t* gh, uint32_t q,
     else atomicAdd(&gh[q], n);
 }
 
-// wave-aggregated histogram update: the two most frequent codes of the wave
-// go in with one atomic each, the rest per lane
+// Histogram update by every active lane of the wave (`valid` says which
+// lanes count a code).  The centre code (zero residual, by far the most
+// frequent) is counted with one ballot and a single atomic; the other codes
+// go in per lane.
 __device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t* gh, uint32_t q, bool valid)
 {
-    bool pending = valid;
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const unsigned long long act = __ballot(pending);
-        if (act == 0ull) break;
-        const int leader = __ffsll((long long)act) - 1;
-        const uint32_t lq = (uint32_t)__shfl((int)q, leader, 64);
-        const bool same = pending && q == lq;
-        const unsigned long long m = __ballot(same);
-        if (__lane_id() == leader) hist_put(sh, gh, lq, (uint32_t)__popcll(m));
-        if (same) pending = false;
-    }
-    if (pending) hist_put(sh, gh, q, 1u);
+    const bool centre = valid && q == (uint32_t)kRadius;
+    const unsigned long long m = __ballot(centre);
+    const unsigned long long act = __ballot(1);
+    if (m && __lane_id() == (unsigned)(__ffsll((long long)act) - 1))
+        atomicAdd(&sh[kRadius - kHLo], (uint32_t)__popcll(m));
+    if (valid && !centre) hist_put(sh, gh, q, 1u);
 }
 
 __device__ __forceinline__ void hist_flush(uint32_t (*sh)[kHW], int nstreams, uint32_t* gh)
@@ -71,45 +76,51 @@ __device__ __forceinline__ void hist_flush(uint32_t (*sh)[kHW], int nstreams, ui
 
 // ---------------------------------------------------------------- predictors
 
-// Interior predictions of all 7 blocks at x on the row `row` (the block grid
-// row of the point, so the point is row[x]); requires 1 <= x <= dx-3 and the
-// same on y and z.  The 34 neighbours sit on 12 rows whose base pointers are
-// wave-uniform (scalar), so every load is scalar base + lane offset + an
-// immediate.  Same expressions, in the same order, as predict<B>'s interior.
+// element k of the row at `base` relative to the lane's byte offset xo: a
+// wave-uniform base plus a 32-bit lane offset, so the load is addressed as
+// scalar base + vector offset + immediate with no 64-bit vector arithmetic
 template <typename T>
-__device__ __forceinline__ void predict7_interior(const T* __restrict__ row, int x, int sx, int sxy, T p[7])
+__device__ __forceinline__ T ldx(const T* __restrict__ base, uint32_t xo, int k)
 {
-    const T* R00 = row;
-    const T* Rm0 = row - sx;             // (dy, dz) = (-1, 0)
-    const T* R10 = row + sx;
-    const T* R20 = row + 2 * sx;
-    const T* R0m = row - sxy;            // (0, -1)
-    const T* R01 = row + sxy;
-    const T* R02 = row + 2 * sxy;
-    const T* Rmm = row - sxy - sx;       // (-1, -1)
-    const T* R11 = row + sxy + sx;
-    const T* R22 = row + 2 * sxy + 2 * sx;
-    const T* R2m = row - sxy + 2 * sx;   // (2, -1)
-    const T* Rm2 = row + 2 * sxy - sx;   // (-1, 2)
-    const T c = R00[x];
-    p[0] = cub(R00[x - 1], c, R00[x + 1], R00[x + 2]);
-    p[1] = cub(Rm0[x], c, R10[x], R20[x]);
-    p[3] = cub(R0m[x], c, R01[x], R02[x]);
+    return *(const T*)((const char*)base + xo + (ptrdiff_t)k * (ptrdiff_t)sizeof(T));
+}
+
+// Interior predictions of all 7 blocks of a point from an accessor:
+// at(dy, dz, k) is the reconstructed value at (x + k, y + dy, z + dz), with
+// dy, dz in -1..2 and k in -1..2 (requires 1 <= x <= dx-3 and the same on y
+// and z).  Same expressions, in the same order, as predict<B>'s interior;
+// the global-memory and LDS forms below both go through it.
+template <typename T, typename At>
+__device__ __forceinline__ void predict7_at(const At& at, T p[7])
+{
+    const T c = at(0, 0, 0);
+    p[0] = cub(at(0, 0, -1), c, at(0, 0, 1), at(0, 0, 2));
+    p[1] = cub(at(-1, 0, 0), c, at(1, 0, 0), at(2, 0, 0));
+    p[3] = cub(at(0, -1, 0), c, at(0, 1, 0), at(0, 2, 0));
     T p0, v0, p1, v1, p2, v2, p3, v3;
-    pv(Rm0[x - 1], c, R10[x + 1], R20[x + 2], p0, v0);
-    pv(Rm0[x + 2], R00[x + 1], R10[x], R20[x - 1], p1, v1);
+    pv(at(-1, 0, -1), c, at(1, 0, 1), at(2, 0, 2), p0, v0);
+    pv(at(-1, 0, 2), at(0, 0, 1), at(1, 0, 0), at(2, 0, -1), p1, v1);
     p[2] = blend2(p0, v0, p1, v1);
-    pv(R0m[x - 1], c, R01[x + 1], R02[x + 2], p0, v0);
-    pv(R0m[x + 2], R00[x + 1], R01[x], R02[x - 1], p1, v1);
+    pv(at(0, -1, -1), c, at(0, 1, 1), at(0, 2, 2), p0, v0);
+    pv(at(0, -1, 2), at(0, 0, 1), at(0, 1, 0), at(0, 2, -1), p1, v1);
     p[4] = blend2(p0, v0, p1, v1);
-    pv(Rmm[x], c, R11[x], R22[x], p0, v0);
-    pv(R2m[x], R10[x], R01[x], Rm2[x], p1, v1);
+    pv(at(-1, -1, 0), c, at(1, 1, 0), at(2, 2, 0), p0, v0);
+    pv(at(2, -1, 0), at(1, 0, 0), at(0, 1, 0), at(-1, 2, 0), p1, v1);
     p[5] = blend2(p0, v0, p1, v1);
-    pv(Rmm[x - 1], c, R11[x + 1], R22[x + 2], p0, v0);
-    pv(Rmm[x + 2], R00[x + 1], R11[x], R22[x - 1], p1, v1);
-    pv(R2m[x - 1], R10[x], R01[x + 1], Rm2[x + 2], p2, v2);
-    pv(Rm2[x - 1], R01[x], R10[x + 1], R2m[x + 2], p3, v3);
+    pv(at(-1, -1, -1), c, at(1, 1, 1), at(2, 2, 2), p0, v0);
+    pv(at(-1, -1, 2), at(0, 0, 1), at(1, 1, 0), at(2, 2, -1), p1, v1);
+    pv(at(2, -1, -1), at(1, 0, 0), at(0, 1, 1), at(-1, 2, 2), p2, v2);
+    pv(at(-1, 2, -1), at(0, 1, 0), at(1, 0, 1), at(2, -1, 2), p3, v3);
     p[6] = blend4(p0, v0, p1, v1, p2, v2, p3, v3);
+}
+
+// global-memory form: `row` = block grid row of the point (wave-uniform in
+// the tile kernels), xo = the point's byte offset in it
+template <typename T>
+__device__ __forceinline__ void predict7_interior(const T* __restrict__ row, uint32_t xo, int sx, int sxy, T p[7])
+{
+    auto at = [&](int dy, int dz, int k) -> T { return ldx(row + (ptrdiff_t)dz * sxy + (ptrdiff_t)dy * sx, xo, k); };
+    predict7_at<T>(at, p);
 }
 
 // all 7 predictions at (x,y,z): the interior form when `inner` (wave-uniform), else checked
@@ -118,7 +129,7 @@ __device__ __forceinline__ void predict7(bool inner, const T* __restrict__ r, in
                                          int dz, T p[7])
 {
     if (inner) {
-        predict7_interior(r + ((size_t)z * dy + y) * dx, x, dx, dx * dy, p);
+        predict7_interior(r + ((size_t)z * dy + y) * dx, (uint32_t)x * (uint32_t)sizeof(T), dx, dx * dy, p);
     } else {
         const int i = (z * dy + y) * dx + x;
         p[0] = predict<0>(r, i, x, y, z, dx, dy, dz);
@@ -131,12 +142,10 @@ __device__ __forceinline__ void predict7(bool inner, const T* __restrict__ r, in
     }
 }
 
-// Work split of a block grid: the main kernels cover x in [1, dx-3] in
-// 64 x 4 tiles (a wave is one row, so y and z are wave-uniform and rows away
-// from the y/z faces take the interior form); the edge kernels cover the
-// columns x in {0, dx-2, dx-1} (every x when dx < 4) with checked predictors.
+// Work split of a block grid: tiles cover x in [1, dx-3]; the edge columns
+// x in {0, dx-2, dx-1} (every x when dx < 4) take the checked predictors.
 struct EdgeMap {
-    int ncol;       // columns handled by the edge kernel
+    int ncol;       // edge columns
     int cols[4];
 };
 
@@ -160,6 +169,28 @@ __device__ __forceinline__ bool row_inner(int y, int z, int dy, int dz)
     return y >= 1 && y <= dy - 3 && z >= 1 && z <= dz - 3;
 }
 
+// the 2x2x2 cell r[k] (k = 4 kz + 2 ky + kx) at (X,Y,Z) of a grid with
+// strides (1, sy, sz): row base + 32-bit byte offset; vec = 2-wide stores
+template <typename T>
+__device__ __forceinline__ void store_cell(T* __restrict__ dst, size_t sy, size_t sz, int X, int Y, int Z,
+                                           const T r[8], bool vec)
+{
+    typedef typename V2<T>::type v2;
+    T* row = dst + (size_t)Z * sz + (size_t)Y * sy;
+    const uint32_t xo = (uint32_t)X * (uint32_t)sizeof(T);
+    if (vec) {
+        v2 w;
+        w.x = r[0]; w.y = r[1]; *(v2*)((char*)row + xo) = w;
+        w.x = r[2]; w.y = r[3]; *(v2*)((char*)(row + sy) + xo) = w;
+        w.x = r[4]; w.y = r[5]; *(v2*)((char*)(row + sz) + xo) = w;
+        w.x = r[6]; w.y = r[7]; *(v2*)((char*)(row + sz + sy) + xo) = w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            *(T*)((char*)(row + (k >> 2) * sz + ((k >> 1) & 1) * sy) + xo + (k & 1) * sizeof(T)) = r[k];
+    }
+}
+
 // ---------------------------------------------------------------- encode
 
 template <typename T>
@@ -169,68 +200,78 @@ __device__ __forceinline__ uint32_t enc1(T tgt, T p, const LevelArgs& a, int b, 
     const T d = tgt - p;
     T dec;
     const uint32_t q = quantize(d, a.eb, a.recip, dec);
-    codes[a.code_off[b] + pos] = (uint16_t)q;
+    // streams hold < 2^31 symbols, so the byte offset 2*pos fits 32 bits
+    *(uint16_t*)((char*)(codes + a.code_off[b]) + 2u * pos) = (uint16_t)q;
     if (!q) side[a.code_off[b] + pos] = d;
     rec = dec + p;  // decoded diff + prediction (F5:1381-1385 then F5:1077-1083)
     return q;
 }
 
-// high level: targets of the 2x2x2 cell of mid point (x,y,z) straight from the field
+// high level: targets of the 2x2x2 cell of mid point (x,y,z) straight from
+// the field, predictions p
 template <typename T>
-__device__ __forceinline__ void high_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ mid,
-                                               const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
-                                               T* __restrict__ side, int x, int y, int z, bool inner, uint32_t q[7])
+__device__ __forceinline__ void high_enc_fin(const T p[7], const T* __restrict__ full, int px, int py, const Geom& g,
+                                             const LevelArgs& a, uint16_t* __restrict__ codes, T* __restrict__ side,
+                                             int x, int y, int z, uint32_t q[7])
 {
     typedef typename V2<T>::type v2;
-    T p[7];
-    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
     const uint32_t pos = stream_pos(g, x, y, z);
     const size_t fy = (size_t)px, fz = (size_t)px * py;
-    const size_t f = ((size_t)(2 * z) * py + 2 * y) * px + 2 * x;
-    const v2 r00 = *(const v2*)(full + f);
-    const v2 r01 = *(const v2*)(full + f + fy);
-    const v2 r10 = *(const v2*)(full + f + fz);
-    const v2 r11 = *(const v2*)(full + f + fz + fy);
+    const T* f = full + ((size_t)(2 * z) * py + 2 * y) * px;  // row base (wave-uniform in tiles)
+    const uint32_t fo = (uint32_t)(2 * x) * (uint32_t)sizeof(T);
+    const v2 r00 = *(const v2*)((const char*)f + fo);
+    const v2 r01 = *(const v2*)((const char*)(f + fy) + fo);
+    const v2 r10 = *(const v2*)((const char*)(f + fz) + fo);
+    const v2 r11 = *(const v2*)((const char*)(f + fz + fy) + fo);
     const T tg[7] = {r00.y, r01.x, r01.y, r10.x, r10.y, r11.x, r11.y};
     T rec;
 #pragma unroll
     for (int b = 0; b < 7; ++b) q[b] = enc1(tg[b], p[b], a, b, pos, codes, side, rec);
 }
 
-// low level: targets from the field at stride 4, reconstruction written into the mid grid
+template <typename T>
+__device__ __forceinline__ void high_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ mid,
+                                               const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
+                                               T* __restrict__ side, int x, int y, int z, bool inner, uint32_t q[7])
+{
+    T p[7];
+    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
+    high_enc_fin(p, full, px, py, g, a, codes, side, x, y, z, q);
+}
+
+// low level: targets from the field at stride 4, predictions p, base value
+// `centre`; the reconstruction goes into the mid grid
+template <typename T>
+__device__ __forceinline__ void low_enc_fin(const T p[7], T centre, const T* __restrict__ full, int px, int py,
+                                            const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
+                                            T* __restrict__ side, T* __restrict__ mid, int x, int y, int z, bool vec,
+                                            uint32_t q[7])
+{
+    const int mx = 2 * g.dx, my = 2 * g.dy;
+    const uint32_t pos = stream_pos(g, x, y, z);
+    // low block k at (x,y,z) = mid[2z+pz][2y+py][2x+px] = full[4z+2pz][4y+2py][4x+2px]
+    const size_t fy = (size_t)2 * px, fz = (size_t)2 * px * py;
+    const T* f = full + ((size_t)(4 * z) * py + 4 * y) * px;  // row base (wave-uniform in tiles)
+    const uint32_t fo = (uint32_t)(4 * x) * (uint32_t)sizeof(T);
+    const T tg[7] = {ldx(f, fo, 2), ldx(f + fy, fo, 0), ldx(f + fy, fo, 2), ldx(f + fz, fo, 0),
+                     ldx(f + fz, fo, 2), ldx(f + fz + fy, fo, 0), ldx(f + fz + fy, fo, 2)};
+    T r[8];
+    r[0] = centre;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) q[b] = enc1(tg[b], p[b], a, b, pos, codes, side, r[b + 1]);
+    store_cell(mid, (size_t)mx, (size_t)mx * my, 2 * x, 2 * y, 2 * z, r, vec);
+}
+
 template <typename T>
 __device__ __forceinline__ void low_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ base,
                                               const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
                                               T* __restrict__ side, T* __restrict__ mid, int x, int y, int z,
                                               bool inner, bool vec, uint32_t q[7])
 {
-    typedef typename V2<T>::type v2;
-    const int lx = g.dx, ly = g.dy;
-    const int mx = 2 * lx, my = 2 * ly;
     T p[7];
-    predict7(inner, base, x, y, z, lx, ly, g.dz, p);
-    const uint32_t pos = stream_pos(g, x, y, z);
-    // low block k at (x,y,z) = mid[2z+pz][2y+py][2x+px] = full[4z+2pz][4y+2py][4x+2px]
-    const size_t fy = (size_t)2 * px, fz = (size_t)2 * px * py;
-    const size_t f = ((size_t)(4 * z) * py + 4 * y) * px + 4 * x;
-    const T tg[7] = {full[f + 2], full[f + fy], full[f + fy + 2], full[f + fz], full[f + fz + 2],
-                     full[f + fz + fy], full[f + fz + fy + 2]};
-    T r[8];
-    r[0] = base[((size_t)z * ly + y) * lx + x];
-#pragma unroll
-    for (int b = 0; b < 7; ++b) q[b] = enc1(tg[b], p[b], a, b, pos, codes, side, r[b + 1]);
-    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
-    const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
-    if (vec) {
-        v2 w;
-        w.x = r[0]; w.y = r[1]; *(v2*)(mid + m) = w;
-        w.x = r[2]; w.y = r[3]; *(v2*)(mid + m + my_) = w;
-        w.x = r[4]; w.y = r[5]; *(v2*)(mid + m + mz_) = w;
-        w.x = r[6]; w.y = r[7]; *(v2*)(mid + m + mz_ + my_) = w;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) mid[m + (k >> 2) * mz_ + ((k >> 1) & 1) * my_ + (k & 1)] = r[k];
-    }
+    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p);
+    const T c = ldx(base + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+    low_enc_fin(p, c, full, px, py, g, a, codes, side, mid, x, y, z, vec, q);
 }
 
 #define STZ_HIST_ADD7(q, ok)                                                    \
@@ -238,113 +279,30 @@ __device__ __forceinline__ void low_enc_point(const T* __restrict__ full, int px
         _Pragma("unroll") for (int b_ = 0; b_ < 7; ++b_) hist_add(sh[b_], hist + b_ * 65536, q[b_], ok); \
     } while (0)
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_high_encode(const T* __restrict__ full, int px, int py,
-                                                     const T* __restrict__ mid, Geom g, LevelArgs a,
-                                                     uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                     uint32_t* __restrict__ hist)
-{
-    __shared__ uint32_t sh[7][kHW];
-    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
-    __syncthreads();
-    const int mx = g.dx, my = g.dy, mz = g.dz;
-    const int nxi = mx - 3;  // interior columns 1 .. mx-3
-    const int ntx = (nxi + 63) >> 6, nty = (my + 3) >> 2;
-    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * mz : 0;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
-        const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
-        const bool ok = x <= mx - 3 && y < my;
-        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (ok) high_enc_point(full, px, py, mid, g, a, codes, side, x, y, z, row_inner(y, z, my, mz), q);
-        STZ_HIST_ADD7(q, ok);
-    }
-    hist_flush(sh, 7, hist);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_high_encode_edge(const T* __restrict__ full, int px, int py,
-                                                          const T* __restrict__ mid, Geom g, LevelArgs a,
-                                                          uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                          uint32_t* __restrict__ hist)
-{
-    __shared__ uint32_t sh[7][kHW];
-    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
-    __syncthreads();
-    const EdgeMap e = edge_map(g.dx);
-    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
-    const uint64_t kend = (n + 255) / 256 * 256;
-    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < kend; k += (uint64_t)gridDim.x * 256ull) {
-        const bool ok = k < n;
-        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (ok) {
-            const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
-            high_enc_point(full, px, py, mid, g, a, codes, side, e.cols[c], y, z, false, q);
-        }
-        STZ_HIST_ADD7(q, ok);
-    }
-    hist_flush(sh, 7, hist);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_low_encode(const T* __restrict__ full, int px, int py,
-                                                    const T* __restrict__ base, Geom g, LevelArgs a,
-                                                    uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                    uint32_t* __restrict__ hist, T* __restrict__ mid)
-{
-    __shared__ uint32_t sh[7][kHW];
-    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
-    __syncthreads();
-    const int lx = g.dx, ly = g.dy, lz = g.dz;
-    const int nxi = lx - 3;
-    const int ntx = (nxi + 63) >> 6, nty = (ly + 3) >> 2;
-    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * lz : 0;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
-        const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
-        const bool ok = x <= lx - 3 && y < ly;
-        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (ok) low_enc_point(full, px, py, base, g, a, codes, side, mid, x, y, z, row_inner(y, z, ly, lz), true, q);
-        STZ_HIST_ADD7(q, ok);
-    }
-    hist_flush(sh, 7, hist);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_low_encode_edge(const T* __restrict__ full, int px, int py,
-                                                         const T* __restrict__ base, Geom g, LevelArgs a,
-                                                         uint16_t* __restrict__ codes, T* __restrict__ side,
-                                                         uint32_t* __restrict__ hist, T* __restrict__ mid)
-{
-    __shared__ uint32_t sh[7][kHW];
-    for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
-    __syncthreads();
-    const EdgeMap e = edge_map(g.dx);
-    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
-    const uint64_t kend = (n + 255) / 256 * 256;
-    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < kend; k += (uint64_t)gridDim.x * 256ull) {
-        const bool ok = k < n;
-        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (ok) {
-            const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
-            low_enc_point(full, px, py, base, g, a, codes, side, mid, e.cols[c], y, z, false, false, q);
-        }
-        STZ_HIST_ADD7(q, ok);
-    }
-    hist_flush(sh, 7, hist);
-}
-
 // ---------------------------------------------------------------- decode
 
 template <typename T>
 __device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, const uint16_t* __restrict__ codes,
-                                  const T* __restrict__ side)
+                                  const T* __restrict__ side, const uint32_t* qpre)
 {
-    const uint32_t q = codes[a.code_off[b] + pos];
+    const uint32_t q = qpre ? qpre[b] : *(const uint16_t*)((const char*)(codes + a.code_off[b]) + 2u * pos);
     const T v = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
     return v + p;  // F5:1080 de_sub = deData + pred
+}
+
+template <typename T>
+__device__ __forceinline__ void low_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a,
+                                            const uint16_t* __restrict__ codes, const T* __restrict__ side,
+                                            T* __restrict__ mid, int x, int y, int z, bool vec,
+                                            const uint32_t* qpre = nullptr)
+{
+    const int mx = 2 * g.dx, my = 2 * g.dy;
+    const uint32_t pos = stream_pos(g, x, y, z);
+    T r[8];
+    r[0] = centre;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, codes, side, qpre);
+    store_cell(mid, (size_t)mx, (size_t)mx * my, 2 * x, 2 * y, 2 * z, r, vec);
 }
 
 template <typename T>
@@ -352,129 +310,341 @@ __device__ __forceinline__ void low_dec_point(const T* __restrict__ base, const 
                                               const uint16_t* __restrict__ codes, const T* __restrict__ side,
                                               T* __restrict__ mid, int x, int y, int z, bool inner, bool vec)
 {
-    typedef typename V2<T>::type v2;
-    const int lx = g.dx, ly = g.dy;
-    const int mx = 2 * lx, my = 2 * ly;
     T p[7];
-    predict7(inner, base, x, y, z, lx, ly, g.dz, p);
+    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p);
+    const T c = ldx(base + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+    low_dec_fin(p, c, g, a, codes, side, mid, x, y, z, vec);
+}
+
+// the 2x2x2 cell of mid point (x,y,z) into the caller's (nx,ny,nz) field
+// from predictions p and the mid value `centre`; vec = 2-wide stores (nx
+// even and the output 2-element aligned)
+template <typename T>
+__device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a,
+                                             const uint16_t* __restrict__ codes, const T* __restrict__ side,
+                                             T* __restrict__ out, int nx, int ny, int nz, int x, int y, int z,
+                                             bool vec, const uint32_t* qpre = nullptr)
+{
+    typedef typename V2<T>::type v2;
+    const int X = 2 * x, Y = 2 * y, Z = 2 * z;
+    if (X >= nx || Y >= ny || Z >= nz) return;  // cell entirely in the padding
     const uint32_t pos = stream_pos(g, x, y, z);
-    T r[8];
-    r[0] = base[((size_t)z * ly + y) * lx + x];
+    T v[8];
+    v[0] = centre;
 #pragma unroll
-    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, codes, side);
-    const size_t my_ = (size_t)mx, mz_ = (size_t)mx * my;
-    const size_t m = ((size_t)(2 * z) * my + 2 * y) * mx + 2 * x;
+    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, codes, side, qpre);
+    const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
+    T* o = out + (size_t)Z * oz + (size_t)Y * oy;  // row base (wave-uniform in tiles)
+    const uint32_t xo = (uint32_t)X * (uint32_t)sizeof(T);
     if (vec) {
         v2 w;
-        w.x = r[0]; w.y = r[1]; *(v2*)(mid + m) = w;
-        w.x = r[2]; w.y = r[3]; *(v2*)(mid + m + my_) = w;
-        w.x = r[4]; w.y = r[5]; *(v2*)(mid + m + mz_) = w;
-        w.x = r[6]; w.y = r[7]; *(v2*)(mid + m + mz_ + my_) = w;
+        w.x = v[0]; w.y = v[1]; *(v2*)((char*)o + xo) = w;
+        if (Y + 1 < ny) { w.x = v[2]; w.y = v[3]; *(v2*)((char*)(o + oy) + xo) = w; }
+        if (Z + 1 < nz) {
+            w.x = v[4]; w.y = v[5]; *(v2*)((char*)(o + oz) + xo) = w;
+            if (Y + 1 < ny) { w.x = v[6]; w.y = v[7]; *(v2*)((char*)(o + oz + oy) + xo) = w; }
+        }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) mid[m + (k >> 2) * mz_ + ((k >> 1) & 1) * my_ + (k & 1)] = r[k];
+        for (int k = 0; k < 8; ++k) {
+            const int kx = k & 1, ky = (k >> 1) & 1, kz = k >> 2;
+            if (X + kx < nx && Y + ky < ny && Z + kz < nz)
+                *(T*)((char*)(o + (size_t)kz * oz + (size_t)ky * oy) + xo + kx * sizeof(T)) = v[k];
+        }
     }
 }
 
-// the 2x2x2 cell of mid point (x,y,z) into the caller's (nx,ny,nz) field;
-// vec = 2-wide stores (nx even and the output 2-element aligned)
 template <typename T>
 __device__ __forceinline__ void high_dec_point(const T* __restrict__ mid, const Geom& g, const LevelArgs& a,
                                                const uint16_t* __restrict__ codes, const T* __restrict__ side,
                                                T* __restrict__ out, int nx, int ny, int nz, int x, int y, int z,
                                                bool inner, bool vec)
 {
-    typedef typename V2<T>::type v2;
-    const int X = 2 * x, Y = 2 * y, Z = 2 * z;
-    if (X >= nx || Y >= ny || Z >= nz) return;  // cell entirely in the padding
+    if (2 * x >= nx || 2 * y >= ny || 2 * z >= nz) return;
     T p[7];
     predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
-    const uint32_t pos = stream_pos(g, x, y, z);
-    T v[8];
-    v[0] = mid[((size_t)z * g.dy + y) * g.dx + x];
-#pragma unroll
-    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, codes, side);
-    const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
-    const size_t o = (size_t)Z * oz + (size_t)Y * oy + X;
-    if (vec) {
-        v2 w;
-        w.x = v[0]; w.y = v[1]; *(v2*)(out + o) = w;
-        if (Y + 1 < ny) { w.x = v[2]; w.y = v[3]; *(v2*)(out + o + oy) = w; }
-        if (Z + 1 < nz) {
-            w.x = v[4]; w.y = v[5]; *(v2*)(out + o + oz) = w;
-            if (Y + 1 < ny) { w.x = v[6]; w.y = v[7]; *(v2*)(out + o + oz + oy) = w; }
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int kx = k & 1, ky = (k >> 1) & 1, kz = k >> 2;
-            if (X + kx < nx && Y + ky < ny && Z + kz < nz) out[o + (size_t)kz * oz + (size_t)ky * oy + kx] = v[k];
-        }
-    }
+    const T c = ldx(mid + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+    high_dec_fin(p, c, g, a, codes, side, out, nx, ny, nz, x, y, z, vec);
 }
 
+// ---------------------------------------------------------------- z-march
+//
+// One launch per level and block grid.  Work items 0 .. nmarch-1 are column
+// items: 64 interior columns (x0 = 1 + 64 tx ..) x kZR rows (y0 = kZR ty ..)
+// over a segment of `seg` planes.  The workgroup walks the segment in z and
+// keeps the predictor grid's planes z-1 .. z+2 of its column (rows y0-1 ..
+// y0+kZR+1, columns x0-1 .. x0+65) in a 5-slot LDS ring: each step stages one
+// new plane (coalesced, fetched into registers a step ahead) behind a single
+// barrier, and every interior point reads its 34 neighbours from LDS.  Wave
+// w owns rows y0 + kZRW w .. y0 + kZRW w + kZRW-1, so y and z are
+// wave-uniform; rows and planes on the grid faces take the checked
+// predictors from global memory.  The items after nmarch hold 256 points
+// each of the edge columns (x = 0, dx-2, dx-1, or every x when dx < 4).  The
+// grid is persistent (one resident wave of workgroups), walking the items in
+// the XCD-aware order.
+constexpr int kZR = 8;
+constexpr int kZRW = kZR / 4;  // rows per wave
+static_assert(kZR % 4 == 0, "four waves share the rows of a column item");
+constexpr int kZRows = kZR + 3;
+constexpr int kZCols = 67;
+constexpr int kZPitch = 68;
+constexpr int kZSlots = 5;
+constexpr int kZPlane = kZRows * kZPitch;
+constexpr int kZLoads = (kZRows * kZCols + 255) / 256;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_low_decode(const T* __restrict__ base, Geom g, LevelArgs a,
-                                                    const uint16_t* __restrict__ codes,
-                                                    const T* __restrict__ side, T* __restrict__ mid)
+struct LevelK {
+    const T* full;        // encode: the padded field (targets)
+    int px, py;
+    const T* ref;         // predictor grid: mid (high level) or base (low level)
+    Geom g;               // block grid = predictor grid dims, stream chunking
+    LevelArgs a;
+    uint16_t* codes;      // encode: written; decode: read
+    T* side;
+    uint32_t* hist;       // encode: 7 x 65536 histogram bins
+    T* dst;               // low level: the mid grid; high decode: the field
+    int nx, ny, nz, vec;  // high decode output dims; 2-wide stores allowed
+    int seg;              // planes per column item
+};
+
+struct ZItems {
+    int ntx, nty;
+    uint64_t nmarch, total;
+};
+
+__host__ __device__ inline ZItems z_items(const Geom& g, int seg)
 {
-    const int lx = g.dx, ly = g.dy, lz = g.dz;
-    const int nxi = lx - 3;
-    const int ntx = (nxi + 63) >> 6, nty = (ly + 3) >> 2;
-    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * lz : 0;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
-        const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
-        if (x > lx - 3 || y >= ly) continue;
-        low_dec_point(base, g, a, codes, side, mid, x, y, z, row_inner(y, z, ly, lz), true);
-    }
+    ZItems Z;
+    const int nxi = g.dx - 3;
+    Z.ntx = nxi > 0 ? (nxi + 63) >> 6 : 0;
+    Z.nty = (g.dy + kZR - 1) / kZR;
+    const int nseg = (g.dz + seg - 1) / seg;
+    Z.nmarch = (uint64_t)Z.ntx * Z.nty * nseg;
+    const uint64_t ne = (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz;
+    Z.total = Z.nmarch + (ne + 255) / 256;
+    return Z;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_low_decode_edge(const T* __restrict__ base, Geom g, LevelArgs a,
-                                                         const uint16_t* __restrict__ codes,
-                                                         const T* __restrict__ side, T* __restrict__ mid)
-{
-    const EdgeMap e = edge_map(g.dx);
-    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
-    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256ull) {
-        const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
-        low_dec_point(base, g, a, codes, side, mid, e.cols[c], y, z, false, false);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_high_decode(const T* __restrict__ mid, Geom g, LevelArgs a,
-                                                     const uint16_t* __restrict__ codes,
-                                                     const T* __restrict__ side, T* __restrict__ out,
-                                                     int nx, int ny, int nz, int vec)
-{
-    const int mx = g.dx, my = g.dy, mz = g.dz;
-    const int nxi = mx - 3;
-    const int ntx = (nxi + 63) >> 6, nty = (my + 3) >> 2;
-    const uint64_t ntiles = nxi > 0 ? (uint64_t)ntx * nty * mz : 0;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-    for (uint64_t t = xcd_block(); t < ntiles; t += gridDim.x) {
-        const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty), z = (int)(t / ((uint64_t)ntx * nty));
-        const int x = 1 + tx * 64 + lane, y = ty * 4 + row;
-        if (x > mx - 3 || y >= my) continue;
-        high_dec_point(mid, g, a, codes, side, out, nx, ny, nz, x, y, z, row_inner(y, z, my, mz), vec != 0);
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_high_decode_edge(const T* __restrict__ mid, Geom g, LevelArgs a,
-                                                          const uint16_t* __restrict__ codes,
-                                                          const T* __restrict__ side, T* __restrict__ out,
-                                                          int nx, int ny, int nz)
+// edge item t (after nmarch): the lane's point; ok = it exists
+__device__ __forceinline__ void edge_point(const Geom& g, uint64_t first, uint64_t t, int& x, int& y, int& z,
+                                           bool& ok)
 {
     const EdgeMap e = edge_map(g.dx);
-    const uint64_t n = (uint64_t)e.ncol * g.dy * g.dz;
-    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256ull) {
-        const int c = (int)(k % e.ncol), y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
-        high_dec_point(mid, g, a, codes, side, out, nx, ny, nz, e.cols[c], y, z, false, false);
+    const uint64_t k = (t - first) * 256u + threadIdx.x;
+    ok = k < (uint64_t)e.ncol * g.dy * g.dz;
+    const int c = (int)(k % e.ncol);
+    y = (int)((k / e.ncol) % g.dy);
+    z = (int)(k / ((uint64_t)e.ncol * g.dy));
+    x = e.cols[c];
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+
+// one point through the level's operation: predictions p and centre value
+// given, or (checked form) evaluated from global memory
+template <typename T, int kMode>
+__device__ __forceinline__ void z_point(const LevelK<T>& K, const T p[7], T c, int x, int y, int z, bool vec,
+                                        uint32_t q[7], const uint32_t* qpre = nullptr)
+{
+    if constexpr (kMode == kHighEnc) high_enc_fin(p, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, x, y, z, q);
+    else if constexpr (kMode == kLowEnc)
+        low_enc_fin(p, c, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, q);
+    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, qpre);
+    else high_dec_fin(p, c, K.g, K.a, K.codes, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0, qpre);
+}
+
+template <typename T, int kMode>
+__device__ __forceinline__ void z_point_checked(const LevelK<T>& K, int x, int y, int z, uint32_t q[7])
+{
+    const Geom& g = K.g;
+    if constexpr (kMode == kHighDec) {
+        if (2 * x >= K.nx || 2 * y >= K.ny || 2 * z >= K.nz) return;
     }
+    T p[7];
+    predict7(false, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+    const T c = K.ref[((size_t)z * g.dy + y) * g.dx + x];
+    z_point<T, kMode>(K, p, c, x, y, z, false, q);
+}
+
+// Tile form of a level launch: work items 0 .. ntiles-1 are 64 x 4 tiles of
+// one plane (wave w owns row y0 + w, so y and z are wave-uniform; interior
+// rows read their 34 neighbours straight from global memory), the items
+// after them the edge columns as above.  No LDS besides the encode
+// histograms, so more workgroups share a CU than in the z-march form.
+__host__ __device__ inline void tile_items(const Geom& g, int& ntx, int& nty, uint64_t& ntiles, uint64_t& total)
+{
+    const int nxi = g.dx - 3;
+    ntx = nxi > 0 ? (nxi + 63) >> 6 : 0;
+    nty = (g.dy + 3) >> 2;
+    ntiles = (uint64_t)ntx * nty * g.dz;
+    const uint64_t ne = (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz;
+    total = ntiles + (ne + 255) / 256;
+}
+
+template <typename T, int kMode>
+__global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
+{
+    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
+    uint32_t* hist = K.hist;
+    if constexpr (kEnc) {
+        for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
+        __syncthreads();
+    }
+    const Geom& g = K.g;
+    int ntx, nty;
+    uint64_t ntiles, total;
+    tile_items(g, ntx, nty, ntiles, total);
+    const int lane = threadIdx.x & 63, row = wave_row();
+    for (uint64_t t = xcd_block(); t < total; t += gridDim.x) {
+        int x, y, z;
+        bool ok;
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (t < ntiles) {
+            const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty);
+            z = (int)(t / ((uint64_t)ntx * nty));
+            x = 1 + tx * 64 + lane;
+            y = ty * 4 + row;
+            ok = x <= g.dx - 3 && y < g.dy;
+            if (row_inner(y, z, g.dy, g.dz)) {
+                if (ok) {
+                    if constexpr (kMode == kHighDec) {
+                        if (2 * x < K.nx && 2 * y < K.ny && 2 * z < K.nz) {
+                            T p[7];
+                            predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+                            const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+                            z_point<T, kMode>(K, p, c, x, y, z, true, q);
+                        }
+                    } else {
+                        T p[7];
+                        predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+                        const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+                        z_point<T, kMode>(K, p, c, x, y, z, true, q);
+                    }
+                }
+            } else if (ok) {
+                z_point_checked<T, kMode>(K, x, y, z, q);
+            }
+        } else {
+            edge_point(g, ntiles, t, x, y, z, ok);
+            if (ok) z_point_checked<T, kMode>(K, x, y, z, q);
+        }
+        if constexpr (kEnc) STZ_HIST_ADD7(q, ok);
+    }
+    if constexpr (kEnc) hist_flush(sh, 7, hist);
+}
+
+template <typename T, int kMode>
+__global__ __launch_bounds__(256) void k_level_z(const LevelK<T> K)
+{
+    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
+    __shared__ T ring[kZSlots * kZPlane];
+    uint32_t* hist = K.hist;
+    if constexpr (kEnc) {
+        for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
+        __syncthreads();
+    }
+    const Geom& g = K.g;
+    const ZItems Z = z_items(g, K.seg);
+    const int lane = threadIdx.x & 63, w = wave_row();
+    const size_t sxy = (size_t)g.dx * g.dy;
+    for (uint64_t t = xcd_block(); t < Z.total; t += gridDim.x) {
+        if (t >= Z.nmarch) {
+            int x, y, z;
+            bool ok;
+            edge_point(g, Z.nmarch, t, x, y, z, ok);
+            uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (ok) z_point_checked<T, kMode>(K, x, y, z, q);
+            if constexpr (kEnc) STZ_HIST_ADD7(q, ok);
+            continue;
+        }
+        const int tx = (int)(t % Z.ntx), ty = (int)((t / Z.ntx) % Z.nty);
+        const int zb = (int)(t / ((uint64_t)Z.ntx * Z.nty)) * K.seg;
+        const int ze = zb + K.seg < g.dz ? zb + K.seg : g.dz;
+        const int x0 = 1 + 64 * tx, y0 = kZR * ty;
+        // this thread's share of a staged plane: element e = threadIdx + 256 k
+        // -> (row, column) of the ring, grid offset inside a plane (clamped;
+        // clamped entries are never read by an interior point)
+        int roff[kZLoads], loff[kZLoads];
+#pragma unroll
+        for (int k = 0; k < kZLoads; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            const int r = e / kZCols, c = e - r * kZCols;
+            const int gy = clampi(y0 - 1 + r, 0, g.dy - 1), gx = clampi(x0 - 1 + c, 0, g.dx - 1);
+            roff[k] = gy * g.dx + gx;
+            loff[k] = e < kZRows * kZCols ? r * kZPitch + c : -1;
+        }
+        auto fetch = [&](int pz, T v[kZLoads]) {
+            const T* P = K.ref + (size_t)clampi(pz, 0, g.dz - 1) * sxy;
+#pragma unroll
+            for (int k = 0; k < kZLoads; ++k) v[k] = loff[k] >= 0 ? P[roff[k]] : T(0);
+        };
+        auto put = [&](int pz, const T v[kZLoads]) {
+            T* S = ring + ((pz + kZSlots) % kZSlots) * kZPlane;
+#pragma unroll
+            for (int k = 0; k < kZLoads; ++k)
+                if (loff[k] >= 0) S[loff[k]] = v[k];
+        };
+        __syncthreads();  // the previous item's readers are done with the ring
+        // decode: the codes of a step's points are fetched a step ahead too
+        uint32_t cq[kEnc ? 1 : kZRW][7], cn[kEnc ? 1 : kZRW][7];
+        auto fetch_codes = [&](int zz) {
+#pragma unroll
+            for (int r = 0; r < kZRW; ++r) {
+                const int y = y0 + kZRW * w + r, x = x0 + lane;
+                if (zz < ze && y < g.dy && x <= g.dx - 3) {
+                    const uint32_t pos = stream_pos(g, x, y, zz);
+#pragma unroll
+                    for (int b = 0; b < 7; ++b)
+                        cn[r][b] = *(const uint16_t*)((const char*)(K.codes + K.a.code_off[b]) + 2u * pos);
+                }
+            }
+        };
+        T v[kZLoads];
+        for (int pz = zb - 1; pz <= zb + 1; ++pz) {
+            fetch(pz, v);
+            put(pz, v);
+        }
+        fetch(zb + 2, v);
+        if constexpr (!kEnc) fetch_codes(zb);
+        for (int z = zb; z < ze; ++z) {
+            put(z + 2, v);
+            if (z + 1 < ze) fetch(z + 3, v);
+            if constexpr (!kEnc) {
+#pragma unroll
+                for (int r = 0; r < kZRW; ++r)
+#pragma unroll
+                    for (int b = 0; b < 7; ++b) cq[r][b] = cn[r][b];
+                fetch_codes(z + 1);
+            }
+            __syncthreads();
+            const T* S[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) S[d] = ring + ((z - 1 + d + kZSlots) % kZSlots) * kZPlane;
+#pragma unroll
+            for (int r = 0; r < kZRW; ++r) {
+                const int y = y0 + kZRW * w + r;
+                if (y >= g.dy) continue;  // wave-uniform
+                const int x = x0 + lane;
+                const bool ok = x <= g.dx - 3;
+                uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+                if (row_inner(y, z, g.dy, g.dz)) {
+                    if (ok) {
+                        const int o = (y - y0 + 1) * kZPitch + lane + 1;
+                        auto at = [&](int dy, int dz, int k) -> T { return S[dz + 1][o + dy * kZPitch + k]; };
+                        T p[7];
+                        predict7_at<T>(at, p);
+                        z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, kEnc ? nullptr : cq[kEnc ? 0 : r]);
+                    }
+                } else if (ok) {
+                    z_point_checked<T, kMode>(K, x, y, z, q);
+                }
+                if constexpr (kEnc) STZ_HIST_ADD7(q, ok);
+            }
+        }
+    }
+    if constexpr (kEnc) hist_flush(sh, 7, hist);
 }
 
 // Random access reads codes through the chunk map: the element of block
@@ -589,6 +759,35 @@ __device__ __forceinline__ uint32_t find_level_job(const LevelJob* __restrict__ 
     return lo;
 }
 
+// one point of a level job: encode (codes, side values, histogram counts of
+// every active lane) or decode; `ok` = the lane owns a point
+template <typename T, int kMode>
+__device__ __forceinline__ void level_point(const LevelJob& J, uint32_t (*sh)[kHW], int x, int y, int z, bool ok,
+                                            bool inner, bool vec)
+{
+    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    const Geom& g = J.g;
+    if constexpr (kEnc) {
+        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (ok) {
+            if (kMode == kHighEnc)
+                high_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side, x, y, z,
+                               inner, q);
+            else
+                low_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side,
+                              (T*)J.mid, x, y, z, inner, vec, q);
+        }
+        uint32_t* hist = J.hist;
+        STZ_HIST_ADD7(q, ok);
+    } else if (ok) {
+        if (kMode == kHighDec)
+            high_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.out, J.nx, J.ny, J.nz, x, y, z,
+                           inner, vec && J.vec != 0);
+        else
+            low_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.mid, x, y, z, inner, vec);
+    }
+}
+
 template <typename T, int kMode, int kWaves>
 __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __restrict__ jobs, uint32_t njobs,
                                                              uint32_t total, uint32_t* __restrict__ counter)
@@ -601,11 +800,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __r
         for (int k = threadIdx.x; k < 7 * kHW; k += kThreads) sh[k / kHW][k % kHW] = 0u;
     }
     int cur = -1;
-    const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, row = wave_row();
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t it = item_s;
+        const uint32_t it = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if (it >= total) break;  // every wave of the workgroup sees the same item
         const int j = (int)find_level_job(jobs, njobs, it);
@@ -621,47 +820,19 @@ __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __r
         }
         const Geom g = J.g;
         const uint32_t local = it - J.item0;
-        int x, y, z;
-        bool ok, inner, vec;
         if (local < J.ntiles) {
+            // tile: y and z are wave-uniform
             const int nxi = g.dx - 3;
             const int ntx = (nxi + 63) >> 6, nty = (g.dy + kWaves - 1) / kWaves;
             const int tx = (int)(local % ntx), ty = (int)((local / ntx) % nty), tz = (int)(local / ((uint32_t)ntx * nty));
-            x = 1 + tx * 64 + lane;
-            y = ty * kWaves + row;
-            z = tz;
-            ok = x <= g.dx - 3 && y < g.dy;
-            inner = row_inner(y, z, g.dy, g.dz);
-            vec = true;
+            const int x = 1 + tx * 64 + lane, y = ty * kWaves + row;
+            level_point<T, kMode>(J, sh, x, y, tz, x <= g.dx - 3 && y < g.dy, row_inner(y, tz, g.dy, g.dz), true);
         } else {
             const EdgeMap e = edge_map(g.dx);
             const uint64_t k = (uint64_t)(local - J.ntiles) * kThreads + threadIdx.x;
-            ok = k < (uint64_t)e.ncol * g.dy * g.dz;
             const int c = (int)(k % e.ncol);
-            y = (int)((k / e.ncol) % g.dy);
-            z = (int)(k / ((uint64_t)e.ncol * g.dy));
-            x = e.cols[c];
-            inner = false;
-            vec = false;
-        }
-        if constexpr (kEnc) {
-            uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
-            if (ok) {
-                if (kMode == kHighEnc)
-                    high_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side, x, y,
-                                   z, inner, q);
-                else
-                    low_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side,
-                                  (T*)J.mid, x, y, z, inner, vec, q);
-            }
-            uint32_t* hist = J.hist;
-            STZ_HIST_ADD7(q, ok);
-        } else if (ok) {
-            if (kMode == kHighDec)
-                high_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.out, J.nx, J.ny, J.nz, x, y,
-                               z, inner, vec && J.vec != 0);
-            else
-                low_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.mid, x, y, z, inner, vec);
+            const int y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
+            level_point<T, kMode>(J, sh, e.cols[c], y, z, k < (uint64_t)e.ncol * g.dy * g.dz, false, false);
         }
     }
     if constexpr (kEnc) {
@@ -669,22 +840,78 @@ __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __r
     }
 }
 
-inline unsigned tile_grid(int dx, int dy, int dz)
+// Persistent grid of a level kernel: the workgroups that are resident at
+// once on the device (CUs x occupancy, asked of the runtime once per
+// kernel).
+inline unsigned resident_slots(const void* kernel)
 {
-    const int nxi = dx - 3 > 0 ? dx - 3 : 1;
-    const uint64_t nt = (uint64_t)((nxi + 63) / 64) * ((dy + 3) / 4) * dz;
-    uint64_t g = nt < 2048 ? nt : 2048;
-    if (g >= 64) g &= ~7ull;  // multiple of 8 for the XCD mapping
-    return (unsigned)(g ? g : 1);
+    static std::mutex mu;
+    static std::map<const void*, unsigned> slots;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = slots.find(kernel);
+    if (it == slots.end()) {
+        int dev = 0, ncu = 0, occ = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) ncu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, 256, 0) != hipSuccess || occ < 1) occ = 1;
+        it = slots.emplace(kernel, (unsigned)(ncu * occ)).first;
+    }
+    return it->second;
 }
 
-inline unsigned flat_grid(uint64_t n)
+// Plane segments so that the column items fill about one resident wave of
+// workgroups (at least 4 planes per segment to amortise the 3-plane halo),
+// then the grid: resident workgroups, at most one per item, a multiple of 8
+// for the XCD order.
+template <typename T, int kMode>
+void launch_level_tile(hipStream_t s, LevelK<T> K)
 {
-    uint64_t b = (n + 255) / 256;
-    return (unsigned)(b < 1 ? 1 : b > 8192 ? 8192 : b);
+    const unsigned slots = resident_slots((const void*)k_level_tile<T, kMode>);
+    int ntx, nty;
+    uint64_t ntiles, total;
+    tile_items(K.g, ntx, nty, ntiles, total);
+    uint64_t G = total < slots ? total : slots;
+    if (G >= 64) G &= ~7ull;
+    if (!G) return;
+    k_level_tile<T, kMode><<<dim3((unsigned)G), dim3(256), 0, s>>>(K);
 }
 
-inline uint64_t edge_points(const Geom& g) { return (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz; }
+template <typename T, int kMode>
+void launch_level_z(hipStream_t s, LevelK<T> K);
+
+// level launch form per mode: encode is VALU-bound and keeps the higher
+// occupancy of the tile form; decode is latency-bound and takes the z-march
+constexpr bool kZMarchEnc = false;
+constexpr bool kZMarchDec = true;
+
+template <typename T, int kMode>
+void launch_level(hipStream_t s, const LevelK<T>& K)
+{
+    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    if ((kEnc && kZMarchEnc) || (!kEnc && kZMarchDec)) launch_level_z<T, kMode>(s, K);
+    else launch_level_tile<T, kMode>(s, K);
+}
+
+template <typename T, int kMode>
+void launch_level_z(hipStream_t s, LevelK<T> K)
+{
+    const void* fn = (const void*)k_level_z<T, kMode>;
+    const unsigned slots = resident_slots(fn);
+    const Geom& g = K.g;
+    const int nxi = g.dx - 3;
+    const uint64_t ncol = (uint64_t)(nxi > 0 ? (nxi + 63) >> 6 : 0) * ((g.dy + kZR - 1) / kZR);
+    uint64_t nseg = ncol ? slots / ncol : 1;
+    if (nseg < 1) nseg = 1;
+    int seg = (int)((g.dz + nseg - 1) / nseg);
+    if (seg < 4) seg = 4;
+    if (seg > g.dz) seg = g.dz;
+    K.seg = seg;
+    const ZItems Z = z_items(g, seg);
+    uint64_t G = Z.total < slots ? Z.total : slots;
+    if (G >= 64) G &= ~7ull;
+    if (!G) return;
+    k_level_z<T, kMode><<<dim3((unsigned)G), dim3(256), 0, s>>>(K);
+}
 
 }  // namespace
 
@@ -693,11 +920,10 @@ void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, con
                        uint16_t* codes, T* side, uint32_t* hist, T* mid)
 {
     (void)pz;
-    if (g.dx >= 4)
-        k_low_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, base, g, a, codes,
-                                                                               side, hist, mid);
-    k_low_encode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(full, px, py, base, g, a, codes, side,
-                                                                             hist, mid);
+    LevelK<T> K{};
+    K.full = full; K.px = px; K.py = py; K.ref = base; K.g = g; K.a = a;
+    K.codes = codes; K.side = side; K.hist = hist; K.dst = mid;
+    launch_level<T, kLowEnc>(s, K);
 }
 
 template <typename T>
@@ -705,32 +931,30 @@ void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, co
                         uint16_t* codes, T* side, uint32_t* hist)
 {
     (void)pz;
-    if (g.dx >= 4)
-        k_high_encode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(full, px, py, mid, g, a, codes,
-                                                                                side, hist);
-    k_high_encode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(full, px, py, mid, g, a, codes,
-                                                                              side, hist);
+    LevelK<T> K{};
+    K.full = full; K.px = px; K.py = py; K.ref = mid; K.g = g; K.a = a;
+    K.codes = codes; K.side = side; K.hist = hist;
+    launch_level<T, kHighEnc>(s, K);
 }
 
 template <typename T>
 void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
                        T* mid)
 {
-    if (g.dx >= 4)
-        k_low_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(base, g, a, codes, side, mid);
-    k_low_decode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(base, g, a, codes, side, mid);
+    LevelK<T> K{};
+    K.ref = base; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = mid;
+    launch_level<T, kLowDec>(s, K);
 }
 
 template <typename T>
 void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
                         T* out, int nx, int ny, int nz)
 {
-    const int vec = ((nx & 1) == 0) && (((uintptr_t)out % (2 * sizeof(T))) == 0);
-    if (g.dx >= 4)
-        k_high_decode<T><<<dim3(tile_grid(g.dx, g.dy, g.dz)), dim3(256), 0, s>>>(mid, g, a, codes, side, out, nx,
-                                                                                ny, nz, vec);
-    k_high_decode_edge<T><<<dim3(flat_grid(edge_points(g))), dim3(256), 0, s>>>(mid, g, a, codes, side, out, nx,
-                                                                             ny, nz);
+    LevelK<T> K{};
+    K.ref = mid; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = out;
+    K.nx = nx; K.ny = ny; K.nz = nz;
+    K.vec = ((nx & 1) == 0) && (((uintptr_t)out % (2 * sizeof(T))) == 0);
+    launch_level<T, kHighDec>(s, K);
 }
 
 template <typename T>
@@ -760,7 +984,7 @@ uint32_t level_tiles(const Geom& g, int waves)
 
 uint32_t level_edge_groups(const Geom& g, int waves)
 {
-    const uint64_t n = edge_points(g);
+    const uint64_t n = (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz;
     return (uint32_t)((n + 64u * waves - 1) / (64u * waves));
 }
 

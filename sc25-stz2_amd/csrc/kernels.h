@@ -41,9 +41,20 @@ struct StreamSet {
     uint32_t pad_[3];
 };
 
+// Multi-symbol decode table entry for one kLutBits-bit window: the codes
+// that lie wholly inside the window, up to three, the first always; symbol 0
+// (unpredictable) only ever as the single symbol of an entry.
+//   bits  0-47  symbols 1..3, 16 bits each (unused slots zero)
+//   bits 48-49  n, the number of symbols; 0: the first code is longer than
+//               the window (canonical search)
+//   bits 50-53  length of the first code
+//   bits 54-57  total length of the n codes
+constexpr int kDecNShift = 48, kDecL1Shift = 50, kDecLtShift = 54;
+constexpr int kDecStageWords = 11264;  // 44 KiB LDS bitstream stage per decode workgroup
+
 // decode table of one stream (device memory)
 struct DecTab {
-    uint32_t lut[1 << kLutBits];       // (len << 16) | sym, or 0xFFFFFFFF
+    uint64_t lut[1 << kLutBits];       // multi-symbol entries (above)
     uint32_t first_code[kLmax + 2];
     uint32_t count[kLmax + 2];
     uint32_t first_index[kLmax + 2];

@@ -23,7 +23,7 @@ constexpr int kLmax = 24;           // Huffman length limit
 constexpr int kSub = 1024;          // symbols per decode sub-chunk
 constexpr int kBaseChunk = 32768;   // symbols per base-stream chunk
 constexpr int kStreams = 15;
-constexpr int kLutBits = 12;        // primary decode table
+constexpr int kLutBits = 12;        // decode table window (bits)
 
 struct FileHeader {
     uint32_t magic, version, dtype, eb_mode;
@@ -64,6 +64,7 @@ struct Geom {
     int chunk;  // C for residual streams, 0 for the base stream (linear chunks)
     int ncx, ncy, ncz;
     uint32_t n_chunks;
+    int cshift;  // log2(C) when C is a power of two, else -1
 };
 
 __host__ __device__ inline int imin_(int a, int b) { return a < b ? a : b; }
@@ -73,7 +74,12 @@ __host__ __device__ inline int imin_(int a, int b) { return a < b ? a : b; }
 __host__ __device__ inline uint32_t stream_pos(const Geom& g, int x, int y, int z)
 {
     const int C = g.chunk;
-    const int cx = x / C, cy = y / C, cz = z / C;
+    int cx, cy, cz;
+    if (g.cshift >= 0) {
+        cx = x >> g.cshift; cy = y >> g.cshift; cz = z >> g.cshift;
+    } else {
+        cx = x / C; cy = y / C; cz = z / C;
+    }
     const int wx = imin_(C, g.dx - cx * C), wy = imin_(C, g.dy - cy * C), wz = imin_(C, g.dz - cz * C);
     const uint32_t start = (uint32_t)cz * C * g.dx * g.dy + (uint32_t)cy * C * g.dx * wz +
                            (uint32_t)cx * C * wy * wz;
@@ -105,6 +111,9 @@ inline Geom make_geom(int dx, int dy, int dz, int chunk)
 {
     Geom g{};
     g.dx = dx; g.dy = dy; g.dz = dz; g.chunk = chunk;
+    g.cshift = -1;
+    for (int k = 0; k < 31 && chunk > 0; ++k)
+        if ((1 << k) == chunk) g.cshift = k;
     if (chunk > 0) {
         g.ncx = (dx + chunk - 1) / chunk;
         g.ncy = (dy + chunk - 1) / chunk;

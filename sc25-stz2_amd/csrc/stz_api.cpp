@@ -724,11 +724,10 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
         }
     }
     // Decode variant for the whole launch, by the symbol-weighted entropy of
-    // what it decodes: up to 2 bits per symbol the LDS-staged kernel, with
-    // sub-chunks per workgroup cut so their expected bits fit the 48 KiB
-    // stage; above, the unstaged kernel with 256 sub-chunks per workgroup
-    // (at 6.5 bits/symbol it decodes 3.4x faster than the staged one at the
-    // 32 sub-chunks per workgroup the stage would allow).
+    // what it decodes: up to 2 bits per symbol the LDS-staged kernel (256
+    // sub-chunks per workgroup, halved while a stream's expected bits with a
+    // 30% margin would not fit the 44 KiB stage); above, the unstaged kernel,
+    // whose smaller LDS footprint fits more workgroups per CU.
     double wbits = 0, wsym = 0;
     for (int s = 0; s < kStreams; ++s) {
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
@@ -741,7 +740,7 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
         d.ipw = 256;
         if (use_stage)
-            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > 12288.0 * 32.0) d.ipw >>= 1;
+            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
         d.item_base = item;
         d.wg_base = wg;
         const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
@@ -1612,7 +1611,7 @@ void plan_huff_setup(stz_ctx* c, stz_dstream* ds, BatchPlan::Field& F)
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
         d.ipw = 256;
         if (F.stage)
-            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > 12288.0 * 32.0) d.ipw >>= 1;
+            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
         d.item_base = item;
         d.wg_base = wg;
         const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
