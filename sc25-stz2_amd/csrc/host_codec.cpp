@@ -130,55 +130,24 @@ int huff_lengths_dense(const uint64_t* freq, int nsym, int lmax, uint8_t* len)
     return (int)leaves.size();
 }
 
-void build_dectab(const uint16_t* sym, const uint8_t* len, uint32_t n, DecTab& t, std::vector<uint16_t>& sorted)
+void build_dechead(const uint16_t* sym, const uint8_t* len, uint32_t n, DecHead& h, std::vector<uint16_t>& sorted)
 {
-    std::memset(&t, 0, sizeof(t));
-    constexpr uint32_t kEsc = 0xFFFFFFFFu, kMask = (1u << kLutBits) - 1u;
-    std::vector<uint32_t> one(1u << kLutBits, kEsc);  // (len << 16) | sym of the window's first code
+    std::memset(&h, 0, sizeof(h));
     uint32_t count[kLmax + 2] = {0};
     for (uint32_t i = 0; i < n; ++i) count[len[i]]++;
     uint32_t c = 0, idx = 0;
-    uint32_t next_code[kLmax + 2] = {0}, fill[kLmax + 2] = {0};
+    uint32_t fill[kLmax + 2] = {0};
     for (int l = 1; l <= kLmax; ++l) {
         c = (l == 1) ? 0u : (c + count[l - 1]) << 1;
-        t.first_code[l] = c;
-        next_code[l] = c;
-        t.count[l] = count[l];
-        t.first_index[l] = idx;
+        h.first_code[l] = c;
+        h.count[l] = count[l];
+        h.first_index[l] = idx;
         fill[l] = idx;
         idx += count[l];
     }
+    // symbols of one length get consecutive codes in symbol order
     sorted.assign(n, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        const int l = len[i];
-        const uint32_t cd = next_code[l]++;
-        sorted[fill[l]++] = sym[i];
-        if (l <= kLutBits) {
-            const uint32_t lo = cd << (kLutBits - l), hi = (cd + 1) << (kLutBits - l);
-            for (uint32_t k = lo; k < hi; ++k) one[k] = ((uint32_t)l << 16) | sym[i];
-        }
-    }
-    // chain the codes that fit in each window: the window shifted left by the
-    // bits already used (zeros below) looks up the next code, which is valid
-    // when its length fits the bits that remain
-    for (uint32_t w = 0; w < (1u << kLutBits); ++w) {
-        const uint32_t e1 = one[w];
-        if (e1 == kEsc) continue;  // n = 0
-        const uint32_t l1 = e1 >> 16, s1 = e1 & 0xFFFFu;
-        uint64_t e = s1;
-        uint32_t cnt = 1, used = l1;
-        while (s1 != 0 && cnt < 3 && used < (uint32_t)kLutBits) {
-            const uint32_t ek = one[(w << used) & kMask];
-            if (ek == kEsc) break;
-            const uint32_t lk = ek >> 16, sk = ek & 0xFFFFu;
-            if (used + lk > (uint32_t)kLutBits || sk == 0) break;
-            e |= (uint64_t)sk << (16 * cnt);
-            ++cnt;
-            used += lk;
-        }
-        e |= (uint64_t)cnt << kDecNShift | (uint64_t)l1 << kDecL1Shift | (uint64_t)used << kDecLtShift;
-        t.lut[w] = e;
-    }
+    for (uint32_t i = 0; i < n; ++i) sorted[fill[len[i]]++] = sym[i];
 }
 
 int interp_levels(int bx, int by, int bz)

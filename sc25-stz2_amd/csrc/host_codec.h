@@ -23,9 +23,9 @@ struct Codebook {
 void build_codebook(const uint64_t* sym_freq, uint32_t n, int lmax, Codebook& cb);
 // lengths for a dense histogram (for the stage entry point)
 int huff_lengths_dense(const uint64_t* freq, int nsym, int lmax, uint8_t* len);
-// canonical decode table from (symbol, length) pairs in symbol order
-void build_dectab(const uint16_t* sym, const uint8_t* len, uint32_t n, DecTab& t,
-                  std::vector<uint16_t>& sorted);
+// canonical decode description from (symbol, length) pairs in symbol order
+// (the table itself is filled on the device from it)
+void build_dechead(const uint16_t* sym, const uint8_t* len, uint32_t n, DecHead& h, std::vector<uint16_t>& sorted);
 
 // ---- base codec schedule ---------------------------------------------------
 int interp_levels(int bx, int by, int bz);

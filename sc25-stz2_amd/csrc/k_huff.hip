@@ -490,6 +490,55 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
     }
 }
 
+// Decode tables: one thread per kLutBits-bit window of one stream
+// (blockIdx.y).  Codes are found by the canonical test in increasing length
+// -- codes of length l are first_code[l] .. first_code[l] + count[l] - 1 --
+// which names the unique code that prefixes the window.  Up to three codes
+// are chained while they fit in the window; symbol 0 only ever stands alone.
+__global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict__ heads,
+                                                       const uint16_t* __restrict__ sorted, DecTab* __restrict__ tabs)
+{
+    const int s = blockIdx.y;
+    const DecHead& h = heads[s];
+    DecTab& t = tabs[s];
+    if (blockIdx.x == 0 && threadIdx.x < kLmax + 2) {
+        t.first_code[threadIdx.x] = h.first_code[threadIdx.x];
+        t.count[threadIdx.x] = h.count[threadIdx.x];
+        t.first_index[threadIdx.x] = h.first_index[threadIdx.x];
+        if (threadIdx.x == 0) t.sorted_off = h.sorted_off;
+    }
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= (1u << kLutBits)) return;
+    const uint16_t* srt = sorted + h.sorted_off;
+    // first code of the `avail` leading bits of the kLutBits-bit window `win`
+    auto code_at = [&](uint32_t win, uint32_t avail, uint32_t& sym, uint32_t& len) -> bool {
+        for (uint32_t l = 1; l <= avail; ++l) {
+            const uint32_t cd = win >> (kLutBits - l);
+            if (cd - h.first_code[l] < h.count[l]) {
+                sym = srt[h.first_index[l] + cd - h.first_code[l]];
+                len = l;
+                return true;
+            }
+        }
+        return false;
+    };
+    uint32_t s1, l1;
+    if (!code_at(w, kLutBits, s1, l1)) {
+        t.lut[w] = 0ull;  // n = 0: longer than the window
+        return;
+    }
+    uint64_t e = s1;
+    uint32_t cnt = 1, used = l1;
+    while (s1 != 0 && cnt < 3 && used < (uint32_t)kLutBits) {
+        uint32_t sk, lk;
+        if (!code_at((w << used) & ((1u << kLutBits) - 1u), kLutBits - used, sk, lk) || sk == 0) break;
+        e |= (uint64_t)sk << (16 * cnt);
+        ++cnt;
+        used += lk;
+    }
+    t.lut[w] = e | (uint64_t)cnt << kDecNShift | (uint64_t)l1 << kDecL1Shift | (uint64_t)used << kDecLtShift;
+}
+
 // Sequential Huffman decode of one lane's sub-chunk.  Bits come from `W`
 // (the workgroup's LDS stage or global memory; word w of the stream is
 // W[w - wbase]); each refill word is fetched one refill ahead.  One table
@@ -843,6 +892,12 @@ void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, ui
 {
     if (!njobs || !total_chunks) return;
     k_dec_prep_multi<<<dim3((total_chunks + 255) / 256), dim3(256), 0, s>>>(jobs, njobs, total_chunks);
+}
+
+void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams)
+{
+    if (nstreams > 0)
+        k_build_dectabs<<<dim3((1u << kLutBits) / 256u, (unsigned)nstreams), dim3(256), 0, s>>>(heads, sorted, tabs);
 }
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,

@@ -52,6 +52,17 @@ struct StreamSet {
 constexpr int kDecNShift = 48, kDecL1Shift = 50, kDecLtShift = 54;
 constexpr int kDecStageWords = 11264;  // 44 KiB LDS bitstream stage per decode workgroup
 
+// canonical decode description of one stream: codes of length l are
+// first_code[l] .. first_code[l] + count[l] - 1, their symbols
+// sorted[sorted_off + first_index[l] ..] in code order
+struct DecHead {
+    uint32_t first_code[kLmax + 2];
+    uint32_t count[kLmax + 2];
+    uint32_t first_index[kLmax + 2];
+    uint32_t sorted_off;
+    uint32_t pad_[3];
+};
+
 // decode table of one stream (device memory)
 struct DecTab {
     uint64_t lut[1 << kLutBits];       // multi-symbol entries (above)
@@ -134,6 +145,8 @@ void launch_zero_bits(hipStream_t s, const StreamSet& ss, uint8_t* payload);
 // cmap[cm_base + chunk] = out_pos
 void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                      const uint32_t* list, uint32_t nlist, DecItem* items, uint32_t* cmap);
+// the decode tables of nstreams streams from their canonical descriptions
+void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams);
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                         const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
                         uint16_t* codes, void* side, uint32_t* err, bool stage);

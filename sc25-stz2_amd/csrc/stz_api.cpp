@@ -16,6 +16,8 @@
 #include <string>
 #include <thread>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <unordered_map>
 #include <vector>
 
@@ -137,6 +139,89 @@ struct DBuf {
 struct ProfRec { const char* name; hipEvent_t a, b; };
 struct BatchPlan;
 
+// Persistent host workers for the per-stream host work of a call (codebook
+// construction): run(n, f) calls f(0..n-1) on the workers and the calling
+// thread and returns when all are done, rethrowing the first exception.
+class TaskPool {
+public:
+    explicit TaskPool(int nworkers)
+    {
+        for (int i = 0; i < nworkers; ++i) th_.emplace_back([this] { loop(); });
+    }
+    ~TaskPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int n, const std::function<void(int)>& f)
+    {
+        if (n <= 1 || th_.empty()) {
+            for (int i = 0; i < n; ++i) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            ntask_ = n;
+            next_.store(0);
+            pending_ = (int)th_.size();
+            err_ = nullptr;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work(f);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        job_ = nullptr;
+        if (err_) std::rethrow_exception(err_);
+    }
+
+private:
+    void work(const std::function<void(int)>& f)
+    {
+        for (;;) {
+            const int i = next_.fetch_add(1);
+            if (i >= ntask_) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!err_) err_ = std::current_exception();
+            }
+        }
+    }
+    void loop()
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)>* f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                f = job_;
+            }
+            work(*f);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    std::atomic<int> next_{0};
+    int ntask_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr err_;
+};
+
 }  // namespace
 
 struct stz_ctx {
@@ -166,6 +251,16 @@ struct stz_ctx {
     DBuf tuner_w;
     bool have_tuner = false;
     float tune_min_conf = 0.f;
+    // host workers (created on first use)
+    std::unique_ptr<TaskPool> workers;
+    TaskPool& tasks()
+    {
+        if (!workers) {
+            const unsigned hc = std::thread::hardware_concurrency();
+            workers.reset(new TaskPool((int)std::min<unsigned>(hc > 1 ? hc - 1 : 0, 3)));
+        }
+        return *workers;
+    }
 
     // Pinned staging for async copies: each request gets a region of its own
     // for the rest of the API call (a bump arena reset when a call starts).
@@ -550,8 +645,9 @@ LevelArgs enc_level_args(const FieldEnc<T>& fe, int level)
     return a;
 }
 
+// file and stream headers of a field from its codebooks (built before)
 template <typename T>
-void enc_phase3_books(FieldEnc<T>& fe)
+void enc_phase3_headers(FieldEnc<T>& fe)
 {
     Meta& m = fe.ds->meta;
     std::memset(&m.fh, 0, sizeof(m.fh));
@@ -582,7 +678,6 @@ void enc_phase3_books(FieldEnc<T>& fe)
     fe.enc_n = 0;
     for (int s = 0; s < kStreams; ++s) {
         const uint32_t n = (uint32_t)fe.lists[s].size();
-        build_codebook(fe.lists[s].data(), n, kLmax, m.cb[s]);
         const uint64_t n_unpred = (n && (fe.lists[s][0] >> 32) == 0) ? (fe.lists[s][0] & 0xFFFFFFFFull) : 0;
         const double ebs = s == 0 ? fe.ebl[0] : s < 8 ? fe.ebl[1] : fe.ebl[2];
         fill_stream_header(m.sh[s], s, fe.L, fe.dtype, ebs, n, n_unpred, m.cb[s].total_bits);
@@ -615,23 +710,26 @@ void ensure_tabs(stz_ctx* c, stz_dstream* ds)
 {
     if (ds->have_tabs) return;
     const Meta& m = ds->meta;
-    std::vector<DecTab> tabs(kStreams);
+    std::vector<DecHead> heads(kStreams);
     std::vector<uint16_t> all;
     for (int s = 0; s < kStreams; ++s) {
         std::vector<uint16_t> sorted;
-        build_dectab(m.cb[s].sym.data(), m.cb[s].len.data(), (uint32_t)m.cb[s].sym.size(), tabs[s], sorted);
-        tabs[s].sorted_off = (uint32_t)all.size();
+        build_dechead(m.cb[s].sym.data(), m.cb[s].len.data(), (uint32_t)m.cb[s].sym.size(), heads[s], sorted);
+        heads[s].sorted_off = (uint32_t)all.size();
         all.insert(all.end(), sorted.begin(), sorted.end());
     }
-    const size_t tb = sizeof(DecTab) * kStreams;
-    ds->tabs = DBuf(ds->pool, tb);
+    const size_t hb = sizeof(DecHead) * kStreams;
+    ds->tabs = DBuf(ds->pool, sizeof(DecTab) * kStreams);
     ds->sorted = DBuf(ds->pool, all.size() * 2 + 16);
-    uint8_t* stage = (uint8_t*)c->host_stage(tb + all.size() * 2 + 64);
-    std::memcpy(stage, tabs.data(), tb);
-    std::memcpy(stage + tb, all.data(), all.size() * 2);
-    HIPCHK(hipMemcpyAsync(ds->tabs.p, stage, tb, hipMemcpyHostToDevice, c->stream));
-    if (!all.empty()) HIPCHK(hipMemcpyAsync(ds->sorted.p, stage + tb, all.size() * 2, hipMemcpyHostToDevice, c->stream));
-    ds->have_tabs = true;
+    DBuf dh(c->pool, hb);
+    uint8_t* stage = (uint8_t*)c->host_stage(hb + all.size() * 2 + 64);
+    std::memcpy(stage, heads.data(), hb);
+    std::memcpy(stage + hb, all.data(), all.size() * 2);
+    HIPCHK(hipMemcpyAsync(dh.p, stage, hb, hipMemcpyHostToDevice, c->stream));
+    if (!all.empty()) HIPCHK(hipMemcpyAsync(ds->sorted.p, stage + hb, all.size() * 2, hipMemcpyHostToDevice, c->stream));
+    K(c, "dectabs", [&] { launch_build_dectabs(c->stream, dh.as<DecHead>(), ds->sorted.as<uint16_t>(),
+                                               ds->tabs.as<DecTab>(), kStreams); });
+    ds->have_tabs = true;  // dh returns to the pool; later users are ordered on the same stream
 }
 
 void layout_of(const Meta& m, HostLayout& L)
@@ -1417,27 +1515,13 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             }
         }
     }
-    // codebooks and headers on host threads
-    {
-        const int nt = std::max(1, std::min(n, 8));
-        if (nt == 1) {
-            enc_phase3_books(fe[0]);
-        } else {
-            std::vector<std::thread> th;
-            std::vector<std::exception_ptr> errs(nt);
-            for (int t = 0; t < nt; ++t)
-                th.emplace_back([&, t] {
-                    try {
-                        for (int i = t; i < n; i += nt) enc_phase3_books(fe[i]);
-                    } catch (...) {
-                        errs[t] = std::current_exception();
-                    }
-                });
-            for (auto& x : th) x.join();
-            for (auto& e : errs)
-                if (e) std::rethrow_exception(e);
-        }
-    }
+    // codebooks of every (field, stream) on the host workers, then headers
+    c->tasks().run(n * kStreams, [&](int k) {
+        FieldEnc<T>& f = fe[k / kStreams];
+        const int s2 = k % kStreams;
+        build_codebook(f.lists[s2].data(), (uint32_t)f.lists[s2].size(), kLmax, f.ds->meta.cb[s2]);
+    });
+    for (int i = 0; i < n; ++i) enc_phase3_headers(fe[i]);
     // payloads: zeroed, host-written sections in one upload + scatter, encoder tables, marked encode
     std::vector<DBuf> enc(n), pre_b(n), pre_u(n), tot_b(n), tot_u(n);
     std::vector<EncJob> ej(n);
