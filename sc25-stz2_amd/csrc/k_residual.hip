@@ -145,22 +145,17 @@ __device__ __forceinline__ void predict7(bool inner, const T* __restrict__ r, in
 // Work split of a block grid: tiles cover x in [1, dx-3]; the edge columns
 // x in {0, dx-2, dx-1} (every x when dx < 4) take the checked predictors.
 struct EdgeMap {
-    int ncol;       // edge columns
-    int cols[4];
+    int ncol;  // edge columns
+    int dx;
+    // column c of the edge set (arithmetic, so no local array is indexed at run time)
+    __host__ __device__ int col(int c) const { return dx < 4 ? c : (c == 0 ? 0 : dx - 3 + c); }
 };
 
 __host__ __device__ inline EdgeMap edge_map(int dx)
 {
-    EdgeMap e{};
-    if (dx < 4) {
-        e.ncol = dx;
-        for (int k = 0; k < dx; ++k) e.cols[k] = k;
-    } else {
-        e.ncol = 3;
-        e.cols[0] = 0;
-        e.cols[1] = dx - 2;
-        e.cols[2] = dx - 1;
-    }
+    EdgeMap e;
+    e.ncol = dx < 4 ? dx : 3;
+    e.dx = dx;
     return e;
 }
 
@@ -435,7 +430,7 @@ __device__ __forceinline__ void edge_point(const Geom& g, uint64_t first, uint64
     const int c = (int)(k % e.ncol);
     y = (int)((k / e.ncol) % g.dy);
     z = (int)(k / ((uint64_t)e.ncol * g.dy));
-    x = e.cols[c];
+    x = e.col(c);
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
@@ -466,19 +461,43 @@ __device__ __forceinline__ void z_point_checked(const LevelK<T>& K, int x, int y
     z_point<T, kMode>(K, p, c, x, y, z, false, q);
 }
 
-// Tile form of a level launch: work items 0 .. ntiles-1 are 64 x 4 tiles of
-// one plane (wave w owns row y0 + w, so y and z are wave-uniform; interior
-// rows read their 34 neighbours straight from global memory), the items
-// after them the edge columns as above.  No LDS besides the encode
-// histograms, so more workgroups share a CU than in the z-march form.
+// Tile form of a level launch: work items 0 .. ntiles-1 are 64 x (4 kTW)
+// tiles of one plane (wave w owns rows y0 + kTW w .., so y and z are
+// wave-uniform; interior rows read their 34 neighbours straight from global
+// memory, the kTW rows of a wave in one straight-line block so their loads
+// overlap), the items after them the edge columns as above.  No LDS besides
+// the encode histograms, so more workgroups share a CU than in the z-march
+// form.  Rows per wave by level (measured on the 512^3 field): the high
+// level's larger grid gains from three, the low level's from one.
+template <int kMode>
+constexpr int tile_rows()
+{
+    return (kMode == kHighEnc || kMode == kHighDec) ? 3 : 1;
+}
+
+template <int kTW>
 __host__ __device__ inline void tile_items(const Geom& g, int& ntx, int& nty, uint64_t& ntiles, uint64_t& total)
 {
     const int nxi = g.dx - 3;
     ntx = nxi > 0 ? (nxi + 63) >> 6 : 0;
-    nty = (g.dy + 3) >> 2;
+    nty = (g.dy + 4 * kTW - 1) / (4 * kTW);
     ntiles = (uint64_t)ntx * nty * g.dz;
     const uint64_t ne = (uint64_t)edge_map(g.dx).ncol * g.dy * g.dz;
     total = ntiles + (ne + 255) / 256;
+}
+
+// interior point (predictions from global memory)
+template <typename T, int kMode>
+__device__ __forceinline__ void tile_point(const LevelK<T>& K, int x, int y, int z, uint32_t q[7])
+{
+    const Geom& g = K.g;
+    if constexpr (kMode == kHighDec) {
+        if (2 * x >= K.nx || 2 * y >= K.ny || 2 * z >= K.nz) return;
+    }
+    T p[7];
+    predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+    const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
+    z_point<T, kMode>(K, p, c, x, y, z, true, q);
 }
 
 template <typename T, int kMode>
@@ -491,51 +510,54 @@ __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
         for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
         __syncthreads();
     }
+    constexpr int kTW = tile_rows<kMode>();
     const Geom& g = K.g;
     int ntx, nty;
     uint64_t ntiles, total;
-    tile_items(g, ntx, nty, ntiles, total);
+    tile_items<kTW>(g, ntx, nty, ntiles, total);
     const int lane = threadIdx.x & 63, row = wave_row();
     for (uint64_t t = xcd_block(); t < total; t += gridDim.x) {
-        int x, y, z;
-        bool ok;
-        uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
         if (t < ntiles) {
             const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty);
-            z = (int)(t / ((uint64_t)ntx * nty));
-            x = 1 + tx * 64 + lane;
-            y = ty * 4 + row;
-            ok = x <= g.dx - 3 && y < g.dy;
-            if (row_inner(y, z, g.dy, g.dz)) {
-                if (ok) {
-                    if constexpr (kMode == kHighDec) {
-                        if (2 * x < K.nx && 2 * y < K.ny && 2 * z < K.nz) {
-                            T p[7];
-                            predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
-                            const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
-                            z_point<T, kMode>(K, p, c, x, y, z, true, q);
-                        }
-                    } else {
-                        T p[7];
-                        predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
-                        const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
-                        z_point<T, kMode>(K, p, c, x, y, z, true, q);
-                    }
+            const int z = (int)(t / ((uint64_t)ntx * nty));
+            const int x = 1 + tx * 64 + lane;
+            const int y0 = ty * 4 * kTW + kTW * row;
+            const bool okx = x <= g.dx - 3;
+            uint32_t q[kTW][7] = {};
+            if (y0 + kTW - 1 < g.dy && row_inner(y0, z, g.dy, g.dz) && row_inner(y0 + kTW - 1, z, g.dy, g.dz)) {
+                if (okx) {
+#pragma unroll
+                    for (int r = 0; r < kTW; ++r) tile_point<T, kMode>(K, x, y0 + r, z, q[r]);
                 }
-            } else if (ok) {
-                z_point_checked<T, kMode>(K, x, y, z, q);
+#pragma unroll
+                for (int r = 0; r < kTW; ++r)
+                    if constexpr (kEnc) STZ_HIST_ADD7(q[r], okx);
+            } else {
+                for (int r = 0; r < kTW; ++r) {
+                    const int y = y0 + r;
+                    const bool ok = okx && y < g.dy;
+                    if (ok) {
+                        if (row_inner(y, z, g.dy, g.dz)) tile_point<T, kMode>(K, x, y, z, q[r]);
+                        else z_point_checked<T, kMode>(K, x, y, z, q[r]);
+                    }
+                    if constexpr (kEnc) STZ_HIST_ADD7(q[r], ok);
+                }
             }
         } else {
+            int x, y, z;
+            bool ok;
+            uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
             edge_point(g, ntiles, t, x, y, z, ok);
             if (ok) z_point_checked<T, kMode>(K, x, y, z, q);
+            if constexpr (kEnc) STZ_HIST_ADD7(q, ok);
         }
-        if constexpr (kEnc) STZ_HIST_ADD7(q, ok);
     }
     if constexpr (kEnc) hist_flush(sh, 7, hist);
 }
 
+// at most 96 VGPRs: five waves per SIMD (measured faster than four or six)
 template <typename T, int kMode>
-__global__ __launch_bounds__(256) void k_level_z(const LevelK<T> K)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_level_z(const LevelK<T> K)
 {
     constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
     __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
@@ -832,7 +854,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __r
             const uint64_t k = (uint64_t)(local - J.ntiles) * kThreads + threadIdx.x;
             const int c = (int)(k % e.ncol);
             const int y = (int)((k / e.ncol) % g.dy), z = (int)(k / ((uint64_t)e.ncol * g.dy));
-            level_point<T, kMode>(J, sh, e.cols[c], y, z, k < (uint64_t)e.ncol * g.dy * g.dz, false, false);
+            level_point<T, kMode>(J, sh, e.col(c), y, z, k < (uint64_t)e.ncol * g.dy * g.dz, false, false);
         }
     }
     if constexpr (kEnc) {
@@ -869,7 +891,7 @@ void launch_level_tile(hipStream_t s, LevelK<T> K)
     const unsigned slots = resident_slots((const void*)k_level_tile<T, kMode>);
     int ntx, nty;
     uint64_t ntiles, total;
-    tile_items(K.g, ntx, nty, ntiles, total);
+    tile_items<tile_rows<kMode>()>(K.g, ntx, nty, ntiles, total);
     uint64_t G = total < slots ? total : slots;
     if (G >= 64) G &= ~7ull;
     if (!G) return;
