@@ -62,9 +62,9 @@ TRAFFIC_FILE = "profiles/hbm_traffic_512cubed.json"
 
 
 def measured_traffic(name, n, s):
-    """HBM bytes per launch of kernel `name` measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE on the 512^3 fp32 workload (calibrated against a 1 GiB copy), or
-    None for another workload or a kernel the profile does not cover."""
+    """Memory bytes per launch of kernel `name` measured with rocprofv3 on the
+    512^3 fp32 workload (read requests by size + WRITE_SIZE), or None for
+    another workload or a kernel the profile does not cover."""
     path = os.path.join(ROOT, TRAFFIC_FILE)
     if n != 512 or s != 4 or not os.path.exists(path):
         return None
@@ -73,8 +73,7 @@ def measured_traffic(name, n, s):
     k = d["kernels"].get(name)
     if k is None:
         return None
-    return {"bytes_per_launch": int(k["bytes_per_launch"]), "fetch_factor": d["fetch_factor"],
-            "write_factor": d["write_factor"]}
+    return {"bytes_per_launch": int(k["bytes_per_launch"]), "method": d.get("method", "")}
 
 
 def main():
@@ -207,8 +206,7 @@ def main():
         if tr is not None:
             roof["traffic"] = tr["bytes_per_launch"]
             roof["traffic_over_algorithmic"] = round(tr["bytes_per_launch"] / kb, 3)
-            roof["traffic_source"] = TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE x %.3f + WRITE_SIZE x %.3f, " \
-                "factors from a 1 GiB device copy)" % (tr["fetch_factor"], tr["write_factor"])
+            roof["traffic_source"] = TRAFFIC_FILE + " (rocprofv3: " + tr["method"] + ")"
     step_alg_bytes = 2.0 * (N * s + payload)
     whole = {"algorithmic_bytes_per_step": int(step_alg_bytes),
              "achieved_GBps": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
