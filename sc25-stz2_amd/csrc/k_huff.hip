@@ -711,10 +711,14 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     const uint8_t* un = payload + d.off_unpred;
     const uint64_t al = (d.code_off + it.p0);
     const bool in_stage = use_lds && it.bit0 >= (w0 << 5) && ((it.bit0 + it.bits + 31) >> 5) + 2 <= w1;
-    const uint32_t* W = in_stage ? stage : bw;
-    const uint64_t wb = in_stage ? w0 : 0;
-    const uint64_t nw = in_stage ? w1 - w0 : d.bits_words;
-    const uint32_t bad = decode_run(W, wb, nw, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+    // two calls rather than one through a pointer chosen per lane: each
+    // sees the address space of its bits (LDS reads for the stage, global
+    // loads otherwise) instead of generic flat loads
+    uint32_t bad;
+    if (in_stage)
+        bad = decode_run(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+    else
+        bad = decode_run(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
     if (bad) atomicOr(err, 1u);
 }
 
