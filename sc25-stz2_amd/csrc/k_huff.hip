@@ -561,7 +561,11 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     uint64_t buf = (((uint64_t)W[wpos] << 32) | W[wpos + 1]) << skip;
     int nb = 64 - skip;
     wpos += 2;
-    uint32_t nextw = wpos < nwords ? W[wpos] : 0u;
+    // the look-ahead word is read unconditionally (index clamped; a word
+    // past the end is never consumed without setting `bad`), so the load
+    // lands in its register and is only waited for at the next refill
+    const uint64_t wlast = nwords - 1;
+    uint32_t nextw = W[wpos < wlast ? wpos : wlast];
     uint64_t u = it.u0;
     const uint32_t cnt = it.cnt;
     auto refill = [&]() {
@@ -570,7 +574,7 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
             bad |= wpos >= nwords ? 1u : 0u;
             ++wpos;
             nb += 32;
-            nextw = wpos < nwords ? W[wpos] : 0u;
+            nextw = W[wpos < wlast ? wpos : wlast];
         }
     };
     // a code longer than the table window: canonical search on the top 32 bits

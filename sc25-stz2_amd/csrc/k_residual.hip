@@ -276,27 +276,37 @@ __device__ __forceinline__ void low_enc_point(const T* __restrict__ full, int px
 
 // ---------------------------------------------------------------- decode
 
-template <typename T>
-__device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, const uint16_t* __restrict__ codes,
-                                  const T* __restrict__ side, const uint32_t* qpre)
+// the 7 block codes of one point
+struct Codes7 {
+    uint32_t q[7];
+};
+
+__device__ __forceinline__ Codes7 load_codes7(const LevelArgs& a, const uint16_t* __restrict__ codes, uint32_t pos)
 {
-    const uint32_t q = qpre ? qpre[b] : *(const uint16_t*)((const char*)(codes + a.code_off[b]) + 2u * pos);
+    Codes7 c;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) c.q[b] = *(const uint16_t*)((const char*)(codes + a.code_off[b]) + 2u * pos);
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, uint32_t q, const T* __restrict__ side)
+{
     const T v = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
     return v + p;  // F5:1080 de_sub = deData + pred
 }
 
 template <typename T>
-__device__ __forceinline__ void low_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a,
-                                            const uint16_t* __restrict__ codes, const T* __restrict__ side,
-                                            T* __restrict__ mid, int x, int y, int z, bool vec,
-                                            const uint32_t* qpre = nullptr)
+__device__ __forceinline__ void low_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a, Codes7 cq,
+                                            const T* __restrict__ side, T* __restrict__ mid, int x, int y, int z,
+                                            bool vec)
 {
     const int mx = 2 * g.dx, my = 2 * g.dy;
     const uint32_t pos = stream_pos(g, x, y, z);
     T r[8];
     r[0] = centre;
 #pragma unroll
-    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, codes, side, qpre);
+    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side);
     store_cell(mid, (size_t)mx, (size_t)mx * my, 2 * x, 2 * y, 2 * z, r, vec);
 }
 
@@ -308,17 +318,16 @@ __device__ __forceinline__ void low_dec_point(const T* __restrict__ base, const 
     T p[7];
     predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p);
     const T c = ldx(base + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
-    low_dec_fin(p, c, g, a, codes, side, mid, x, y, z, vec);
+    low_dec_fin(p, c, g, a, load_codes7(a, codes, stream_pos(g, x, y, z)), side, mid, x, y, z, vec);
 }
 
 // the 2x2x2 cell of mid point (x,y,z) into the caller's (nx,ny,nz) field
 // from predictions p and the mid value `centre`; vec = 2-wide stores (nx
 // even and the output 2-element aligned)
 template <typename T>
-__device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a,
-                                             const uint16_t* __restrict__ codes, const T* __restrict__ side,
-                                             T* __restrict__ out, int nx, int ny, int nz, int x, int y, int z,
-                                             bool vec, const uint32_t* qpre = nullptr)
+__device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a, Codes7 cq,
+                                             const T* __restrict__ side, T* __restrict__ out, int nx, int ny, int nz,
+                                             int x, int y, int z, bool vec)
 {
     typedef typename V2<T>::type v2;
     const int X = 2 * x, Y = 2 * y, Z = 2 * z;
@@ -327,7 +336,7 @@ __device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom&
     T v[8];
     v[0] = centre;
 #pragma unroll
-    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, codes, side, qpre);
+    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side);
     const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
     T* o = out + (size_t)Z * oz + (size_t)Y * oy;  // row base (wave-uniform in tiles)
     const uint32_t xo = (uint32_t)X * (uint32_t)sizeof(T);
@@ -359,7 +368,7 @@ __device__ __forceinline__ void high_dec_point(const T* __restrict__ mid, const 
     T p[7];
     predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
     const T c = ldx(mid + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
-    high_dec_fin(p, c, g, a, codes, side, out, nx, ny, nz, x, y, z, vec);
+    high_dec_fin(p, c, g, a, load_codes7(a, codes, stream_pos(g, x, y, z)), side, out, nx, ny, nz, x, y, z, vec);
 }
 
 // ---------------------------------------------------------------- z-march
@@ -437,15 +446,25 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 
 // one point through the level's operation: predictions p and centre value
 // given, or (checked form) evaluated from global memory
+// decode modes take the point's codes in cq (the caller loads or prefetched
+// them); encode modes ignore it
 template <typename T, int kMode>
 __device__ __forceinline__ void z_point(const LevelK<T>& K, const T p[7], T c, int x, int y, int z, bool vec,
-                                        uint32_t q[7], const uint32_t* qpre = nullptr)
+                                        uint32_t q[7], Codes7 cq)
 {
     if constexpr (kMode == kHighEnc) high_enc_fin(p, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, x, y, z, q);
     else if constexpr (kMode == kLowEnc)
         low_enc_fin(p, c, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, q);
-    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, qpre);
-    else high_dec_fin(p, c, K.g, K.a, K.codes, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0, qpre);
+    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, x, y, z, vec);
+    else high_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0);
+}
+
+// the codes a decode mode needs at block point (x,y,z) (nothing for encode)
+template <typename T, int kMode>
+__device__ __forceinline__ Codes7 point_codes(const LevelK<T>& K, int x, int y, int z)
+{
+    if constexpr (kMode == kLowDec || kMode == kHighDec) return load_codes7(K.a, K.codes, stream_pos(K.g, x, y, z));
+    else return Codes7{};
 }
 
 template <typename T, int kMode>
@@ -458,7 +477,7 @@ __device__ __forceinline__ void z_point_checked(const LevelK<T>& K, int x, int y
     T p[7];
     predict7(false, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
     const T c = K.ref[((size_t)z * g.dy + y) * g.dx + x];
-    z_point<T, kMode>(K, p, c, x, y, z, false, q);
+    z_point<T, kMode>(K, p, c, x, y, z, false, q, point_codes<T, kMode>(K, x, y, z));
 }
 
 // Tile form of a level launch: work items 0 .. ntiles-1 are 64 x (4 kTW)
@@ -497,7 +516,7 @@ __device__ __forceinline__ void tile_point(const LevelK<T>& K, int x, int y, int
     T p[7];
     predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
     const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
-    z_point<T, kMode>(K, p, c, x, y, z, true, q);
+    z_point<T, kMode>(K, p, c, x, y, z, true, q, point_codes<T, kMode>(K, x, y, z));
 }
 
 template <typename T, int kMode>
@@ -610,17 +629,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         };
         __syncthreads();  // the previous item's readers are done with the ring
         // decode: the codes of a step's points are fetched a step ahead too
-        uint32_t cq[kEnc ? 1 : kZRW][7], cn[kEnc ? 1 : kZRW][7];
+        Codes7 cq[kZRW] = {}, cn[kZRW] = {};
         auto fetch_codes = [&](int zz) {
 #pragma unroll
             for (int r = 0; r < kZRW; ++r) {
                 const int y = y0 + kZRW * w + r, x = x0 + lane;
-                if (zz < ze && y < g.dy && x <= g.dx - 3) {
-                    const uint32_t pos = stream_pos(g, x, y, zz);
-#pragma unroll
-                    for (int b = 0; b < 7; ++b)
-                        cn[r][b] = *(const uint16_t*)((const char*)(K.codes + K.a.code_off[b]) + 2u * pos);
-                }
+                if (zz < ze && y < g.dy && x <= g.dx - 3) cn[r] = point_codes<T, kMode>(K, x, y, zz);
             }
         };
         T v[kZLoads];
@@ -635,9 +649,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             if (z + 1 < ze) fetch(z + 3, v);
             if constexpr (!kEnc) {
 #pragma unroll
-                for (int r = 0; r < kZRW; ++r)
-#pragma unroll
-                    for (int b = 0; b < 7; ++b) cq[r][b] = cn[r][b];
+                for (int r = 0; r < kZRW; ++r) cq[r] = cn[r];
                 fetch_codes(z + 1);
             }
             __syncthreads();
@@ -657,7 +669,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                         auto at = [&](int dy, int dz, int k) -> T { return S[dz + 1][o + dy * kZPitch + k]; };
                         T p[7];
                         predict7_at<T>(at, p);
-                        z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, kEnc ? nullptr : cq[kEnc ? 0 : r]);
+                        z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, cq[r]);
                     }
                 } else if (ok) {
                     z_point_checked<T, kMode>(K, x, y, z, q);

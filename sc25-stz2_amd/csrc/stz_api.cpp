@@ -1596,7 +1596,20 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     DBuf dencl = upload_vec(c, encl);
     K(c, "enc_scatter", [&] { launch_enc_scatter_multi(st, dencl.as<uint32_t>(), (uint32_t)(encl.size() / 3), dej.as<EncJob>()); });
     // marked encode (F5:1412 encode_marked_fast): lengths -> chunk scan -> pack
-    K(c, "huff_encode", [&] { launch_huff_encode_multi(st, dej.as<EncJob>(), (uint32_t)n, chunk0); });
+    // one field: the single-stream-set kernels, whose buffers arrive as
+    // kernel pointer arguments (global accesses rather than the generic ones
+    // a pointer read out of a job record gets)
+    if (n == 1) {
+        const EncJob& j = ej[0];
+        K(c, "huff_encode", [&] {
+            launch_huff_len(st, j.ss, j.codes, j.enc, j.payload, j.sub_pre_bits, j.sub_pre_un, j.chunk_tot_bits,
+                            j.chunk_tot_un);
+            launch_huff_scan(st, j.ss, j.chunk_tot_bits, j.chunk_tot_un, j.payload);
+            launch_huff_pack(st, j.ss, j.codes, j.side, j.enc, j.sub_pre_bits, j.sub_pre_un, j.payload);
+        });
+    } else {
+        K(c, "huff_encode", [&] { launch_huff_encode_multi(st, dej.as<EncJob>(), (uint32_t)n, chunk0); });
+    }
     // the staging buffers feed async copies: finish before they can be reused
     HIPCHK(hipStreamSynchronize(st));
 }
