@@ -61,6 +61,22 @@ __device__ __forceinline__ int find_stream_by_chunk(const StreamSet& ss, uint32_
     return s;
 }
 
+// The encoder table entries of the codes near the centre (where nearly all
+// codes fall) staged in LDS per workgroup; the rest read global memory.
+constexpr int kEncWin = 2048;
+constexpr uint32_t kEncLo = (uint32_t)kRadius - kEncWin / 2;
+
+__device__ __forceinline__ void stage_enc_window(const uint32_t* __restrict__ es, uint32_t* tab)
+{
+    for (int k = threadIdx.x; k < kEncWin; k += blockDim.x) tab[k] = es[kEncLo + k];
+}
+
+__device__ __forceinline__ uint32_t enc_entry(const uint32_t* tab, const uint32_t* __restrict__ es, uint32_t q)
+{
+    const uint32_t r = q - kEncLo;
+    return r < (uint32_t)kEncWin ? tab[r] : es[q];
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
@@ -133,6 +149,7 @@ __device__ __forceinline__ void huff_len_body(const StreamSet& ss, const uint16_
                                               uint64_t* __restrict__ chunk_tot_un, uint32_t gc)
 {
     __shared__ uint32_t lb[256], lu[256];
+    __shared__ uint32_t tab[kEncWin];
     const int s = find_stream_by_chunk(ss, gc);
     const StreamDesc& d = ss.s[s];
     const uint32_t c = gc - d.chunk_base;
@@ -142,6 +159,8 @@ __device__ __forceinline__ void huff_len_body(const StreamSet& ss, const uint16_
     const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
     const uint16_t* cs = codes + d.code_off;
     const uint32_t* es = enc + (size_t)s * 65536;
+    stage_enc_window(es, tab);
+    __syncthreads();
     uint16_t* sbits = (uint16_t*)(payload + d.off_sub_bits);
     uint16_t* sun = (uint16_t*)(payload + d.off_sub_unpred);
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
@@ -159,7 +178,7 @@ __device__ __forceinline__ void huff_len_body(const StreamSet& ss, const uint16_
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             if (q[k] != 0xFFFFFFFFu) {
-                bits += es[q[k]] >> 24;
+                bits += enc_entry(tab, es, q[k]) >> 24;
                 un += q[k] == 0u;
             }
         }
@@ -255,6 +274,7 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
 {
     constexpr int kWords = (31 + kSub * kLmax + 31) / 32 + 2;  // 772
     __shared__ uint32_t wbuf[4][kWords];
+    __shared__ uint32_t tab[kEncWin];
     const int s = find_stream_by_chunk(ss, gc);
     const StreamDesc& d = ss.s[s];
     const uint32_t c = gc - d.chunk_base;
@@ -266,6 +286,8 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
     const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
     const uint16_t* cs = codes + d.code_off;
     const uint32_t* es = enc + (size_t)s * 65536;
+    stage_enc_window(es, tab);
+    __syncthreads();
     uint32_t* bw = (uint32_t*)(payload + d.off_bits);
     uint8_t* un_out = payload + d.off_unpred;
     const uint8_t* side_s = side + d.code_off * d.tsize;
@@ -303,7 +325,7 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const bool in = pl + k < p1;
-                e[k] = in ? es[q[k]] : 0u;
+                e[k] = in ? enc_entry(tab, es, q[k]) : 0u;
                 lbits += e[k] >> 24;
                 lzero += (in && q[k] == 0u) ? 1u : 0u;
             }
