@@ -644,19 +644,31 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     uint64_t lo = 0;
     uint32_t pc = 0;
     while (i + pc + 3 <= cnt) {
-        refill();
+        // refill without a branch: the word after the look-ahead one is read
+        // every step (beside the table read) and taken by selects
+        {
+            const bool need = nb < 32;
+            const uint64_t wn = wpos + 1;
+            const uint32_t w2 = W[wn < wlast ? wn : wlast];
+            buf |= need ? (uint64_t)nextw << ((32 - nb) & 63) : 0ull;
+            bad |= (need && wpos >= nwords) ? 1u : 0u;
+            wpos += need ? 1u : 0u;
+            nb += need ? 32 : 0;
+            nextw = need ? w2 : nextw;
+        }
         const uint64_t e = lut[buf >> (64 - kLutBits)];
         uint32_t n = (uint32_t)(e >> kDecNShift) & 3u;
-        int l;
-        uint64_t v;
-        if (n) {
-            l = (int)((e >> kDecLtShift) & 15u);
-            v = e & ((1ull << (16 * n)) - 1ull);
-        } else {
-            v = slow(l);
-            n = 1;
+        int l = (int)((e >> kDecLtShift) & 15u);
+        uint64_t v = e & ((1ull << (16 * n)) - 1ull);
+        // the two rare cases under one branch: a code longer than the
+        // window (n = 0), and symbol 0, which always comes alone (v = 0)
+        if (v == 0ull) {
+            if (n == 0) {
+                v = slow(l);
+                n = 1;
+            }
+            if (v == 0ull) unpred(i + pc);
         }
-        if (v == 0ull) unpred(i + pc);  // symbol 0 comes alone
         buf <<= l;
         nb -= l;
         const uint64_t spill = pc ? v >> (64 - 16 * pc) : 0ull;
