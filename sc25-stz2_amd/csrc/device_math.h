@@ -126,18 +126,20 @@ __device__ __forceinline__ T predict_rt(int b, const T* __restrict__ r, int i, i
 // bit-exact with SZ3 LinearQuantizer::quantize_and_overwrite(d, 0)).
 // Returns the code (0 = unpredictable) and the decompressed value (d itself
 // when unpredictable).
+// Branch-free: every step is evaluated and the two rejections (out of range,
+// NaN/Inf included; error above eb after rounding) select at the end.
 template <typename T>
 __device__ __forceinline__ uint32_t quantize(T d, double eb, double recip, T& dec_out)
 {
     const double ad = fabs((double)d) * recip;
-    if (!(ad < (double)(2 * kRadius - 1))) { dec_out = d; return 0u; }
-    const int half = ((int)ad + 1) >> 1;
+    const bool in_range = ad < (double)(2 * kRadius - 1);
+    const int half = ((int)(in_range ? ad : 0.0) + 1) >> 1;
     T dec = (T)((double)(2 * half) * eb);  // F5:1353-1356 dec_lut[h]
     if (d < T(0)) dec = -dec;
     const T derr = dec - d;
-    if (!(fabs((double)derr) <= eb)) { dec_out = d; return 0u; }
-    dec_out = dec;
-    return (uint32_t)((d < T(0)) ? kRadius - half : kRadius + half);
+    const bool ok = in_range && fabs((double)derr) <= eb;
+    dec_out = ok ? dec : d;
+    return ok ? (uint32_t)((d < T(0)) ? kRadius - half : kRadius + half) : 0u;
 }
 
 // F5:1531-1535 rec_lut[q] = (T)(2*(q-r)*eb)

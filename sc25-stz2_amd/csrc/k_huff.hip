@@ -408,10 +408,11 @@ __device__ __forceinline__ uint32_t find_enc_job(const EncJob* __restrict__ jobs
     return lo;
 }
 
-__global__ __launch_bounds__(1024) void k_hist_compact_multi(const EncJob* __restrict__ jobs)
+// streams s0 .. s0+ns-1 of every field
+__global__ __launch_bounds__(1024) void k_hist_compact_multi(const EncJob* __restrict__ jobs, int s0, int ns)
 {
-    const EncJob& J = jobs[blockIdx.x / kStreams];
-    hist_compact_body(J.hist, J.compact, J.counts, (int)(blockIdx.x % kStreams));
+    const EncJob& J = jobs[blockIdx.x / ns];
+    hist_compact_body(J.hist, J.compact, J.counts, s0 + (int)(blockIdx.x % ns));
 }
 
 __global__ void k_enc_scatter_multi(const uint32_t* __restrict__ list, uint32_t n, const EncJob* __restrict__ jobs)
@@ -900,9 +901,9 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
     k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items, cmap);
 }
 
-void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs)
+void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, int s0, int ns)
 {
-    if (njobs) k_hist_compact_multi<<<dim3(njobs * kStreams), dim3(1024), 0, s>>>(jobs);
+    if (njobs && ns > 0) k_hist_compact_multi<<<dim3(njobs * (unsigned)ns), dim3(1024), 0, s>>>(jobs, s0, ns);
 }
 
 void launch_enc_scatter_multi(hipStream_t s, const uint32_t* list, uint32_t n, const EncJob* jobs)

@@ -113,7 +113,8 @@ struct EncJob {
     uint32_t* counts;          // symbols used per stream
     uint32_t chunk0;           // first chunk of this field in the per-chunk launches
 };
-void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs);
+// compaction of the histograms of streams s0 .. s0+ns-1 of every job
+void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, int s0, int ns);
 // list: (field, stream << 16 | symbol, code | len << 24) triples
 void launch_enc_scatter_multi(hipStream_t s, const uint32_t* list, uint32_t n, const EncJob* jobs);
 // huff_len, huff_scan, huff_pack of every field (chunk prefix in chunk0)

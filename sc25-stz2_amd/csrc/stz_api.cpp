@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -382,7 +383,7 @@ struct HostLayout {
     uint64_t total_codes;
 };
 
-void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
+void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
 {
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.px = px; L.py = py; L.pz = pz;
@@ -408,6 +409,33 @@ void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size
         off += align_up(L.n[s], 64);
     }
     L.total_codes = off;
+}
+
+// Layouts by shape: building one walks every chunk of every stream (tens of
+// microseconds at 512^3), and fields of one shape come back again and again
+// (steps, serving, batches), so the process keeps the recent ones.
+void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
+{
+    typedef std::array<uint64_t, 8> Key;
+    static std::mutex mu;
+    static std::map<Key, std::shared_ptr<const HostLayout>> cache;
+    const Key k = {nx, ny, nz, px, py, pz, (uint64_t)(uint32_t)ch, (uint64_t)(uint32_t)cl};
+    std::shared_ptr<const HostLayout> hit;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(k);
+        if (it != cache.end()) hit = it->second;
+    }
+    if (hit) {
+        L = *hit;
+        return;
+    }
+    auto fresh = std::make_shared<HostLayout>();
+    build_layout(*fresh, nx, ny, nz, px, py, pz, ch, cl);
+    L = *fresh;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 64) cache.clear();
+    cache.emplace(k, std::move(fresh));
 }
 
 void check_dims(size_t nx, size_t ny, size_t nz, int ch, int cl)
@@ -581,7 +609,6 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     fe.py = pad4(fe.ny);
     fe.pz = pad4(fe.nz);
     fe.nin = (uint64_t)fe.nx * fe.ny * fe.nz;
-    make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low);
     // input on device, 2-element aligned, padded to multiples of 4
     fe.din = field;
     const bool padded = fe.px != fe.nx || fe.py != fe.ny || fe.pz != fe.nz;
@@ -609,6 +636,8 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     K(c, "minmax", [&] { launch_minmax<T>(st, fe.din, fe.nin, fe.mm_part.template as<double>(), outp); });
     fe.mm_host = (double*)c->host_stage(64);
     HIPCHK(hipMemcpyAsync(fe.mm_host, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    // host work while the range is computed
+    make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low);
 }
 
 // after a wait: level bounds, scratch, histograms cleared, base grid gathered
@@ -1429,6 +1458,13 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         DBuf dfj = upload_vec(c, fj);
         K(c, "interp_encode_fused", [&] { launch_interp_encode_fused_multi<T>(st, dfj.as<FusedEncJob>(), (uint32_t)fj.size()); });
         for (auto& rj : rounds) {
+            if (rj.size() == 1) {
+                // one field: the pass goes by value (no job upload, whose copy
+                // costs more GPU time than a small pass)
+                const PassEncJob& q = rj[0];
+                K(c, "interp_encode", [&] { launch_interp_encode<T>(st, q.p, (T*)q.v, q.codes, (T*)q.side, q.hist); });
+                continue;
+            }
             uint32_t wg = 0;
             for (PassEncJob& q : rj) {
                 q.wg0 = wg;
@@ -1438,8 +1474,63 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             K(c, "interp_encode", [&] { launch_interp_encode_multi<T>(st, dr.as<PassEncJob>(), (uint32_t)rj.size(), wg); });
         }
     }
+    // Histograms -> host (F5:1271 / F5:1402 preprocess_encode_fast).  With
+    // few fields the base and low streams' histograms are final once the low
+    // level has run: they are compacted and read back then, and their
+    // codebooks are built on the host while the GPU runs the high level.
+    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
+    DBuf compact(pool, (size_t)n * kStreams * 65536 * 8), counts(pool, (size_t)n * 64 * 4);
+    std::vector<EncJob> hj(n);
+    for (int i = 0; i < n; ++i) {
+        hj[i].hist = fe[i].hist.template as<uint32_t>();
+        hj[i].compact = compact.as<uint64_t>() + (size_t)i * kStreams * 65536;
+        hj[i].counts = counts.as<uint32_t>() + (size_t)i * 64;
+        fe[i].lists.assign(kStreams, {});
+    }
+    DBuf dhj = upload_vec(c, hj);
+    const bool early = n <= 4;
+    const size_t hs_bytes = (size_t)n * 256 + (size_t)n * kStreams * kFirst * 8;
+    uint8_t* hs_lo = early ? (uint8_t*)c->host_stage(hs_bytes) : nullptr;
+    uint8_t* hs_hi = (uint8_t*)c->host_stage(hs_bytes);
+    // compact streams [s0, s0+ns) of every field and copy counts + first entries into hs
+    auto readback = [&](uint8_t* hs, int s0, int ns) {
+        K(c, "hist_compact", [&] { launch_hist_compact_multi(st, dhj.as<EncJob>(), (uint32_t)n, s0, ns); });
+        HIPCHK(hipMemcpyAsync(hs, counts.p, (size_t)n * 256, hipMemcpyDeviceToHost, st));
+        uint64_t* hf = (uint64_t*)(hs + (size_t)n * 256);
+        for (int i = 0; i < n; ++i) {
+            const size_t row = (size_t)i * kStreams + s0;
+            HIPCHK(hipMemcpy2DAsync(hf + row * kFirst, kFirst * 8, compact.as<uint64_t>() + row * 65536, 65536 * 8,
+                                    kFirst * 8, (size_t)ns, hipMemcpyDeviceToHost, st));
+        }
+    };
+    // after the copies have landed: symbol lists, then codebooks on the host workers
+    auto books = [&](const uint8_t* hs, int s0, int ns) {
+        const uint64_t* hf = (const uint64_t*)(hs + (size_t)n * 256);
+        for (int i = 0; i < n; ++i) {
+            FieldEnc<T>& f = fe[i];
+            const uint32_t* hc = (const uint32_t*)(hs + (size_t)i * 256);
+            for (int s2 = s0; s2 < s0 + ns; ++s2) {
+                const uint32_t m = hc[s2];
+                f.lists[s2].resize(m);
+                const uint64_t* src = hf + ((size_t)i * kStreams + s2) * kFirst;
+                if (m <= kFirst) std::memcpy(f.lists[s2].data(), src, (size_t)m * 8);
+                else
+                    HIPCHK(hipMemcpy(f.lists[s2].data(), compact.as<uint64_t>() + ((size_t)i * kStreams + s2) * 65536,
+                                     (size_t)m * 8, hipMemcpyDeviceToHost));
+            }
+        }
+        c->tasks().run(n * ns, [&](int k) {
+            FieldEnc<T>& f = fe[k / ns];
+            const int s2 = s0 + k % ns;
+            build_codebook(f.lists[s2].data(), (uint32_t)f.lists[s2].size(), kLmax, f.ds->meta.cb[s2]);
+        });
+    };
     // residual levels
     for (int level = 1; level <= 2; ++level) {
+        if (early && level == 2) {
+            readback(hs_lo, 0, 8);
+            HIPCHK(hipEventRecord(c->ev_fork, st));
+        }
         if (n == 1) {
             FieldEnc<T>& f = fe[0];
             const LevelArgs a = enc_level_args(f, level);
@@ -1482,58 +1573,40 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         });
         jobs_launch<T>(c, md, js, cfg);
     }
-    // histograms -> host (F5:1271 / F5:1402 preprocess_encode_fast)
-    constexpr uint32_t kFirst = FieldEnc<T>::kFirst;
-    DBuf compact(pool, (size_t)n * kStreams * 65536 * 8), counts(pool, (size_t)n * 64 * 4);
-    {
-        std::vector<EncJob> hj(n);
-        for (int i = 0; i < n; ++i) {
-            hj[i].hist = fe[i].hist.template as<uint32_t>();
-            hj[i].compact = compact.as<uint64_t>() + (size_t)i * kStreams * 65536;
-            hj[i].counts = counts.as<uint32_t>() + (size_t)i * 64;
-        }
-        DBuf dh = upload_vec(c, hj);
-        K(c, "hist_compact", [&] { launch_hist_compact_multi(st, dh.as<EncJob>(), (uint32_t)n); });
-        uint8_t* hs = (uint8_t*)c->host_stage((size_t)n * 256 + (size_t)n * kStreams * kFirst * 8);
-        HIPCHK(hipMemcpyAsync(hs, counts.p, (size_t)n * 256, hipMemcpyDeviceToHost, st));
-        uint64_t* hf = (uint64_t*)(hs + (size_t)n * 256);
-        HIPCHK(hipMemcpy2DAsync(hf, kFirst * 8, compact.p, 65536 * 8, kFirst * 8, (size_t)n * kStreams,
-                                hipMemcpyDeviceToHost, st));
+    if (early) {
+        readback(hs_hi, 8, kStreams - 8);
+        HIPCHK(hipEventSynchronize(c->ev_fork));
+        books(hs_lo, 0, 8);  // overlaps the high level on the GPU
         HIPCHK(hipStreamSynchronize(st));
-        for (int i = 0; i < n; ++i) {
-            FieldEnc<T>& f = fe[i];
-            const uint32_t* hc = (const uint32_t*)(hs + (size_t)i * 256);
-            f.lists.assign(kStreams, {});
-            for (int s2 = 0; s2 < kStreams; ++s2) {
-                const uint32_t m = hc[s2];
-                f.lists[s2].resize(m);
-                const uint64_t* src = hf + ((size_t)i * kStreams + s2) * kFirst;
-                if (m <= kFirst) std::memcpy(f.lists[s2].data(), src, (size_t)m * 8);
-                else
-                    HIPCHK(hipMemcpy(f.lists[s2].data(), compact.as<uint64_t>() + ((size_t)i * kStreams + s2) * 65536,
-                                     (size_t)m * 8, hipMemcpyDeviceToHost));
-            }
-        }
+        books(hs_hi, 8, kStreams - 8);
+    } else {
+        readback(hs_hi, 0, kStreams);
+        HIPCHK(hipStreamSynchronize(st));
+        books(hs_hi, 0, kStreams);
     }
-    // codebooks of every (field, stream) on the host workers, then headers
-    c->tasks().run(n * kStreams, [&](int k) {
-        FieldEnc<T>& f = fe[k / kStreams];
-        const int s2 = k % kStreams;
-        build_codebook(f.lists[s2].data(), (uint32_t)f.lists[s2].size(), kLmax, f.ds->meta.cb[s2]);
-    });
     for (int i = 0; i < n; ++i) enc_phase3_headers(fe[i]);
     // payloads: zeroed, host-written sections in one upload + scatter, encoder tables, marked encode
     std::vector<DBuf> enc(n), pre_b(n), pre_u(n), tot_b(n), tot_u(n);
     std::vector<EncJob> ej(n);
     std::vector<Segment> segs;
-    std::vector<uint32_t> encl;
-    size_t img_bytes = 0;
+    // one upload: the host-written payload sections, then the segment list,
+    // then the encoder-table entries (field, stream<<16|symbol, code|len<<24;
+    // (stream<<16|symbol, code|len<<24) pairs for one field)
+    size_t img_bytes = 0, nsym = 0;
     for (int i = 0; i < n; ++i) {
         const Meta& m = fe[i].ds->meta;
         img_bytes += align_up(sizeof(FileHeader), 16);
-        for (int s2 = 0; s2 < kStreams; ++s2) img_bytes += align_up(m.sh[s2].off_sub_bits, 16);
+        for (int s2 = 0; s2 < kStreams; ++s2) {
+            img_bytes += align_up(m.sh[s2].off_sub_bits, 16);
+            nsym += m.cb[s2].sym.size();
+        }
     }
-    uint8_t* img = (uint8_t*)c->host_stage(img_bytes + 64);
+    const uint32_t ew = n == 1 ? 2u : 3u;
+    const size_t seg_off = img_bytes, enc_off = align_up(seg_off + (size_t)n * (kStreams + 1) * sizeof(Segment), 16);
+    const size_t up_bytes = enc_off + nsym * ew * 4;
+    uint8_t* img = (uint8_t*)c->host_stage(up_bytes + 64);
+    uint32_t* encl = (uint32_t*)(img + enc_off);
+    size_t ne = 0;
     size_t io = 0;
     uint32_t chunk0 = 0;
     for (int i = 0; i < n; ++i) {
@@ -1564,9 +1637,9 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             segs.push_back(Segment{io, pay + fh.stream_off[s2], sh.off_sub_bits});
             io += align_up(sh.off_sub_bits, 16);
             for (size_t k = 0; k < m.cb[s2].sym.size(); ++k) {
-                encl.push_back((uint32_t)i);
-                encl.push_back(((uint32_t)s2 << 16) | m.cb[s2].sym[k]);
-                encl.push_back(m.cb[s2].code[k] | ((uint32_t)m.cb[s2].len[k] << 24));
+                if (ew == 3) encl[ne++] = (uint32_t)i;
+                encl[ne++] = ((uint32_t)s2 << 16) | m.cb[s2].sym[k];
+                encl[ne++] = m.cb[s2].code[k] | ((uint32_t)m.cb[s2].len[k] << 24);
             }
         }
         const StreamSet ss = make_streamset(f.L, m, f.dtype);
@@ -1588,13 +1661,20 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         j.chunk0 = chunk0;
         chunk0 += ss.total_chunks;
     }
-    DBuf dimg(pool, img_bytes + 64);
-    HIPCHK(hipMemcpyAsync(dimg.p, img, img_bytes, hipMemcpyHostToDevice, st));
-    DBuf dsegs = upload_vec(c, segs);
-    K(c, "copy_segments", [&] { launch_copy_segments(st, dimg.as<uint8_t>(), dsegs.as<Segment>(), (uint32_t)segs.size()); });
-    DBuf dej = upload_vec(c, ej);
-    DBuf dencl = upload_vec(c, encl);
-    K(c, "enc_scatter", [&] { launch_enc_scatter_multi(st, dencl.as<uint32_t>(), (uint32_t)(encl.size() / 3), dej.as<EncJob>()); });
+    std::memcpy(img + seg_off, segs.data(), segs.size() * sizeof(Segment));
+    DBuf dimg(pool, up_bytes + 64);
+    HIPCHK(hipMemcpyAsync(dimg.p, img, up_bytes, hipMemcpyHostToDevice, st));
+    uint8_t* dup = dimg.as<uint8_t>();
+    K(c, "copy_segments", [&] { launch_copy_segments(st, dup, (const Segment*)(dup + seg_off), (uint32_t)segs.size()); });
+    DBuf dej;
+    if (n == 1) {
+        K(c, "enc_scatter", [&] { launch_enc_scatter(st, (const uint32_t*)(dup + enc_off), (uint32_t)nsym, ej[0].enc); });
+    } else {
+        dej = upload_vec(c, ej);
+        K(c, "enc_scatter", [&] {
+            launch_enc_scatter_multi(st, (const uint32_t*)(dup + enc_off), (uint32_t)nsym, dej.as<EncJob>());
+        });
+    }
     // marked encode (F5:1412 encode_marked_fast): lengths -> chunk scan -> pack
     // one field: the single-stream-set kernels, whose buffers arrive as
     // kernel pointer arguments (global accesses rather than the generic ones
