@@ -548,8 +548,10 @@ __global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict
     uint32_t s1, l1;
     if (!code_at(w, kLutBits, s1, l1)) {
         t.lut[w] = 0ull;  // n = 0: longer than the window
+        t.lut1[w] = 0xFFFFFFFFu;
         return;
     }
+    t.lut1[w] = (l1 << 16) | s1;
     uint64_t e = s1;
     uint32_t cnt = 1, used = l1;
     while (s1 != 0 && cnt < 3 && used < (uint32_t)kLutBits) {
@@ -569,6 +571,7 @@ __global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict
 // kLutBits bits).  Codes leave in 8-byte groups once the destination is
 // 8-byte aligned (`al` = element offset of co[0]); the first symbols up to
 // that alignment and the last ones go one at a time.
+template <bool kLdsBits>
 __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
                                                const DecItem& it, const uint64_t* __restrict__ lut,
                                                const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
@@ -645,9 +648,14 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     uint64_t lo = 0;
     uint32_t pc = 0;
     while (i + pc + 3 <= cnt) {
-        // refill without a branch: the word after the look-ahead one is read
-        // every step (beside the table read) and taken by selects
-        {
+        if constexpr (!kLdsBits) {
+            // bits in global memory: a read is a long round trip, so only
+            // when the buffer runs low, one word ahead
+            refill();
+        } else {
+            // bits in LDS: refill without a branch -- the word after the
+            // look-ahead one is read every step (beside the table read) and
+            // taken by selects
             const bool need = nb < 32;
             const uint64_t wn = wpos + 1;
             const uint32_t w2 = W[wn < wlast ? wn : wlast];
@@ -688,6 +696,104 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     return bad;
 }
 
+// Single-symbol form for dense payloads (a kLutBits window rarely holds two
+// codes): one table entry (len << 16 | symbol, 0xFFFFFFFF for a longer code)
+// per step from a 16 KiB table, bits from global memory fetched one refill
+// ahead.  Codes leave in 64-byte bursts (32 codes held in registers) when the
+// destination is 64-byte aligned, in 8-byte groups otherwise.
+template <int kGroup>
+__device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
+                                               const DecItem& it, const uint32_t* __restrict__ lut,
+                                               const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
+                                               const uint32_t* __restrict__ fi, const uint16_t* __restrict__ srt,
+                                               uint16_t* __restrict__ co, uint8_t* __restrict__ so,
+                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred)
+{
+    uint64_t wpos = (it.bit0 >> 5) - wbase;
+    const int skip = (int)(it.bit0 & 31u);
+    uint32_t bad = 0;
+    if (wpos + 2 > nwords) return 1u;
+    uint64_t buf = (((uint64_t)W[wpos] << 32) | W[wpos + 1]) << skip;
+    int nb = 64 - skip;
+    wpos += 2;
+    uint32_t nextw = wpos < nwords ? W[wpos] : 0u;
+    uint64_t u = it.u0;
+    const uint32_t cnt = it.cnt;
+    uint32_t i = 0;
+    auto step = [&]() -> uint32_t {
+        if (nb < 32) {
+            buf |= (uint64_t)nextw << (32 - nb);
+            bad |= wpos >= nwords ? 1u : 0u;
+            ++wpos;
+            nb += 32;
+            nextw = wpos < nwords ? W[wpos] : 0u;
+        }
+        const uint32_t e = lut[(uint32_t)(buf >> (64 - kLutBits))];
+        uint32_t sym;
+        int l;
+        if (e != 0xFFFFFFFFu) {
+            sym = e & 0xFFFFu;
+            l = (int)(e >> 16);
+        } else {
+            const uint32_t win = (uint32_t)(buf >> 32);
+            for (l = kLutBits + 1; l <= kLmax; ++l)
+                if ((win >> (32 - l)) - fc[l] < cn[l]) break;
+            if (l > kLmax) {
+                bad = 1;
+                l = 1;
+                sym = 0;
+            } else {
+                sym = srt[fi[l] + (win >> (32 - l)) - fc[l]];
+            }
+        }
+        buf <<= l;
+        nb -= l;
+        if (sym == 0u) {
+            if (u < n_unpred) {
+                if (tsize == 4) ((uint32_t*)so)[i] = ((const uint32_t*)un)[u];
+                else ((uint64_t*)so)[i] = ((const uint64_t*)un)[u];
+            } else {
+                bad = 1;
+            }
+            u++;
+        }
+        return sym;
+    };
+    if (kGroup == 32) {
+        // 32 codes -> 4 x 16-byte stores
+        for (; i + 32 <= cnt;) {
+            uint32_t r[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t a = step();
+                ++i;
+                const uint32_t b = step();
+                ++i;
+                r[k] = a | (b << 16);
+            }
+            uint4* d = (uint4*)(co + i - 32);
+            d[0] = make_uint4(r[0], r[1], r[2], r[3]);
+            d[1] = make_uint4(r[4], r[5], r[6], r[7]);
+            d[2] = make_uint4(r[8], r[9], r[10], r[11]);
+            d[3] = make_uint4(r[12], r[13], r[14], r[15]);
+        }
+    } else if (kGroup == 4) {
+        for (; i + 4 <= cnt;) {
+            const uint32_t a = step();
+            ++i;
+            const uint32_t b = step();
+            ++i;
+            const uint32_t c = step();
+            ++i;
+            const uint32_t d = step();
+            ++i;
+            *(uint2*)(co + i - 4) = make_uint2(a | (b << 16), c | (d << 16));
+        }
+    }
+    for (; i < cnt; ++i) co[i] = (uint16_t)step();
+    return bad;
+}
+
 // One workgroup = `ipw` consecutive sub-chunks of one stream, one per lane.
 // Their bits form one contiguous range.  kStage: the workgroup first copies
 // that range (up to kDecStageWords words of it) into LDS with coalesced
@@ -707,7 +813,11 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     if (s < 0) return;  // uniform across the workgroup
     const StreamDesc& d = ss.s[s];
     const DecTab* t = tabs + s;
-    for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
+    uint32_t* lut1 = (uint32_t*)lut;  // the single-symbol table of the unstaged form
+    if constexpr (kStage)
+        for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
+    else
+        for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut1[k] = t->lut1[k];
     if (threadIdx.x < kLmax + 2) {
         fc[threadIdx.x] = t->first_code[threadIdx.x];
         cn[threadIdx.x] = t->count[threadIdx.x];
@@ -754,10 +864,18 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     // sees the address space of its bits (LDS reads for the stage, global
     // loads otherwise) instead of generic flat loads
     uint32_t bad;
-    if (in_stage)
-        bad = decode_run(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
-    else
-        bad = decode_run(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+    if constexpr (!kStage) {
+        if ((al & 31u) == 0)
+            bad = decode_run1<32>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+        else if ((al & 3u) == 0)
+            bad = decode_run1<4>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+        else
+            bad = decode_run1<1>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+    } else if (in_stage) {
+        bad = decode_run<true>(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+    } else {
+        bad = decode_run<false>(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+    }
     if (bad) atomicOr(err, 1u);
 }
 
@@ -771,7 +889,7 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
                                                      uint8_t* __restrict__ side,
                                                      uint32_t* __restrict__ err)
 {
-    __shared__ uint64_t lut[1 << kLutBits];
+    __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage);
@@ -791,7 +909,7 @@ __device__ __forceinline__ uint32_t find_dec_job(const DecJob* __restrict__ jobs
 template <bool kStage>
 __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restrict__ jobs, uint32_t njobs)
 {
-    __shared__ uint64_t lut[1 << kLutBits];
+    __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];

@@ -383,9 +383,7 @@ __device__ __forceinline__ void high_dec_point(const T* __restrict__ mid, const 
 // w owns rows y0 + kZRW w .. y0 + kZRW w + kZRW-1, so y and z are
 // wave-uniform; rows and planes on the grid faces take the checked
 // predictors from global memory.  The items after nmarch hold 256 points
-// each of the edge columns (x = 0, dx-2, dx-1, or every x when dx < 4).  The
-// grid is persistent (one resident wave of workgroups), walking the items in
-// the XCD-aware order.
+// each of the edge columns (x = 0, dx-2, dx-1, or every x when dx < 4).
 constexpr int kZR = 8;
 constexpr int kZRW = kZR / 4;  // rows per wave
 static_assert(kZR % 4 == 0, "four waves share the rows of a column item");
@@ -893,10 +891,8 @@ inline unsigned resident_slots(const void* kernel)
     return it->second;
 }
 
-// Plane segments so that the column items fill about one resident wave of
-// workgroups (at least 4 planes per segment to amortise the 3-plane halo),
-// then the grid: resident workgroups, at most one per item, a multiple of 8
-// for the XCD order.
+// Tile form: a persistent grid of resident workgroups, at most one per item,
+// a multiple of 8 for the XCD order.
 template <typename T, int kMode>
 void launch_level_tile(hipStream_t s, LevelK<T> K)
 {
@@ -926,6 +922,11 @@ void launch_level(hipStream_t s, const LevelK<T>& K)
     else launch_level_tile<T, kMode>(s, K);
 }
 
+// Z-march: plane segments so that the column items fill about one resident
+// wave of workgroups (4 to 32 planes per segment: the 3-plane halo is
+// amortised, and on a large grid the workgroups stay short-lived, so launches
+// from other streams -- concurrent requests -- get CUs between them), one
+// workgroup per item.
 template <typename T, int kMode>
 void launch_level_z(hipStream_t s, LevelK<T> K)
 {
@@ -938,13 +939,12 @@ void launch_level_z(hipStream_t s, LevelK<T> K)
     if (nseg < 1) nseg = 1;
     int seg = (int)((g.dz + nseg - 1) / nseg);
     if (seg < 4) seg = 4;
+    if (seg > 32) seg = 32;
     if (seg > g.dz) seg = g.dz;
     K.seg = seg;
     const ZItems Z = z_items(g, seg);
-    uint64_t G = Z.total < slots ? Z.total : slots;
-    if (G >= 64) G &= ~7ull;
-    if (!G) return;
-    k_level_z<T, kMode><<<dim3((unsigned)G), dim3(256), 0, s>>>(K);
+    if (!Z.total) return;
+    k_level_z<T, kMode><<<dim3((unsigned)Z.total), dim3(256), 0, s>>>(K);
 }
 
 }  // namespace

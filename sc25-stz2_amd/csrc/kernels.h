@@ -66,6 +66,7 @@ struct DecHead {
 // decode table of one stream (device memory)
 struct DecTab {
     uint64_t lut[1 << kLutBits];       // multi-symbol entries (above)
+    uint32_t lut1[1 << kLutBits];      // the first code only: (len << 16) | symbol, 0xFFFFFFFF if longer
     uint32_t first_code[kLmax + 2];
     uint32_t count[kLmax + 2];
     uint32_t first_index[kLmax + 2];
