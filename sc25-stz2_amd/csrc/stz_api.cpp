@@ -228,9 +228,9 @@ private:
 struct stz_ctx {
     int device = 0;
     hipStream_t own = nullptr, stream = nullptr;
-    // second stream for work that overlaps the main one, joined by events
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // marks a mid-compress readback, so host codebook builds start while
+    // the stream still runs the later kernels
+    hipEvent_t ev_fork = nullptr;
     std::shared_ptr<Pool> pool;
     void* pinned = nullptr;
     size_t pinned_cap = 0;
@@ -2387,9 +2387,7 @@ int stz_create(stz_ctx** out, int device)
         // a blocking stream: it orders after work on the legacy default
         // stream, where torch puts the caller's tensors by default
         HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
-        HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         c->stream = c->own;
         c->pool = std::make_shared<Pool>();
         c->pool->device = device;
@@ -2412,11 +2410,8 @@ void stz_destroy(stz_ctx* c)
     c->plan.reset();
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (void* p : c->retired) (void)hipHostFree(p);
-    if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->own) (void)hipStreamDestroy(c->own);
-    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     delete c;
 }
 

@@ -12,6 +12,13 @@ all their high boxes.  Requests that arrive while batches are on the GPU wait
 only for the next free worker; batches of different workers overlap on the
 device (the library call releases the GIL).  Latency is arrival to
 completion of the batch that served the request.
+
+The workers' batches overlap only when their streams sit on different
+hardware queues: HIP spreads streams over GPU_MAX_HW_QUEUES queues (4 by
+default, shared with torch's streams), and streams that share one run in
+order, so a box batch then waits for a full decode queued before it.  Set
+GPU_MAX_HW_QUEUES to at least workers + 4 before HIP starts (RoiServer warns
+otherwise).
 """
 import collections
 import threading
@@ -40,8 +47,15 @@ class _Pending:
 class RoiServer:
     def __init__(self, ds, workers=3, max_batch=32, mem_budget=16 << 30, device=0, keep_base=True,
                  keep_results=False):
+        import os
+        import warnings
         import torch
         from .api import Context
+        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        if workers > 1 and hwq < workers + 4:
+            warnings.warn("GPU_MAX_HW_QUEUES=%d: %d worker streams share hardware queues, so batches of "
+                          "different workers run in order; set it to >= %d before HIP starts"
+                          % (hwq, workers, workers + 4))
         self.torch = torch
         self.ds = ds
         self.dev = torch.device("cuda", device)
