@@ -61,20 +61,72 @@ __device__ __forceinline__ int find_stream_by_chunk(const StreamSet& ss, uint32_
     return s;
 }
 
-// The encoder table entries of the codes near the centre (where nearly all
-// codes fall) staged in LDS per workgroup; the rest read global memory.
-constexpr int kEncWin = 2048;
-constexpr uint32_t kEncLo = (uint32_t)kRadius - kEncWin / 2;
+// The encoder table entries of a stream's used symbols (at most kEncWin of
+// them, those around the centre when the range is wider; nearly all codes
+// fall there) staged in LDS per workgroup; the rest read global memory.
+constexpr int kEncWin = (int)kEncWindow;
 
-__device__ __forceinline__ void stage_enc_window(const uint32_t* __restrict__ es, uint32_t* tab)
+__device__ __forceinline__ void stage_enc_window(const uint32_t* __restrict__ es, uint32_t* tab, uint32_t lo,
+                                                 uint32_t n)
 {
-    for (int k = threadIdx.x; k < kEncWin; k += blockDim.x) tab[k] = es[kEncLo + k];
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) tab[k] = es[lo + k];
 }
 
-__device__ __forceinline__ uint32_t enc_entry(const uint32_t* tab, const uint32_t* __restrict__ es, uint32_t q)
+// a position past the sub-chunk's end: encoder entry 0 (no bits), not a zero code
+constexpr uint32_t kNoCode = 0xFFFFFFFFu;
+
+// A lane's 16 codes of sub-chunk j (positions pstart + 1024 j + 16 lane ..
+// +15): two 16-byte loads for a whole aligned sub-chunk, else code by code
+// with kNoCode past its end
+__device__ __forceinline__ void load16(const uint16_t* __restrict__ cs, uint64_t pstart, uint64_t pend, bool al8,
+                                       uint32_t j, int lane, uint32_t (&q)[16])
 {
-    const uint32_t r = q - kEncLo;
-    return r < (uint32_t)kEncWin ? tab[r] : es[q];
+    const uint64_t p0 = pstart + (uint64_t)j * kSub;
+    const uint64_t p1 = p0 + kSub < pend ? p0 + kSub : pend;
+    const uint64_t pl = p0 + (uint64_t)lane * 16;
+    if (al8 && p1 - p0 == kSub) {
+        const uint4 a = *(const uint4*)(cs + pl);
+        const uint4 b = *(const uint4*)(cs + pl + 8);
+        const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            q[2 * k] = w8[k] & 0xFFFFu;
+            q[2 * k + 1] = w8[k] >> 16;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) q[k] = pl + k < p1 ? (uint32_t)cs[pl + k] : kNoCode;
+    }
+}
+
+// encoder entries of a lane's 16 codes: every code reads the LDS window
+// (clamped index), and only a lane holding a code outside it takes the
+// branch to global memory -- one branch per 16 codes instead of one each
+__device__ __forceinline__ void lookup16(const uint32_t* tab, const uint32_t* __restrict__ es, uint32_t lo,
+                                         uint32_t n, const uint32_t (&q)[16], uint32_t (&e)[16])
+{
+    uint32_t outm = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t r = q[k] - lo;
+        const bool inw = r < n;
+        e[k] = tab[inw ? r : 0u];
+        outm |= inw ? 0u : (1u << k);
+    }
+    if (outm) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if ((outm >> k) & 1u) e[k] = q[k] <= 0xFFFFu ? es[q[k]] : 0u;
+    }
+}
+
+// orders one wave's LDS accesses around this point (a wave's LDS operations
+// execute in issue order, so only the compiler has to be held back)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
@@ -159,28 +211,23 @@ __device__ __forceinline__ void huff_len_body(const StreamSet& ss, const uint16_
     const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
     const uint16_t* cs = codes + d.code_off;
     const uint32_t* es = enc + (size_t)s * 65536;
-    stage_enc_window(es, tab);
+    stage_enc_window(es, tab, d.enc_lo, d.enc_n);
     __syncthreads();
     uint16_t* sbits = (uint16_t*)(payload + d.off_sub_bits);
     uint16_t* sun = (uint16_t*)(payload + d.off_sub_unpred);
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const bool al8 = ((d.code_off + pstart) & 7) == 0;  // sub-chunks start 1024 codes apart
     for (uint32_t j = wid; j < nsub; j += 4) {
-        const uint64_t p0 = pstart + (uint64_t)j * kSub;
-        const uint64_t p1 = p0 + kSub < pend ? p0 + kSub : pend;
+        // the lane's 16 codes first (two 16-byte loads for a whole aligned
+        // sub-chunk), then 16 independent table reads
+        uint32_t q[16], e[16];
+        load16(cs, pstart, pend, al8, j, lane, q);
+        lookup16(tab, es, d.enc_lo, d.enc_n, q, e);
         uint32_t bits = 0, un = 0;
-        // all 16 codes of the lane first, then 16 independent table reads
-        uint32_t q[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const uint64_t p = p0 + lane + (uint64_t)k * kWave;
-            q[k] = p < p1 ? (uint32_t)cs[p] : 0xFFFFFFFFu;
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (q[k] != 0xFFFFFFFFu) {
-                bits += enc_entry(tab, es, q[k]) >> 24;
-                un += q[k] == 0u;
-            }
+            bits += e[k] >> 24;
+            un += q[k] == 0u ? 1u : 0u;
         }
         bits = wave_sum_u32(bits);
         un = wave_sum_u32(un);
@@ -286,65 +333,56 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
     const uint64_t pend = pstart + chunk_volume(d.g, c, d.n_symbols);
     const uint16_t* cs = codes + d.code_off;
     const uint32_t* es = enc + (size_t)s * 65536;
-    stage_enc_window(es, tab);
+    stage_enc_window(es, tab, d.enc_lo, d.enc_n);
     __syncthreads();
     uint32_t* bw = (uint32_t*)(payload + d.off_bits);
     uint8_t* un_out = payload + d.off_unpred;
     const uint8_t* side_s = side + d.code_off * d.tsize;
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
     uint32_t* wb = wbuf[wid];
-    const uint32_t rounds = (nsub + 3) / 4;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint32_t j = r * 4 + wid;
-        const bool active = j < nsub;
-        uint32_t q[16];
-        uint32_t e[16];
-        uint32_t lbits = 0, lzero = 0, nW = 0, off = 0;
-        uint64_t bit0 = 0, u0 = 0, pl = 0, p1 = 0;
-        uint32_t eb_ = 0, eu_ = 0;
-        if (active) {
-            const uint32_t gs = d.sub_base + sb + j;
-            bit0 = cbit0 + sub_pre_bits[gs];
-            u0 = cun0 + sub_pre_un[gs];
-            const uint64_t p0 = pstart + (uint64_t)j * kSub;
-            p1 = p0 + kSub < pend ? p0 + kSub : pend;
-            pl = p0 + (uint64_t)lane * 16;
-            if (((d.code_off + pl) & 7) == 0 && pl + 16 <= p1) {
-                const uint4 a = *(const uint4*)(cs + pl);
-                const uint4 b = *(const uint4*)(cs + pl + 8);
-                const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    const bool al8 = ((d.code_off + pstart) & 7) == 0;  // sub-chunks start 1024 codes apart
+    // each wave packs its own sub-chunks in its own word buffer: no
+    // workgroup barriers between the rounds
+    for (uint32_t j = wid; j < nsub; j += 4) {
+        const uint32_t gs = d.sub_base + sb + j;
+        const uint64_t bit0 = cbit0 + sub_pre_bits[gs];
+        const uint64_t u0 = cun0 + sub_pre_un[gs];
+        const uint64_t pl = pstart + (uint64_t)j * kSub + (uint64_t)lane * 16;
+        uint32_t q[16], e[16];
+        load16(cs, pstart, pend, al8, j, lane, q);
+        lookup16(tab, es, d.enc_lo, d.enc_n, q, e);
+        uint32_t lbits = 0, lzero = 0;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    q[2 * k] = w8[k] & 0xFFFFu;
-                    q[2 * k + 1] = w8[k] >> 16;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) q[k] = pl + k < p1 ? (uint32_t)cs[pl + k] : 0u;
-            }
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const bool in = pl + k < p1;
-                e[k] = in ? enc_entry(tab, es, q[k]) : 0u;
-                lbits += e[k] >> 24;
-                lzero += (in && q[k] == 0u) ? 1u : 0u;
-            }
-            const uint32_t ib = wave_incl_scan_u32(lbits);
-            const uint32_t iu = wave_incl_scan_u32(lzero);
-            const uint32_t tot = __shfl((int)ib, kWave - 1, kWave);
-            eb_ = ib - lbits;
-            eu_ = iu - lzero;
-            off = (uint32_t)(bit0 & 31u);
-            nW = (off + tot + 31) >> 5;
-            for (uint32_t w = lane; w < nW; w += kWave) wb[w] = 0u;
+        for (int k = 0; k < 16; ++k) {
+            lbits += e[k] >> 24;
+            lzero += q[k] == 0u ? 1u : 0u;
         }
-        __syncthreads();
-        if (active) {
-            // assemble whole words in registers; the first may be shared with the previous lane
-            uint32_t pos = off + eb_;
+        const uint32_t ib = wave_incl_scan_u32(lbits);
+        const uint32_t iu = wave_incl_scan_u32(lzero);
+        const uint32_t tot = __shfl((int)ib, kWave - 1, kWave);
+        const uint32_t off = (uint32_t)(bit0 & 31u);
+        const uint32_t nW = (off + tot + 31) >> 5;
+        for (uint32_t w = lane; w < nW; w += kWave) wb[w] = 0u;
+        wave_lds_sync();
+        // the lane's bits go to [pos, pos + lbits) of the word buffer, MSB first
+        const uint32_t pos = off + ib - lbits;
+        const uint32_t o = pos & 31u;
+        if (o + lbits <= 64u) {
+            // they span at most two words: assemble one 64-bit value
+            uint64_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc = (acc << (e[k] >> 24)) | (e[k] & 0xFFFFFFu);
+            if (lbits) {
+                const uint64_t v = acc << (64u - lbits);
+                const uint32_t w0 = pos >> 5;
+                atomicOr(&wb[w0], (uint32_t)(v >> (32u + o)));
+                if (o + lbits > 32u) atomicOr(&wb[w0 + 1], (uint32_t)(v >> o));
+            }
+        } else {
+            // whole words as they fill; the first may be shared with the previous lane
             uint32_t wcur = pos >> 5;
             uint64_t acc = 0;
-            int nacc = (int)(pos & 31u);
+            int nacc = (int)o;
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int l = (int)(e[k] >> 24);
@@ -358,11 +396,13 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
                 }
             }
             if (nacc > 0) atomicOr(&wb[wcur], (uint32_t)(acc << (32 - nacc)));
-            // unpredictable values of this lane, in stream order
-            uint64_t u = u0 + eu_;
+        }
+        // unpredictable values of this lane, in stream order
+        if (lzero) {
+            uint64_t u = u0 + iu - lzero;
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
-                if (pl + k < p1 && q[k] == 0u) {
+                if (q[k] == 0u) {
                     if (d.tsize == 4)
                         ((uint32_t*)un_out)[u] = ((const uint32_t*)side_s)[pl + k];
                     else
@@ -371,19 +411,17 @@ __device__ __forceinline__ void huff_pack_body(const StreamSet& ss, const uint16
                 }
             }
         }
-        __syncthreads();
-        if (active) {
-            const uint64_t W0 = bit0 >> 5;
-            for (uint32_t w = lane; w < nW; w += kWave) {
-                const uint32_t v = wb[w];
-                if (w == 0 || w == nW - 1) {
-                    if (v) atomicOr(&bw[W0 + w], v);
-                } else {
-                    bw[W0 + w] = v;
-                }
+        wave_lds_sync();
+        const uint64_t W0 = bit0 >> 5;
+        for (uint32_t w = lane; w < nW; w += kWave) {
+            const uint32_t v = wb[w];
+            if (w == 0 || w == nW - 1) {
+                if (v) atomicOr(&bw[W0 + w], v);
+            } else {
+                bw[W0 + w] = v;
             }
         }
-        __syncthreads();
+        wave_lds_sync();  // the next round clears the buffer
     }
 }
 

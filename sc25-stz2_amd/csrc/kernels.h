@@ -10,6 +10,9 @@ namespace stz {
 // Per-stream view used by the entropy-coding kernels.  Offsets *_off are byte
 // offsets into the payload buffer; code_off is the element offset of the
 // stream inside the codes / side arrays.
+// encoder-table entries an encode workgroup stages in LDS (per stream)
+constexpr uint32_t kEncWindow = 2048;
+
 struct StreamDesc {
     uint64_t code_off;
     uint64_t n_symbols;
@@ -23,12 +26,12 @@ struct StreamDesc {
     uint32_t item_base;      // first decode work item of this stream (256-aligned)
     uint32_t item_count;     // decode work items of this stream
     uint32_t tsize;          // bytes per raw value (4 or 8)
-    uint32_t pad_;
+    uint32_t enc_lo;         // encode: the symbols enc_lo .. enc_lo + enc_n - 1 have
     uint64_t n_unpred;       // unpredictable values stored in the stream
     uint32_t ipw;            // decode work items per workgroup (<= 256)
     uint32_t wg_base;        // first decode workgroup of this stream
     uint32_t cm_base;        // first entry of this stream in a chunk map
-    uint32_t pad2_;
+    uint32_t enc_n;          //   their encoder-table entries staged in LDS (<= 2048)
 };
 
 struct StreamSet {

@@ -1642,7 +1642,26 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 encl[ne++] = m.cb[s2].code[k] | ((uint32_t)m.cb[s2].len[k] << 24);
             }
         }
-        const StreamSet ss = make_streamset(f.L, m, f.dtype);
+        StreamSet ss = make_streamset(f.L, m, f.dtype);
+        // encoder-table window staged in LDS by the encode kernels: the used
+        // symbols, or the kEncWindow of them nearest the centre
+        for (int s2 = 0; s2 < kStreams; ++s2) {
+            const std::vector<uint16_t>& sym = m.cb[s2].sym;  // ascending
+            StreamDesc& d = ss.s[s2];
+            d.enc_lo = 0;
+            d.enc_n = 0;
+            if (sym.empty()) continue;
+            const uint32_t a = sym.front(), b = sym.back();
+            if (b - a + 1 <= kEncWindow) {
+                d.enc_lo = a;
+                d.enc_n = b - a + 1;
+            } else {
+                // the centre window, moved inside [a, b]
+                const uint32_t c0 = (uint32_t)kRadius - kEncWindow / 2;
+                d.enc_lo = c0 < a ? a : (c0 > b + 1 - kEncWindow ? b + 1 - kEncWindow : c0);
+                d.enc_n = kEncWindow;
+            }
+        }
         enc[i] = DBuf(pool, (size_t)kStreams * 65536 * 4);
         pre_b[i] = DBuf(pool, (size_t)ss.total_sub * 4 + 16);
         pre_u[i] = DBuf(pool, (size_t)ss.total_sub * 4 + 16);
