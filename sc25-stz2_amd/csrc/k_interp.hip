@@ -315,7 +315,8 @@ template <typename T>
 void launch_interp_encode(hipStream_t s, const InterpPass& p, T* v, uint16_t* codes, T* side, uint32_t* hist)
 {
     if (!p.n) return;
-    k_interp_encode<T><<<dim3(pass_grid(p.n)), dim3(256), 0, s>>>(p, v, codes, side, hist);
+    const unsigned g = pass_grid(p.n);
+    k_interp_encode<T><<<dim3(g < kPassEncMaxWgs ? g : kPassEncMaxWgs), dim3(256), 0, s>>>(p, v, codes, side, hist);
 }
 
 template <typename T>

@@ -895,10 +895,18 @@ inline unsigned resident_slots(const void* kernel)
 
 // Tile form: a persistent grid of resident workgroups, at most one per item,
 // a multiple of 8 for the XCD order.
-template <typename T, int kMode>
-void launch_level_tile(hipStream_t s, LevelK<T> K)
+// at most `share` of the slots, at least one XCD-ordered round of 8
+inline unsigned shared_slots(unsigned slots, float share)
 {
-    const unsigned slots = resident_slots((const void*)k_level_tile<T, kMode>);
+    if (share >= 1.0f) return slots;
+    const unsigned s = (unsigned)((float)slots * share);
+    return s < 8u ? 8u : s;
+}
+
+template <typename T, int kMode>
+void launch_level_tile(hipStream_t s, LevelK<T> K, float share)
+{
+    const unsigned slots = shared_slots(resident_slots((const void*)k_level_tile<T, kMode>), share);
     int ntx, nty;
     uint64_t ntiles, total;
     tile_items<tile_rows<kMode>()>(K.g, ntx, nty, ntiles, total);
@@ -909,7 +917,7 @@ void launch_level_tile(hipStream_t s, LevelK<T> K)
 }
 
 template <typename T, int kMode>
-void launch_level_z(hipStream_t s, LevelK<T> K);
+void launch_level_z(hipStream_t s, LevelK<T> K, float share);
 
 // level launch form per mode: encode is VALU-bound and keeps the higher
 // occupancy of the tile form; decode is latency-bound and takes the z-march
@@ -917,11 +925,11 @@ constexpr bool kZMarchEnc = false;
 constexpr bool kZMarchDec = true;
 
 template <typename T, int kMode>
-void launch_level(hipStream_t s, const LevelK<T>& K)
+void launch_level(hipStream_t s, const LevelK<T>& K, float share = 1.0f)
 {
     constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
-    if ((kEnc && kZMarchEnc) || (!kEnc && kZMarchDec)) launch_level_z<T, kMode>(s, K);
-    else launch_level_tile<T, kMode>(s, K);
+    if ((kEnc && kZMarchEnc) || (!kEnc && kZMarchDec)) launch_level_z<T, kMode>(s, K, share);
+    else launch_level_tile<T, kMode>(s, K, share);
 }
 
 // Z-march: plane segments so that the column items fill about one resident
@@ -930,7 +938,7 @@ void launch_level(hipStream_t s, const LevelK<T>& K)
 // persistent grid of resident workgroups, a multiple of 8 for the XCD order.
 // (One workgroup per item measured 30% slower at 256^3 per sub-band.)
 template <typename T, int kMode>
-void launch_level_z(hipStream_t s, LevelK<T> K)
+void launch_level_z(hipStream_t s, LevelK<T> K, float share)
 {
     const void* fn = (const void*)k_level_z<T, kMode>;
     const unsigned slots = resident_slots(fn);
@@ -945,7 +953,8 @@ void launch_level_z(hipStream_t s, LevelK<T> K)
     if (seg > g.dz) seg = g.dz;
     K.seg = seg;
     const ZItems Z = z_items(g, seg);
-    uint64_t G = Z.total < slots ? Z.total : slots;
+    const unsigned gs = shared_slots(slots, share);
+    uint64_t G = Z.total < gs ? Z.total : gs;
     if (G >= 64) G &= ~7ull;
     if (!G) return;
     k_level_z<T, kMode><<<dim3((unsigned)G), dim3(256), 0, s>>>(K);
@@ -977,22 +986,22 @@ void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, co
 
 template <typename T>
 void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
-                       T* mid)
+                       T* mid, float share)
 {
     LevelK<T> K{};
     K.ref = base; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = mid;
-    launch_level<T, kLowDec>(s, K);
+    launch_level<T, kLowDec>(s, K, share);
 }
 
 template <typename T>
 void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
-                        T* out, int nx, int ny, int nz)
+                        T* out, int nx, int ny, int nz, float share)
 {
     LevelK<T> K{};
     K.ref = mid; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = out;
     K.nx = nx; K.ny = ny; K.nz = nz;
     K.vec = ((nx & 1) == 0) && (((uintptr_t)out % (2 * sizeof(T))) == 0);
-    launch_level<T, kHighDec>(s, K);
+    launch_level<T, kHighDec>(s, K, share);
 }
 
 template <typename T>
@@ -1053,9 +1062,10 @@ void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t 
                                        uint16_t*, T*, uint32_t*, T*);                                            \
     template void launch_high_encode<T>(hipStream_t, const T*, int, int, int, const T*, Geom, LevelArgs,        \
                                         uint16_t*, T*, uint32_t*);                                               \
-    template void launch_low_decode<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, T*);  \
+    template void launch_low_decode<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, T*,  \
+                                       float);                                                                   \
     template void launch_high_decode<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, T*,  \
-                                        int, int, int);                                                          \
+                                        int, int, int, float);                                                   \
     template void launch_low_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*, const T*, \
                                              T*, int, int, const BoxJob*, uint32_t, uint32_t);                  \
     template void launch_high_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*,         \

@@ -206,12 +206,14 @@ void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, con
 template <typename T>
 void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, const T* mid, Geom g,
                         LevelArgs a, uint16_t* codes, T* side, uint32_t* hist);
+// share: the persistent grid takes at most that fraction of the resident
+// workgroup slots (the rest stay free for other streams' launches)
 template <typename T>
 void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
-                       const T* side, T* mid);
+                       const T* side, T* mid, float share = 1.0f);
 template <typename T>
 void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const uint16_t* codes,
-                        const T* side, T* out, int nx, int ny, int nz);
+                        const T* side, T* out, int nx, int ny, int nz, float share = 1.0f);
 // random access: mid values on boxes of the mid grid -> mid array (full mid dims)
 template <typename T>
 void launch_low_decode_boxes(hipStream_t s, const T* base, Geom g, LevelArgs a, const uint16_t* codes,
@@ -271,6 +273,14 @@ struct InterpPassList {
     InterpPass p[kMaxFusedPasses];  // start/step/s/dims in lattice units
 };
 constexpr int kLatticeBytes = 128 * 1024;
+// The fused kernel is one workgroup, so its lattice is also held to
+// kFusedPoints (16^3): the finer levels run as passes over the whole chip
+// (at 128^3, 23 us for the 16^3 lattice plus three stride-4 passes, where the
+// 32^3 lattice took 86 us in one workgroup).
+constexpr int kFusedPoints = 4096;
+// An encode pass flushes one LDS histogram per workgroup with global
+// atomics on the same bins: at most this many workgroups per pass.
+constexpr unsigned kPassEncMaxWgs = 512;
 // many fields: the fused coarse passes one workgroup per field; a fine pass
 // of every field in one launch (workgroups wg0 .. wg0 + nwg - 1 per field)
 struct FusedJob {

@@ -246,6 +246,9 @@ struct stz_ctx {
     std::map<std::string, LaunchCfg> tuned;
     bool autotune = true;
     int n_cu = 0;
+    // fraction of the resident workgroup slots the persistent full-level
+    // decode kernels take (stz_set_grid_share)
+    float grid_share = 1.0f;
     // captured batch decode (graph + the buffers it reads and writes)
     std::unique_ptr<BatchPlan> plan;
     // learned tuner model (fp32 MLP parameters on the device)
@@ -533,7 +536,7 @@ StreamSet make_streamset(const HostLayout& L, const Meta& m, int dtype)
 }
 
 // base-codec schedule: the coarse passes (stride >= S0, S0 the smallest
-// power of two whose lattice fits the LDS budget) run in the fused
+// power of two whose lattice fits the LDS budget and kFusedPoints) run in the fused
 // single-workgroup kernel on an LDS copy of the lattice; every finer pass gets
 // a launch of its own.  Stream order is unchanged.
 template <typename FF, typename FP>
@@ -541,7 +544,7 @@ void run_passes(const std::vector<InterpPass>& passes, size_t tsize, FF&& fused,
 {
     if (passes.empty()) return;
     const int bx = passes[0].bx, by = passes[0].by, bz = passes[0].bz;
-    const size_t cap = kLatticeBytes / tsize;
+    const size_t cap = std::min<size_t>(kLatticeBytes / tsize, (size_t)kFusedPoints);
     int S0 = 1;
     auto lat = [&](int S, int d) { return (d - 1) / S + 1; };
     while ((size_t)lat(S0, bx) * lat(S0, by) * lat(S0, bz) > cap) S0 <<= 1;
@@ -971,11 +974,14 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));
     decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
     const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
-    K(c, "low_decode", [&] { launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
+    K(c, "low_decode", [&] {
+        launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                             c->grid_share);
+    });
     OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
     K(c, "high_decode", [&] {
         launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
-                              (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz);
+                              (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, c->grid_share);
     });
     ob.copy_back(c);
     err.fetch(c);
@@ -1108,7 +1114,10 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
         if (need_mid) mid = DBuf(c->pool, (size_t)mx * my * mz * sizeof(T));
         const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
         if (need_lowfull)
-            K(c, "low_decode", [&] { launch_low_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>()); });
+            K(c, "low_decode", [&] {
+                launch_low_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                                     c->grid_share);
+            });
         // box jobs: low regions first, then the high boxes
         std::vector<BoxJob> lowj, highj;
         uint32_t lowb = 0, highb = 0;
@@ -1450,7 +1459,7 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 j.codes = c0;
                 j.side = s0;
                 j.hist = f.hist.template as<uint32_t>();
-                j.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, 4096);
+                j.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, kPassEncMaxWgs);
                 rounds[r].push_back(j);
                 ++r;
             });
@@ -2442,6 +2451,13 @@ int stz_set_stream(stz_ctx* c, void* s)
 }
 
 void* stz_get_stream(stz_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int stz_set_grid_share(stz_ctx* c, float share)
+{
+    if (!c || !(share > 0.0f && share <= 1.0f)) return STZ_E_ARG;
+    c->grid_share = share;
+    return STZ_OK;
+}
 
 int stz_profile(stz_ctx* c, int enable)
 {

@@ -15,7 +15,8 @@ ERRORS = {-1: "STZ_E_ARG", -2: "STZ_E_DIMS", -3: "STZ_E_STREAM", -4: "STZ_E_OOM"
 
 # every entry point include/stz.h declares
 EXPORTS = [
-    "stz_create", "stz_destroy", "stz_set_stream", "stz_get_stream", "stz_profile", "stz_profile_report",
+    "stz_create", "stz_destroy", "stz_set_stream", "stz_get_stream", "stz_set_grid_share", "stz_profile",
+    "stz_profile_report",
     "stz_default_params", "stz_strerror", "stz_free",
     "stz_compress", "stz_decompress", "stz_decompress_roi", "stz_decompress_slice", "stz_decompress_level",
     "stz_query_info",
@@ -90,6 +91,7 @@ def lib():
     L.stz_set_stream.argtypes = [vp, vp]
     L.stz_get_stream.argtypes = [vp]
     L.stz_get_stream.restype = vp
+    L.stz_set_grid_share.argtypes = [vp, C.c_float]
     L.stz_profile.argtypes = [vp, i32]
     L.stz_profile_report.argtypes = [vp, C.c_char_p, sz]
     L.stz_default_params.argtypes = [P(Params)]

@@ -120,11 +120,12 @@ class DStream:
         lib().stz_free(p)
         return out
 
-    def decompress(self, out=None, like=None):
+    def decompress(self, out=None, like=None, ctx=None):
         i = self.info
+        c = ctx if ctx is not None else self.ctx
         if out is None:
             out = _empty_like_kind(like if like is not None else np.empty(0), (i.nz, i.ny, i.nx), i.dtype)
-        check(lib().stz_dstream_decompress(self.ctx.h, self.h, _ptr(out)), "stz_dstream_decompress")
+        check(lib().stz_dstream_decompress(c.h, self.h, _ptr(out)), "stz_dstream_decompress")
         return out
 
     def decompress_roi(self, x0, y0, z0, x1, y1, z1, like=None):
@@ -226,6 +227,11 @@ class Context:
 
     def stream(self):
         return lib().stz_get_stream(self.h)
+
+    def set_grid_share(self, share):
+        """Persistent full-level decode kernels of this context take at most
+        this fraction of the resident workgroup slots (serving)."""
+        check(lib().stz_set_grid_share(self.h, float(share)), "stz_set_grid_share")
 
     def profile(self, enable=True):
         check(lib().stz_profile(self.h, 1 if enable else 0), "stz_profile")

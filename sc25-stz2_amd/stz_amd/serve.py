@@ -18,7 +18,9 @@ hardware queues: HIP spreads streams over GPU_MAX_HW_QUEUES queues (4 by
 default, shared with torch's streams), and streams that share one run in
 order, so a box batch then waits for a full decode queued before it.  Set
 GPU_MAX_HW_QUEUES to at least workers + 4 before HIP starts (RoiServer warns
-otherwise).
+otherwise).  The full-decode worker's persistent level kernels take only
+`full_share` of the resident workgroup slots (stz_set_grid_share), so a box
+batch finds free CUs while a full decode runs.
 """
 import collections
 import threading
@@ -46,7 +48,7 @@ class _Pending:
 
 class RoiServer:
     def __init__(self, ds, workers=3, max_batch=32, mem_budget=16 << 30, device=0, keep_base=True,
-                 keep_results=False):
+                 keep_results=False, full_share=0.5):
         import os
         import warnings
         import torch
@@ -66,6 +68,10 @@ class RoiServer:
         self.mem_budget = mem_budget
         self.keep_results = keep_results
         self.ctxs = [Context(device) for _ in range(workers)]
+        if workers > 1:
+            # the full-resolution decodes' persistent kernels leave part of
+            # the GPU to the other workers' batches
+            self.ctxs[0].set_grid_share(full_share)
         ds.prepare(keep_base=keep_base, ctx=self.ctxs[0])
         self.queue = collections.deque()
         self.cv = threading.Condition()

@@ -45,6 +45,7 @@ def test_null_arguments_rejected(stz):
     assert L.stz_query_info(None, 0, C.byref(info)) == -1
     assert L.stz_compress(None, 0, None, 1, 1, 1, 1e-3, 1, None, None, None) == -1
     assert L.stz_decompress(None, None, 0, None) == -1
+    assert L.stz_set_grid_share(None, 0.5) == -1
 
 
 def test_no_device_is_an_error_not_a_fallback(stz):

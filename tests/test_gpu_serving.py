@@ -92,6 +92,31 @@ def test_box_past_the_field_is_an_error(stz, ctx):
     ds.free()
 
 
+def test_grid_share_leaves_results_unchanged(stz, ctx):
+    # 256^3: the persistent level grids exceed a quarter of the resident slots
+    import torch
+    from stz_amd import fields
+    dev = torch.device("cuda:0")
+    g = fields.grf_device((256, 256, 256), -3.0, 21, dev, torch.float32)
+    ds = ctx.compress_device(g, 1e-3, "rel")
+    want = ds.decompress()
+    want2 = ds.decompress_batch([("level", 2)])[0]
+    c2 = stz.Context(0)
+    try:
+        c2.set_grid_share(0.25)
+        got = ds.decompress(ctx=c2)
+        got2 = ds.decompress_batch([("level", 2)], ctx=c2)[0]
+        with pytest.raises(stz.StzError):
+            c2.set_grid_share(0.0)
+        with pytest.raises(stz.StzError):
+            c2.set_grid_share(1.5)
+    finally:
+        c2.close()
+    assert np.array_equal(_bits(got), _bits(want))
+    assert np.array_equal(_bits(got2), _bits(want2))
+    ds.free()
+
+
 def test_server_results_identical_alone_and_in_mix(stz, ctx):
     import torch
     from stz_amd import fields, serve
