@@ -16,6 +16,7 @@ namespace stz {
 namespace {
 
 constexpr int kHW = 1024;
+constexpr int kFusedDecThreads = 1024;
 constexpr uint32_t kHLo = kRadius - kHW / 2;
 
 template <typename T>
@@ -172,7 +173,7 @@ template <typename T>
 __device__ __forceinline__ void lattice_io(const InterpPassList& l, T* __restrict__ L, T* __restrict__ v, bool load)
 {
     const uint32_t n = (uint32_t)l.lx * l.ly * l.lz;
-    for (uint32_t k = threadIdx.x; k < n; k += 1024u) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
         const uint32_t t = k / (uint32_t)l.lx;
         const int x = (int)(k - t * (uint32_t)l.lx);
         const int z = (int)(t / (uint32_t)l.ly);
@@ -189,7 +190,7 @@ __device__ __forceinline__ void interp_encode_fused_wg(const InterpPassList& l, 
                                                        uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t sh[kHW];
-    __shared__ T L[kLatticeBytes / sizeof(T)];
+    __shared__ T L[kFusedPoints];
     for (int k = threadIdx.x; k < kHW; k += 1024) sh[k] = 0u;
     lattice_io(l, L, v, true);
     __syncthreads();
@@ -245,7 +246,7 @@ __device__ __forceinline__ void interp_decode_fused_wg(const InterpPassList& l, 
 {
     for (int pi = 0; pi < l.n; ++pi) {
         const InterpPass& p = l.p[pi];
-        for (uint64_t k = threadIdx.x; k < p.n; k += 1024u) {
+        for (uint64_t k = threadIdx.x; k < p.n; k += blockDim.x) {
             const uint32_t q = codes[p.pos + k];
             if (p.axis < 0) {
                 L[0] = q ? dequantize<T>(q, p.eb) : side[p.pos + k];
@@ -261,18 +262,18 @@ __device__ __forceinline__ void interp_decode_fused_wg(const InterpPassList& l, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_interp_decode_fused(InterpPassList l, T* __restrict__ v,
+__global__ __launch_bounds__(kFusedDecThreads) void k_interp_decode_fused(InterpPassList l, T* __restrict__ v,
                                                               const uint16_t* __restrict__ codes,
                                                               const T* __restrict__ side)
 {
-    __shared__ T L[kLatticeBytes / sizeof(T)];
+    __shared__ T L[kFusedPoints];
     interp_decode_fused_wg(l, v, codes, side, L);
 }
 
 template <typename T>
-__global__ __launch_bounds__(1024) void k_interp_decode_fused_multi(const FusedJob* __restrict__ jobs)
+__global__ __launch_bounds__(kFusedDecThreads) void k_interp_decode_fused_multi(const FusedJob* __restrict__ jobs)
 {
-    __shared__ T L[kLatticeBytes / sizeof(T)];
+    __shared__ T L[kFusedPoints];
     const FusedJob& J = jobs[blockIdx.x];
     interp_decode_fused_wg(J.l, (T*)J.v, J.codes, (const T*)J.side, L);
 }
@@ -337,14 +338,14 @@ template <typename T>
 void launch_interp_decode_fused(hipStream_t s, const InterpPassList& l, T* v, const uint16_t* codes, const T* side)
 {
     if (l.n <= 0) return;
-    k_interp_decode_fused<T><<<dim3(1), dim3(1024), 0, s>>>(l, v, codes, side);
+    k_interp_decode_fused<T><<<dim3(1), dim3(kFusedDecThreads), 0, s>>>(l, v, codes, side);
 }
 
 template <typename T>
 void launch_interp_decode_fused_multi(hipStream_t s, const FusedJob* jobs, uint32_t njobs)
 {
     if (!njobs) return;
-    k_interp_decode_fused_multi<T><<<dim3(njobs), dim3(1024), 0, s>>>(jobs);
+    k_interp_decode_fused_multi<T><<<dim3(njobs), dim3(kFusedDecThreads), 0, s>>>(jobs);
 }
 
 template <typename T>

@@ -231,6 +231,20 @@ struct stz_ctx {
     // marks a mid-compress readback, so host codebook builds start while
     // the stream still runs the later kernels
     hipEvent_t ev_fork = nullptr;
+    // second stream of a full decode (created on first use): the high
+    // streams' Huffman decode runs there while the base and low levels
+    // decode on `stream`; ev_items / ev_high order it
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_items = nullptr, ev_high = nullptr;
+    hipStream_t second()
+    {
+        if (!side_stream) {
+            HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&ev_items, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&ev_high, hipEventDisableTiming));
+        }
+        return side_stream;
+    }
     std::shared_ptr<Pool> pool;
     void* pinned = nullptr;
     size_t pinned_cap = 0;
@@ -776,6 +790,8 @@ void layout_of(const Meta& m, HostLayout& L)
 // cmap[cm_base[s] + chunk].
 struct Decoded {
     DBuf codes, side, cmap;
+    DBuf items;             // decode work items (read by a launch on the second stream)
+    bool split = false;     // streams kSplitStream.. decode on the second stream: join before use
     uint64_t code_base[kStreams] = {};
     uint32_t cm_base[kStreams] = {};
     bool compact = false;
@@ -800,10 +816,16 @@ struct ErrFlag {
     }
 };
 
+// Streams 0 .. kSplitStream-1 (base and low) feed the base and low levels;
+// the high streams, about 7/8 of the symbols, only the high level.
+constexpr int kSplitStream = 8;
+
 // Enqueue the Huffman decode of full[s] streams (every chunk) and of the
-// chunks lists[s] of the others.
+// chunks lists[s] of the others.  split: the high streams decode on the
+// context's second stream, overlapping the base and low levels' decode on
+// `stream`; join_high() orders `stream` after them.
 void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* full,
-                 const std::vector<int64_t>* lists, Decoded& out, uint32_t* err)
+                 const std::vector<int64_t>* lists, Decoded& out, uint32_t* err, bool split = false)
 {
     ensure_tabs(c, ds);
     hipStream_t st = c->stream;
@@ -885,7 +907,8 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
     out.cmap = DBuf(c->pool, (size_t)cm * 4 + 16);
     if (!all_full) K(c, "cmap_fill", [&] { launch_fill_u32(st, out.cmap.as<uint32_t>(), cm, kNoChunk); });
     if (!item) return;
-    DBuf items(c->pool, (size_t)item * sizeof(DecItem));
+    out.items = DBuf(c->pool, (size_t)item * sizeof(DecItem));
+    DBuf& items = out.items;
     uint8_t* pay = ds->payload.as<uint8_t>();
     if (all_full) {
         K(c, "dec_prep", [&] { launch_dec_prep(st, ss, pay, nullptr, 0, items.as<DecItem>(), nullptr); });
@@ -899,11 +922,60 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
                             out.cmap.as<uint32_t>());
         });
     }
+    if (split && ss.s[kSplitStream].wg_base > 0 && ss.s[kSplitStream].wg_base < ss.total_wgs) {
+        // two launches over one item array: streams 0 .. kSplitStream-1 here,
+        // the rest (their workgroup ids rebased, their tables offset) on the
+        // second stream once the items exist
+        StreamSet lo = ss, hi = ss;
+        const uint32_t w0 = ss.s[kSplitStream].wg_base;
+        lo.nstreams = kSplitStream;
+        lo.total_wgs = w0;
+        hi.nstreams = kStreams - kSplitStream;
+        for (int k = 0; k < (int)hi.nstreams; ++k) {
+            hi.s[k] = ss.s[kSplitStream + k];
+            hi.s[k].wg_base -= w0;
+        }
+        hi.total_wgs = ss.total_wgs - w0;
+        hipStream_t s2 = c->second();
+        HIPCHK(hipEventRecord(c->ev_items, st));
+        HIPCHK(hipStreamWaitEvent(s2, c->ev_items, 0));
+        K(c, "huff_decode", [&] {
+            launch_huff_decode(st, lo, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
+                               out.codes.as<uint16_t>(), out.side.p, err, use_stage);
+            launch_huff_decode(s2, hi, pay, ds->tabs.as<DecTab>() + kSplitStream, ds->sorted.as<uint16_t>(),
+                               items.as<DecItem>(), out.codes.as<uint16_t>(), out.side.p, err, use_stage);
+        });
+        HIPCHK(hipEventRecord(c->ev_high, s2));
+        out.split = true;
+        return;
+    }
     K(c, "huff_decode", [&] {
         launch_huff_decode(st, ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
                            out.codes.as<uint16_t>(), out.side.p, err, use_stage);
     });
 }
+
+// `stream` waits for the high streams' decode of a split huff_decode
+void join_high(stz_ctx* c, Decoded& dd)
+{
+    if (!dd.split) return;
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_high, 0));
+    dd.split = false;
+}
+
+// on an error between the split and the join: `stream` still waits for the
+// second stream before the decode buffers go back to the pool
+struct JoinGuard {
+    stz_ctx* c;
+    Decoded& dd;
+    ~JoinGuard()
+    {
+        if (dd.split) {
+            (void)hipStreamWaitEvent(c->stream, c->ev_high, 0);
+            (void)hipStreamSynchronize(c->side_stream);
+        }
+    }
+};
 
 template <typename T>
 void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& dd, T* v, int stop_level)
@@ -968,7 +1040,10 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     for (int s = 0; s < kStreams; ++s) all[s] = true;
     ErrFlag err(c);
     Decoded dd;
-    huff_decode(c, ds, L, all, nullptr, dd, err.ptr());
+    // the high streams' Huffman decode overlaps the base and low levels
+    // (one stream when profiling, so per-kernel events stay meaningful)
+    JoinGuard guard{c, dd};
+    huff_decode(c, ds, L, all, nullptr, dd, err.ptr(), !c->prof);
     hipStream_t st = c->stream;
     const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
     DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));
@@ -979,6 +1054,7 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
                              c->grid_share);
     });
     OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
+    join_high(c, dd);
     K(c, "high_decode", [&] {
         launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
                               (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, c->grid_share);
@@ -2440,6 +2516,12 @@ void stz_destroy(stz_ctx* c)
     for (void* p : c->retired) (void)hipHostFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side_stream) {
+        (void)hipStreamSynchronize(c->side_stream);
+        (void)hipStreamDestroy(c->side_stream);
+        (void)hipEventDestroy(c->ev_items);
+        (void)hipEventDestroy(c->ev_high);
+    }
     delete c;
 }
 
