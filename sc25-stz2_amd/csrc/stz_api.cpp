@@ -142,7 +142,10 @@ struct BatchPlan;
 
 // Persistent host workers for the per-stream host work of a call (codebook
 // construction): run(n, f) calls f(0..n-1) on the workers and the calling
-// thread and returns when all are done, rethrowing the first exception.
+// thread and returns when the n calls are done (not when every worker has
+// woken up: a worker that wakes late finds the tasks taken), rethrowing the
+// first exception.  Each run has its own record, so a late worker never
+// takes a task of a later run.
 class TaskPool {
 public:
     explicit TaskPool(int nworkers)
@@ -164,34 +167,45 @@ public:
             for (int i = 0; i < n; ++i) f(i);
             return;
         }
+        auto job = std::make_shared<Job>();
+        job->f = &f;
+        job->n = n;
         {
             std::lock_guard<std::mutex> lk(mu_);
-            job_ = &f;
-            ntask_ = n;
-            next_.store(0);
-            pending_ = (int)th_.size();
-            err_ = nullptr;
+            cur_ = job;
             ++gen_;
         }
         cv_.notify_all();
-        work(f);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [&] { return pending_ == 0; });
-        job_ = nullptr;
-        if (err_) std::rethrow_exception(err_);
+        work(*job);
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            done_.wait(lk, [&] { return job->finished.load() == n; });
+            if (cur_ == job) cur_.reset();
+        }
+        if (job->err) std::rethrow_exception(job->err);
     }
 
 private:
-    void work(const std::function<void(int)>& f)
+    struct Job {
+        const std::function<void(int)>* f = nullptr;  // valid while tasks remain unclaimed
+        int n = 0;
+        std::atomic<int> next{0}, finished{0};
+        std::exception_ptr err;  // written under the pool's mutex
+    };
+    void work(Job& j)
     {
         for (;;) {
-            const int i = next_.fetch_add(1);
-            if (i >= ntask_) return;
+            const int i = j.next.fetch_add(1);
+            if (i >= j.n) return;
             try {
-                f(i);
+                (*j.f)(i);
             } catch (...) {
                 std::lock_guard<std::mutex> lk(mu_);
-                if (!err_) err_ = std::current_exception();
+                if (!j.err) j.err = std::current_exception();
+            }
+            if (j.finished.fetch_add(1) + 1 == j.n) {
+                std::lock_guard<std::mutex> lk(mu_);
+                done_.notify_all();
             }
         }
     }
@@ -199,28 +213,23 @@ private:
     {
         uint64_t seen = 0;
         for (;;) {
-            const std::function<void(int)>* f;
+            std::shared_ptr<Job> j;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
                 if (stop_) return;
                 seen = gen_;
-                f = job_;
+                j = cur_;
             }
-            work(*f);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--pending_ == 0) done_.notify_one();
+            if (j) work(*j);
         }
     }
     std::vector<std::thread> th_;
     std::mutex mu_;
     std::condition_variable cv_, done_;
-    const std::function<void(int)>* job_ = nullptr;
-    std::atomic<int> next_{0};
-    int ntask_ = 0, pending_ = 0;
+    std::shared_ptr<Job> cur_;
     uint64_t gen_ = 0;
     bool stop_ = false;
-    std::exception_ptr err_;
 };
 
 }  // namespace
