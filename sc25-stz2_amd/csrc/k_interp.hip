@@ -190,7 +190,7 @@ __device__ __forceinline__ void interp_encode_fused_wg(const InterpPassList& l, 
                                                        uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t sh[kHW];
-    __shared__ T L[kFusedPoints];
+    __shared__ T L[kLatticeBytes / sizeof(T)];
     for (int k = threadIdx.x; k < kHW; k += 1024) sh[k] = 0u;
     lattice_io(l, L, v, true);
     __syncthreads();
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kFusedDecThreads) void k_interp_decode_fused(Interp
 template <typename T>
 __global__ __launch_bounds__(kFusedDecThreads) void k_interp_decode_fused_multi(const FusedJob* __restrict__ jobs)
 {
-    __shared__ T L[kFusedPoints];
+    __shared__ T L[kLatticeBytes / sizeof(T)];
     const FusedJob& J = jobs[blockIdx.x];
     interp_decode_fused_wg(J.l, (T*)J.v, J.codes, (const T*)J.side, L);
 }

@@ -272,24 +272,30 @@ __global__ void k_adam(float* __restrict__ P, float* __restrict__ G, float* __re
 // cubic (-1, 9, 9, -1)/16 interpolation along axis a (the base codec's
 // finest level); lin1: stride-1 linear residual (the high level); d1: first
 // difference.  Points whose stencil leaves the field are skipped.
+// Each 64^3 block is split into kStatParts z-slabs of its sample lattice,
+// one 256-lane workgroup each (about 80 VGPRs: six workgroups per CU), whose
+// partial sums k_block_stats_combine adds up.
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_stats(const T* __restrict__ f, int nx, int ny, int nz, int bx,
-                                                     int by, double eb, double* __restrict__ out)
+                                                     int by, double eb, double* __restrict__ part)
 {
-    __shared__ double red[256];
-    const int b = blockIdx.x;
+    __shared__ double red[4][kFeatSums];
+    const int b = blockIdx.x / kStatParts, q = blockIdx.x % kStatParts;
     const int cx = b % bx, cy = (b / bx) % by, cz = b / (bx * by);
     const int x0 = cx * 64, y0 = cy * 64, z0 = cz * 64;
     const int x1 = min(x0 + 64, nx), y1 = min(y0 + 64, ny), z1 = min(z0 + 64, nz);
     const int sx = (x1 - x0 + 1) / 2, sy = (y1 - y0 + 1) / 2, sz = (z1 - z0 + 1) / 2;
     const int ns = sx * sy * sz;
+    // this part's samples: lattice planes [q sz / P, (q + 1) sz / P)
+    const int k0 = sx * sy * (q * sz / kStatParts), k1 = sx * sy * ((q + 1) * sz / kStatParts);
     double acc[kFeatSums];
     for (int k = 0; k < kFeatSums; ++k) acc[k] = 0.0;
     acc[4] = 1e300;
     acc[5] = -1e300;
     const size_t st[3] = {1, (size_t)nx, (size_t)nx * ny};
     const int dim[3] = {nx, ny, nz};
-    for (int k = threadIdx.x; k < ns; k += 256) {
+    for (int k = k0 + threadIdx.x; k < k1 && k < ns; k += 256) {
         const int x = x0 + 2 * (k % sx), y = y0 + 2 * ((k / sx) % sy), z = z0 + 2 * (k / (sx * sy));
         const int c[3] = {x, y, z};
         const size_t i = ((size_t)z * ny + y) * nx + x;
@@ -322,20 +328,41 @@ __global__ __launch_bounds__(256) void k_block_stats(const T* __restrict__ f, in
         acc[16] += mc4 <= eb ? 1.0 : 0.0;
         acc[17] += d1 / 3.0;
     }
+    // a shuffle reduction per wave, then one pass over the four waves
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
     for (int k = 0; k < kFeatSums; ++k) {
-        red[threadIdx.x] = acc[k];
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) {
-                const double o = red[threadIdx.x + w];
-                double& r = red[threadIdx.x];
-                r = k == 4 ? fmin(r, o) : k == 5 ? fmax(r, o) : r + o;
-            }
-            __syncthreads();
+        double v = acc[k];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const double o = __shfl_xor(v, d, 64);
+            v = k == 4 ? fmin(v, o) : k == 5 ? fmax(v, o) : v + o;
         }
-        if (threadIdx.x == 0) out[(size_t)b * kFeatSums + k] = red[0];
-        __syncthreads();
+        if (lane == 0) red[wv][k] = v;
     }
+    __syncthreads();
+    if (threadIdx.x < kFeatSums) {
+        const int k = threadIdx.x;
+        double r = red[0][k];
+        for (int w = 1; w < 4; ++w) {
+            const double o = red[w][k];
+            r = k == 4 ? fmin(r, o) : k == 5 ? fmax(r, o) : r + o;
+        }
+        part[(size_t)blockIdx.x * kFeatSums + k] = r;
+    }
+}
+
+__global__ void k_block_stats_combine(const double* __restrict__ part, uint32_t nb, double* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb * kFeatSums) return;
+    const uint32_t b = i / kFeatSums, k = i % kFeatSums;
+    double r = part[((size_t)b * kStatParts) * kFeatSums + k];
+    for (int q = 1; q < kStatParts; ++q) {
+        const double o = part[((size_t)b * kStatParts + q) * kFeatSums + k];
+        r = k == 4 ? fmin(r, o) : k == 5 ? fmax(r, o) : r + o;
+    }
+    out[i] = r;
 }
 
 }  // namespace
@@ -360,12 +387,15 @@ void launch_adam(hipStream_t s, float* P, float* G, float* M, float* V, uint32_t
 }
 
 template <typename T>
-void launch_block_stats(hipStream_t s, const T* f, int nx, int ny, int nz, double eb, double* out)
+void launch_block_stats(hipStream_t s, const T* f, int nx, int ny, int nz, double eb, double* out,
+                        double* part)
 {
     const int bx = (nx + 63) / 64, by = (ny + 63) / 64, bz = (nz + 63) / 64;
-    k_block_stats<T><<<dim3(bx * by * bz), dim3(256), 0, s>>>(f, nx, ny, nz, bx, by, eb, out);
+    const uint32_t nb = (uint32_t)(bx * by * bz);
+    k_block_stats<T><<<dim3(nb * kStatParts), dim3(256), 0, s>>>(f, nx, ny, nz, bx, by, eb, part);
+    k_block_stats_combine<<<dim3((nb * kFeatSums + 255) / 256), dim3(256), 0, s>>>(part, nb, out);
 }
-template void launch_block_stats<float>(hipStream_t, const float*, int, int, int, double, double*);
-template void launch_block_stats<double>(hipStream_t, const double*, int, int, int, double, double*);
+template void launch_block_stats<float>(hipStream_t, const float*, int, int, int, double, double*, double*);
+template void launch_block_stats<double>(hipStream_t, const double*, int, int, int, double, double*, double*);
 
 }  // namespace stz

@@ -341,7 +341,10 @@ void launch_mlp_grad(hipStream_t s, const float* P, const float* X, const float*
                      float* loss);
 void launch_adam(hipStream_t s, float* P, float* G, float* M, float* V, uint32_t n, float lr, float b1, float b2,
                  float eps, int step, float wd);
+// 64^3 blocks' statistics are summed in this many partial workgroups each
+constexpr int kStatParts = 4;
 template <typename T>
-void launch_block_stats(hipStream_t s, const T* f, int nx, int ny, int nz, double eb, double* out);
+void launch_block_stats(hipStream_t s, const T* f, int nx, int ny, int nz, double eb, double* out,
+                        double* part);  // part: kStatParts x kFeatSums doubles per block
 
 }  // namespace stz

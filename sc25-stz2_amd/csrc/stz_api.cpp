@@ -563,11 +563,13 @@ StreamSet make_streamset(const HostLayout& L, const Meta& m, int dtype)
 // single-workgroup kernel on an LDS copy of the lattice; every finer pass gets
 // a launch of its own.  Stream order is unchanged.
 template <typename FF, typename FP>
-void run_passes(const std::vector<InterpPass>& passes, size_t tsize, FF&& fused, FP&& single)
+void run_passes(const std::vector<InterpPass>& passes, size_t tsize, FF&& fused, FP&& single, bool many = false)
 {
     if (passes.empty()) return;
     const int bx = passes[0].bx, by = passes[0].by, bz = passes[0].bz;
-    const size_t cap = std::min<size_t>(kLatticeBytes / tsize, (size_t)kFusedPoints);
+    // many fields: one fused workgroup per field runs side by side, so each
+    // keeps the whole LDS budget; one field: at most kFusedPoints
+    const size_t cap = many ? kLatticeBytes / tsize : std::min<size_t>(kLatticeBytes / tsize, (size_t)kFusedPoints);
     int S0 = 1;
     auto lat = [&](int S, int d) { return (d - 1) / S + 1; };
     while ((size_t)lat(S0, bx) * lat(S0, by) * lat(S0, bz) > cap) S0 <<= 1;
@@ -749,7 +751,7 @@ LevelArgs enc_level_args(const FieldEnc<T>& fe, int level);
 
 template <typename T>
 void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
-                     int mode, const stz_params& prm, stz_dstream* const* out);
+                     int mode, const stz_params& prm, stz_dstream* const* out, bool* low_conf);
 
 template <typename T>
 void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
@@ -757,7 +759,7 @@ void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, 
 {
     const void* f = field;
     const size_t dims[3] = {nx, ny, nz};
-    compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, prm, &ds);
+    compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, prm, &ds, nullptr);
 }
 
 // ------------------------------------------------------------ decode helpers
@@ -1429,7 +1431,7 @@ DBuf upload_vec(stz_ctx* c, const std::vector<V>& v)
 // statistics of all fields, one MLP launch over all their blocks, then per
 // field the same weighted vote as a single-field call.
 template <typename T>
-void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe)
+void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, std::vector<float>* confs = nullptr)
 {
     if (!c->have_tuner) throw StzError(STZ_E_ARG, "no tuner model set (stz_tune_set_model)");
     hipStream_t st = c->stream;
@@ -1439,14 +1441,15 @@ void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe)
         off[i + 1] = off[i] + ((fe[i].nx + 63) / 64) * ((fe[i].ny + 63) / 64) * ((fe[i].nz + 63) / 64);
     const size_t nb = off[n];
     std::vector<double> ebabs(n);
-    DBuf sums(c->pool, nb * kFeatSums * sizeof(double));
+    DBuf sums(c->pool, nb * kFeatSums * sizeof(double)), part(c->pool, nb * kStatParts * kFeatSums * sizeof(double));
     for (size_t i = 0; i < n; ++i) {
         const FieldEnc<T>& f = fe[i];
         ebabs[i] = f.mode == STZ_EB_REL ? f.eb * (f.mm_host[1] - f.mm_host[0]) : f.eb;
         if (!(ebabs[i] > 0)) ebabs[i] = 1e-300;
         K(c, "block_stats", [&] {
             launch_block_stats<T>(st, f.din, (int)f.nx, (int)f.ny, (int)f.nz, ebabs[i],
-                                  sums.as<double>() + off[i] * kFeatSums);
+                                  sums.as<double>() + off[i] * kFeatSums,
+                                  part.as<double>() + off[i] * kStatParts * kFeatSums);
         });
     }
     double* hs = (double*)c->host_stage(nb * kFeatSums * sizeof(double));
@@ -1463,24 +1466,33 @@ void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe)
     float* hy = hx + X.size();
     HIPCHK(hipMemcpyAsync(hy, y.p, nb * kMlpOut * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (confs) confs->assign(n, 0.f);
     for (size_t i = 0; i < n; ++i) {
         double wsum = 0;
         for (size_t b = off[i]; b < off[i + 1]; ++b) wsum += hs[b * kFeatSums];
-        std::vector<double> lp(kMlpOut, 0.0);
+        std::vector<double> lp(kMlpOut, 0.0), pr(kMlpOut, 0.0);
+        double e[kMlpOut];
         for (size_t b = off[i]; b < off[i + 1]; ++b) {
             const double w = wsum > 0 ? hs[b * kFeatSums] / wsum : 1.0 / (double)(off[i + 1] - off[i]);
             const float* r = hy + b * kMlpOut;
             double mx = r[0];
             for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
             double se = 0;
-            for (int k = 0; k < kMlpOut; ++k) se += std::exp(r[k] - mx);
+            for (int k = 0; k < kMlpOut; ++k) {
+                e[k] = std::exp(r[k] - mx);
+                se += e[k];
+            }
             const double lse = mx + std::log(se);
-            for (int k = 0; k < kMlpOut; ++k) lp[k] += w * (r[k] - lse);
+            for (int k = 0; k < kMlpOut; ++k) {
+                lp[k] += w * (r[k] - lse);
+                pr[k] += w * e[k] / se;
+            }
         }
         int best = 0;
         for (int k = 1; k < kMlpOut; ++k)
             if (lp[k] > lp[best]) best = k;
         tune_config(best, fe[i].prm);
+        if (confs) (*confs)[i] = (float)pr[best];
     }
 }
 
@@ -1493,7 +1505,7 @@ void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe)
 // marked Huffman encode (lengths, chunk scan, pack).
 template <typename T>
 void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
-                     int mode, const stz_params& prm, stz_dstream* const* out)
+                     int mode, const stz_params& prm, stz_dstream* const* out, bool* low_conf)
 {
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -1512,7 +1524,16 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         enc_phase0(c, f, (const T*)fields[i]);
     }
     HIPCHK(hipStreamSynchronize(st));
-    if (prm.tuner == 1) learned_choice_batch<T>(c, fe);
+    if (prm.tuner == 1) {
+        // one field: below the context's confidence threshold the caller
+        // falls back to the exhaustive trial
+        std::vector<float> conf;
+        learned_choice_batch<T>(c, fe, &conf);
+        if (low_conf && n == 1 && conf[0] < c->tune_min_conf) {
+            *low_conf = true;
+            return;
+        }
+    }
     for (int i = 0; i < n; ++i) enc_phase1(c, fe[i]);
     // base level: coarse passes of every field, then the fine-pass rounds
     {
@@ -1547,7 +1568,7 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 j.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, kPassEncMaxWgs);
                 rounds[r].push_back(j);
                 ++r;
-            });
+            }, n > 1);
         }
         DBuf dfj = upload_vec(c, fj);
         K(c, "interp_encode_fused", [&] { launch_interp_encode_fused_multi<T>(st, dfj.as<FusedEncJob>(), (uint32_t)fj.size()); });
@@ -1830,12 +1851,12 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
                 tmp[i].reset(new stz_dstream());
                 traw[i] = tmp[i].get();
             }
-            compress_fields<T>(c, dtype, n, fields, dims, eb, mode, pk, traw.data());
+            compress_fields<T>(c, dtype, n, fields, dims, eb, mode, pk, traw.data(), nullptr);
             for (int i = 0; i < n; ++i)
                 if (k == 0 || traw[i]->bytes < raw[i]->bytes) std::swap(dss[i], tmp[i]), raw[i] = dss[i].get();
         }
     } else {
-        compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data());
+        compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data(), nullptr);
     }
     for (int i = 0; i < n; ++i) out[i] = dss[i].release();
 }
@@ -1997,7 +2018,7 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
             q.nwg = (uint32_t)std::min<uint64_t>((p.n + 255) / 256, 1024);
             rounds[r].push_back(q);
             ++r;
-        });
+        }, n > 1);
     }
     P.chunks = chunk;
     P.n_st = (uint32_t)stj.size();
@@ -2332,8 +2353,10 @@ std::vector<float> tune_features(stz_ctx* c, const T* d, size_t nx, size_t ny, s
                                  std::vector<double>* counts = nullptr)
 {
     const size_t nb = ((nx + 63) / 64) * ((ny + 63) / 64) * ((nz + 63) / 64);
-    DBuf sums(c->pool, nb * kFeatSums * sizeof(double));
-    K(c, "block_stats", [&] { launch_block_stats<T>(c->stream, d, (int)nx, (int)ny, (int)nz, eb_abs, sums.as<double>()); });
+    DBuf sums(c->pool, nb * kFeatSums * sizeof(double)), part(c->pool, nb * kStatParts * kFeatSums * sizeof(double));
+    K(c, "block_stats", [&] {
+        launch_block_stats<T>(c->stream, d, (int)nx, (int)ny, (int)nz, eb_abs, sums.as<double>(), part.as<double>());
+    });
     double* h = (double*)c->host_stage(nb * kFeatSums * sizeof(double));
     HIPCHK(hipMemcpyAsync(h, sums.p, nb * kFeatSums * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2379,13 +2402,16 @@ int tune_learned(stz_ctx* c, const T* d, size_t nx, size_t ny, size_t nz, double
         const float* r = hy + b * kMlpOut;
         double mx = r[0];
         for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
-        double se = 0;
-        for (int k = 0; k < kMlpOut; ++k) se += std::exp(r[k] - mx);
+        double se = 0, e[kMlpOut];
+        for (int k = 0; k < kMlpOut; ++k) {
+            e[k] = std::exp(r[k] - mx);
+            se += e[k];
+        }
         const double lse = mx + std::log(se);
         const double w = cnt[b] / wsum;
         for (int k = 0; k < kMlpOut; ++k) {
             lp[k] += w * (r[k] - lse);
-            pr[k] += w * std::exp(r[k] - lse);
+            pr[k] += w * e[k] / se;
         }
     }
     int best = 0;
@@ -2435,14 +2461,13 @@ void compress_tuned(stz_ctx* c, int dtype, const void* field, size_t nx, size_t 
     FieldView fv(c, field, nx * ny * nz * sizeof(T));
     const T* d = (const T*)fv.p;
     if (p.tuner == 1) {
-        float conf = 0.f;
-        const int k = tune_learned<T>(c, d, nx, ny, nz, eb, mode, conf);
-        if (conf >= c->tune_min_conf) {
-            tune_config(k, p);
-            p.tuner = 0;
-            compress_impl<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds);
-            return;
-        }
+        // the model's choice is made inside the compress, after its value
+        // range pass (the same statistics, MLP and vote as stz_tune)
+        bool low = false;
+        const void* f = d;
+        const size_t dims[3] = {nx, ny, nz};
+        compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, p, &ds, &low);
+        if (!low) return;
     }
     tune_trial<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds, nullptr);
 }
