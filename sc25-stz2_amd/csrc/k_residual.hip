@@ -535,7 +535,20 @@ __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
     uint64_t ntiles, total;
     tile_items<kTW>(g, ntx, nty, ntiles, total);
     const int lane = threadIdx.x & 63, row = wave_row();
-    for (uint64_t t = xcd_block(); t < total; t += gridDim.x) {
+    // item walk: the high level in the interleaved XCD order; the low level
+    // with each XCD sweeping one contiguous range of items (z-planes), so
+    // the base-grid planes a tile's stencil shares with the next ones stay
+    // in that XCD's L2 (low encode 104 -> 94 us at 512^3; the high encode
+    // measured slower that way, 416 -> 447 us)
+    constexpr bool kSweep = kMode == kLowEnc;
+    const uint32_t G = gridDim.x;
+    const bool sweep = kSweep && (G % 8u) == 0;
+    const uint64_t per = sweep ? (total + 7) / 8 : total;
+    const uint64_t base = sweep ? (uint64_t)(blockIdx.x % 8u) * per : 0;
+    const uint64_t first = sweep ? blockIdx.x / 8u : xcd_block(), step = sweep ? G / 8u : G;
+    for (uint64_t j = first; j < per; j += step) {
+        const uint64_t t = base + j;
+        if (t >= total) break;
         if (t < ntiles) {
             const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty);
             const int z = (int)(t / ((uint64_t)ntx * nty));

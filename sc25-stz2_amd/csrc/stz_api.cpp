@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -157,6 +158,7 @@ public:
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
+            stop_flag_.store(true);
         }
         cv_.notify_all();
         for (auto& t : th_) t.join();
@@ -173,7 +175,7 @@ public:
         {
             std::lock_guard<std::mutex> lk(mu_);
             cur_ = job;
-            ++gen_;
+            gen_.store(gen_.load() + 1);
         }
         cv_.notify_all();
         work(*job);
@@ -213,12 +215,20 @@ private:
     {
         uint64_t seen = 0;
         for (;;) {
+            // a compress calls run() twice about half a millisecond apart
+            // (base/low codebooks, then high): spin up to 1 ms on the
+            // generation before sleeping, so the second call finds the
+            // workers awake
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load(std::memory_order_acquire) == seen && !stop_flag_.load(std::memory_order_relaxed) &&
+                   std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(1))
+                std::this_thread::yield();
             std::shared_ptr<Job> j;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                cv_.wait(lk, [&] { return stop_ || gen_.load() != seen; });
                 if (stop_) return;
-                seen = gen_;
+                seen = gen_.load();
                 j = cur_;
             }
             if (j) work(*j);
@@ -228,8 +238,9 @@ private:
     std::mutex mu_;
     std::condition_variable cv_, done_;
     std::shared_ptr<Job> cur_;
-    uint64_t gen_ = 0;
+    std::atomic<uint64_t> gen_{0};  // written under mu_, read without it while spinning
     bool stop_ = false;
+    std::atomic<bool> stop_flag_{false};
 };
 
 }  // namespace
