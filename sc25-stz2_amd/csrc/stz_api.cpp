@@ -897,23 +897,27 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
             if (pos >= (1ull << 32)) throw StzError(STZ_E_DIMS, "random-access decode larger than 2^32 symbols");
         }
     }
-    // Decode variant for the whole launch, by the symbol-weighted entropy of
-    // what it decodes: up to 2 bits per symbol the LDS-staged kernel (256
-    // sub-chunks per workgroup, halved while a stream's expected bits with a
-    // 30% margin would not fit the 44 KiB stage); above, the unstaged kernel,
-    // whose smaller LDS footprint fits more workgroups per CU.
-    double wbits = 0, wsym = 0;
+    // Decode variant per launch (a split decode has two: streams below
+    // kSplitStream, and the rest), by the symbol-weighted entropy of what it
+    // decodes: up to 2 bits per symbol the LDS-staged kernel (256 sub-chunks
+    // per workgroup, halved while a stream's expected bits with a 30% margin
+    // would not fit the 44 KiB stage); above, the unstaged kernel, whose
+    // smaller LDS footprint fits more workgroups per CU.
+    const bool two = split;
+    double wbits[2] = {0, 0}, wsym[2] = {0, 0};
     for (int s = 0; s < kStreams; ++s) {
+        const int gi = two && s >= kSplitStream ? 1 : 0;
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
-        wbits += bps * ss.s[s].item_count;
-        wsym += ss.s[s].item_count;
+        wbits[gi] += bps * ss.s[s].item_count;
+        wsym[gi] += ss.s[s].item_count;
     }
-    const bool use_stage = wsym > 0 && wbits / wsym <= 2.0;
+    const bool stage_of[2] = {wsym[0] > 0 && wbits[0] / wsym[0] <= 2.0, wsym[1] > 0 && wbits[1] / wsym[1] <= 2.0};
+    const bool use_stage = stage_of[0];
     for (int s = 0; s < kStreams; ++s) {
         StreamDesc& d = ss.s[s];
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
         d.ipw = 256;
-        if (use_stage)
+        if (stage_of[two && s >= kSplitStream ? 1 : 0])
             while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
         d.item_base = item;
         d.wg_base = wg;
@@ -963,9 +967,9 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
         HIPCHK(hipStreamWaitEvent(s2, c->ev_items, 0));
         K(c, "huff_decode", [&] {
             launch_huff_decode(st, lo, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
-                               out.codes.as<uint16_t>(), out.side.p, err, use_stage);
+                               out.codes.as<uint16_t>(), out.side.p, err, stage_of[0]);
             launch_huff_decode(s2, hi, pay, ds->tabs.as<DecTab>() + kSplitStream, ds->sorted.as<uint16_t>(),
-                               items.as<DecItem>(), out.codes.as<uint16_t>(), out.side.p, err, use_stage);
+                               items.as<DecItem>(), out.codes.as<uint16_t>(), out.side.p, err, stage_of[1]);
         });
         HIPCHK(hipEventRecord(c->ev_high, s2));
         out.split = true;
