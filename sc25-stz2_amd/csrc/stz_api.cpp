@@ -778,26 +778,33 @@ void ensure_tabs(stz_ctx* c, stz_dstream* ds)
 {
     if (ds->have_tabs) return;
     const Meta& m = ds->meta;
+    // canonical descriptions built on the host workers, one stream each
     std::vector<DecHead> heads(kStreams);
-    std::vector<uint16_t> all;
+    std::vector<std::vector<uint16_t>> sorted(kStreams);
+    c->tasks().run(kStreams, [&](int s) {
+        build_dechead(m.cb[s].sym.data(), m.cb[s].len.data(), (uint32_t)m.cb[s].sym.size(), heads[s], sorted[s]);
+    });
+    size_t ns = 0;
     for (int s = 0; s < kStreams; ++s) {
-        std::vector<uint16_t> sorted;
-        build_dechead(m.cb[s].sym.data(), m.cb[s].len.data(), (uint32_t)m.cb[s].sym.size(), heads[s], sorted);
-        heads[s].sorted_off = (uint32_t)all.size();
-        all.insert(all.end(), sorted.begin(), sorted.end());
+        heads[s].sorted_off = (uint32_t)ns;
+        ns += sorted[s].size();
     }
-    const size_t hb = sizeof(DecHead) * kStreams;
+    // one upload: the sorted symbols, then (16-byte aligned) the descriptions,
+    // which stay behind them in the stream's buffer
+    const size_t sb = align_up(ns * 2, 16), hb = sizeof(DecHead) * kStreams;
     ds->tabs = DBuf(ds->pool, sizeof(DecTab) * kStreams);
-    ds->sorted = DBuf(ds->pool, all.size() * 2 + 16);
-    DBuf dh(c->pool, hb);
-    uint8_t* stage = (uint8_t*)c->host_stage(hb + all.size() * 2 + 64);
-    std::memcpy(stage, heads.data(), hb);
-    std::memcpy(stage + hb, all.data(), all.size() * 2);
-    HIPCHK(hipMemcpyAsync(dh.p, stage, hb, hipMemcpyHostToDevice, c->stream));
-    if (!all.empty()) HIPCHK(hipMemcpyAsync(ds->sorted.p, stage + hb, all.size() * 2, hipMemcpyHostToDevice, c->stream));
-    K(c, "dectabs", [&] { launch_build_dectabs(c->stream, dh.as<DecHead>(), ds->sorted.as<uint16_t>(),
-                                               ds->tabs.as<DecTab>(), kStreams); });
-    ds->have_tabs = true;  // dh returns to the pool; later users are ordered on the same stream
+    ds->sorted = DBuf(ds->pool, sb + hb + 16);
+    uint8_t* stage = (uint8_t*)c->host_stage(sb + hb + 64);
+    size_t o = 0;
+    for (int s = 0; s < kStreams; ++s) {
+        std::memcpy(stage + o, sorted[s].data(), sorted[s].size() * 2);
+        o += sorted[s].size() * 2;
+    }
+    std::memcpy(stage + sb, heads.data(), hb);
+    HIPCHK(hipMemcpyAsync(ds->sorted.p, stage, sb + hb, hipMemcpyHostToDevice, c->stream));
+    const DecHead* dh = (const DecHead*)(ds->sorted.as<uint8_t>() + sb);
+    K(c, "dectabs", [&] { launch_build_dectabs(c->stream, dh, ds->sorted.as<uint16_t>(), ds->tabs.as<DecTab>(), kStreams); });
+    ds->have_tabs = true;
 }
 
 void layout_of(const Meta& m, HostLayout& L)
