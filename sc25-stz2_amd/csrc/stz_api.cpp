@@ -251,6 +251,8 @@ struct stz_ctx {
     // marks a mid-compress readback, so host codebook builds start while
     // the stream still runs the later kernels
     hipEvent_t ev_fork = nullptr;
+    // the value range of a compress has reached the host
+    hipEvent_t ev_range = nullptr;
     // second stream of a full decode (created on first use): the high
     // streams' Huffman decode runs there while the base and low levels
     // decode on `stream`; ev_items / ev_high order it
@@ -675,20 +677,12 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     K(c, "minmax", [&] { launch_minmax<T>(st, fe.din, fe.nin, fe.mm_part.template as<double>(), outp); });
     fe.mm_host = (double*)c->host_stage(64);
     HIPCHK(hipMemcpyAsync(fe.mm_host, outp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    // host work while the range is computed
+    HIPCHK(hipEventRecord(c->ev_range, st));
+    // the work that does not depend on the range is queued before the
+    // caller waits for it (on ev_range): scratch, histograms cleared, base
+    // grid gathered
     make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low);
-}
-
-// after a wait: level bounds, scratch, histograms cleared, base grid gathered
-template <typename T>
-void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
-{
-    hipStream_t st = c->stream;
-    auto pool = c->pool;
     const HostLayout& L = fe.L;
-    fe.vmin = fe.mm_host[0];
-    fe.vmax = fe.mm_host[1];
-    level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.prm, fe.ebl, fe.unit);
     const int lx = (int)(fe.px / 4), ly = (int)(fe.py / 4), lz = (int)(fe.pz / 4);
     const int mx = (int)(fe.px / 2), my = (int)(fe.py / 2), mz = (int)(fe.pz / 2);
     fe.codes = DBuf(pool, L.total_codes * 2);
@@ -697,10 +691,21 @@ void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
     fe.vbase = DBuf(pool, (size_t)lx * ly * lz * sizeof(T));
     fe.mid = DBuf(pool, (size_t)mx * my * mz * sizeof(T));
     K(c, "hist_zero", [&] { launch_fill_u32(st, fe.hist.template as<uint32_t>(), (uint64_t)kStreams * 65536, 0u); });
-    // base level (F5:2132-2145): gather low block 0, then the pass schedule
+    // base level (F5:2132-2145): gather low block 0; the pass schedule follows
+    // once the bounds are known
     K(c, "gather_low0", [&] {
         launch_gather_low0<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>());
     });
+}
+
+// after the wait: level bounds from the range
+template <typename T>
+void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
+{
+    (void)c;
+    fe.vmin = fe.mm_host[0];
+    fe.vmax = fe.mm_host[1];
+    level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.prm, fe.ebl, fe.unit);
 }
 
 template <typename T>
@@ -1545,7 +1550,9 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         f.ds = out[i];
         enc_phase0(c, f, (const T*)fields[i]);
     }
-    HIPCHK(hipStreamSynchronize(st));
+    // every field's range is on the host once the last one is (stream order);
+    // the gathers queued after them keep the GPU busy meanwhile
+    HIPCHK(hipEventSynchronize(c->ev_range));
     if (prm.tuner == 1) {
         // one field: below the context's confidence threshold the caller
         // falls back to the exhaustive trial
@@ -2548,6 +2555,7 @@ int stz_create(stz_ctx** out, int device)
         // stream, where torch puts the caller's tensors by default
         HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
         HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_range, hipEventDisableTiming));
         c->stream = c->own;
         c->pool = std::make_shared<Pool>();
         c->pool->device = device;
@@ -2572,6 +2580,7 @@ void stz_destroy(stz_ctx* c)
     for (void* p : c->retired) (void)hipHostFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_range) (void)hipEventDestroy(c->ev_range);
     if (c->side_stream) {
         (void)hipStreamSynchronize(c->side_stream);
         (void)hipStreamDestroy(c->side_stream);
