@@ -1637,14 +1637,14 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     uint8_t* hs_lo = early ? (uint8_t*)c->host_stage(hs_bytes) : nullptr;
     uint8_t* hs_hi = (uint8_t*)c->host_stage(hs_bytes);
     // compact streams [s0, s0+ns) of every field and copy counts + first entries into hs
-    auto readback = [&](uint8_t* hs, int s0, int ns) {
-        K(c, "hist_compact", [&] { launch_hist_compact_multi(st, dhj.as<EncJob>(), (uint32_t)n, s0, ns); });
-        HIPCHK(hipMemcpyAsync(hs, counts.p, (size_t)n * 256, hipMemcpyDeviceToHost, st));
+    auto readback = [&](uint8_t* hs, int s0, int ns, hipStream_t rs) {
+        K(c, "hist_compact", [&] { launch_hist_compact_multi(rs, dhj.as<EncJob>(), (uint32_t)n, s0, ns); });
+        HIPCHK(hipMemcpyAsync(hs, counts.p, (size_t)n * 256, hipMemcpyDeviceToHost, rs));
         uint64_t* hf = (uint64_t*)(hs + (size_t)n * 256);
         for (int i = 0; i < n; ++i) {
             const size_t row = (size_t)i * kStreams + s0;
             HIPCHK(hipMemcpy2DAsync(hf + row * kFirst, kFirst * 8, compact.as<uint64_t>() + row * 65536, 65536 * 8,
-                                    kFirst * 8, (size_t)ns, hipMemcpyDeviceToHost, st));
+                                    kFirst * 8, (size_t)ns, hipMemcpyDeviceToHost, rs));
         }
     };
     // after the copies have landed: symbol lists, then codebooks on the host workers
@@ -1669,11 +1669,31 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             build_codebook(f.lists[s2].data(), (uint32_t)f.lists[s2].size(), kLmax, f.ds->meta.cb[s2]);
         });
     };
+    // on an error after the low readback forked to the second stream,
+    // `stream` still waits for it before the buffers go back to the pool
+    struct ForkGuard {
+        stz_ctx* c;
+        bool on = false;
+        ~ForkGuard()
+        {
+            if (on) (void)hipStreamWaitEvent(c->stream, c->ev_fork, 0);
+        }
+    } fork_guard{c};
     // residual levels
     for (int level = 1; level <= 2; ++level) {
         if (early && level == 2) {
-            readback(hs_lo, 0, 8);
-            HIPCHK(hipEventRecord(c->ev_fork, st));
+            // the base and low streams' histograms are final: compacted and
+            // read back on the second stream (beside the high level; on
+            // `stream` when profiling), their codebooks built meanwhile
+            hipStream_t rs = st;
+            if (!c->prof) {
+                rs = c->second();
+                HIPCHK(hipEventRecord(c->ev_items, st));
+                HIPCHK(hipStreamWaitEvent(rs, c->ev_items, 0));
+            }
+            readback(hs_lo, 0, 8, rs);
+            HIPCHK(hipEventRecord(c->ev_fork, rs));
+            fork_guard.on = rs != st;
         }
         if (n == 1) {
             FieldEnc<T>& f = fe[0];
@@ -1718,13 +1738,15 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         jobs_launch<T>(c, md, js, cfg);
     }
     if (early) {
-        readback(hs_hi, 8, kStreams - 8);
+        HIPCHK(hipStreamWaitEvent(st, c->ev_fork, 0));  // the low readback's buffers, before they are reused
+        fork_guard.on = false;
+        readback(hs_hi, 8, kStreams - 8, st);
         HIPCHK(hipEventSynchronize(c->ev_fork));
         books(hs_lo, 0, 8);  // overlaps the high level on the GPU
         HIPCHK(hipStreamSynchronize(st));
         books(hs_hi, 8, kStreams - 8);
     } else {
-        readback(hs_hi, 0, kStreams);
+        readback(hs_hi, 0, kStreams, st);
         HIPCHK(hipStreamSynchronize(st));
         books(hs_hi, 0, kStreams);
     }
