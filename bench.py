@@ -31,10 +31,11 @@ sys.path.insert(0, os.path.join(ROOT, "sc25-stz2_amd"))
 
 # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and
 # two streams on one queue run in order: the serving run's worker streams each
-# need their own queue, or a box waits behind a full-resolution decode.  Read
-# once when HIP starts, so it is set before torch is imported.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# need their own queue, or a box waits behind a full-resolution decode (each
+# library context holds up to two streams).  Read once when HIP starts, so it
+# is set before torch is imported.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "GB/s comp+decomp at rel eb 1e-3 (learned vs plain, same GPU); ROI p99 ms"
 HBM_PEAK_GBS = 8000.0
@@ -500,6 +501,7 @@ def serve_bench(args, dev):
     info = ds.info
     del g
     torch.cuda.empty_cache()
+    ctx.close()  # its streams would take hardware queues the workers need; the stream keeps its memory
     reqs, arr = serve.poisson_trace((n, n, n), args.serve_requests, args.serve_rate, seed=1005)
     srv = serve.RoiServer(ds, workers=args.serve_workers, max_batch=32, mem_budget=32 << 30)
     log("serving %d requests at %.0f/s on %d workers" % (len(reqs), args.serve_rate, args.serve_workers))

@@ -76,7 +76,8 @@ void* stz_get_stream(stz_ctx* ctx);
 /* Serving: this context's persistent full-level decode kernels (a
  * full-resolution or 1/2-resolution decode) take at most `share` (0 < share
  * <= 1, default 1) of the GPU's resident workgroup slots, so launches from
- * other contexts' streams find free CUs meanwhile.  Results are unchanged.
+ * other contexts' streams find free CUs meanwhile, and its full decodes keep
+ * their Huffman stage on one stream.  Results are unchanged.
  * No counterpart in the reference, which runs one query at a time. */
 int stz_set_grid_share(stz_ctx* ctx, float share);
 /* per-kernel timing with HIP events on the context stream */

@@ -1079,9 +1079,10 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     ErrFlag err(c);
     Decoded dd;
     // the high streams' Huffman decode overlaps the base and low levels
-    // (one stream when profiling, so per-kernel events stay meaningful)
+    // (one stream when profiling, so per-kernel events stay meaningful, and
+    // for a context held to part of the GPU, which should not widen)
     JoinGuard guard{c, dd};
-    huff_decode(c, ds, L, all, nullptr, dd, err.ptr(), !c->prof);
+    huff_decode(c, ds, L, all, nullptr, dd, err.ptr(), !c->prof && c->grid_share >= 1.0f);
     hipStream_t st = c->stream;
     const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
     DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));

@@ -16,9 +16,10 @@ completion of the batch that served the request.
 The workers' batches overlap only when their streams sit on different
 hardware queues: HIP spreads streams over GPU_MAX_HW_QUEUES queues (4 by
 default, shared with torch's streams), and streams that share one run in
-order, so a box batch then waits for a full decode queued before it.  Set
-GPU_MAX_HW_QUEUES to at least workers + 4 before HIP starts (RoiServer warns
-otherwise).  The full-decode worker's persistent level kernels take only
+order, so a box batch then waits for a full decode queued before it.  A
+context holds up to two streams (the second for the overlapped parts of a
+full decode or compress).  Set GPU_MAX_HW_QUEUES to at least 2 x workers + 4
+before HIP starts (RoiServer warns otherwise).  The full-decode worker's persistent level kernels take only
 `full_share` of the resident workgroup slots (stz_set_grid_share), so a box
 batch finds free CUs while a full decode runs.
 """
@@ -54,10 +55,10 @@ class RoiServer:
         import torch
         from .api import Context
         hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-        if workers > 1 and hwq < workers + 4:
-            warnings.warn("GPU_MAX_HW_QUEUES=%d: %d worker streams share hardware queues, so batches of "
+        if workers > 1 and hwq < 2 * workers + 4:
+            warnings.warn("GPU_MAX_HW_QUEUES=%d: %d workers' streams share hardware queues, so batches of "
                           "different workers run in order; set it to >= %d before HIP starts"
-                          % (hwq, workers, workers + 4))
+                          % (hwq, workers, 2 * workers + 4))
         self.torch = torch
         self.ds = ds
         self.dev = torch.device("cuda", device)
