@@ -29,6 +29,8 @@ def _field(kind, shape, dtype, seed):
         return fields.lognormal(shape=shape, seed=seed, dtype=dtype)
     if kind == "grf":
         return fields.grf(shape, -3.0, seed).astype(dtype)
+    if kind == "noise":      # white noise: every stream's codes span thousands of bins
+        return np.random.default_rng(seed).standard_normal(shape).astype(dtype)
     raise ValueError(kind)
 
 
@@ -101,6 +103,10 @@ CASES = [
     ("grf", (16, 16, 16), np.float32, 0.05, "abs"),
     ("sin", (8, 8, 8), np.float32, 1e-3, "ref"),
     ("sin", (5, 3, 7), np.float64, 1e-3, "rel"),        # dims below 4
+    # used symbols wider than the encoder's 2048-entry LDS window (the window
+    # placed around the centre, the rest read from global memory)
+    ("noise", (40, 48, 56), np.float32, 1e-5, "rel"),
+    ("noise", (24, 32, 40), np.float64, 1e-7, "rel"),
 ]
 
 
