@@ -103,11 +103,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-workers", type=int, default=4,
-                    help="worker contexts (HIP streams); 4 = the box's hardware queues")
+                    help="worker contexts of the request server (one HIP stream each, two for the full-decode worker)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the path is "
-              "single-GPU (multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.",
+        print("bench.py measures one GPU: the compress/decompress path runs on a single MI355X (how it would "
+              "shard over several GPUs is a design note, not a measured run). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
