@@ -1054,7 +1054,8 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
 {
     const uint32_t n = list ? nlist : ss.total_chunks;
     if (!n) return;
-    k_dec_prep<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(ss, payload, list, nlist, items, cmap);
+    // one thread per chunk (a few thousand): 64-lane workgroups spread them over more CUs
+    k_dec_prep<<<dim3((n + 63) / 64), dim3(64), 0, s>>>(ss, payload, list, nlist, items, cmap);
 }
 
 void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, int s0, int ns)
@@ -1090,7 +1091,7 @@ void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs,
 void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks)
 {
     if (!njobs || !total_chunks) return;
-    k_dec_prep_multi<<<dim3((total_chunks + 255) / 256), dim3(256), 0, s>>>(jobs, njobs, total_chunks);
+    k_dec_prep_multi<<<dim3((total_chunks + 63) / 64), dim3(64), 0, s>>>(jobs, njobs, total_chunks);
 }
 
 void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams)
