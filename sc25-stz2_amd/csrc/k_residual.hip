@@ -289,9 +289,20 @@ __device__ __forceinline__ Codes7 load_codes7(const LevelArgs& a, const uint16_t
     return c;
 }
 
+// dequantized values of the kDqW codes around the centre, staged in LDS by
+// the z-march decode (the same dequantize() per code, so the same values;
+// F5:1531-1535 keeps such a table too)
+constexpr int kDqW = 2048;
+constexpr uint32_t kDqLo = (uint32_t)kRadius - kDqW / 2;
+
 template <typename T>
-__device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, uint32_t q, const T* __restrict__ side)
+__device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, uint32_t q, const T* __restrict__ side,
+                                  const T* dq)
 {
+    if (dq) {
+        const uint32_t r = q - kDqLo;  // symbol 0 (unpredictable) falls outside
+        if (r < (uint32_t)kDqW) return dq[r] + p;
+    }
     const T v = q ? dequantize<T>(q, a.eb) : side[a.code_off[b] + pos];
     return v + p;  // F5:1080 de_sub = deData + pred
 }
@@ -299,14 +310,14 @@ __device__ __forceinline__ T dec1(T p, const LevelArgs& a, int b, uint32_t pos, 
 template <typename T>
 __device__ __forceinline__ void low_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a, Codes7 cq,
                                             const T* __restrict__ side, T* __restrict__ mid, int x, int y, int z,
-                                            bool vec)
+                                            bool vec, const T* dq = nullptr)
 {
     const int mx = 2 * g.dx, my = 2 * g.dy;
     const uint32_t pos = stream_pos(g, x, y, z);
     T r[8];
     r[0] = centre;
 #pragma unroll
-    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side);
+    for (int b = 0; b < 7; ++b) r[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side, dq);
     store_cell(mid, (size_t)mx, (size_t)mx * my, 2 * x, 2 * y, 2 * z, r, vec);
 }
 
@@ -327,7 +338,7 @@ __device__ __forceinline__ void low_dec_point(const T* __restrict__ base, const 
 template <typename T>
 __device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom& g, const LevelArgs& a, Codes7 cq,
                                              const T* __restrict__ side, T* __restrict__ out, int nx, int ny, int nz,
-                                             int x, int y, int z, bool vec)
+                                             int x, int y, int z, bool vec, const T* dq = nullptr)
 {
     typedef typename V2<T>::type v2;
     const int X = 2 * x, Y = 2 * y, Z = 2 * z;
@@ -336,7 +347,7 @@ __device__ __forceinline__ void high_dec_fin(const T p[7], T centre, const Geom&
     T v[8];
     v[0] = centre;
 #pragma unroll
-    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side);
+    for (int b = 0; b < 7; ++b) v[b + 1] = dec1(p[b], a, b, pos, cq.q[b], side, dq);
     const size_t oy = (size_t)nx, oz = (size_t)nx * ny;
     T* o = out + (size_t)Z * oz + (size_t)Y * oy;  // row base (wave-uniform in tiles)
     const uint32_t xo = (uint32_t)X * (uint32_t)sizeof(T);
@@ -450,13 +461,13 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // them); encode modes ignore it
 template <typename T, int kMode>
 __device__ __forceinline__ void z_point(const LevelK<T>& K, const T p[7], T c, int x, int y, int z, bool vec,
-                                        uint32_t q[7], Codes7 cq)
+                                        uint32_t q[7], Codes7 cq, const T* dq = nullptr)
 {
     if constexpr (kMode == kHighEnc) high_enc_fin(p, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, x, y, z, q);
     else if constexpr (kMode == kLowEnc)
         low_enc_fin(p, c, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, q);
-    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, x, y, z, vec);
-    else high_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0);
+    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, x, y, z, vec, dq);
+    else high_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0, dq);
 }
 
 // the codes a decode mode needs at block point (x,y,z) (nothing for encode)
@@ -594,10 +605,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
     __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
     __shared__ T ring[kZSlots * kZPlane];
+    __shared__ T dq[kEnc ? 1 : kDqW];
     uint32_t* hist = K.hist;
     if constexpr (kEnc) {
         for (int k = threadIdx.x; k < 7 * kHW; k += 256) sh[k / kHW][k % kHW] = 0u;
         __syncthreads();
+    } else {
+        // read after the first ring barrier
+        for (int k = threadIdx.x; k < kDqW; k += 256) dq[k] = dequantize<T>(kDqLo + (uint32_t)k, K.a.eb);
     }
     const Geom& g = K.g;
     const ZItems Z = z_items(g, K.seg);
@@ -682,7 +697,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                         auto at = [&](int dy, int dz, int k) -> T { return S[dz + 1][o + dy * kZPitch + k]; };
                         T p[7];
                         predict7_at<T>(at, p);
-                        z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, cq[r]);
+                        z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, cq[r], kEnc ? nullptr : dq);
                     }
                 } else if (ok) {
                     z_point_checked<T, kMode>(K, x, y, z, q);
