@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libstz_oracle.so")
+# STZ_ORACLE_LIB: another build of the oracle (the sanitized one, `make -C oracle asan`)
+_LIB_PATH = os.environ.get("STZ_ORACLE_LIB") or os.path.join(_HERE, "_build", "libstz_oracle.so")
 _lib = None
 
 F32, F64 = 0, 1

@@ -2,6 +2,13 @@
 
     python sc25-stz2_amd/build.py          # -> sc25-stz2_amd/lib/libstz_hip.so
                                            #    sc25-stz2_amd/lib/stz_cli
+    python sc25-stz2_amd/build.py --sanitize-host
+        # -> sc25-stz2_amd/_build/asan/libstz_hip.so: host code (C ABI, codebooks,
+        #    planner, header parsing) with AddressSanitizer + UBSan, device code as
+        #    usual; for the CPU tests, e.g.
+        #    STZ_HIP_LIB=sc25-stz2_amd/_build/asan/libstz_hip.so \
+        #    LD_PRELOAD=$(hipcc --print-file-name=libclang_rt.asan-x86_64.so) \
+        #    ASAN_OPTIONS=detect_leaks=0 python -m pytest tests/test_abi.py
 
 Device code is built with -ffp-contract=off (no FMA contraction, matching the
 reference's x86 build without -march) and -fgpu-flush-denormals-to-zero
@@ -68,6 +75,40 @@ def build(verbose=False):
     return so
 
 
+# host code only: each sanitizer flag applies to the host compilation
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def build_sanitized_host(verbose=False):
+    out_dir = os.path.join(OBJ, "asan")
+    os.makedirs(out_dir, exist_ok=True)
+
+    def comp(src):
+        out = os.path.join(out_dir, src + ".o")
+        extra = ["-g", "-O1"] + HOST_SANITIZE if src.endswith(".cpp") else []
+        cmd = [HIPCC] + CFLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+        return out
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(comp, SOURCES))
+    so = os.path.join(out_dir, "libstz_hip.so")
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + HOST_SANITIZE + ["-o", so] + objs + \
+        ["-l:libzstd.so.1", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    if verbose:
+        print("built", so)
+    return so
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize-host" in sys.argv[1:]:
+        build_sanitized_host(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)

@@ -6,7 +6,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libstz_hip.so")
+# STZ_HIP_LIB: another build of the same library (the host-sanitized one, for instance)
+LIB_PATH = os.environ.get("STZ_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libstz_hip.so")
 _lib = None
 
 STZ_OK = 0
