@@ -136,6 +136,25 @@ def test_payload_and_decode_bitexact(ctx, oracle, case):
     ds.free()
 
 
+def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
+    """All 48 configurations the learned tuner chooses among (base interpolator
+    x axis order x level-bound scaling): payload bytes equal the oracle's, f32
+    and f64, and a spread of them decode bit-exactly."""
+    from stz_amd import tuner
+    f32 = _field("grf", (36, 40, 44), np.float32, seed=17)
+    f64 = _field("sin", (20, 24, 28), np.float64, seed=5)
+    for k in range(48):
+        prm = tuner.config_params(k)
+        for f, eb in ((f32, 1e-3), (f64, 1e-5)):
+            _say("tuner configuration", (k, f.dtype.name))
+            ds, gpu, want = _compress_both(ctx, oracle, f, eb, "rel", **prm)
+            diff = first_difference(gpu, want)
+            assert diff is None, (k, f.dtype.name, diff)
+            if k in (0, 13, 30, 47):
+                _assert_bitexact(ds.decompress(), oracle.decompress(want), "decompress config %d" % k)
+            ds.free()
+
+
 @pytest.mark.parametrize("params", [dict(interp=0, order=3), dict(interp=1, order=5, alpha=1.5, beta=4.0),
                                     dict(chunk_high=8, chunk_low=4), dict(mult_base=1.0, mult_low=2.5, mult_high=6.25)])
 def test_params_bitexact(ctx, oracle, params):
