@@ -18,6 +18,12 @@ namespace {
 constexpr int kHW = 1024;
 constexpr int kFusedDecThreads = 1024;
 constexpr uint32_t kHLo = kRadius - kHW / 2;
+// A one-field encode pass: at most kPassEncWgs workgroups (each flushes its
+// LDS histogram with atomics on the same few bins) of kPassEncThreads
+// threads, so the CUs stay occupied (512^3 field, base passes per step:
+// 256-thread x 512 workgroups 108 us, 1024 x 512 93 us, 1024 x 256 91 us)
+constexpr int kPassEncThreads = 1024;
+constexpr unsigned kPassEncWgs = 256;
 
 template <typename T>
 __device__ __forceinline__ T interp_pred(const T* __restrict__ v, long c, long st, int pos, int s, int dim,
@@ -90,17 +96,17 @@ __device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t* gh, uint32_t q,
     if (pending) hist_put(sh, gh, q, 1u);
 }
 
-template <typename T>
+template <typename T, int kThreads>
 __device__ __forceinline__ void interp_encode_wg(const InterpPass& p, T* __restrict__ v, uint16_t* __restrict__ codes,
                                                  T* __restrict__ side, uint32_t* __restrict__ hist, uint32_t wg,
                                                  uint32_t nwg)
 {
     __shared__ uint32_t sh[kHW];
-    for (int k = threadIdx.x; k < kHW; k += 256) sh[k] = 0u;
+    for (int k = threadIdx.x; k < kHW; k += kThreads) sh[k] = 0u;
     __syncthreads();
-    const uint64_t stride = (uint64_t)nwg * 256u;
+    const uint64_t stride = (uint64_t)nwg * kThreads;
     const uint64_t kend = (p.n + 255u) / 256u * 256u;  // every lane of a wave runs the same trip count
-    for (uint64_t k = wg * 256ull + threadIdx.x; k < kend; k += stride) {
+    for (uint64_t k = (uint64_t)wg * kThreads + threadIdx.x; k < kend; k += stride) {
         const bool ok = k < p.n;
         uint32_t q = 0;
         if (ok) {
@@ -122,16 +128,16 @@ __device__ __forceinline__ void interp_encode_wg(const InterpPass& p, T* __restr
         hist_add(sh, hist, q, ok);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kHW; k += 256)
+    for (int k = threadIdx.x; k < kHW; k += kThreads)
         if (sh[k]) atomicAdd(&hist[kHLo + k], sh[k]);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp_encode(InterpPass p, T* __restrict__ v,
+__global__ __launch_bounds__(kPassEncThreads) void k_interp_encode(InterpPass p, T* __restrict__ v,
                                                        uint16_t* __restrict__ codes, T* __restrict__ side,
                                                        uint32_t* __restrict__ hist)
 {
-    interp_encode_wg(p, v, codes, side, hist, blockIdx.x, gridDim.x);
+    interp_encode_wg<T, kPassEncThreads>(p, v, codes, side, hist, blockIdx.x, gridDim.x);
 }
 
 template <typename T>
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(256) void k_interp_encode_multi(const PassEncJob* _
         if (jobs[m].wg0 <= blockIdx.x) lo = m; else hi = m;
     }
     const PassEncJob& J = jobs[lo];
-    interp_encode_wg(J.p, (T*)J.v, J.codes, (T*)J.side, J.hist, blockIdx.x - J.wg0, J.nwg);
+    interp_encode_wg<T, 256>(J.p, (T*)J.v, J.codes, (T*)J.side, J.hist, blockIdx.x - J.wg0, J.nwg);
 }
 
 template <typename T>
@@ -316,8 +322,9 @@ template <typename T>
 void launch_interp_encode(hipStream_t s, const InterpPass& p, T* v, uint16_t* codes, T* side, uint32_t* hist)
 {
     if (!p.n) return;
-    const unsigned g = pass_grid(p.n);
-    k_interp_encode<T><<<dim3(g < kPassEncMaxWgs ? g : kPassEncMaxWgs), dim3(256), 0, s>>>(p, v, codes, side, hist);
+    const uint64_t g = (p.n + kPassEncThreads - 1) / kPassEncThreads;
+    k_interp_encode<T><<<dim3((unsigned)(g < kPassEncWgs ? g : kPassEncWgs)), dim3(kPassEncThreads), 0, s>>>(
+        p, v, codes, side, hist);
 }
 
 template <typename T>

@@ -278,8 +278,9 @@ constexpr int kLatticeBytes = 128 * 1024;
 // (at 128^3, 23 us for the 16^3 lattice plus three stride-4 passes, where the
 // 32^3 lattice took 86 us in one workgroup).
 constexpr int kFusedPoints = 4096;
-// An encode pass flushes one LDS histogram per workgroup with global
-// atomics on the same bins: at most this many workgroups per pass.
+// An encode pass of many fields flushes one LDS histogram per workgroup
+// with global atomics on the same bins: at most this many workgroups per
+// field and pass.
 constexpr unsigned kPassEncMaxWgs = 512;
 // many fields: the fused coarse passes one workgroup per field; a fine pass
 // of every field in one launch (workgroups wg0 .. wg0 + nwg - 1 per field)
