@@ -506,12 +506,49 @@ __device__ __forceinline__ int find_stream_by_wg(const StreamSet& ss, uint32_t w
     return s;
 }
 
-// expand chunks into 1024-symbol decode work items
-__global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
-                           const uint32_t* __restrict__ list, uint32_t nlist,
-                           DecItem* __restrict__ items, uint32_t* __restrict__ cmap)
+// expand one chunk into its 1024-symbol decode work items: one wave per
+// chunk, its lanes over the sub-chunks (64 per round), bit and
+// unpredictable offsets by wave prefix sums with the running total carried
+// between rounds
+__device__ __forceinline__ void dec_prep_chunk(const StreamDesc& d, uint32_t s, uint32_t c, uint32_t ib,
+                                               uint64_t pstart, const uint8_t* __restrict__ payload,
+                                               DecItem* __restrict__ items)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = __lane_id();
+    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
+    const uint16_t* sbits = (const uint16_t*)(payload + d.off_sub_bits);
+    const uint16_t* sun = (const uint16_t*)(payload + d.off_sub_unpred);
+    uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
+    uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
+    const uint32_t sb = csub[c], nsub = csub[c + 1] - sb;
+    const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
+    for (uint32_t j0 = 0; j0 < nsub; j0 += kWave) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        const bool ok = j < nsub;
+        const uint32_t b = ok ? (uint32_t)sbits[sb + j] : 0u, n = ok ? (uint32_t)sun[sb + j] : 0u;
+        const uint32_t ibits = wave_incl_scan_u32(b), iun = wave_incl_scan_u32(n);
+        if (ok) {
+            DecItem it;
+            it.bit0 = bit + (ibits - b);
+            it.u0 = u + (iun - n);
+            it.p0 = (uint32_t)(pstart + (uint64_t)j * kSub);
+            const uint64_t rem = vol - (uint64_t)j * kSub;
+            it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
+            it.stream = s;
+            it.bits = b;
+            items[ib + j] = it;
+        }
+        bit += (uint32_t)__shfl((int)ibits, kWave - 1, kWave);
+        u += (uint32_t)__shfl((int)iun, kWave - 1, kWave);
+    }
+}
+
+// one 64-lane workgroup per chunk of the set, or per listed chunk
+__global__ __launch_bounds__(64) void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
+                                                 const uint32_t* __restrict__ list, uint32_t nlist,
+                                                 DecItem* __restrict__ items, uint32_t* __restrict__ cmap)
+{
+    const uint32_t i = blockIdx.x;
     uint32_t s, c, ib;
     uint64_t pstart;
     if (list) {
@@ -520,7 +557,7 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
         c = list[4 * i + 1];
         ib = ss.s[s].item_base + list[4 * i + 2];
         pstart = list[4 * i + 3];
-        cmap[ss.s[s].cm_base + c] = (uint32_t)pstart;
+        if (threadIdx.x == 0) cmap[ss.s[s].cm_base + c] = (uint32_t)pstart;
     } else {
         if (i >= ss.total_chunks) return;
         s = (uint32_t)find_stream_by_chunk(ss, i);
@@ -528,27 +565,7 @@ __global__ void k_dec_prep(StreamSet ss, const uint8_t* __restrict__ payload,
         ib = ss.s[s].item_base + ((const uint32_t*)(payload + ss.s[s].off_chunk_sub))[c];
         pstart = chunk_start(ss.s[s].g, c);
     }
-    const StreamDesc& d = ss.s[s];
-    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
-    const uint16_t* sbits = (const uint16_t*)(payload + d.off_sub_bits);
-    const uint16_t* sun = (const uint16_t*)(payload + d.off_sub_unpred);
-    uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
-    uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
-    const uint32_t sb = csub[c], se = csub[c + 1];
-    const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
-    for (uint32_t j = 0; j < se - sb; ++j) {
-        DecItem it;
-        it.bit0 = bit;
-        it.u0 = u;
-        it.p0 = (uint32_t)(pstart + (uint64_t)j * kSub);
-        const uint64_t rem = vol - (uint64_t)j * kSub;
-        it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
-        it.stream = s;
-        it.bits = sbits[sb + j];
-        items[ib + j] = it;
-        bit += sbits[sb + j];
-        u += sun[sb + j];
-    }
+    dec_prep_chunk(ss.s[s], s, c, ib, pstart, payload, items);
 }
 
 // Decode tables: one thread per kLutBits-bit window of one stream
@@ -956,9 +973,10 @@ __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restr
 }
 
 // many fields, every chunk: thread -> (field, chunk)
-__global__ void k_dec_prep_multi(const DecJob* __restrict__ jobs, uint32_t njobs, uint32_t total)
+__global__ __launch_bounds__(64) void k_dec_prep_multi(const DecJob* __restrict__ jobs, uint32_t njobs,
+                                                       uint32_t total)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x;
     if (i >= total) return;
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
@@ -971,29 +989,8 @@ __global__ void k_dec_prep_multi(const DecJob* __restrict__ jobs, uint32_t njobs
     const uint32_t s = (uint32_t)find_stream_by_chunk(ss, gc);
     const uint32_t c = gc - ss.s[s].chunk_base;
     const StreamDesc& d = ss.s[s];
-    const uint8_t* payload = J.payload;
-    const uint32_t* csub = (const uint32_t*)(payload + d.off_chunk_sub);
-    const uint16_t* sbits = (const uint16_t*)(payload + d.off_sub_bits);
-    const uint16_t* sun = (const uint16_t*)(payload + d.off_sub_unpred);
-    uint64_t bit = ((const uint64_t*)(payload + d.off_chunk_bit))[c];
-    uint64_t u = ((const uint64_t*)(payload + d.off_chunk_unpred))[c];
-    const uint32_t sb = csub[c], se = csub[c + 1];
-    const uint32_t ib = d.item_base + sb;
-    const uint64_t pstart = chunk_start(d.g, c);
-    const uint64_t vol = chunk_volume(d.g, c, d.n_symbols);
-    for (uint32_t j = 0; j < se - sb; ++j) {
-        DecItem it;
-        it.bit0 = bit;
-        it.u0 = u;
-        it.p0 = (uint32_t)(pstart + (uint64_t)j * kSub);
-        const uint64_t rem = vol - (uint64_t)j * kSub;
-        it.cnt = (uint32_t)(rem < (uint64_t)kSub ? rem : (uint64_t)kSub);
-        it.stream = s;
-        it.bits = sbits[sb + j];
-        J.items[ib + j] = it;
-        bit += sbits[sb + j];
-        u += sun[sb + j];
-    }
+    const uint32_t sb = ((const uint32_t*)(J.payload + d.off_chunk_sub))[c];
+    dec_prep_chunk(d, s, c, d.item_base + sb, chunk_start(d.g, c), J.payload, J.items);
 }
 
 }  // namespace
@@ -1054,8 +1051,7 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
 {
     const uint32_t n = list ? nlist : ss.total_chunks;
     if (!n) return;
-    // one thread per chunk (a few thousand): 64-lane workgroups spread them over more CUs
-    k_dec_prep<<<dim3((n + 63) / 64), dim3(64), 0, s>>>(ss, payload, list, nlist, items, cmap);
+    k_dec_prep<<<dim3(n), dim3(64), 0, s>>>(ss, payload, list, nlist, items, cmap);
 }
 
 void launch_hist_compact_multi(hipStream_t s, const EncJob* jobs, uint32_t njobs, int s0, int ns)
@@ -1091,7 +1087,7 @@ void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs,
 void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks)
 {
     if (!njobs || !total_chunks) return;
-    k_dec_prep_multi<<<dim3((total_chunks + 63) / 64), dim3(64), 0, s>>>(jobs, njobs, total_chunks);
+    k_dec_prep_multi<<<dim3(total_chunks), dim3(64), 0, s>>>(jobs, njobs, total_chunks);
 }
 
 void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams)
