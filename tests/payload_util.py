@@ -48,3 +48,41 @@ def first_difference(a, b):
             return msg + " first diff at %d: stream %d section %s (+%d), %d bytes differ" % (
                 i, s, sec, rel, d.size)
     return msg + " first diff at %d" % i
+
+
+def _clear_nan_sign(words, bits):
+    """Sign bit cleared in place for every NaN bit pattern of a uint32/uint64 view."""
+    exp = np.uint64((1 << (bits - 1)) - 1) if bits == 64 else np.uint32((1 << 31) - 1)
+    inf = np.uint64(0x7FF0000000000000) if bits == 64 else np.uint32(0x7F800000)
+    nan = (words & exp) > inf
+    words[nan] &= exp
+
+
+def nan_sign_cleared(payload, dtype):
+    """A copy of a payload whose unpredictable-value sections hold every NaN
+    with its sign bit cleared.  A subtraction whose subtrahend is NaN keeps
+    the NaN's sign on x86 and flips it on the GPU (its subtract is an add with
+    a negated operand), so fields holding NaNs differ in that bit only."""
+    out = np.array(payload, dtype=np.uint8, copy=True)
+    size = np.dtype(dtype).itemsize
+    so, sb = stream_table(out)
+    for s in range(15):
+        if sb[s] == 0:
+            continue
+        base, offs = stream_sections(out, s)
+        lo, hi = base + offs[6], base + offs[7]
+        n = (hi - lo) // size
+        if n == 0:
+            continue
+        assert lo % size == 0, "unaligned unpredictable section"
+        view = out[lo:lo + n * size].view(np.uint64 if size == 8 else np.uint32)
+        _clear_nan_sign(view, 8 * size)
+    return out
+
+
+def nan_sign_cleared_values(a):
+    """A copy of a float array with every NaN's sign bit cleared."""
+    a = np.array(a, copy=True)
+    w = a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+    _clear_nan_sign(w.reshape(-1), 8 * a.dtype.itemsize)
+    return a

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from payload_util import first_difference
+from payload_util import first_difference, nan_sign_cleared, nan_sign_cleared_values
 
 pytestmark = pytest.mark.gpu
 
@@ -206,6 +206,54 @@ def test_roi_slice_level(ctx, oracle, dtype):
     for lv in range(4):
         _say("level", lv)
         _assert_bitexact(ds.decompress_level(lv), oracle.decompress_level(want, lv), "level %d" % lv)
+
+
+def _degenerate(kind, dtype):
+    rng = np.random.default_rng(23)
+    shape = (20, 24, 28)
+    if kind == "constant":
+        return np.full(shape, 3.5, dtype)
+    if kind == "zeros":
+        return np.zeros(shape, dtype)
+    if kind == "point":
+        return np.array([[[2.0]]], dtype)
+    g = rng.standard_normal(shape).astype(dtype)
+    if kind == "nan-first":          # a NaN where a sequential range scan starts
+        g[0, 0, 0] = np.nan
+    elif kind == "nan-inf":
+        g[3, 4, 5] = np.nan
+        g[10, 1, 2] = np.inf
+        g[19, 23, 27] = -np.inf
+    return g
+
+
+# (field, eb, mode): a zero value range under a relative bound (bound 0: every
+# value stored verbatim), one symbol per stream, a 1-point field, a bound so
+# large that every code is the centre, and non-finite inputs (skipped by the
+# value range; unpredictable, and their neighbours' predictions with them)
+DEGENERATE = [("constant", 1e-3, "rel"), ("zeros", 1e-3, "abs"), ("point", 1e-3, "rel"), ("normal", 1e30, "abs"),
+              ("nan-first", 1e-3, "rel"), ("nan-inf", 1e-3, "abs")]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", DEGENERATE, ids=[c[0] + "-" + c[2] for c in DEGENERATE])
+def test_degenerate_fields_bitexact(ctx, oracle, case, dtype):
+    kind, eb, mode = case
+    _say("degenerate field", kind, np.dtype(dtype).name)
+    f = _degenerate(kind, dtype)
+    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode)
+    got, ref = ds.decompress(), oracle.decompress(want)
+    if kind.startswith("nan"):
+        # NaN side values and decoded NaNs may differ in the sign bit only
+        # (x86 keeps a NaN subtrahend's sign, the GPU flips it): equal with
+        # that bit cleared, NaNs at the same points, every other bit equal
+        gpu, want = nan_sign_cleared(gpu, dtype), nan_sign_cleared(want, dtype)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        got, ref = nan_sign_cleared_values(got), nan_sign_cleared_values(ref)
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    _assert_bitexact(got, ref, "decompress " + kind)
+    ds.free()
 
 
 def test_c1_config_bitexact(ctx, oracle):
