@@ -75,6 +75,10 @@ __device__ __forceinline__ void stage_enc_window(const uint32_t* __restrict__ es
 // a position past the sub-chunk's end: encoder entry 0 (no bits), not a zero code
 constexpr uint32_t kNoCode = 0xFFFFFFFFu;
 
+// symbols of codes longer than the decode table window held in LDS by a
+// staged decode workgroup (2 KiB; two workgroups still share a CU)
+constexpr int kLongSyms = 1024;
+
 // A lane's 16 codes of sub-chunk j (positions pstart + 1024 j + 16 lane ..
 // +15): two 16-byte loads for a whole aligned sub-chunk, else code by code
 // with kNoCode past its end
@@ -633,7 +637,7 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
                                                const uint32_t* __restrict__ fi, const uint16_t* __restrict__ srt,
                                                uint16_t* __restrict__ co, uint8_t* __restrict__ so,
                                                const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
-                                               uint64_t al)
+                                               uint64_t al, const uint16_t* lsrt, uint32_t llo, uint32_t ln)
 {
     uint64_t wpos = (it.bit0 >> 5) - wbase;
     const int skip = (int)(it.bit0 & 31u);
@@ -668,7 +672,9 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
             l = 1;
             return 0u;
         }
-        return srt[fi[l] + (win >> (32 - l)) - fc[l]];
+        // the long codes' symbols: the workgroup's LDS copy of their first ln
+        const uint32_t idx = fi[l] + (win >> (32 - l)) - fc[l];
+        return idx - llo < ln ? (uint32_t)lsrt[idx - llo] : (uint32_t)srt[idx];
     };
     // symbol 0 at stream position `pos` of the sub-chunk: the next raw value
     auto unpred = [&](uint32_t pos) {
@@ -862,16 +868,25 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
                                                const DecItem* __restrict__ items, uint16_t* __restrict__ codes,
                                                uint8_t* __restrict__ side, uint32_t* __restrict__ err,
                                                uint32_t wgid, uint64_t* lut, uint32_t* fc, uint32_t* cn,
-                                               uint32_t* fi, uint32_t* stage)
+                                               uint32_t* fi, uint32_t* stage, uint16_t* lsrt)
 {
     const int s = find_stream_by_wg(ss, wgid);
     if (s < 0) return;  // uniform across the workgroup
     const StreamDesc& d = ss.s[s];
     const DecTab* t = tabs + s;
     uint32_t* lut1 = (uint32_t*)lut;  // the single-symbol table of the unstaged form
-    if constexpr (kStage)
+    // staged form: the symbols of the codes longer than the table window
+    // (the tail of the canonical order) up to kLongSyms of them in LDS, so
+    // that the long-code path waits on no global load
+    uint32_t llo = 0, ln = 0;
+    if constexpr (kStage) {
         for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
-    else
+        llo = t->first_index[kLutBits + 1];
+        const uint32_t lhi = t->first_index[kLmax] + t->count[kLmax];
+        ln = lhi > llo ? (lhi - llo < (uint32_t)kLongSyms ? lhi - llo : (uint32_t)kLongSyms) : 0u;
+        const uint16_t* s0 = sorted + t->sorted_off + llo;
+        for (uint32_t k = threadIdx.x; k < ln; k += 256) lsrt[k] = s0[k];
+    } else
         for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut1[k] = t->lut1[k];
     if (threadIdx.x < kLmax + 2) {
         fc[threadIdx.x] = t->first_code[threadIdx.x];
@@ -927,9 +942,11 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
         else
             bad = decode_run1<1>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
     } else if (in_stage) {
-        bad = decode_run<true>(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+        bad = decode_run<true>(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al,
+                               lsrt, llo, ln);
     } else {
-        bad = decode_run<false>(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al);
+        bad = decode_run<false>(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al,
+                                lsrt, llo, ln);
     }
     if (bad) atomicOr(err, 1u);
 }
@@ -947,7 +964,9 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
-    huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage);
+    __shared__ uint16_t lsrt[kStage ? kLongSyms : 1];
+    huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage,
+                           lsrt);
 }
 
 // many fields: workgroups wg0 .. of job j decode field j
@@ -967,9 +986,10 @@ __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restr
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    __shared__ uint16_t lsrt[kStage ? kLongSyms : 1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];
     huff_decode_wg<kStage>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
-                           blockIdx.x - J.wg0, lut, fc, cn, fi, stage);
+                           blockIdx.x - J.wg0, lut, fc, cn, fi, stage, lsrt);
 }
 
 // many fields, every chunk: thread -> (field, chunk)
