@@ -78,6 +78,8 @@ constexpr uint32_t kNoCode = 0xFFFFFFFFu;
 // symbols of codes longer than the decode table window held in LDS by a
 // staged decode workgroup (2 KiB; two workgroups still share a CU)
 constexpr int kLongSyms = 1024;
+// the same for the unstaged form, whose 16 KiB table leaves room
+constexpr int kLongSyms1 = 2048;
 
 // A lane's 16 codes of sub-chunk j (positions pstart + 1024 j + 16 lane ..
 // +15): two 16-byte loads for a whole aligned sub-chunk, else code by code
@@ -768,7 +770,8 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
                                                const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
                                                const uint32_t* __restrict__ fi, const uint16_t* __restrict__ srt,
                                                uint16_t* __restrict__ co, uint8_t* __restrict__ so,
-                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred)
+                                               const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
+                                               const uint16_t* lsrt, uint32_t llo, uint32_t ln)
 {
     uint64_t wpos = (it.bit0 >> 5) - wbase;
     const int skip = (int)(it.bit0 & 31u);
@@ -804,7 +807,8 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
                 l = 1;
                 sym = 0;
             } else {
-                sym = srt[fi[l] + (win >> (32 - l)) - fc[l]];
+                const uint32_t idx = fi[l] + (win >> (32 - l)) - fc[l];
+                sym = idx - llo < ln ? (uint32_t)lsrt[idx - llo] : (uint32_t)srt[idx];
             }
         }
         buf <<= l;
@@ -875,19 +879,19 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     const StreamDesc& d = ss.s[s];
     const DecTab* t = tabs + s;
     uint32_t* lut1 = (uint32_t*)lut;  // the single-symbol table of the unstaged form
-    // staged form: the symbols of the codes longer than the table window
-    // (the tail of the canonical order) up to kLongSyms of them in LDS, so
-    // that the long-code path waits on no global load
-    uint32_t llo = 0, ln = 0;
-    if constexpr (kStage) {
+    if constexpr (kStage)
         for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
-        llo = t->first_index[kLutBits + 1];
-        const uint32_t lhi = t->first_index[kLmax] + t->count[kLmax];
-        ln = lhi > llo ? (lhi - llo < (uint32_t)kLongSyms ? lhi - llo : (uint32_t)kLongSyms) : 0u;
-        const uint16_t* s0 = sorted + t->sorted_off + llo;
-        for (uint32_t k = threadIdx.x; k < ln; k += 256) lsrt[k] = s0[k];
-    } else
+    else
         for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut1[k] = t->lut1[k];
+    // the symbols of the codes longer than the table window (the tail of
+    // the canonical order), the first kLongSyms (kLongSyms1) of them, in LDS:
+    // the long-code path then waits on no global load
+    constexpr uint32_t kCap = kStage ? kLongSyms : kLongSyms1;
+    const uint32_t llo = t->first_index[kLutBits + 1];
+    const uint32_t lhi = t->first_index[kLmax] + t->count[kLmax];
+    const uint32_t ln = lhi > llo ? (lhi - llo < kCap ? lhi - llo : kCap) : 0u;
+    const uint16_t* s0 = sorted + t->sorted_off + llo;
+    for (uint32_t k = threadIdx.x; k < ln; k += 256) lsrt[k] = s0[k];
     if (threadIdx.x < kLmax + 2) {
         fc[threadIdx.x] = t->first_code[threadIdx.x];
         cn[threadIdx.x] = t->count[threadIdx.x];
@@ -936,11 +940,14 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     uint32_t bad;
     if constexpr (!kStage) {
         if ((al & 31u) == 0)
-            bad = decode_run1<32>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+            bad = decode_run1<32>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
+                                  lsrt, llo, ln);
         else if ((al & 3u) == 0)
-            bad = decode_run1<4>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+            bad = decode_run1<4>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
+                                  lsrt, llo, ln);
         else
-            bad = decode_run1<1>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred);
+            bad = decode_run1<1>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
+                                  lsrt, llo, ln);
     } else if (in_stage) {
         bad = decode_run<true>(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al,
                                lsrt, llo, ln);
@@ -964,7 +971,7 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
-    __shared__ uint16_t lsrt[kStage ? kLongSyms : 1];
+    __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
     huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage,
                            lsrt);
 }
@@ -986,7 +993,7 @@ __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restr
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
-    __shared__ uint16_t lsrt[kStage ? kLongSyms : 1];
+    __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];
     huff_decode_wg<kStage>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
                            blockIdx.x - J.wg0, lut, fc, cn, fi, stage, lsrt);
