@@ -115,6 +115,12 @@ int stz_decompress_slice(stz_ctx* ctx, const void* stream, size_t bytes, int axi
 int stz_decompress_level(stz_ctx* ctx, const void* stream, size_t bytes, int level, void* out,
                          size_t out_dims[3]);
 int stz_query_info(const void* stream, size_t bytes, stz_info* info);
+/* Host-only validation of a container or bare payload: every zstd frame,
+ * header, codebook and chunk table is checked as a decode would check it
+ * before its first kernel launch.  STZ_OK or STZ_E_STREAM; needs no GPU.
+ * Replaces the nullptr returns of SZ_decompress_nopred_marked_chunks on a
+ * bad stream (F5:1855-1861) with an up-front check. */
+int stz_stream_check(const void* stream, size_t bytes);
 
 /* ---- device-resident streams (benchmark, serving) ---------------------- */
 int stz_compress_device(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny,

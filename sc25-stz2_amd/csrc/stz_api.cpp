@@ -33,6 +33,7 @@ size_t ZSTD_compress(void* dst, size_t cap, const void* src, size_t n, int level
 size_t ZSTD_decompress(void* dst, size_t cap, const void* src, size_t n);
 size_t ZSTD_compressBound(size_t n);
 unsigned ZSTD_isError(size_t code);
+unsigned long long ZSTD_getFrameContentSize(const void* src, size_t n);
 }
 
 using namespace stz;
@@ -2184,16 +2185,32 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     check_dims(fh.nx, fh.ny, fh.nz, (int)fh.chunk_high, (int)fh.chunk_low);
     HostLayout L;
     make_layout(L, fh.nx, fh.ny, fh.nz, fh.px, fh.py, fh.pz, (int)fh.chunk_high, (int)fh.chunk_low);
+    // streams are packed back to back after the file header (the writer's
+    // layout); every bound below is checked without overflow
+    uint64_t next = sizeof(FileHeader);
     for (int s = 0; s < kStreams; ++s) {
-        if (fh.stream_off[s] + sizeof(StreamHeader) > bytes || fh.stream_off[s] + fh.stream_bytes[s] > bytes)
+        if (fh.stream_off[s] != next || fh.stream_off[s] > bytes || fh.stream_bytes[s] < sizeof(StreamHeader) ||
+            fh.stream_bytes[s] > bytes - fh.stream_off[s])
             throw StzError(STZ_E_STREAM, "stream out of bounds");
+        next = fh.stream_off[s] + fh.stream_bytes[s];
         StreamHeader& sh = m.sh[s];
         std::memcpy(&sh, p + fh.stream_off[s], sizeof(StreamHeader));
+        // counts bounded before they size the sections (no wrap in the offsets)
+        if (sh.n_used > 65536 || sh.n_unpred > L.n[s] || sh.total_bits > (uint64_t)L.n[s] * kLmax)
+            throw StzError(STZ_E_STREAM, "bad stream header");
         StreamHeader chk;
         fill_stream_header(chk, s, L, (int)fh.dtype, sh.eb, sh.n_used, sh.n_unpred, sh.total_bits);
+        // every section offset is the one the geometry and counts imply: the
+        // host reads and the decode kernels' reads go through them
         if (sh.magic != kStreamMagic || sh.n_symbols != L.n[s] || sh.n_chunks != L.g[s].n_chunks ||
-            sh.stream_bytes != chk.stream_bytes || sh.off_bits != chk.off_bits || sh.n_used > 65536 ||
-            sh.stream_bytes != fh.stream_bytes[s])
+            sh.n_sub != chk.n_sub || sh.stream_bytes != chk.stream_bytes || sh.off_codebook != chk.off_codebook ||
+            sh.off_chunk_bit != chk.off_chunk_bit || sh.off_chunk_unpred != chk.off_chunk_unpred ||
+            sh.off_chunk_sub != chk.off_chunk_sub || sh.off_sub_bits != chk.off_sub_bits ||
+            sh.off_sub_unpred != chk.off_sub_unpred || sh.off_unpred != chk.off_unpred ||
+            sh.off_bits != chk.off_bits || sh.stream_bytes != fh.stream_bytes[s] || sh.kind != chk.kind ||
+            sh.block != chk.block || sh.dtype != chk.dtype || sh.dx != chk.dx || sh.dy != chk.dy ||
+            sh.dz != chk.dz || sh.chunk != chk.chunk || sh.lmax != (uint32_t)kLmax ||
+            sh.radius != (uint32_t)kRadius || sh.sub_size != (uint32_t)kSub)
             throw StzError(STZ_E_STREAM, "bad stream header");
         const uint32_t* cb = (const uint32_t*)(p + fh.stream_off[s] + sh.off_codebook);
         Codebook& b = m.cb[s];
@@ -2231,6 +2248,7 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
             cun[sh.n_chunks] != au || au != sh.n_unpred)
             throw StzError(STZ_E_STREAM, "bad chunk table totals");
     }
+    if (next != fh.total_bytes) throw StzError(STZ_E_STREAM, "bad payload size");
 }
 
 // ------------------------------------------------------------ host container
@@ -2295,18 +2313,24 @@ void from_container(const uint8_t* in, size_t n, std::vector<uint8_t>& pay)
     if (ch.magic != kContMagic || ch.version != 1 || ch.n_frames != kStreams + 1 || ch.raw[0] != sizeof(FileHeader))
         throw StzError(STZ_E_STREAM, "bad container");
     size_t o = sizeof(ch), po = 0;
-    uint64_t sum = 0;
-    for (int f = 0; f <= kStreams; ++f) sum += ch.raw[f];
-    if (sum != ch.payload_bytes || ch.payload_bytes > (1ull << 40)) throw StzError(STZ_E_STREAM, "bad container sizes");
+    if (ch.payload_bytes > (1ull << 40)) throw StzError(STZ_E_STREAM, "bad container sizes");
     // validate every frame before starting any thread: a throw with joinable
-    // std::threads alive would terminate the process
+    // std::threads alive would terminate the process.  Each frame's raw size
+    // must fit what is left of the payload (no wrapped sums) and equal the
+    // content size its zstd frame header declares.
     {
         size_t oo = sizeof(ch);
+        uint64_t left = ch.payload_bytes;
         for (int f = 0; f <= kStreams; ++f) {
-            if (ch.comp[f] > n || oo + ch.comp[f] > n) throw StzError(STZ_E_STREAM, "truncated container");
+            if (ch.comp[f] > n - oo) throw StzError(STZ_E_STREAM, "truncated container");
+            if (ch.raw[f] > left) throw StzError(STZ_E_STREAM, "bad container sizes");
             if (f == 0 && ch.comp[f] != ch.raw[f]) throw StzError(STZ_E_STREAM, "bad header frame");
+            if (f > 0 && ZSTD_getFrameContentSize(in + oo, ch.comp[f]) != ch.raw[f])
+                throw StzError(STZ_E_STREAM, "bad zstd frame size");
+            left -= ch.raw[f];
             oo += ch.comp[f];
         }
+        if (left != 0) throw StzError(STZ_E_STREAM, "bad container sizes");
     }
     pay.resize(ch.payload_bytes);
     std::memcpy(pay.data(), in + o, ch.raw[0]);
@@ -3002,6 +3026,17 @@ int stz_decompress_level(stz_ctx* c, const void* stream, size_t bytes, int level
     rc = stz_dstream_decompress_level(c, ds, level, out, od);
     stz_dstream_free(ds);
     return rc;
+}
+
+int stz_stream_check(const void* stream, size_t bytes)
+{
+    if (!stream) return STZ_E_ARG;
+    return guard(nullptr, [&] {
+        std::vector<uint8_t> pay;
+        from_container((const uint8_t*)stream, bytes, pay);
+        Meta m;
+        parse_meta(pay.data(), pay.size(), m);
+    });
 }
 
 int stz_query_info(const void* stream, size_t bytes, stz_info* info)

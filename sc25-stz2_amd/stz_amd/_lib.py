@@ -20,7 +20,7 @@ EXPORTS = [
     "stz_profile_report",
     "stz_default_params", "stz_strerror", "stz_free",
     "stz_compress", "stz_decompress", "stz_decompress_roi", "stz_decompress_slice", "stz_decompress_level",
-    "stz_query_info",
+    "stz_query_info", "stz_stream_check",
     "stz_compress_device", "stz_dstream_upload", "stz_dstream_download", "stz_dstream_info",
     "stz_dstream_decompress", "stz_dstream_decompress_roi", "stz_dstream_decompress_level",
     "stz_dstream_payload", "stz_dstream_free", "stz_dstream_prepare", "stz_dstream_decompress_batch",
@@ -107,6 +107,7 @@ def lib():
     L.stz_decompress_slice.argtypes = [vp, vp, sz, i32, i32, vp, P(sz)]
     L.stz_decompress_level.argtypes = [vp, vp, sz, i32, vp, P(sz)]
     L.stz_query_info.argtypes = [vp, sz, P(Info)]
+    L.stz_stream_check.argtypes = [vp, sz]
     L.stz_compress_device.argtypes = [vp, i32, vp, sz, sz, sz, dbl, i32, P(Params), P(vp)]
     L.stz_dstream_upload.argtypes = [vp, vp, sz, P(vp)]
     L.stz_dstream_download.argtypes = [vp, vp, i32, P(vp), P(sz)]

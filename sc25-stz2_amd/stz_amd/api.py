@@ -76,6 +76,12 @@ def stream_info(stream):
     return info
 
 
+def stream_check(stream):
+    """Host-only validation of a container or payload (stz_stream_check); raises StzError."""
+    buf = np.frombuffer(stream, dtype=np.uint8) if not isinstance(stream, np.ndarray) else stream
+    check(lib().stz_stream_check(_ptr(buf), buf.size), "stz_stream_check")
+
+
 class DStream:
     """A compressed stream resident in HBM (stz_dstream)."""
 

@@ -87,17 +87,26 @@ def test_plan_query_matches_golden(stz):
     from stz_amd import api
     g = np.load(os.path.join(ROOT, "tests", "golden", "ref_vectors.npz"))
     pin = g["plan_in"]
-    for k in range(0, pin.shape[0], 3):
+    # the golden chunk ids of case k, block b follow those of every earlier
+    # (case, block) in the concatenated arrays
+    offs = {"high": 0, "low": 0}
+    for k in range(pin.shape[0]):
         dims = tuple(int(v) for v in pin[k, :3])
         q = [int(v) for v in pin[k, 5:11]]
         qa, sl, high, low = api.plan_query(dims, q, int(pin[k, 3]), int(pin[k, 4]), 4096)
         gq = g["plan_query"][k]
-        assert qa == tuple(int(v) for v in gq[:6]) and sl == bool(gq[6])
-        for plans, table in ((high, g["plan_high"][k]), (low, g["plan_low"][k])):
-            assert set(plans) == {b for b in range(7) if table[b, 0]}
-            for b in plans:
-                assert plans[b][0] == tuple(int(v) for v in table[b, 1:7])
-                assert len(plans[b][1]) == table[b, 7]
+        assert qa == tuple(int(v) for v in gq[:6]) and sl == bool(gq[6]), k
+        for name, plans, table in (("high", high, g["plan_high"][k]), ("low", low, g["plan_low"][k])):
+            allc = g["plan_%s_chunks" % name]
+            assert set(plans) == {b for b in range(7) if table[b, 0]}, (k, name)
+            for b in range(7):
+                if not table[b, 0]:
+                    continue
+                assert plans[b][0] == tuple(int(v) for v in table[b, 1:7]), (k, name, b)
+                n = int(table[b, 7])
+                assert np.array_equal(plans[b][1], allc[offs[name]:offs[name] + n]), (k, name, b)
+                offs[name] += n
+    assert offs["high"] == g["plan_high_chunks"].size and offs["low"] == g["plan_low_chunks"].size
 
 
 def _cli():

@@ -525,3 +525,26 @@ def test_bf16_rounding_reference():
         assert mlp_ref.bf16(np.float32(x)) == want, x
     v = np.float32(np.pi)
     assert mlp_ref.bf16(v) == 3.140625          # 0x4049 (pi rounds down)
+
+
+def test_mlp_reference_adam_matches_torch_adamw():
+    """The reference Adam step (decoupled weight decay) against
+    torch.optim.AdamW in float64 over several steps with changing gradients."""
+    import torch
+
+    import mlp_ref
+    rng = np.random.default_rng(3)
+    n = 257
+    P = rng.normal(size=n)
+    M = np.zeros(n)
+    V = np.zeros(n)
+    lr, wd = 3e-3, 1e-2
+    t = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.AdamW([t], lr=lr, betas=(0.9, 0.999), eps=1e-8, weight_decay=wd)
+    for step in range(1, 8):
+        G = rng.normal(size=n) * (10.0 ** rng.integers(-3, 2))
+        P, M, V = mlp_ref.adam(P, G, M, V, step, lr, wd)
+        opt.zero_grad()
+        t.grad = torch.tensor(G, dtype=torch.float64)
+        opt.step()
+        np.testing.assert_allclose(P, t.detach().numpy(), rtol=1e-12, atol=1e-14)
