@@ -36,6 +36,14 @@ def config_params(k):
             "mult_high": p.mult_high, "alpha": p.alpha, "beta": p.beta}
 
 
+def config_index(prm):
+    """The configuration whose parameters are prm (a dict like config_params'), or -1."""
+    for k in range(TUNE_CONFIGS):
+        if config_params(k) == {key: prm[key] for key in config_params(0)}:
+            return k
+    return -1
+
+
 def features(ctx, field, eb_abs):
     """16 statistics per 64^3 sub-block -> (n_blocks, 16) float32."""
     nz, ny, nx = field.shape

@@ -1,0 +1,248 @@
+"""The benchmark configurations at their own sizes on the GPU:
+
+* 512^3 fp32 log-normal field (seed 1002) at rel eb 1e-2, 1e-3 and 1e-4:
+  payload bytes equal the oracle's, decode bit-exact, error within the bound;
+* 256x384x384 fp64 turbulence-like field (seed 1003) at rel eb 1e-6 with the
+  learned tuner and with the exhaustive trial: the payload equals the
+  oracle's for the chosen configuration, the trial's choice is the smallest
+  of its 48 sizes and those sizes match the oracle's, decode within the bound;
+* 96 uneven fp32 fields (seeds 1004 + i) in one batch call at rel eb 1e-3:
+  every stream and decoded field equals its one-call result, every eighth
+  field's payload equals the oracle's, every field within the bound;
+* 1024^3 fp32 GRF (seed 1005) compressed once at rel eb 1e-3: payload and
+  full decode equal the oracle's; a Poisson mix of 200 box / slice / level
+  requests through the continuous-batching server, every result equal to
+  the request run alone and to the crop of the full decode (F5:2609-2640).
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from payload_util import first_difference
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+def _equal_bits(got, want, what):
+    g, w = _bits(got).ravel(), _bits(want).ravel()
+    assert g.shape == w.shape, "%s: %s vs %s" % (what, got.shape, want.shape)
+    bad = np.flatnonzero(g != w)
+    assert bad.size == 0, "%s: %d of %d values differ, first at %d" % (what, bad.size, g.size, bad[0])
+
+
+def _bound_ok(rec, f, dstream_or_header):
+    """max |rec - f| <= the level bound plus the reconstruction's T rounding
+    (2 ulp of the largest magnitude)."""
+    h = dstream_or_header
+    bound = max(h["eb_base"], h["eb_low"], h["eb_high"])
+    err = float(np.max(np.abs(rec.astype(np.float64) - f.astype(np.float64))))
+    slack = 2 * float(np.spacing(np.abs(f).max().astype(f.dtype)))
+    return err <= bound + slack, err, bound
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    return torch.device("cuda:0")
+
+
+# ------------------------------------------------------------------ 512^3 log-normal
+
+@pytest.fixture(scope="module")
+def c2_field(dev):
+    from stz_amd import fields
+    t0 = time.time()
+    f = fields.lognormal(n=512, seed=1002, dtype=np.float32, device=dev)
+    _say("512^3 log-normal generated in %.1fs" % (time.time() - t0))
+    return f
+
+
+@pytest.mark.parametrize("eb", [1e-2, 1e-3, 1e-4])
+def test_c2_512cubed_lognormal_bitexact(ctx, oracle, dev, c2_field, eb):
+    import torch
+    f = c2_field
+    _say("512^3 log-normal, rel eb", eb)
+    t = torch.from_numpy(f).to(dev)
+    ds = ctx.compress_device(t, eb, "rel")
+    gpu = ds.download(zstd_level=-1)
+    t0 = time.time()
+    want = oracle.compress(f, eb, oracle.EB_REL)
+    _say("  oracle compress %.1fs, payload %d bytes" % (time.time() - t0, want.size))
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    out = torch.empty_like(t)
+    ds.decompress(out)
+    rec = out.cpu().numpy()
+    ref = oracle.decompress(want)
+    _equal_bits(rec, ref, "512^3 decompress")
+    ok, err, bound = _bound_ok(rec, f, oracle.header(want))
+    assert ok, (err, bound)
+    ds.free()
+
+
+# ------------------------------------------------------------------ 256x384x384 fp64, tuned
+
+@pytest.fixture(scope="module")
+def c3(ctx, dev):
+    import stz_amd
+    from stz_amd import fields, tuner
+    t0 = time.time()
+    f = fields.turbulence(shape=(256, 384, 384), seed=1003, device=dev)
+    _say("256x384x384 turbulence generated in %.1fs" % (time.time() - t0))
+    c = stz_amd.Context(0)
+    params, meta = tuner.load_params()
+    tuner.set_model(c, params, min_confidence=float(meta.get("min_confidence", 0.0)))
+    yield f, c
+    c.close()
+
+
+def _chosen(info):
+    return dict(interp=int(info.interp), order=int(info.order), mult_base=info.mult_base,
+                mult_low=info.mult_low, mult_high=info.mult_high, alpha=info.alpha, beta=info.beta)
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["learned", "trial"])
+def test_c3_turbulence_fp64_tuned(oracle, dev, c3, mode):
+    import torch
+    f, c = c3
+    _say("256x384x384 fp64, rel eb 1e-6, tuner", mode)
+    t = torch.from_numpy(f).to(dev)
+    ds = c.compress_device(t, 1e-6, "rel", tuner=mode)
+    prm = _chosen(ds.info)
+    gpu = ds.download(zstd_level=-1)
+    want = oracle.compress(f, 1e-6, oracle.EB_REL, **prm)
+    diff = first_difference(gpu, want)
+    assert diff is None, (prm, diff)
+    out = torch.empty_like(t)
+    ds.decompress(out)
+    rec = out.cpu().numpy()
+    _equal_bits(rec, oracle.decompress(want), "256x384x384 decompress")
+    ok, err, bound = _bound_ok(rec, f, oracle.header(want))
+    assert ok, (err, bound)
+    # the requested bound holds whatever the choice
+    assert bound <= 1e-6 * float(f.max() - f.min()) * (1 + 1e-12)
+    if mode == 2:
+        from stz_amd import tuner
+        sizes = tuner.trial_sizes(c, t, 1e-6)
+        assert int(np.argmin(sizes)) == tuner.config_index(prm), (int(np.argmin(sizes)), prm)
+        assert gpu.size == int(sizes.min())
+        for k in (0, 17, 40):   # the trial's sizes are the oracle's payload sizes
+            assert int(sizes[k]) == oracle.compress(f, 1e-6, oracle.EB_REL, **tuner.config_params(k)).size, k
+    ds.free()
+
+
+# ------------------------------------------------------------------ 96 uneven fields
+
+def test_c4_batch_of_96_uneven_fields(ctx, oracle, dev):
+    import torch
+    from stz_amd import fields
+    t0 = time.time()
+    host = fields.uneven_batch(96)   # seeds 1004 + i, dims U{32..384}, slopes U[-4,-2]
+    fs = [torch.from_numpy(h).to(dev) for h in host]
+    pts = sum(h.size for h in host)
+    _say("96 fields, %d points, generated in %.1fs" % (pts, time.time() - t0))
+    batch = ctx.compress_batch(fs, 1e-3, "rel")
+    outs = [torch.empty_like(t) for t in fs]
+    ctx.decompress_batch(batch, outs)
+    torch.cuda.synchronize()
+    _say("batch compress + decompress done")
+    for i, (h, t, b, o) in enumerate(zip(host, fs, batch, outs)):
+        s = ctx.compress_device(t, 1e-3, "rel")
+        pb, ps = b.download(-1), s.download(-1)
+        assert pb.size == ps.size and np.array_equal(pb, ps), (i, h.shape)
+        single = s.decompress(like=t)
+        rec = o.cpu().numpy()
+        _equal_bits(rec, single.cpu().numpy(), "field %d %s batch vs single decode" % (i, h.shape))
+        ok, err, bound = _bound_ok(rec, h, oracle.header(pb))
+        assert ok, (i, h.shape, err, bound)
+        if i % 8 == 0:
+            want = oracle.compress(h, 1e-3, oracle.EB_REL)
+            diff = first_difference(pb, want)
+            assert diff is None, (i, h.shape, diff)
+        s.free()
+    _say("96 fields checked")
+    for b in batch:
+        b.free()
+
+
+# ------------------------------------------------------------------ 1024^3 serving
+
+@pytest.fixture(scope="module")
+def c5(ctx, dev):
+    import torch
+    from stz_amd import fields
+    t0 = time.time()
+    g = fields.grf_device((1024, 1024, 1024), -3.0, 1005, dev, torch.float32)
+    torch.cuda.synchronize()
+    _say("1024^3 GRF generated in %.1fs" % (time.time() - t0))
+    ds = ctx.compress_device(g, 1e-3, "rel")
+    full = torch.empty_like(g)
+    ds.decompress(full)
+    torch.cuda.synchronize()
+    yield g, ds, full
+    ds.free()
+
+
+def test_c5_1024cubed_payload_and_decode_bitexact(oracle, c5):
+    g, ds, full = c5
+    f = g.cpu().numpy()
+    gpu = ds.download(zstd_level=-1)
+    t0 = time.time()
+    want = oracle.compress(f, 1e-3, oracle.EB_REL)
+    _say("1024^3 oracle compress %.1fs, payload %d bytes" % (time.time() - t0, want.size))
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    del gpu
+    rec = full.cpu().numpy()
+    _equal_bits(rec, oracle.decompress(want), "1024^3 decompress")
+    ok, err, bound = _bound_ok(rec, f, oracle.header(want))
+    assert ok, (err, bound)
+
+
+def _crop(full, req):
+    nz, ny, nx = full.shape
+    if req[0] == "box":
+        x0, y0, z0, x1, y1, z1 = req[1:]
+        return full[min(z0, z1):max(z0, z1) + 1, min(y0, y1):max(y0, y1) + 1, min(x0, x1):max(x0, x1) + 1]
+    if req[0] == "slice":
+        sl = [slice(None)] * 3
+        sl[2 - req[1]] = slice(req[2], req[2] + 1)
+        return full[tuple(sl)]
+    step = {0: 8, 1: 4, 2: 2, 3: 1}[req[1]]
+    return full[::step, ::step, ::step]
+
+
+def test_c5_1024cubed_serving_mix(stz, c5):
+    import torch
+    from stz_amd import serve
+    g, ds, full = c5
+    reqs, arr = serve.poisson_trace((1024, 1024, 1024), 200, rate=1000.0, seed=1005)
+    srv = serve.RoiServer(ds, workers=3, max_batch=32, mem_budget=32 << 30, keep_results=True)
+    try:
+        recs, t0 = srv.run_trace(reqs, arr)
+    finally:
+        srv.close()
+    _say("served %d requests in %d batches" % (len(recs), len(srv.batches)))
+    fv = full.view(torch.int32)
+    for p in recs:
+        got = p.out
+        want = _crop(fv, p.req)
+        assert tuple(got.shape) == tuple(want.shape), p.req
+        assert torch.equal(got.view(torch.int32), want), p.req
+        alone = ds.decompress_batch([p.req], like=got)[0]
+        assert torch.equal(alone.view(torch.int32), got.view(torch.int32)), p.req
+        p.out = None
+    stats, gbs, wall = serve.summarize(recs, t0, 4)
+    _say("serving mix:", stats["all"])
