@@ -227,6 +227,47 @@ int stz_quantize(stz_ctx* ctx, stz_dtype dtype, const void* data, size_t n, doub
 /* decode_marked_transform's recover LUT (F5:1531-1544) */
 int stz_dequantize(stz_ctx* ctx, stz_dtype dtype, const uint16_t* codes, size_t n, double eb,
                    const void* unpred, void* out);
+/* Fused predict + quantize of one residual block (the two halves of the
+ * reference's per-block compress: preprocess_block_zslab, F5:1161-1175, then
+ * SZ_compress's quantizer, F5:1314-1397): codes[i] (row-major, 0 =
+ * unpredictable) of diff = sub - pred(ref) under bound eb; the unpredictable
+ * residuals in element order into unpred (capacity dx*dy*dz values), their
+ * count in n_unpred.  rec (may be NULL) receives every point's decoded value
+ * dec + pred: the closed-loop reconstruction the low level writes back
+ * (writeDec, F5:1381-1385, then F5:1077-1083).  The level only sets eb
+ * (mult_low / mult_high x the unit, F5:2201, F5:2259), so it is not an
+ * argument here. */
+int stz_residual_encode(stz_ctx* ctx, stz_dtype dtype, int block, const void* sub, const void* ref, size_t dx,
+                        size_t dy, size_t dz, double eb, uint16_t* codes, void* unpred, size_t* n_unpred,
+                        void* rec);
+/* Its inverse: out = dequantize(codes, unpred) + pred(ref), the recover of
+ * SZ_decompress_separated (F5:1531-1562) followed by depreprocess_block_zslab
+ * (F5:1177-1191); unpred in element order (NULL when no code is 0). */
+int stz_residual_decode(stz_ctx* ctx, stz_dtype dtype, int block, const uint16_t* codes, const void* unpred,
+                        const void* ref, size_t dx, size_t dy, size_t dz, double eb, void* out);
+/* SZ_compress_finish (F5:1258-1307): one (dx,dy,dz) block's row-major codes
+ * plus its unpredictable values (element order, n_unpred = the number of zero
+ * codes) -> one chunk-marked stream of `chunk`^3 chunks: a stream header, the
+ * canonical codebook, the per-chunk bit offsets (the markers, F5:50-55) and
+ * unpredictable counts, the per-sub-chunk tables and the bitstream (this
+ * project's stream format, as inside a payload).  Histogram, codebook and
+ * marked encode run on the device.  zstd is not applied (stz_compress's
+ * container does that per stream).  *stream is host memory (stz_free). */
+int stz_huff_encode_chunks(stz_ctx* ctx, stz_dtype dtype, const uint16_t* codes, const void* unpred,
+                           size_t n_unpred, size_t dx, size_t dy, size_t dz, int chunk, double eb, void** stream,
+                           size_t* bytes);
+/* SZ_decompress_nopred_marked_chunks (F5:1841-1911): decode only the chunks
+ * chunk_ids[0 .. n_ids) (row-major ids (cz*ncy + cy)*ncx + cx, F5:1578-1582)
+ * of a stream from stz_huff_encode_chunks and write their recovered values
+ * (and, when codes_out is not NULL, their codes) at their points of the
+ * row-major block `out` (dx*dy*dz values of the stream's dtype); other points
+ * keep what they held.  chunk_ids NULL or n_ids < 0 decodes every chunk
+ * (SZ_decompress_separated, F5:1466-1576).  decoded_symbols and unpred_zeros
+ * count what the reference counts (F5:1870-1905); unlike the reference,
+ * which consumes its row-major unpredictable list in chunk order, every
+ * unpredictable point gets its own stored value. */
+int stz_huff_decode_chunks(stz_ctx* ctx, const void* stream, size_t bytes, const int64_t* chunk_ids, int64_t n_ids,
+                           void* out, uint16_t* codes_out, size_t* decoded_symbols, size_t* unpred_zeros);
 /* Huffman code lengths (HuffmanEncoder::preprocess_encode_fast, F5:1271) --
  * this project's canonical length-limited code */
 int stz_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len);

@@ -179,6 +179,22 @@ template <typename T>
 void launch_crop(hipStream_t s, const T* in, int px, int py, int ox, int oy, int oz, int sx, int sy,
                  int sz, int step, T* out);
 
+// ---- k_stage.hip (per-stream stage entry points) ----
+// codes / diff (the residual, for the unpredictable list) / rec (decoded
+// value, may be null) of one block's points in row-major order
+template <typename T>
+void launch_residual_encode(hipStream_t s, int block, const T* sub, const T* ref, int dx, int dy, int dz, double eb,
+                            uint16_t* codes, T* diff, T* rec);
+// row-major codes and element-order unpredictables -> stream order
+template <typename T>
+void launch_block_to_stream(hipStream_t s, const uint16_t* codes, const T* unpred, uint64_t n_unpred, const Geom& g,
+                            uint16_t* codes_s, T* side_s);
+void launch_hist_codes(hipStream_t s, const uint16_t* codes, uint64_t n, uint32_t* hist);
+// list: (chunk, first element in codes/side) pairs
+template <typename T>
+void launch_chunks_to_block(hipStream_t s, const uint32_t* list, uint32_t nlist, const Geom& g, const uint16_t* codes,
+                            const T* side, double eb, T* out, uint16_t* codes_out);
+
 // ---- k_residual.hip ----
 struct LevelArgs {
     double eb, recip;

@@ -412,6 +412,20 @@ size_t tsize_of(int dtype) { return dtype == STZ_F32 ? 4 : 8; }
 size_t pad4(size_t n) { size_t p = (n + 3) & ~(size_t)3; return p < 4 ? 4 : p; }
 
 // ------------------------------------------------------------ geometry
+// sub-chunk index of every chunk of one stream grid (kSub symbols each, a
+// chunk's last one partial): chunk_sub[c] .. chunk_sub[c+1]-1
+uint32_t chunk_subs(const Geom& g, uint64_t n, std::vector<uint32_t>& chunk_sub)
+{
+    chunk_sub.resize(g.n_chunks + 1);
+    uint32_t sub = 0;
+    for (uint32_t c = 0; c < g.n_chunks; ++c) {
+        chunk_sub[c] = sub;
+        sub += (uint32_t)((chunk_volume(g, c, n) + kSub - 1) / kSub);
+    }
+    chunk_sub[g.n_chunks] = sub;
+    return sub;
+}
+
 struct HostLayout {
     size_t nx, ny, nz, px, py, pz;
     int chunk_high, chunk_low;
@@ -437,14 +451,7 @@ void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, siz
         else L.g[s] = make_geom(mx, my, mz, ch);
         const Geom& g = L.g[s];
         L.n[s] = (uint64_t)g.dx * g.dy * g.dz;
-        L.chunk_sub[s].resize(g.n_chunks + 1);
-        uint32_t sub = 0;
-        for (uint32_t c = 0; c < g.n_chunks; ++c) {
-            L.chunk_sub[s][c] = sub;
-            sub += (uint32_t)((chunk_volume(g, c, L.n[s]) + kSub - 1) / kSub);
-        }
-        L.chunk_sub[s][g.n_chunks] = sub;
-        L.n_sub[s] = sub;
+        L.n_sub[s] = chunk_subs(g, L.n[s], L.chunk_sub[s]);
         L.code_off[s] = off;
         off += align_up(L.n[s], 64);
     }
@@ -503,21 +510,21 @@ void level_ebs(double eb, int mode, double vmin, double vmax, const stz_params& 
     ebl[2] = p.mult_high * unit;
 }
 
-// section offsets of one stream, the same formula the payload reader uses
-void fill_stream_header(StreamHeader& sh, int s, const HostLayout& L, int dtype, double eb, uint32_t n_used,
-                        uint64_t n_unpred, uint64_t total_bits)
+// section offsets of one stream, the same formula the payload reader uses;
+// kind 0 base, 1 low, 2 high (payload streams), 3 a standalone block stream
+void fill_stream_header_g(StreamHeader& sh, uint32_t kind, uint32_t block, const Geom& g, uint64_t n, uint32_t n_sub,
+                          int dtype, double eb, uint32_t n_used, uint64_t n_unpred, uint64_t total_bits)
 {
     std::memset(&sh, 0, sizeof(sh));
-    const Geom& g = L.g[s];
     sh.magic = kStreamMagic;
-    sh.kind = s == 0 ? 0u : s < 8 ? 1u : 2u;
-    sh.block = s == 0 ? 0u : (uint32_t)((s - 1) % 7);
+    sh.kind = kind;
+    sh.block = block;
     sh.dtype = (uint32_t)dtype;
-    sh.n_symbols = L.n[s];
+    sh.n_symbols = n;
     sh.dx = (uint32_t)g.dx; sh.dy = (uint32_t)g.dy; sh.dz = (uint32_t)g.dz;
     sh.chunk = g.chunk > 0 ? (uint32_t)g.chunk : (uint32_t)kBaseChunk;
     sh.n_chunks = g.n_chunks;
-    sh.n_sub = L.n_sub[s];
+    sh.n_sub = n_sub;
     sh.n_used = n_used;
     sh.lmax = kLmax;
     sh.n_unpred = n_unpred;
@@ -536,6 +543,72 @@ void fill_stream_header(StreamHeader& sh, int s, const HostLayout& L, int dtype,
     sh.off_unpred = off;       off += align8(ts * n_unpred);
     sh.off_bits = off;         off += align8(4ull * ((total_bits + 31) / 32 + 2));
     sh.stream_bytes = off;
+}
+
+void fill_stream_header(StreamHeader& sh, int s, const HostLayout& L, int dtype, double eb, uint32_t n_used,
+                        uint64_t n_unpred, uint64_t total_bits)
+{
+    fill_stream_header_g(sh, s == 0 ? 0u : s < 8 ? 1u : 2u, s == 0 ? 0u : (uint32_t)((s - 1) % 7), L.g[s], L.n[s],
+                         L.n_sub[s], dtype, eb, n_used, n_unpred, total_bits);
+}
+
+// One stream of `avail` bytes at p checked against what its geometry
+// implies (header fields, every section offset, the codebook's order and
+// Kraft sum, the chunk and sub-chunk tables' monotone totals); the header
+// and codebook are returned.  Every read the decode kernels make goes
+// through offsets checked here.
+void check_stream(const uint8_t* p, uint64_t avail, uint32_t kind, uint32_t block, const Geom& g, uint64_t n,
+                  const std::vector<uint32_t>& chunk_sub, int dtype, StreamHeader& sh, Codebook& b)
+{
+    if (avail < sizeof(StreamHeader)) throw StzError(STZ_E_STREAM, "short stream");
+    std::memcpy(&sh, p, sizeof(StreamHeader));
+    const uint32_t n_sub = chunk_sub[g.n_chunks];
+    // counts bounded before they size the sections (no wrap in the offsets)
+    if (sh.n_used > 65536 || sh.n_unpred > n || sh.total_bits > n * kLmax)
+        throw StzError(STZ_E_STREAM, "bad stream header");
+    StreamHeader chk;
+    fill_stream_header_g(chk, kind, block, g, n, n_sub, dtype, sh.eb, sh.n_used, sh.n_unpred, sh.total_bits);
+    if (sh.magic != kStreamMagic || sh.n_symbols != n || sh.n_chunks != g.n_chunks || sh.n_sub != chk.n_sub ||
+        sh.stream_bytes != chk.stream_bytes || sh.off_codebook != chk.off_codebook ||
+        sh.off_chunk_bit != chk.off_chunk_bit || sh.off_chunk_unpred != chk.off_chunk_unpred ||
+        sh.off_chunk_sub != chk.off_chunk_sub || sh.off_sub_bits != chk.off_sub_bits ||
+        sh.off_sub_unpred != chk.off_sub_unpred || sh.off_unpred != chk.off_unpred || sh.off_bits != chk.off_bits ||
+        sh.kind != chk.kind || sh.block != chk.block || sh.dtype != chk.dtype || sh.dx != chk.dx ||
+        sh.dy != chk.dy || sh.dz != chk.dz || sh.chunk != chk.chunk || sh.lmax != (uint32_t)kLmax ||
+        sh.radius != (uint32_t)kRadius || sh.sub_size != (uint32_t)kSub || sh.stream_bytes > avail)
+        throw StzError(STZ_E_STREAM, "bad stream header");
+    const uint32_t* cb = (const uint32_t*)(p + sh.off_codebook);
+    b.sym.resize(sh.n_used);
+    b.len.resize(sh.n_used);
+    double kraft = 0;
+    for (uint32_t i = 0; i < sh.n_used; ++i) {
+        b.sym[i] = (uint16_t)(cb[i] & 0xFFFF);
+        b.len[i] = (uint8_t)(cb[i] >> 16);
+        if (b.len[i] == 0 || b.len[i] > kLmax || (i && b.sym[i] <= b.sym[i - 1]))
+            throw StzError(STZ_E_STREAM, "bad codebook");
+        kraft += std::ldexp(1.0, -b.len[i]);
+    }
+    if (kraft > 1.0 + 1e-12) throw StzError(STZ_E_STREAM, "codebook violates Kraft");
+    // chunk tables: geometry-derived sub-chunk map, monotone offsets that
+    // add up to the sub-chunk sizes (they steer every decode read)
+    const uint64_t* cbit = (const uint64_t*)(p + sh.off_chunk_bit);
+    const uint64_t* cun = (const uint64_t*)(p + sh.off_chunk_unpred);
+    const uint32_t* csub = (const uint32_t*)(p + sh.off_chunk_sub);
+    const uint16_t* sbits = (const uint16_t*)(p + sh.off_sub_bits);
+    const uint16_t* sun = (const uint16_t*)(p + sh.off_sub_unpred);
+    uint64_t ab = 0, au = 0;
+    for (uint32_t c = 0; c < sh.n_chunks; ++c) {
+        if (csub[c] != chunk_sub[c] || cbit[c] != ab || cun[c] != au) throw StzError(STZ_E_STREAM, "bad chunk table");
+        for (uint32_t j = csub[c]; j < chunk_sub[c + 1]; ++j) {
+            if (sbits[j] > (uint32_t)kSub * kLmax || sun[j] > (uint32_t)kSub)
+                throw StzError(STZ_E_STREAM, "bad sub-chunk table");
+            ab += sbits[j];
+            au += sun[j];
+        }
+    }
+    if (csub[sh.n_chunks] != chunk_sub[sh.n_chunks] || cbit[sh.n_chunks] != ab || ab != sh.total_bits ||
+        cun[sh.n_chunks] != au || au != sh.n_unpred)
+        throw StzError(STZ_E_STREAM, "bad chunk table totals");
 }
 
 StreamSet make_streamset(const HostLayout& L, const Meta& m, int dtype)
@@ -2193,60 +2266,10 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
             fh.stream_bytes[s] > bytes - fh.stream_off[s])
             throw StzError(STZ_E_STREAM, "stream out of bounds");
         next = fh.stream_off[s] + fh.stream_bytes[s];
-        StreamHeader& sh = m.sh[s];
-        std::memcpy(&sh, p + fh.stream_off[s], sizeof(StreamHeader));
-        // counts bounded before they size the sections (no wrap in the offsets)
-        if (sh.n_used > 65536 || sh.n_unpred > L.n[s] || sh.total_bits > (uint64_t)L.n[s] * kLmax)
-            throw StzError(STZ_E_STREAM, "bad stream header");
-        StreamHeader chk;
-        fill_stream_header(chk, s, L, (int)fh.dtype, sh.eb, sh.n_used, sh.n_unpred, sh.total_bits);
-        // every section offset is the one the geometry and counts imply: the
-        // host reads and the decode kernels' reads go through them
-        if (sh.magic != kStreamMagic || sh.n_symbols != L.n[s] || sh.n_chunks != L.g[s].n_chunks ||
-            sh.n_sub != chk.n_sub || sh.stream_bytes != chk.stream_bytes || sh.off_codebook != chk.off_codebook ||
-            sh.off_chunk_bit != chk.off_chunk_bit || sh.off_chunk_unpred != chk.off_chunk_unpred ||
-            sh.off_chunk_sub != chk.off_chunk_sub || sh.off_sub_bits != chk.off_sub_bits ||
-            sh.off_sub_unpred != chk.off_sub_unpred || sh.off_unpred != chk.off_unpred ||
-            sh.off_bits != chk.off_bits || sh.stream_bytes != fh.stream_bytes[s] || sh.kind != chk.kind ||
-            sh.block != chk.block || sh.dtype != chk.dtype || sh.dx != chk.dx || sh.dy != chk.dy ||
-            sh.dz != chk.dz || sh.chunk != chk.chunk || sh.lmax != (uint32_t)kLmax ||
-            sh.radius != (uint32_t)kRadius || sh.sub_size != (uint32_t)kSub)
-            throw StzError(STZ_E_STREAM, "bad stream header");
-        const uint32_t* cb = (const uint32_t*)(p + fh.stream_off[s] + sh.off_codebook);
-        Codebook& b = m.cb[s];
-        b.sym.resize(sh.n_used);
-        b.len.resize(sh.n_used);
-        double kraft = 0;
-        for (uint32_t i = 0; i < sh.n_used; ++i) {
-            b.sym[i] = (uint16_t)(cb[i] & 0xFFFF);
-            b.len[i] = (uint8_t)(cb[i] >> 16);
-            if (b.len[i] == 0 || b.len[i] > kLmax || (i && b.sym[i] <= b.sym[i - 1]))
-                throw StzError(STZ_E_STREAM, "bad codebook");
-            kraft += std::ldexp(1.0, -b.len[i]);
-        }
-        if (kraft > 1.0 + 1e-12) throw StzError(STZ_E_STREAM, "codebook violates Kraft");
-        // chunk tables: geometry-derived sub-chunk map, monotone offsets that
-        // add up to the sub-chunk sizes (they steer every decode read)
-        const uint8_t* sb = p + fh.stream_off[s];
-        const uint64_t* cbit = (const uint64_t*)(sb + sh.off_chunk_bit);
-        const uint64_t* cun = (const uint64_t*)(sb + sh.off_chunk_unpred);
-        const uint32_t* csub = (const uint32_t*)(sb + sh.off_chunk_sub);
-        const uint16_t* sbits = (const uint16_t*)(sb + sh.off_sub_bits);
-        const uint16_t* sun = (const uint16_t*)(sb + sh.off_sub_unpred);
-        if (sh.n_sub != L.n_sub[s]) throw StzError(STZ_E_STREAM, "bad sub-chunk count");
-        uint64_t ab = 0, au = 0;
-        for (uint32_t c = 0; c < sh.n_chunks; ++c) {
-            if (csub[c] != L.chunk_sub[s][c] || cbit[c] != ab || cun[c] != au)
-                throw StzError(STZ_E_STREAM, "bad chunk table");
-            for (uint32_t j = csub[c]; j < L.chunk_sub[s][c + 1]; ++j) {
-                if (sbits[j] > (uint32_t)kSub * kLmax || sun[j] > (uint32_t)kSub) throw StzError(STZ_E_STREAM, "bad sub-chunk table");
-                ab += sbits[j];
-                au += sun[j];
-            }
-        }
-        if (csub[sh.n_chunks] != L.chunk_sub[s][sh.n_chunks] || cbit[sh.n_chunks] != ab || ab != sh.total_bits ||
-            cun[sh.n_chunks] != au || au != sh.n_unpred)
-            throw StzError(STZ_E_STREAM, "bad chunk table totals");
+        const uint32_t kind = s == 0 ? 0u : s < 8 ? 1u : 2u, block = s == 0 ? 0u : (uint32_t)((s - 1) % 7);
+        check_stream(p + fh.stream_off[s], fh.stream_bytes[s], kind, block, L.g[s], L.n[s], L.chunk_sub[s],
+                     (int)fh.dtype, m.sh[s], m.cb[s]);
+        if (m.sh[s].stream_bytes != fh.stream_bytes[s]) throw StzError(STZ_E_STREAM, "bad stream size");
     }
     if (next != fh.total_bytes) throw StzError(STZ_E_STREAM, "bad payload size");
 }
@@ -3154,6 +3177,335 @@ template <typename T>
 void launch_dequant_list(hipStream_t s, const uint16_t* codes, const T* unpred, uint64_t n, double eb, T* out);
 }  // namespace stz
 
+namespace {
+
+// ---- per-stream stage entry points ------------------------------------------
+
+// number of zero codes of n codes the caller holds (host or device)
+size_t zero_codes(stz_ctx* c, const uint16_t* codes, size_t n)
+{
+    if (!is_device_ptr(codes)) {
+        size_t nz = 0;
+        for (size_t i = 0; i < n; ++i) nz += codes[i] == 0;
+        return nz;
+    }
+    std::vector<uint16_t> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), codes, n * 2, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    size_t nz = 0;
+    for (size_t i = 0; i < n; ++i) nz += h[i] == 0;
+    return nz;
+}
+
+void check_block(size_t dx, size_t dy, size_t dz)
+{
+    if (!dx || !dy || !dz || dx * dy * dz >= (1ull << 31) || dx * dy >= (1ull << 31))
+        throw StzError(STZ_E_DIMS, "block dims must be nonzero and hold < 2^31 points");
+}
+
+template <typename T>
+void residual_encode_impl(stz_ctx* c, int block, const void* sub, const void* ref, size_t dx, size_t dy, size_t dz,
+                          double eb, uint16_t* codes, void* unpred, size_t* n_unpred, void* rec)
+{
+    if (block < 0 || block > 6) throw StzError(STZ_E_ARG, "block must be 0..6");
+    check_block(dx, dy, dz);
+    const size_t n = dx * dy * dz, nb = n * sizeof(T);
+    DView vs(c, sub, nb, true), vr(c, ref, nb, true), vc(c, codes, n * 2, false), vu(c, unpred, nb, false);
+    std::unique_ptr<DView> vrec(rec ? new DView(c, rec, nb, false) : nullptr);
+    DBuf diff(c->pool, nb + 64), cnt(c->pool, 64);
+    K(c, "residual_encode", [&] {
+        launch_residual_encode<T>(c->stream, block, (const T*)vs.ptr, (const T*)vr.ptr, (int)dx, (int)dy, (int)dz,
+                                  eb, (uint16_t*)vc.ptr, diff.as<T>(), vrec ? (T*)vrec->ptr : nullptr);
+    });
+    K(c, "compact_unpred", [&] {
+        launch_compact_unpred<T>(c->stream, (const uint16_t*)vc.ptr, diff.as<T>(), n, (T*)vu.ptr, cnt.as<uint64_t>());
+    });
+    uint64_t* h = (uint64_t*)c->host_stage(64);
+    HIPCHK(hipMemcpyAsync(h, cnt.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    vc.back();
+    if (!vu.dev) HIPCHK(hipMemcpyAsync(unpred, vu.ptr, h[0] * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    if (vrec) vrec->back();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *n_unpred = (size_t)h[0];
+}
+
+template <typename T>
+void residual_decode_impl(stz_ctx* c, int block, const uint16_t* codes, const void* unpred, const void* ref,
+                          size_t dx, size_t dy, size_t dz, double eb, void* out)
+{
+    if (block < 0 || block > 6) throw StzError(STZ_E_ARG, "block must be 0..6");
+    check_block(dx, dy, dz);
+    const size_t n = dx * dy * dz, nb = n * sizeof(T);
+    const size_t nz = zero_codes(c, codes, n);
+    if (nz && !unpred) throw StzError(STZ_E_ARG, "unpredictable list missing");
+    DView vc(c, codes, n * 2, true), vr(c, ref, nb, true), vo(c, out, nb, false);
+    DView vu(c, unpred ? unpred : (const void*)codes, (nz ? nz : 1) * sizeof(T), unpred != nullptr && nz > 0);
+    DBuf de(c->pool, nb + 64);
+    K(c, "dequant_list", [&] {
+        launch_dequant_list<T>(c->stream, (const uint16_t*)vc.ptr, (const T*)vu.ptr, n, eb, de.as<T>());
+    });
+    K(c, "residual", [&] {
+        launch_residual_predict<T>(c->stream, block, de.as<T>(), (const T*)vr.ptr, (int)dx, (int)dy, (int)dz,
+                                   (T*)vo.ptr, -1);
+    });
+    vo.back();
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// stream descriptor of a standalone block stream (its bytes start at 0)
+StreamDesc block_desc(const StreamHeader& sh, const Geom& g, int dtype)
+{
+    StreamDesc d{};
+    d.code_off = 0;
+    d.n_symbols = sh.n_symbols;
+    d.off_chunk_bit = sh.off_chunk_bit;
+    d.off_chunk_unpred = sh.off_chunk_unpred;
+    d.off_chunk_sub = sh.off_chunk_sub;
+    d.off_sub_bits = sh.off_sub_bits;
+    d.off_sub_unpred = sh.off_sub_unpred;
+    d.off_unpred = sh.off_unpred;
+    d.off_bits = sh.off_bits;
+    d.bits_words = (sh.stream_bytes - sh.off_bits) / 4;
+    d.g = g;
+    d.n_chunks = g.n_chunks;
+    d.n_sub = sh.n_sub;
+    d.tsize = (uint32_t)tsize_of(dtype);
+    d.n_unpred = sh.n_unpred;
+    return d;
+}
+
+// SZ_compress_finish (F5:1258-1307) on the device: histogram, codebook,
+// chunk-marked encode of one block's codes into a standalone stream
+template <typename T>
+void huff_encode_chunks_impl(stz_ctx* c, int dtype, const uint16_t* codes, const void* unpred, size_t n_unpred,
+                             size_t dx, size_t dy, size_t dz, int chunk, double eb, std::vector<uint8_t>& outv)
+{
+    check_block(dx, dy, dz);
+    if (chunk <= 0 || chunk > 64) throw StzError(STZ_E_ARG, "chunk dims must be in 1..64");
+    if (!(eb >= 0)) throw StzError(STZ_E_ARG, "bad error bound");
+    if (n_unpred && !unpred) throw StzError(STZ_E_ARG, "unpredictable list missing");
+    hipStream_t st = c->stream;
+    const size_t n = dx * dy * dz, ts = sizeof(T);
+    const Geom g = make_geom((int)dx, (int)dy, (int)dz, chunk);
+    std::vector<uint32_t> csub;
+    const uint32_t n_sub = chunk_subs(g, n, csub);
+    DView vc(c, codes, n * 2, true);
+    // histogram -> host: the zero count must equal the caller's list length
+    // before the list is read on the device
+    DBuf hist(c->pool, 65536 * 4), compact(c->pool, 65536 * 8 + 64), counts(c->pool, 256);
+    K(c, "hist_zero", [&] { launch_fill_u32(st, hist.as<uint32_t>(), 65536, 0u); });
+    K(c, "hist_codes", [&] { launch_hist_codes(st, (const uint16_t*)vc.ptr, n, hist.as<uint32_t>()); });
+    K(c, "hist_compact", [&] { launch_hist_compact(st, hist.as<uint32_t>(), 1, compact.as<uint64_t>(), counts.as<uint32_t>()); });
+    uint32_t* hc = (uint32_t*)c->host_stage(64);
+    HIPCHK(hipMemcpyAsync(hc, counts.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint32_t m = hc[0];
+    std::vector<uint64_t> list(m);
+    if (m) HIPCHK(hipMemcpy(list.data(), compact.p, (size_t)m * 8, hipMemcpyDeviceToHost));
+    const uint64_t zeros = (m && (list[0] >> 32) == 0) ? (list[0] & 0xFFFFFFFFull) : 0;
+    if (zeros != n_unpred) throw StzError(STZ_E_ARG, "n_unpred differs from the number of zero codes");
+    Codebook cb;
+    build_codebook(list.data(), m, kLmax, cb);
+    StreamHeader sh;
+    fill_stream_header_g(sh, 3u, 0u, g, n, n_sub, dtype, eb, m, n_unpred, cb.total_bits);
+    // stream buffer: zeroed, host-written sections (header, codebook, chunk
+    // sub-chunk map) uploaded, then the marked encode fills the rest
+    DBuf pay(c->pool, align_up(sh.stream_bytes, 4) + 64);
+    K(c, "payload_zero", [&] { launch_fill_u32(st, pay.as<uint32_t>(), align_up(sh.stream_bytes, 4) / 4, 0u); });
+    const size_t pre = sh.off_sub_bits;
+    const size_t enc_off = align_up(pre, 16);
+    uint8_t* img = (uint8_t*)c->host_stage(enc_off + (size_t)m * 8 + 64);
+    std::memset(img, 0, pre);
+    std::memcpy(img, &sh, sizeof(sh));
+    uint32_t* cbw = (uint32_t*)(img + sh.off_codebook);
+    uint32_t* encl = (uint32_t*)(img + enc_off);
+    for (uint32_t k = 0; k < m; ++k) {
+        cbw[k] = (uint32_t)cb.sym[k] | ((uint32_t)cb.len[k] << 16);
+        encl[2 * k] = cb.sym[k];
+        encl[2 * k + 1] = cb.code[k] | ((uint32_t)cb.len[k] << 24);
+    }
+    std::memcpy(img + sh.off_chunk_sub, csub.data(), 4 * csub.size());
+    DBuf up(c->pool, enc_off + (size_t)m * 8 + 64);
+    HIPCHK(hipMemcpyAsync(up.p, img, enc_off + (size_t)m * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(pay.p, up.p, pre, hipMemcpyDeviceToDevice, st));
+    DBuf enc(c->pool, 65536 * 4);
+    K(c, "enc_zero", [&] { launch_fill_u32(st, enc.as<uint32_t>(), 65536, 0u); });
+    if (m) K(c, "enc_scatter", [&] { launch_enc_scatter(st, (const uint32_t*)(up.as<uint8_t>() + enc_off), m, enc.as<uint32_t>()); });
+    // codes and unpredictables into stream order
+    DBuf cs(c->pool, n * 2 + 64), side(c->pool, n * ts + 64);
+    DView vu(c, n_unpred ? unpred : (const void*)codes, (n_unpred ? n_unpred : 1) * ts, n_unpred > 0);
+    K(c, "block_to_stream", [&] {
+        launch_block_to_stream<T>(st, (const uint16_t*)vc.ptr, (const T*)vu.ptr, n_unpred, g, cs.as<uint16_t>(),
+                                  side.as<T>());
+    });
+    StreamSet ss{};
+    ss.nstreams = 1;
+    ss.total_chunks = g.n_chunks;
+    ss.total_sub = n_sub;
+    StreamDesc& d = ss.s[0];
+    d = block_desc(sh, g, dtype);
+    if (m) {
+        const uint32_t a = cb.sym.front(), b = cb.sym.back();
+        if (b - a + 1 <= kEncWindow) {
+            d.enc_lo = a;
+            d.enc_n = b - a + 1;
+        } else {
+            const uint32_t c0 = (uint32_t)kRadius - kEncWindow / 2;
+            d.enc_lo = c0 < a ? a : (c0 > b + 1 - kEncWindow ? b + 1 - kEncWindow : c0);
+            d.enc_n = kEncWindow;
+        }
+    }
+    DBuf pre_b(c->pool, (size_t)n_sub * 4 + 16), pre_u(c->pool, (size_t)n_sub * 4 + 16);
+    DBuf tot_b(c->pool, (size_t)g.n_chunks * 8 + 16), tot_u(c->pool, (size_t)g.n_chunks * 8 + 16);
+    K(c, "huff_encode", [&] {
+        launch_huff_len(st, ss, cs.as<uint16_t>(), enc.as<uint32_t>(), pay.as<uint8_t>(), pre_b.as<uint32_t>(),
+                        pre_u.as<uint32_t>(), tot_b.as<uint64_t>(), tot_u.as<uint64_t>());
+        launch_huff_scan(st, ss, tot_b.as<uint64_t>(), tot_u.as<uint64_t>(), pay.as<uint8_t>());
+        launch_huff_pack(st, ss, cs.as<uint16_t>(), side.p, enc.as<uint32_t>(), pre_b.as<uint32_t>(),
+                         pre_u.as<uint32_t>(), pay.as<uint8_t>());
+    });
+    outv.resize(sh.stream_bytes);
+    HIPCHK(hipMemcpyAsync(outv.data(), pay.p, sh.stream_bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+// SZ_decompress_nopred_marked_chunks (F5:1841-1911) / SZ_decompress_separated
+// (F5:1466-1576) on the device for a standalone block stream
+void huff_decode_chunks_impl(stz_ctx* c, const void* stream, size_t bytes, const int64_t* ids, int64_t n_ids,
+                             void* out, uint16_t* codes_out, size_t* dsym, size_t* uzero)
+{
+    hipStream_t st = c->stream;
+    std::vector<uint8_t> hbuf;
+    const uint8_t* p = (const uint8_t*)stream;
+    if (is_device_ptr(stream)) {
+        hbuf.resize(bytes);
+        HIPCHK(hipMemcpy(hbuf.data(), stream, bytes, hipMemcpyDeviceToHost));
+        p = hbuf.data();
+    }
+    if (bytes < sizeof(StreamHeader)) throw StzError(STZ_E_STREAM, "short stream");
+    StreamHeader h0;
+    std::memcpy(&h0, p, sizeof(h0));
+    if (h0.magic != kStreamMagic || h0.kind != 3u || (h0.dtype != STZ_F32 && h0.dtype != STZ_F64) ||
+        h0.chunk == 0 || h0.chunk > 64)
+        throw StzError(STZ_E_STREAM, "not a block stream");
+    check_block(h0.dx, h0.dy, h0.dz);
+    const int dtype = (int)h0.dtype;
+    const size_t ts = tsize_of(dtype);
+    const Geom g = make_geom((int)h0.dx, (int)h0.dy, (int)h0.dz, (int)h0.chunk);
+    const uint64_t n = (uint64_t)g.dx * g.dy * g.dz;
+    std::vector<uint32_t> csub;
+    chunk_subs(g, n, csub);
+    StreamHeader sh;
+    Codebook cb;
+    check_stream(p, bytes, 3u, 0u, g, n, csub, dtype, sh, cb);
+    // chunk list (row-major ids, F5:1578-1582); none given: every chunk
+    std::vector<int64_t> all;
+    if (!ids || n_ids < 0) {
+        all.resize(g.n_chunks);
+        for (uint32_t k = 0; k < g.n_chunks; ++k) all[k] = k;
+        ids = all.data();
+        n_ids = (int64_t)g.n_chunks;
+    }
+    const uint64_t* cun = (const uint64_t*)(p + sh.off_chunk_unpred);
+    std::vector<uint32_t> quads, pairs;
+    uint64_t pos = 0, syms = 0, zeros = 0;
+    uint32_t item = 0;
+    for (int64_t k = 0; k < n_ids; ++k) {
+        const int64_t ch = ids[k];
+        if (ch < 0 || ch >= (int64_t)g.n_chunks) throw StzError(STZ_E_ARG, "chunk id out of range");
+        quads.push_back(0u);
+        quads.push_back((uint32_t)ch);
+        quads.push_back(item);
+        quads.push_back((uint32_t)pos);
+        pairs.push_back((uint32_t)ch);
+        pairs.push_back((uint32_t)pos);
+        item += csub[ch + 1] - csub[ch];
+        const uint64_t vol = chunk_volume(g, (uint32_t)ch, n);
+        syms += vol;
+        zeros += cun[ch + 1] - cun[ch];
+        pos += align_up(vol, 32);
+        if (pos >= (1ull << 32)) throw StzError(STZ_E_DIMS, "chunk list decodes more than 2^32 symbols");
+    }
+    *dsym = (size_t)syms;
+    *uzero = (size_t)zeros;
+    if (!n_ids) return;
+    // stream bytes and decode tables on the device
+    DBuf pay(c->pool, align_up(bytes, 4) + 64);
+    uint8_t* stg = (uint8_t*)c->host_stage(sh.stream_bytes);
+    std::memcpy(stg, p, sh.stream_bytes);
+    HIPCHK(hipMemcpyAsync(pay.p, stg, sh.stream_bytes, hipMemcpyHostToDevice, st));
+    DecHead head;
+    std::vector<uint16_t> sorted;
+    build_dechead(cb.sym.data(), cb.len.data(), (uint32_t)cb.sym.size(), head, sorted);
+    head.sorted_off = 0;
+    const size_t sb = align_up(sorted.size() * 2, 16);
+    DBuf srt(c->pool, sb + sizeof(DecHead) + 16), tabs(c->pool, sizeof(DecTab));
+    uint8_t* hs = (uint8_t*)c->host_stage(sb + sizeof(DecHead) + 16);
+    std::memcpy(hs, sorted.data(), sorted.size() * 2);
+    std::memcpy(hs + sb, &head, sizeof(head));
+    HIPCHK(hipMemcpyAsync(srt.p, hs, sb + sizeof(DecHead), hipMemcpyHostToDevice, st));
+    K(c, "dectabs", [&] {
+        launch_build_dectabs(st, (const DecHead*)(srt.as<uint8_t>() + sb), srt.as<uint16_t>(), tabs.as<DecTab>(), 1);
+    });
+    StreamSet ss{};
+    ss.nstreams = 1;
+    ss.total_chunks = g.n_chunks;
+    ss.total_sub = sh.n_sub;
+    StreamDesc& d = ss.s[0];
+    d = block_desc(sh, g, dtype);
+    d.item_count = item;
+    // decode variant as for payload streams: up to 2 bits per symbol the
+    // LDS-staged kernel (fewer items per workgroup while a workgroup's bits
+    // would overflow the stage), above it the unstaged one
+    const double bps = (double)sh.total_bits / (double)n;
+    const bool stage = bps <= 2.0;
+    d.ipw = 256;
+    if (stage)
+        while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
+    const uint32_t nwg = (item + d.ipw - 1) / d.ipw;
+    ss.total_items = nwg * d.ipw;
+    ss.total_wgs = nwg;
+    DBuf dq(c->pool, quads.size() * 4 + 16), dp(c->pool, pairs.size() * 4 + 16);
+    uint32_t* lst = (uint32_t*)c->host_stage((quads.size() + pairs.size()) * 4 + 16);
+    std::memcpy(lst, quads.data(), quads.size() * 4);
+    std::memcpy(lst + quads.size(), pairs.data(), pairs.size() * 4);
+    HIPCHK(hipMemcpyAsync(dq.p, lst, quads.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dp.p, lst + quads.size(), pairs.size() * 4, hipMemcpyHostToDevice, st));
+    DBuf items(c->pool, (size_t)ss.total_items * sizeof(DecItem) + 16), cmap(c->pool, (size_t)g.n_chunks * 4 + 16);
+    DBuf codes(c->pool, pos * 2 + 64), side(c->pool, pos * ts + 64);
+    ErrFlag err(c);
+    K(c, "dec_prep", [&] {
+        launch_dec_prep(st, ss, pay.as<uint8_t>(), dq.as<uint32_t>(), (uint32_t)n_ids, items.as<DecItem>(),
+                        cmap.as<uint32_t>());
+    });
+    K(c, "huff_decode", [&] {
+        launch_huff_decode(st, ss, pay.as<uint8_t>(), tabs.as<DecTab>(), srt.as<uint16_t>(), items.as<DecItem>(),
+                           codes.as<uint16_t>(), side.p, err.ptr(), stage);
+    });
+    // points outside the listed chunks keep the caller's values
+    DView vo(c, out, n * ts, true);
+    std::unique_ptr<DView> vco(codes_out ? new DView(c, codes_out, n * 2, true) : nullptr);
+    K(c, "chunks_to_block", [&] {
+        if (dtype == STZ_F32)
+            launch_chunks_to_block<float>(st, dp.as<uint32_t>(), (uint32_t)n_ids, g, codes.as<uint16_t>(),
+                                          side.as<float>(), sh.eb, (float*)vo.ptr,
+                                          vco ? (uint16_t*)vco->ptr : nullptr);
+        else
+            launch_chunks_to_block<double>(st, dp.as<uint32_t>(), (uint32_t)n_ids, g, codes.as<uint16_t>(),
+                                           side.as<double>(), sh.eb, (double*)vo.ptr,
+                                           vco ? (uint16_t*)vco->ptr : nullptr);
+    });
+    err.fetch(c);
+    HIPCHK(hipStreamSynchronize(st));
+    err.check();
+    vo.back();
+    if (vco) vco->back();
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
 extern "C" {
 
 int stz_split(stz_ctx* c, stz_dtype dtype, const void* full, size_t nx, size_t ny, size_t nz, void* const sub[8])
@@ -3240,6 +3592,51 @@ int stz_dequantize(stz_ctx* c, stz_dtype dtype, const uint16_t* codes, size_t n,
             K(c, "dequant_list", [&] { launch_dequant_list<double>(c->stream, (const uint16_t*)vc.ptr, (const double*)vu.ptr, n, eb, (double*)vo.ptr); });
         vo.back();
         HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int stz_residual_encode(stz_ctx* c, stz_dtype dtype, int block, const void* sub, const void* ref, size_t dx,
+                        size_t dy, size_t dz, double eb, uint16_t* codes, void* unpred, size_t* n_unpred, void* rec)
+{
+    if (!c || !sub || !ref || !codes || !unpred || !n_unpred || !(eb >= 0)) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) residual_encode_impl<float>(c, block, sub, ref, dx, dy, dz, eb, codes, unpred, n_unpred, rec);
+        else residual_encode_impl<double>(c, block, sub, ref, dx, dy, dz, eb, codes, unpred, n_unpred, rec);
+    });
+}
+
+int stz_residual_decode(stz_ctx* c, stz_dtype dtype, int block, const uint16_t* codes, const void* unpred,
+                        const void* ref, size_t dx, size_t dy, size_t dz, double eb, void* out)
+{
+    if (!c || !codes || !ref || !out) return STZ_E_ARG;
+    return guard(c, [&] {
+        if (dtype == STZ_F32) residual_decode_impl<float>(c, block, codes, unpred, ref, dx, dy, dz, eb, out);
+        else residual_decode_impl<double>(c, block, codes, unpred, ref, dx, dy, dz, eb, out);
+    });
+}
+
+int stz_huff_encode_chunks(stz_ctx* c, stz_dtype dtype, const uint16_t* codes, const void* unpred, size_t n_unpred,
+                           size_t dx, size_t dy, size_t dz, int chunk, double eb, void** stream, size_t* bytes)
+{
+    if (!c || !codes || !stream || !bytes) return STZ_E_ARG;
+    return guard(c, [&] {
+        std::vector<uint8_t> v;
+        if (dtype == STZ_F32) huff_encode_chunks_impl<float>(c, dtype, codes, unpred, n_unpred, dx, dy, dz, chunk, eb, v);
+        else huff_encode_chunks_impl<double>(c, dtype, codes, unpred, n_unpred, dx, dy, dz, chunk, eb, v);
+        void* o = std::malloc(v.size());
+        if (!o) throw std::bad_alloc();
+        std::memcpy(o, v.data(), v.size());
+        *stream = o;
+        *bytes = v.size();
+    });
+}
+
+int stz_huff_decode_chunks(stz_ctx* c, const void* stream, size_t bytes, const int64_t* chunk_ids, int64_t n_ids,
+                           void* out, uint16_t* codes_out, size_t* decoded_symbols, size_t* unpred_zeros)
+{
+    if (!c || !stream || !out || !decoded_symbols || !unpred_zeros) return STZ_E_ARG;
+    return guard(c, [&] {
+        huff_decode_chunks_impl(c, stream, bytes, chunk_ids, n_ids, out, codes_out, decoded_symbols, unpred_zeros);
     });
 }
 

@@ -397,6 +397,64 @@ class Context:
               "stz_dequantize")
         return out
 
+    def residual_encode(self, block, sub, ref, eb, with_rec=True):
+        """Fused predict + quantize of one block: (codes (z,y,x) uint16,
+        unpredictable residuals in element order, reconstruction or None)."""
+        sub = np.ascontiguousarray(sub)
+        ref = np.ascontiguousarray(ref, dtype=sub.dtype)
+        dz, dy, dx = sub.shape
+        codes = np.empty(sub.shape, dtype=np.uint16)
+        un = np.empty(sub.size, dtype=sub.dtype)
+        rec = np.empty_like(sub) if with_rec else None
+        n = C.c_size_t()
+        check(lib().stz_residual_encode(self.h, _dtype_id(sub), block, _ptr(sub), _ptr(ref), dx, dy, dz, float(eb),
+                                        _ptr(codes), _ptr(un), C.byref(n),
+                                        _ptr(rec) if rec is not None else C.c_void_p()), "stz_residual_encode")
+        return codes, un[: n.value].copy(), rec
+
+    def residual_decode(self, block, codes, unpred, ref, eb):
+        codes = np.ascontiguousarray(codes, dtype=np.uint16)
+        ref = np.ascontiguousarray(ref)
+        unpred = np.ascontiguousarray(unpred, dtype=ref.dtype)
+        dz, dy, dx = ref.shape
+        out = np.empty_like(ref)
+        up = _ptr(unpred) if unpred.size else C.c_void_p()
+        check(lib().stz_residual_decode(self.h, _dtype_id(ref), block, _ptr(codes), up, _ptr(ref), dx, dy, dz,
+                                        float(eb), _ptr(out)), "stz_residual_decode")
+        return out
+
+    def huff_encode_chunks(self, codes, unpred, eb, chunk, dtype):
+        """One block's codes (z,y,x) + unpredictables -> a chunk-marked stream (uint8 array)."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint16)
+        unpred = np.ascontiguousarray(unpred, dtype=dtype)
+        dz, dy, dx = codes.shape
+        dt = F32 if np.dtype(dtype) == np.float32 else F64
+        s = C.c_void_p()
+        n = C.c_size_t()
+        up = _ptr(unpred) if unpred.size else C.c_void_p()
+        check(lib().stz_huff_encode_chunks(self.h, dt, _ptr(codes), up, unpred.size, dx, dy, dz, int(chunk),
+                                           float(eb), C.byref(s), C.byref(n)), "stz_huff_encode_chunks")
+        out = np.ctypeslib.as_array(C.cast(s, C.POINTER(C.c_uint8)), shape=(n.value,)).copy()
+        lib().stz_free(s)
+        return out
+
+    def huff_decode_chunks(self, stream, dims, dtype, chunk_ids=None, out=None, codes_out=None):
+        """Decode the listed chunks (row-major ids; None: all) of a block stream
+        into out (z,y,x), leaving other points as they were.  Returns
+        (out, codes_out, decoded_symbols, unpred_zeros)."""
+        stream = np.ascontiguousarray(stream, dtype=np.uint8)
+        dx, dy, dz = dims
+        if out is None:
+            out = np.zeros((dz, dy, dx), dtype=dtype)
+        ids = None if chunk_ids is None else np.ascontiguousarray(chunk_ids, dtype=np.int64)
+        ns, nz = C.c_size_t(), C.c_size_t()
+        check(lib().stz_huff_decode_chunks(self.h, _ptr(stream), stream.size,
+                                           _ptr(ids) if ids is not None else C.c_void_p(),
+                                           -1 if ids is None else ids.size, _ptr(out),
+                                           _ptr(codes_out) if codes_out is not None else C.c_void_p(),
+                                           C.byref(ns), C.byref(nz)), "stz_huff_decode_chunks")
+        return out, codes_out, ns.value, nz.value
+
 
 def huff_lengths(freq, lmax=24):
     freq = np.ascontiguousarray(freq, dtype=np.uint64)
