@@ -36,6 +36,7 @@ extern "C" {
 #define STZ_E_HIP -5     /* HIP runtime error */
 #define STZ_E_ZSTD -6    /* host zstd failure */
 #define STZ_E_NODEV -7   /* no usable GPU */
+#define STZ_E_SCRATCH -8 /* input larger than the allocation-free entry point's scratch */
 
 typedef enum { STZ_F32 = 0, STZ_F64 = 1 } stz_dtype; /* F5:3057-3076 -f / -d */
 
@@ -268,6 +269,23 @@ int stz_huff_encode_chunks(stz_ctx* ctx, stz_dtype dtype, const uint16_t* codes,
  * unpredictable point gets its own stored value. */
 int stz_huff_decode_chunks(stz_ctx* ctx, const void* stream, size_t bytes, const int64_t* chunk_ids, int64_t n_ids,
                            void* out, uint16_t* codes_out, size_t* decoded_symbols, size_t* unpred_zeros);
+/* Allocation-free variant of stz_residual_encode.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It allocates nothing (no hipMalloc / hipFree, no host staging, no
+ * synchronisation): the residuals the unpredictable compaction reads live in
+ * a per-GPU static device array of STZ_NOALLOC_SCRATCH_BYTES, so a block may
+ * hold at most that many bytes of values (8 Mi f32 or 4 Mi f64 points; larger
+ * is STZ_E_SCRATCH).  Every pointer is device memory of the context's device
+ * (STZ_E_ARG otherwise); the unpredictable count goes to the device word
+ * *n_unpred.  The work is enqueued on the context's stream (stz_get_stream)
+ * and the outputs are ready when it completes.  All calls on one GPU share
+ * the scratch, which keeps the last call's contents: calls that overlap on
+ * one GPU (two contexts' streams, or two graph replays) race. */
+#define STZ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int stz_residual_encode_noalloc(stz_ctx* ctx, stz_dtype dtype, int block, const void* sub, const void* ref,
+                                size_t dx, size_t dy, size_t dz, double eb, uint16_t* codes, void* unpred,
+                                uint64_t* n_unpred, void* rec);
 /* Huffman code lengths (HuffmanEncoder::preprocess_encode_fast, F5:1271) --
  * this project's canonical length-limited code */
 int stz_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len);

@@ -10,10 +10,41 @@
 //   marked-chunk placement  SZ_decompress_nopred_marked_chunks (F5:1870-1905)
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "device_math.h"
 #include "kernels.h"
 
 namespace stz {
+
+// Device scratch of the allocation-free residual encode: the block's
+// residuals, which the ordered unpredictable compaction reads back.  Every
+// GPU holds its own copy for the life of the process; its address on a
+// device comes from hipGetSymbolAddress on that device (the name itself is a
+// host-side stand-in, never a device address).  All calls on one GPU share
+// it, so calls that overlap on one GPU race.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+int scratch_setup(int device, void** ptr)
+{
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[device]) {
+        // the caller has made `device` current
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return -1;
+        scratch_dev[device] = p;
+    }
+    *ptr = scratch_dev[device];
+    return 0;
+}
 
 namespace {
 
@@ -165,6 +196,15 @@ void launch_residual_encode(hipStream_t s, int block, const T* sub, const T* ref
 }
 
 template <typename T>
+void launch_residual_encode_noalloc(hipStream_t s, void* scratch_ptr, int block, const T* sub, const T* ref, int dx,
+                                    int dy, int dz, double eb, uint16_t* codes, T* unpred, uint64_t* count, T* rec)
+{
+    T* diff = (T*)scratch_ptr;   // the per-device address, never the array's name
+    launch_residual_encode<T>(s, block, sub, ref, dx, dy, dz, eb, codes, diff, rec);
+    launch_compact_unpred<T>(s, codes, diff, (uint64_t)dx * dy * dz, unpred, count);
+}
+
+template <typename T>
 void launch_block_to_stream(hipStream_t s, const uint16_t* codes, const T* unpred, uint64_t n_unpred, const Geom& g,
                             uint16_t* codes_s, T* side_s)
 {
@@ -188,6 +228,8 @@ void launch_chunks_to_block(hipStream_t s, const uint32_t* list, uint32_t nlist,
 #define STZ_INST(T)                                                                                               \
     template void launch_residual_encode<T>(hipStream_t, int, const T*, const T*, int, int, int, double, uint16_t*, \
                                             T*, T*);                                                              \
+    template void launch_residual_encode_noalloc<T>(hipStream_t, void*, int, const T*, const T*, int, int, int,   \
+                                                    double, uint16_t*, T*, uint64_t*, T*);                       \
     template void launch_block_to_stream<T>(hipStream_t, const uint16_t*, const T*, uint64_t, const Geom&,          \
                                             uint16_t*, T*);                                                       \
     template void launch_chunks_to_block<T>(hipStream_t, const uint32_t*, uint32_t, const Geom&, const uint16_t*,   \

@@ -190,6 +190,16 @@ template <typename T>
 void launch_block_to_stream(hipStream_t s, const uint16_t* codes, const T* unpred, uint64_t n_unpred, const Geom& g,
                             uint16_t* codes_s, T* side_s);
 void launch_hist_codes(hipStream_t s, const uint16_t* codes, uint64_t n, uint32_t* hist);
+// the allocation-free residual encode: residuals in the per-device scratch
+// array (kScratchBytes), its address from scratch_setup (called once per
+// device with that device current; 0 on success)
+constexpr size_t kScratchBytes = 32u << 20;
+int scratch_setup(int device, void** ptr);
+template <typename T>
+void launch_residual_encode_noalloc(hipStream_t s, void* scratch_ptr, int block, const T* sub, const T* ref, int dx,
+                                    int dy, int dz, double eb, uint16_t* codes, T* unpred, uint64_t* count, T* rec);
+template <typename T>
+void launch_compact_unpred(hipStream_t s, const uint16_t* codes, const T* data, uint64_t n, T* out, uint64_t* count);
 // list: (chunk, first element in codes/side) pairs
 template <typename T>
 void launch_chunks_to_block(hipStream_t s, const uint32_t* list, uint32_t nlist, const Geom& g, const uint16_t* codes,

@@ -248,6 +248,8 @@ private:
 
 struct stz_ctx {
     int device = 0;
+    // this device's copy of the allocation-free entry point's scratch array
+    void* scratch = nullptr;
     hipStream_t own = nullptr, stream = nullptr;
     // marks a mid-compress readback, so host codebook builds start while
     // the stream still runs the later kernels
@@ -2602,6 +2604,7 @@ const char* stz_strerror(int code)
     case STZ_E_HIP: return "HIP runtime error";
     case STZ_E_ZSTD: return "zstd error";
     case STZ_E_NODEV: return "no GPU device";
+    case STZ_E_SCRATCH: return "input exceeds the allocation-free entry point's scratch (STZ_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown error";
     }
 }
@@ -2632,6 +2635,8 @@ int stz_create(stz_ctx** out, int device)
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         c->n_cu = prop.multiProcessorCount;
+        // the device is current (guard): its scratch address, taken once per device
+        if (scratch_setup(device, &c->scratch) != 0) throw StzError(STZ_E_HIP, "scratch symbol address");
         *out = c.release();
     });
 }
@@ -3638,6 +3643,47 @@ int stz_huff_decode_chunks(stz_ctx* c, const void* stream, size_t bytes, const i
     return guard(c, [&] {
         huff_decode_chunks_impl(c, stream, bytes, chunk_ids, n_ids, out, codes_out, decoded_symbols, unpred_zeros);
     });
+}
+
+static_assert(kScratchBytes == STZ_NOALLOC_SCRATCH_BYTES, "scratch size in the header");
+
+int stz_residual_encode_noalloc(stz_ctx* c, stz_dtype dtype, int block, const void* sub, const void* ref, size_t dx,
+                                size_t dy, size_t dz, double eb, uint16_t* codes, void* unpred, uint64_t* n_unpred,
+                                void* rec)
+{
+    if (!c || !sub || !ref || !codes || !unpred || !n_unpred || !(eb >= 0) || block < 0 || block > 6 ||
+        (dtype != STZ_F32 && dtype != STZ_F64))
+        return STZ_E_ARG;
+    if (!dx || !dy || !dz) return STZ_E_DIMS;
+    // points bounded by the scratch, checked without overflow
+    const size_t cap = kScratchBytes / tsize_of(dtype);
+    if (dx > cap || dy > cap || dz > cap || dx * dy > cap || dx * dy * dz > cap) return STZ_E_SCRATCH;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        (void)hipGetLastError();
+        return STZ_E_HIP;
+    }
+    if (!c->scratch) return STZ_E_HIP;
+    // every buffer is device memory of this context's device: nothing is staged
+    const void* ptrs[6] = {sub, ref, codes, unpred, n_unpred, rec};
+    for (int k = 0; k < 6; ++k) {
+        if (k == 5 && !rec) break;
+        hipPointerAttribute_t a;
+        if (hipPointerGetAttributes(&a, ptrs[k]) != hipSuccess) {
+            (void)hipGetLastError();
+            return STZ_E_ARG;
+        }
+        if (a.type != hipMemoryTypeDevice || a.device != c->device) return STZ_E_ARG;
+    }
+    (void)hipGetLastError();
+    const int ix = (int)dx, iy = (int)dy, iz = (int)dz;
+    if (dtype == STZ_F32)
+        launch_residual_encode_noalloc<float>(c->stream, c->scratch, block, (const float*)sub, (const float*)ref, ix,
+                                              iy, iz, eb, codes, (float*)unpred, n_unpred, (float*)rec);
+    else
+        launch_residual_encode_noalloc<double>(c->stream, c->scratch, block, (const double*)sub, (const double*)ref,
+                                               ix, iy, iz, eb, codes, (double*)unpred, n_unpred, (double*)rec);
+    if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
+    return STZ_OK;
 }
 
 int stz_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len)

@@ -12,7 +12,7 @@ _lib = None
 
 STZ_OK = 0
 ERRORS = {-1: "STZ_E_ARG", -2: "STZ_E_DIMS", -3: "STZ_E_STREAM", -4: "STZ_E_OOM", -5: "STZ_E_HIP",
-          -6: "STZ_E_ZSTD", -7: "STZ_E_NODEV"}
+          -6: "STZ_E_ZSTD", -7: "STZ_E_NODEV", -8: "STZ_E_SCRATCH"}
 
 # every entry point include/stz.h declares
 EXPORTS = [
@@ -27,6 +27,7 @@ EXPORTS = [
     "stz_split", "stz_merge", "stz_residual_predict", "stz_residual_reconstruct", "stz_quantize",
     "stz_dequantize", "stz_huff_lengths", "stz_plan_query",
     "stz_residual_encode", "stz_residual_decode", "stz_huff_encode_chunks", "stz_huff_decode_chunks",
+    "stz_residual_encode_noalloc",
     "stz_tune_config", "stz_tune_features", "stz_tune_set_model", "stz_tune", "stz_tune_trial",
     "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
     "stz_compress_batch", "stz_decompress_batch", "stz_autotune",
@@ -131,6 +132,7 @@ def lib():
     L.stz_residual_encode.argtypes = [vp, i32, i32, vp, vp, sz, sz, sz, dbl, vp, vp, P(sz), vp]
     L.stz_residual_decode.argtypes = [vp, i32, i32, vp, vp, vp, sz, sz, sz, dbl, vp]
     L.stz_huff_encode_chunks.argtypes = [vp, i32, vp, vp, sz, sz, sz, sz, i32, dbl, P(vp), P(sz)]
+    L.stz_residual_encode_noalloc.argtypes = [vp, i32, i32, vp, vp, sz, sz, sz, dbl, vp, vp, vp, vp]
     L.stz_huff_decode_chunks.argtypes = [vp, vp, sz, vp, C.c_int64, vp, vp, P(sz), P(sz)]
     L.stz_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
     L.stz_compress_batch.argtypes = [vp, i32, i32, P(vp), P(sz), dbl, i32, P(Params), P(vp)]

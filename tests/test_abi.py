@@ -31,7 +31,7 @@ def test_header_and_exports_agree(stz):
 
 def test_strerror_and_defaults(stz):
     L = stz.lib()
-    for code in [0, -1, -2, -3, -4, -5, -6, -7]:
+    for code in [0, -1, -2, -3, -4, -5, -6, -7, -8]:
         assert L.stz_strerror(code)
     p = stz.default_params()
     # F5:2127-2129 level multipliers, F5:34-35 chunk dims
