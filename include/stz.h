@@ -40,9 +40,16 @@ extern "C" {
 
 typedef enum { STZ_F32 = 0, STZ_F64 = 1 } stz_dtype; /* F5:3057-3076 -f / -d */
 
-/* STZ_EB_ABS / STZ_EB_REL: eb is the maximum pointwise error (REL: times
- * the value range, computed in double).  STZ_EB_REF: eb has the meaning of
- * F5's argv[1]: level bounds are mult_* x eb (F5:2139, F5:2201, F5:2259). */
+/* STZ_EB_ABS / STZ_EB_REL: the bound E = eb (REL: eb times the value range
+ * of the finite values, in double) is a strict maximum of |decoded - input|
+ * at every finite point.  The levels share E - 4u(M + E), u the unit
+ * roundoff of the field's type and M its largest finite |value|: that much
+ * covers the rounding of the reconstruction in the field's type.  A bound
+ * below 4u(M + E) (a few ulps of the data) leaves 0: every value is stored,
+ * and the error is then that rounding, at most 3.25u(M + E).
+ * STZ_EB_REF: eb has the meaning of F5's argv[1]: level bounds are
+ * mult_* x eb (F5:2139, F5:2201, F5:2259), and the decoded error is at most
+ * the level bound plus the same rounding, as in the reference. */
 typedef enum { STZ_EB_ABS = 0, STZ_EB_REL = 1, STZ_EB_REF = 2 } stz_eb_mode;
 
 typedef struct {

@@ -173,20 +173,28 @@ void ora_default_params(ora_params* p)
     p->beta = 1.0;
 }
 
-/* ABS/REL: the requested bound is the maximum pointwise error, so the level
- * unit is eb/max(mult).  REF keeps F5's CLI meaning (argv[1] times the level
+/* ABS/REL: the requested bound E is the maximum pointwise error.  The
+ * reconstruction's two roundings in T (residual and value) and the rounded
+ * difference the quantizer checks add at most 3.25 u (M + E), u the unit
+ * roundoff of T and M = max |value|, so the levels share E - 4 u (M + E)
+ * (0 when the bound is below that), and the level unit is that over
+ * max(mult).  REF keeps F5's CLI meaning (argv[1] times the level
  * multiplier, F5:2139, F5:2201, F5:2259). */
-static void ora_level_ebs(double eb, int mode, double vmin, double vmax, const ora_params* p,
+static void ora_level_ebs(double eb, int mode, double vmin, double vmax, int dtype, const ora_params* p,
                           double ebl[3], double* unit)
 {
     if (mode == ORA_EB_REF) {
         *unit = eb;
     } else {
         const double E = (mode == ORA_EB_REL) ? eb * (vmax - vmin) : eb;
+        const double M = fabs(vmin) > fabs(vmax) ? fabs(vmin) : fabs(vmax);
+        const double u = dtype == ORA_F32 ? 0x1p-24 : 0x1p-53;
+        double Es = E - 4.0 * u * (M + E);
+        if (!(Es > 0)) Es = 0;
         double mm = p->mult_base;
         if (p->mult_low > mm) mm = p->mult_low;
         if (p->mult_high > mm) mm = p->mult_high;
-        *unit = E / mm;
+        *unit = Es / mm;
     }
     ebl[0] = p->mult_base * *unit;
     ebl[1] = p->mult_low * *unit;

@@ -496,16 +496,30 @@ void check_dims(size_t nx, size_t ny, size_t nz, int ch, int cl)
         throw StzError(STZ_E_DIMS, "field too large for one stream grid");
 }
 
-void level_ebs(double eb, int mode, double vmin, double vmax, const stz_params& p, double ebl[3], double& unit)
+// ABS / REL: the requested bound E is the maximum pointwise error.  The
+// reconstruction rounds twice in the field's type (the residual
+// fl(f - pred) and the value fl(dec + pred)), and the quantizer's check
+// |dec - d| <= eb sees a difference rounded once more: together at most
+// 3.25 u (M + E), u the unit roundoff, M = max |value| (predictions are
+// weighted means of reconstructed values with weights summing to 1 and
+// absolute sum <= 1.25).  The levels get E - 4 u (M + E), so the error is
+// strictly at most E; a bound below that slack leaves 0 (every value
+// stored).  REF keeps F5's meaning: level bounds are mult_* x eb.
+void level_ebs(double eb, int mode, double vmin, double vmax, int dtype, const stz_params& p, double ebl[3],
+               double& unit)
 {
     if (mode == STZ_EB_REF) {
         unit = eb;
     } else {
         const double E = (mode == STZ_EB_REL) ? eb * (vmax - vmin) : eb;
+        const double M = std::max(std::fabs(vmin), std::fabs(vmax));
+        const double u = dtype == STZ_F32 ? 0x1p-24 : 0x1p-53;
+        double Es = E - 4.0 * u * (M + E);
+        if (!(Es > 0)) Es = 0;
         double mm = p.mult_base;
         if (p.mult_low > mm) mm = p.mult_low;
         if (p.mult_high > mm) mm = p.mult_high;
-        unit = E / mm;
+        unit = Es / mm;
     }
     ebl[0] = p.mult_base * unit;
     ebl[1] = p.mult_low * unit;
@@ -781,7 +795,7 @@ void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
     (void)c;
     fe.vmin = fe.mm_host[0];
     fe.vmax = fe.mm_host[1];
-    level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.prm, fe.ebl, fe.unit);
+    level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.dtype, fe.prm, fe.ebl, fe.unit);
 }
 
 template <typename T>

@@ -86,3 +86,22 @@ def nan_sign_cleared_values(a):
     w = a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
     _clear_nan_sign(w.reshape(-1), 8 * a.dtype.itemsize)
     return a
+
+
+def requested_bound(f, eb, mode):
+    """The maximum pointwise error an ABS / REL compress guarantees: eb, or
+    eb times the value range of the finite values (in double)."""
+    if mode == "abs":
+        return float(eb)
+    g = np.asarray(f, dtype=np.float64)
+    g = g[np.isfinite(g)]
+    return float(eb) * (float(g.max()) - float(g.min())) if g.size else 0.0
+
+
+def max_error(rec, f):
+    """max |rec - f| over the finite points of f (non-finite points must match)."""
+    a = np.asarray(rec, dtype=np.float64)
+    b = np.asarray(f, dtype=np.float64)
+    fin = np.isfinite(b)
+    assert np.array_equal(np.isnan(a[~fin]), np.isnan(b[~fin]))
+    return float(np.max(np.abs(a[fin] - b[fin]))) if fin.any() else 0.0

@@ -2,16 +2,17 @@
 
 Bar: bit-exact codes, payload bytes and reconstructions (integer/byte work
 and the FP predictors are reproduced operation for operation), max pointwise
-error <= eb (plus the reference quantizer's T-rounding of pred+dec, at most
-2 ulp of the largest magnitude), ROI / slice / level == the oracle's, which
-itself equals a crop of the full decode.
+error <= the requested ABS / REL bound (strictly; under F5's REF meaning the
+level bound plus the T rounding of pred+dec, at most 2 ulp of the largest
+magnitude), ROI / slice / level == the oracle's, which itself equals a crop
+of the full decode.
 """
 import sys
 
 import numpy as np
 import pytest
 
-from payload_util import first_difference, nan_sign_cleared, nan_sign_cleared_values
+from payload_util import first_difference, max_error, nan_sign_cleared, nan_sign_cleared_values, requested_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -128,11 +129,18 @@ def test_payload_and_decode_bitexact(ctx, oracle, case):
     rec = ds.decompress()
     ref = oracle.decompress(want)
     _assert_bitexact(rec, ref, "decompress")
-    h = oracle.header(want)
-    err = np.max(np.abs(rec.astype(np.float64) - f.astype(np.float64)))
-    bound = max(h["eb_base"], h["eb_low"], h["eb_high"])
-    slack = 2 * np.spacing(np.abs(f).max().astype(dtype)).astype(np.float64)
-    assert err <= bound + slack, (err, bound)
+    err = max_error(rec, f)
+    if mode in ("abs", "rel"):
+        # the requested bound holds strictly (the level unit leaves room for
+        # the reconstruction's rounding)
+        E = requested_bound(f, eb, mode)
+        assert err <= E, (err, E)
+    else:
+        # REF: F5's level bounds, plus the T rounding of pred + dec
+        h = oracle.header(want)
+        bound = max(h["eb_base"], h["eb_low"], h["eb_high"])
+        slack = 2 * np.spacing(np.abs(f).max().astype(dtype)).astype(np.float64)
+        assert err <= bound + slack, (err, bound)
     ds.free()
 
 

@@ -548,3 +548,20 @@ def test_mlp_reference_adam_matches_torch_adamw():
         t.grad = torch.tensor(G, dtype=torch.float64)
         opt.step()
         np.testing.assert_allclose(P, t.detach().numpy(), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("dt,ebs", [(np.float32, (1e-2, 1e-3, 1e-4, 1e-5)), (np.float64, (1e-4, 1e-7, 1e-10))])
+def test_oracle_requested_bound_is_strict(oracle, dt, ebs):
+    """White noise (every residual a large fraction of the bound) and an
+    offset field (|values| far above the range): the decoded error never
+    exceeds the requested ABS / REL bound, without any rounding slack."""
+    from payload_util import max_error, requested_bound
+    rng = np.random.default_rng(17)
+    noise = rng.standard_normal((24, 28, 32))
+    for f in (noise.astype(dt), (1000.0 + noise).astype(dt)):
+        for eb in ebs:
+            for mode, code in (("rel", oracle.EB_REL), ("abs", oracle.EB_ABS)):
+                e = eb if mode == "rel" else eb * float(f.max() - f.min())
+                p = oracle.compress(f, e, code)
+                err = max_error(oracle.decompress(p), f)
+                assert err <= requested_bound(f, e, mode), (dt.__name__, eb, mode, err)
