@@ -641,7 +641,8 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
                                                const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
                                                uint64_t al, const uint16_t* lsrt, uint32_t llo, uint32_t ln)
 {
-    uint64_t wpos = (it.bit0 >> 5) - wbase;
+    const uint64_t wpos0 = (it.bit0 >> 5) - wbase;
+    uint64_t wpos = wpos0;
     const int skip = (int)(it.bit0 & 31u);
     uint32_t bad = 0;
     if (wpos + 2 > nwords) return 1u;
@@ -756,6 +757,10 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     for (uint32_t k = 0; k < pc; ++k) co[i + k] = (uint16_t)(lo >> (16 * k));
     i += pc;
     for (; i < cnt; ++i) co[i] = (uint16_t)single(i);
+    // the sub-chunk uses exactly its recorded bit count (else corrupt):
+    // bits used = words loaded into buf x 32 - the start offset - bits left
+    const int64_t used = (int64_t)(wpos - wpos0) * 32 - skip - nb;
+    bad |= used != (int64_t)it.bits ? 1u : 0u;
     return bad;
 }
 
@@ -773,7 +778,8 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
                                                const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
                                                const uint16_t* lsrt, uint32_t llo, uint32_t ln)
 {
-    uint64_t wpos = (it.bit0 >> 5) - wbase;
+    const uint64_t wpos0 = (it.bit0 >> 5) - wbase;
+    uint64_t wpos = wpos0;
     const int skip = (int)(it.bit0 & 31u);
     uint32_t bad = 0;
     if (wpos + 2 > nwords) return 1u;
@@ -856,6 +862,10 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
         }
     }
     for (; i < cnt; ++i) co[i] = (uint16_t)step();
+    // the sub-chunk uses exactly its recorded bit count (else corrupt):
+    // bits used = words loaded into buf x 32 - the start offset - bits left
+    const int64_t used = (int64_t)(wpos - wpos0) * 32 - skip - nb;
+    bad |= used != (int64_t)it.bits ? 1u : 0u;
     return bad;
 }
 
@@ -866,7 +876,7 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
 // payloads); lanes past the staged words read global memory.  Without the
 // stage (dense payloads) every refill reads global memory, and the smaller
 // LDS footprint lets more workgroups share a CU.
-template <bool kStage>
+template <int kForm>
 __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_t* __restrict__ payload,
                                                const DecTab* __restrict__ tabs, const uint16_t* __restrict__ sorted,
                                                const DecItem* __restrict__ items, uint16_t* __restrict__ codes,
@@ -876,9 +886,10 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
 {
     const int s = find_stream_by_wg(ss, wgid);
     if (s < 0) return;  // uniform across the workgroup
+    constexpr bool kStage = kForm == kDecStaged;
     const StreamDesc& d = ss.s[s];
     const DecTab* t = tabs + s;
-    uint32_t* lut1 = (uint32_t*)lut;  // the single-symbol table of the unstaged form
+    uint32_t* lut1 = (uint32_t*)lut;  // the single-symbol table of the dense form
     if constexpr (kStage)
         for (int k = threadIdx.x; k < (1 << kLutBits); k += 256) lut[k] = t->lut[k];
     else
@@ -959,7 +970,7 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
 }
 
 
-template <bool kStage>
+template <int kForm>
 __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t* __restrict__ payload,
                                                      const DecTab* __restrict__ tabs,
                                                      const uint16_t* __restrict__ sorted,
@@ -968,11 +979,12 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
                                                      uint8_t* __restrict__ side,
                                                      uint32_t* __restrict__ err)
 {
+    constexpr bool kStage = kForm == kDecStaged;
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
-    huff_decode_wg<kStage>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage,
+    huff_decode_wg<kForm>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage,
                            lsrt);
 }
 
@@ -987,15 +999,16 @@ __device__ __forceinline__ uint32_t find_dec_job(const DecJob* __restrict__ jobs
     return lo;
 }
 
-template <bool kStage>
+template <int kForm>
 __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restrict__ jobs, uint32_t njobs)
 {
+    constexpr bool kStage = kForm == kDecStaged;
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
     __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
     __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
     __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];
-    huff_decode_wg<kStage>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
+    huff_decode_wg<kForm>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
                            blockIdx.x - J.wg0, lut, fc, cn, fi, stage, lsrt);
 }
 
@@ -1104,11 +1117,11 @@ void launch_copy_segments(hipStream_t s, const uint8_t* src, const Segment* segs
     if (n) k_copy_segments<<<dim3(n), dim3(256), 0, s>>>(src, segs);
 }
 
-void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage)
+void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, int form)
 {
     if (!njobs || !wgs) return;
-    if (stage) k_huff_decode_multi<true><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
-    else k_huff_decode_multi<false><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+    if (form == kDecStaged) k_huff_decode_multi<kDecStaged><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
+    else k_huff_decode_multi<kDecDense><<<dim3(wgs), dim3(256), 0, s>>>(jobs, njobs);
 }
 
 void launch_dec_prep_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t total_chunks)
@@ -1125,15 +1138,15 @@ void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* s
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,
                         const uint16_t* sorted, const DecItem* items, uint16_t* codes, void* side,
-                        uint32_t* err, bool stage)
+                        uint32_t* err, int form)
 {
     if (!ss.total_wgs) return;
-    if (stage)
-        k_huff_decode<true><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
-                                                                     (uint8_t*)side, err);
+    if (form == kDecStaged)
+        k_huff_decode<kDecStaged><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
+                                                                           (uint8_t*)side, err);
     else
-        k_huff_decode<false><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
-                                                                      (uint8_t*)side, err);
+        k_huff_decode<kDecDense><<<dim3(ss.total_wgs), dim3(256), 0, s>>>(ss, payload, tabs, sorted, items, codes,
+                                                                          (uint8_t*)side, err);
 }
 
 }  // namespace stz

@@ -129,7 +129,11 @@ struct Segment {
     uint64_t bytes;
 };
 void launch_copy_segments(hipStream_t s, const uint8_t* src, const Segment* segs, uint32_t n);
-void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, bool stage);
+// Huffman decode forms: kDecStaged, the multi-symbol table with the
+// workgroup's bits staged in LDS; kDecDense, the single-symbol table with
+// bits from global memory
+enum DecForm { kDecStaged = 0, kDecDense = 1 };
+void launch_huff_decode_multi(hipStream_t s, const DecJob* jobs, uint32_t njobs, uint32_t wgs, int form);
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint64_t n, uint32_t v);
 void launch_hist_compact(hipStream_t s, const uint32_t* hist, int nstreams, uint64_t* out,
@@ -154,7 +158,7 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
 void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams);
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                         const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
-                        uint16_t* codes, void* side, uint32_t* err, bool stage);
+                        uint16_t* codes, void* side, uint32_t* err, int form);
 
 // ---- k_field.hip ----
 template <typename T>

@@ -944,6 +944,23 @@ struct ErrFlag {
 // the high streams, about 7/8 of the symbols, only the high level.
 constexpr int kSplitStream = 8;
 
+// Decode form of a launch by the symbol-weighted bits per symbol of what it
+// decodes: up to 2 the multi-symbol table, above it the single-symbol one
+int dec_form(double bps)
+{
+    return bps <= 2.0 ? (int)kDecStaged : (int)kDecDense;
+}
+
+// sub-chunks per decode workgroup: 256, fewer for the staged form while a
+// stream's expected bits with a 30% margin would overflow the LDS stage
+uint32_t dec_ipw(int form, double bps)
+{
+    uint32_t ipw = 256;
+    if (form == kDecStaged)
+        while (ipw > 32 && ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) ipw >>= 1;
+    return ipw;
+}
+
 // Enqueue the Huffman decode of full[s] streams (every chunk) and of the
 // chunks lists[s] of the others.  split: the high streams decode on the
 // context's second stream, overlapping the base and low levels' decode on
@@ -1013,14 +1030,13 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
         wbits[gi] += bps * ss.s[s].item_count;
         wsym[gi] += ss.s[s].item_count;
     }
-    const bool stage_of[2] = {wsym[0] > 0 && wbits[0] / wsym[0] <= 2.0, wsym[1] > 0 && wbits[1] / wsym[1] <= 2.0};
-    const bool use_stage = stage_of[0];
+    const int form_of[2] = {dec_form(wsym[0] > 0 ? wbits[0] / wsym[0] : 0.0),
+                            dec_form(wsym[1] > 0 ? wbits[1] / wsym[1] : 0.0)};
+    const int use_form = form_of[0];
     for (int s = 0; s < kStreams; ++s) {
         StreamDesc& d = ss.s[s];
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
-        d.ipw = 256;
-        if (stage_of[two && s >= kSplitStream ? 1 : 0])
-            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
+        d.ipw = dec_ipw(form_of[two && s >= kSplitStream ? 1 : 0], bps);
         d.item_base = item;
         d.wg_base = wg;
         const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
@@ -1069,9 +1085,9 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
         HIPCHK(hipStreamWaitEvent(s2, c->ev_items, 0));
         K(c, "huff_decode", [&] {
             launch_huff_decode(st, lo, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
-                               out.codes.as<uint16_t>(), out.side.p, err, stage_of[0]);
+                               out.codes.as<uint16_t>(), out.side.p, err, form_of[0]);
             launch_huff_decode(s2, hi, pay, ds->tabs.as<DecTab>() + kSplitStream, ds->sorted.as<uint16_t>(),
-                               items.as<DecItem>(), out.codes.as<uint16_t>(), out.side.p, err, stage_of[1]);
+                               items.as<DecItem>(), out.codes.as<uint16_t>(), out.side.p, err, form_of[1]);
         });
         HIPCHK(hipEventRecord(c->ev_high, s2));
         out.split = true;
@@ -1079,7 +1095,7 @@ void huff_decode(stz_ctx* c, stz_dstream* ds, const HostLayout& L, const bool* f
     }
     K(c, "huff_decode", [&] {
         launch_huff_decode(st, ss, pay, ds->tabs.as<DecTab>(), ds->sorted.as<uint16_t>(), items.as<DecItem>(),
-                           out.codes.as<uint16_t>(), out.side.p, err, use_stage);
+                           out.codes.as<uint16_t>(), out.side.p, err, use_form);
     });
 }
 
@@ -2062,9 +2078,7 @@ void plan_huff_setup(stz_ctx* c, stz_dstream* ds, BatchPlan::Field& F)
     for (int s = 0; s < kStreams; ++s) {
         StreamDesc& d = F.ss.s[s];
         const double bps = L.n[s] ? (double)m.sh[s].total_bits / (double)L.n[s] : 0.0;
-        d.ipw = 256;
-        if (F.stage)
-            while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
+        d.ipw = dec_ipw(F.stage ? kDecStaged : kDecDense, bps);
         d.item_base = item;
         d.wg_base = wg;
         const uint32_t nwg = (d.item_count + d.ipw - 1) / d.ipw;
@@ -2090,8 +2104,8 @@ void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
     K(c, "err_zero", [&] { launch_fill_u32(st, P.err.as<uint32_t>(), (uint64_t)n, 0u); });
     K(c, "dec_prep", [&] { launch_dec_prep_multi(st, P.dec_all.as<DecJob>(), (uint32_t)n, P.chunks); });
     K(c, "huff_decode", [&] {
-        launch_huff_decode_multi(st, P.dec_st.as<DecJob>(), P.n_st, P.wg_st, true);
-        launch_huff_decode_multi(st, P.dec_un.as<DecJob>(), P.n_un, P.wg_un, false);
+        launch_huff_decode_multi(st, P.dec_st.as<DecJob>(), P.n_st, P.wg_st, kDecStaged);
+        launch_huff_decode_multi(st, P.dec_un.as<DecJob>(), P.n_un, P.wg_un, kDecDense);
     });
     K(c, "interp_decode_fused", [&] { launch_interp_decode_fused_multi<T>(st, P.fused.as<FusedJob>(), P.n_fused); });
     for (size_t r = 0; r < P.rounds.size(); ++r)
@@ -3479,9 +3493,7 @@ void huff_decode_chunks_impl(stz_ctx* c, const void* stream, size_t bytes, const
     // would overflow the stage), above it the unstaged one
     const double bps = (double)sh.total_bits / (double)n;
     const bool stage = bps <= 2.0;
-    d.ipw = 256;
-    if (stage)
-        while (d.ipw > 32 && d.ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) d.ipw >>= 1;
+    d.ipw = dec_ipw(stage ? kDecStaged : kDecDense, bps);
     const uint32_t nwg = (item + d.ipw - 1) / d.ipw;
     ss.total_items = nwg * d.ipw;
     ss.total_wgs = nwg;
@@ -3500,7 +3512,7 @@ void huff_decode_chunks_impl(stz_ctx* c, const void* stream, size_t bytes, const
     });
     K(c, "huff_decode", [&] {
         launch_huff_decode(st, ss, pay.as<uint8_t>(), tabs.as<DecTab>(), srt.as<uint16_t>(), items.as<DecItem>(),
-                           codes.as<uint16_t>(), side.p, err.ptr(), stage);
+                           codes.as<uint16_t>(), side.p, err.ptr(), stage ? kDecStaged : kDecDense);
     });
     // points outside the listed chunks keep the caller's values
     DView vo(c, out, n * ts, true);
