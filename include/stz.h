@@ -60,7 +60,16 @@ typedef struct {
     double alpha, beta;                    /* base codec level eb: eb/min(alpha^(l-1),beta) */
     int zstd_level;                        /* host zstd level (F5:3048: SZ3 default 3) */
     int tuner;                             /* 0 fixed params, 1 learned, 2 exhaustive trial */
+    int pred;                              /* STZ_PRED_* bits: F5's predictor build options */
 } stz_params;
+
+/* F5's compile-time predictor options as run-time bits of stz_params.pred
+ * (0 = F5's default build).  They change the interior predictions of the
+ * plane blocks (2, 4, 5) and the cube block (6), and are written to the
+ * stream header, which the decoder follows. */
+#define STZ_PRED_WPOW2 1      /* inverse-square variance weights (F5:177-203) */
+#define STZ_PRED_PLANE_AXIS 2 /* two axis-pair candidates more for blocks 2, 4, 5 (F5:579-607, F5:627-633) */
+#define STZ_PRED_CUBE_AXIS 4  /* three axis-pair candidates more for block 6 (F5:609-622, F5:675-686) */
 
 typedef struct {
     size_t nx, ny, nz;        /* caller's dims, x fastest */
@@ -70,6 +79,7 @@ typedef struct {
     size_t payload_bytes;     /* device payload (before zstd) */
     int interp, order;
     double mult_base, mult_low, mult_high, alpha, beta;
+    int pred;                 /* STZ_PRED_* bits */
 } stz_info;
 
 typedef struct stz_ctx stz_ctx;

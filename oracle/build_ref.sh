@@ -21,10 +21,21 @@ if [ ! -f "$F5" ]; then
     exit 0
 fi
 mkdir -p "$HERE/_ref"
-{
-    echo "#line 1 \"F5\""; sed -n '1,17p' "$F5"
-    echo "#line 24 \"F5\""; sed -n '24,1253p' "$F5"
-    echo "#line 1577 \"F5\""; sed -n '1577,1839p' "$F5"
-    echo "#line 1 \"ref_harness.cpp\""; cat "$HERE/ref_harness.cpp"
-} | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w -o "$HERE/_ref/libstz_ref.so" -
-echo "built $HERE/_ref/libstz_ref.so"
+# one build per predictor option set of F5 (its own compile-time switches,
+# F5:177-203 PRED_WPOW2, F5:579-607 PRED_PLANE_AXIS, F5:609-686 PRED_CUBE_AXIS)
+build() {  # $1 = output name, the rest = -D flags
+    out="$HERE/_ref/$1"; shift
+    {
+        echo "#line 1 \"F5\""; sed -n '1,17p' "$F5"
+        echo "#line 24 \"F5\""; sed -n '24,1253p' "$F5"
+        echo "#line 1577 \"F5\""; sed -n '1577,1839p' "$F5"
+        echo "#line 1 \"ref_harness.cpp\""; cat "$HERE/ref_harness.cpp"
+    } | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w "$@" -o "$out" -
+    echo "built $out"
+}
+build libstz_ref.so & p0=$!
+build libstz_ref_wpow2.so -DPRED_WPOW2 & p1=$!
+build libstz_ref_plane.so -DPRED_PLANE_AXIS & p2=$!
+build libstz_ref_cube.so -DPRED_CUBE_AXIS & p3=$!
+build libstz_ref_all.so -DPRED_WPOW2 -DPRED_PLANE_AXIS -DPRED_CUBE_AXIS & p4=$!
+for p in $p0 $p1 $p2 $p3 $p4; do wait $p || exit 1; done

@@ -25,7 +25,11 @@ class Params(C.Structure):
         ("chunk_high", C.c_int), ("chunk_low", C.c_int),
         ("interp", C.c_int), ("order", C.c_int),
         ("alpha", C.c_double), ("beta", C.c_double),
+        ("pred", C.c_int),
     ]
+
+# F5's predictor build options as bits of Params.pred (PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS)
+PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS = 1, 2, 4
 
 
 def build():
@@ -51,6 +55,8 @@ def lib():
             getattr(L, "ora_merge_" + t).argtypes = [C.POINTER(vp), vp, i32, i32, i32]
             getattr(L, "ora_preprocess_" + t).argtypes = [i32, vp, vp, vp, i32, i32, i32]
             getattr(L, "ora_depreprocess_" + t).argtypes = [i32, vp, vp, vp, i32, i32, i32]
+            getattr(L, "ora_preprocess_pv_" + t).argtypes = [i32, i32, vp, vp, vp, i32, i32, i32]
+            getattr(L, "ora_depreprocess_pv_" + t).argtypes = [i32, i32, vp, vp, vp, i32, i32, i32]
             getattr(L, "ora_quantize_" + t).restype = sz
             getattr(L, "ora_quantize_" + t).argtypes = [vp, sz, dbl, i32, vp, vp, vp]
             getattr(L, "ora_dequantize_" + t).argtypes = [vp, sz, dbl, i32, vp, vp]
@@ -171,21 +177,21 @@ def merge(subs):
     return out
 
 
-def preprocess(block, sub, ref):
+def preprocess(block, sub, ref, pred=0):
     sub = np.ascontiguousarray(sub)
     ref = np.ascontiguousarray(ref)
     dz, dy, dx = sub.shape
     out = np.empty_like(sub)
-    getattr(lib(), "ora_preprocess_" + _tsuf(sub.dtype))(block, _ptr(sub), _ptr(ref), _ptr(out), dx, dy, dz)
+    getattr(lib(), "ora_preprocess_pv_" + _tsuf(sub.dtype))(block, pred, _ptr(sub), _ptr(ref), _ptr(out), dx, dy, dz)
     return out
 
 
-def depreprocess(block, de, ref):
+def depreprocess(block, de, ref, pred=0):
     de = np.ascontiguousarray(de)
     ref = np.ascontiguousarray(ref)
     dz, dy, dx = de.shape
     out = np.empty_like(de)
-    getattr(lib(), "ora_depreprocess_" + _tsuf(de.dtype))(block, _ptr(de), _ptr(ref), _ptr(out), dx, dy, dz)
+    getattr(lib(), "ora_depreprocess_pv_" + _tsuf(de.dtype))(block, pred, _ptr(de), _ptr(ref), _ptr(out), dx, dy, dz)
     return out
 
 

@@ -35,7 +35,8 @@ typedef struct {
     uint32_t interp, order, lmax, n_streams;
     double alpha, beta;
     uint64_t total_bytes;
-    uint64_t reserved[3];
+    uint64_t pred;          /* predictor options (ORA_PRED_* bits), 0 = F5's default build */
+    uint64_t reserved[2];
     uint64_t stream_off[STZF_NSTREAMS];
     uint64_t stream_bytes[STZF_NSTREAMS];
     uint8_t pad[48];

@@ -171,6 +171,7 @@ void ora_default_params(ora_params* p)
     p->order = 0;
     p->alpha = 1.0;
     p->beta = 1.0;
+    p->pred = 0;         /* F5's default build: no PRED_* option */
 }
 
 /* ABS/REL: the requested bound E is the maximum pointwise error.  The
@@ -358,6 +359,7 @@ static void ora_fill_header(stzf_file_header* fh, const ora_layout* L, int dtype
     fh->chunk_low = (uint32_t)L->chunk_low;
     fh->sub_size = STZF_SUB;
     fh->interp = (uint32_t)p->interp;
+    fh->pred = (uint64_t)p->pred;
     fh->order = (uint32_t)p->order;
     fh->lmax = STZF_LMAX;
     fh->n_streams = STZF_NSTREAMS;
@@ -510,7 +512,8 @@ static int ora_open_view(ora_view* v, const uint8_t* p, size_t bytes)
     memset(v, 0, sizeof(*v));
     if (bytes < sizeof(stzf_file_header)) return -1;
     const stzf_file_header* fh = (const stzf_file_header*)p;
-    if (fh->magic != STZF_MAGIC || fh->version != STZF_VERSION || fh->n_streams != STZF_NSTREAMS)
+    if (fh->magic != STZF_MAGIC || fh->version != STZF_VERSION || fh->n_streams != STZF_NSTREAMS ||
+        fh->pred > 7)
         return -1;
     if (fh->total_bytes > bytes) return -1;
     v->base = p;

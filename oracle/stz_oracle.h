@@ -34,7 +34,13 @@ typedef struct {
     int interp;                              /* base codec: 0 linear, 1 cubic */
     int order;                               /* base codec axis order 0..5 */
     double alpha, beta;                      /* base codec level eb scaling */
+    int pred;                                /* ORA_PRED_* bits: F5's compile-time predictor options */
 } ora_params;
+
+/* F5's predictor build options, as run-time bits: PRED_WPOW2 (F5:177-203),
+ * PRED_PLANE_AXIS (F5:579-607, F5:627-633), PRED_CUBE_AXIS (F5:609-622,
+ * F5:675-686) */
+enum { ORA_PRED_WPOW2 = 1, ORA_PRED_PLANE_AXIS = 2, ORA_PRED_CUBE_AXIS = 4 };
 
 void ora_default_params(ora_params* p);
 
@@ -62,6 +68,15 @@ void ora_preprocess_f32(int block, const float* sub, const float* ref, float* di
                         int dx, int dy, int dz);
 void ora_preprocess_f64(int block, const double* sub, const double* ref, double* diff,
                         int dx, int dy, int dz);
+/* the same under F5's predictor options (pv: ORA_PRED_* bits) */
+void ora_preprocess_pv_f32(int block, int pv, const float* sub, const float* ref, float* diff,
+                           int dx, int dy, int dz);
+void ora_preprocess_pv_f64(int block, int pv, const double* sub, const double* ref, double* diff,
+                           int dx, int dy, int dz);
+void ora_depreprocess_pv_f32(int block, int pv, const float* de, const float* ref, float* out,
+                             int dx, int dy, int dz);
+void ora_depreprocess_pv_f64(int block, int pv, const double* de, const double* ref, double* out,
+                             int dx, int dy, int dz);
 /* out = de + pred(ref) */
 void ora_depreprocess_f32(int block, const float* de, const float* ref, float* out,
                           int dx, int dy, int dz);

@@ -90,56 +90,78 @@ __device__ __forceinline__ T ldx(const T* __restrict__ base, uint32_t xo, int k)
 // dy, dz in -1..2 and k in -1..2 (requires 1 <= x <= dx-3 and the same on y
 // and z).  Same expressions, in the same order, as predict<B>'s interior;
 // the global-memory and LDS forms below both go through it.
+// `opt`: F5's predictor options (kPred* bits; a constant 0 in the default
+// kernels, so their code is F5's default build only).
 template <typename T, typename At>
-__device__ __forceinline__ void predict7_at(const At& at, T p[7])
+__device__ __forceinline__ void predict7_at(const At& at, T p[7], int opt)
 {
+    // A(ox, oy, oz): the value at that offset from the point
+    auto A = [&](int ox, int oy, int oz) -> T { return at(oy, oz, ox); };
     const T c = at(0, 0, 0);
     p[0] = cub(at(0, 0, -1), c, at(0, 0, 1), at(0, 0, 2));
     p[1] = cub(at(-1, 0, 0), c, at(1, 0, 0), at(2, 0, 0));
     p[3] = cub(at(0, -1, 0), c, at(0, 1, 0), at(0, 2, 0));
     T p0, v0, p1, v1, p2, v2, p3, v3;
-    pv(at(-1, 0, -1), c, at(1, 0, 1), at(2, 0, 2), p0, v0);
-    pv(at(-1, 0, 2), at(0, 0, 1), at(1, 0, 0), at(2, 0, -1), p1, v1);
-    p[2] = blend2(p0, v0, p1, v1);
-    pv(at(0, -1, -1), c, at(0, 1, 1), at(0, 2, 2), p0, v0);
-    pv(at(0, -1, 2), at(0, 0, 1), at(0, 1, 0), at(0, 2, -1), p1, v1);
-    p[4] = blend2(p0, v0, p1, v1);
-    pv(at(-1, -1, 0), c, at(1, 1, 0), at(2, 2, 0), p0, v0);
-    pv(at(2, -1, 0), at(1, 0, 0), at(0, 1, 0), at(-1, 2, 0), p1, v1);
-    p[5] = blend2(p0, v0, p1, v1);
+    if (opt & kPredPlaneAxis) {
+        p[2] = plane_dyn<T>([&](int o1, int o2) { return A(o1, o2, 0); });
+        p[4] = plane_dyn<T>([&](int o1, int o2) { return A(o1, 0, o2); });
+        p[5] = plane_dyn<T>([&](int o1, int o2) { return A(0, o1, o2); });
+    } else {
+        pv(at(-1, 0, -1), c, at(1, 0, 1), at(2, 0, 2), p0, v0);
+        pv(at(-1, 0, 2), at(0, 0, 1), at(1, 0, 0), at(2, 0, -1), p1, v1);
+        p[2] = blend2(p0, v0, p1, v1, opt);
+        pv(at(0, -1, -1), c, at(0, 1, 1), at(0, 2, 2), p0, v0);
+        pv(at(0, -1, 2), at(0, 0, 1), at(0, 1, 0), at(0, 2, -1), p1, v1);
+        p[4] = blend2(p0, v0, p1, v1, opt);
+        pv(at(-1, -1, 0), c, at(1, 1, 0), at(2, 2, 0), p0, v0);
+        pv(at(2, -1, 0), at(1, 0, 0), at(0, 1, 0), at(-1, 2, 0), p1, v1);
+        p[5] = blend2(p0, v0, p1, v1, opt);
+    }
     pv(at(-1, -1, -1), c, at(1, 1, 1), at(2, 2, 2), p0, v0);
     pv(at(-1, -1, 2), at(0, 0, 1), at(1, 1, 0), at(2, 2, -1), p1, v1);
     pv(at(2, -1, -1), at(1, 0, 0), at(0, 1, 1), at(-1, 2, 2), p2, v2);
     pv(at(-1, 2, -1), at(0, 1, 0), at(1, 0, 1), at(2, -1, 2), p3, v3);
-    p[6] = blend4(p0, v0, p1, v1, p2, v2, p3, v3);
+    if (opt & kPredCubeAxis) p[6] = cube7<T>(A, p0, v0, p1, v1, p2, v2, p3, v3);
+    else p[6] = blend4(p0, v0, p1, v1, p2, v2, p3, v3, opt);
 }
 
 // global-memory form: `row` = block grid row of the point (wave-uniform in
 // the tile kernels), xo = the point's byte offset in it
 template <typename T>
-__device__ __forceinline__ void predict7_interior(const T* __restrict__ row, uint32_t xo, int sx, int sxy, T p[7])
+__device__ __forceinline__ void predict7_interior(const T* __restrict__ row, uint32_t xo, int sx, int sxy, T p[7],
+                                                  int opt)
 {
     auto at = [&](int dy, int dz, int k) -> T { return ldx(row + (ptrdiff_t)dz * sxy + (ptrdiff_t)dy * sx, xo, k); };
-    predict7_at<T>(at, p);
+    predict7_at<T>(at, p, opt);
 }
 
 // all 7 predictions at (x,y,z): the interior form when `inner` (wave-uniform), else checked
 template <typename T>
 __device__ __forceinline__ void predict7(bool inner, const T* __restrict__ r, int x, int y, int z, int dx, int dy,
-                                         int dz, T p[7])
+                                         int dz, T p[7], int opt)
 {
     if (inner) {
-        predict7_interior(r + ((size_t)z * dy + y) * dx, (uint32_t)x * (uint32_t)sizeof(T), dx, dx * dy, p);
+        predict7_interior(r + ((size_t)z * dy + y) * dx, (uint32_t)x * (uint32_t)sizeof(T), dx, dx * dy, p, opt);
     } else {
         const int i = (z * dy + y) * dx + x;
-        p[0] = predict<0>(r, i, x, y, z, dx, dy, dz);
-        p[1] = predict<1>(r, i, x, y, z, dx, dy, dz);
-        p[2] = predict<2>(r, i, x, y, z, dx, dy, dz);
-        p[3] = predict<3>(r, i, x, y, z, dx, dy, dz);
-        p[4] = predict<4>(r, i, x, y, z, dx, dy, dz);
-        p[5] = predict<5>(r, i, x, y, z, dx, dy, dz);
-        p[6] = predict<6>(r, i, x, y, z, dx, dy, dz);
+        p[0] = predict<0>(r, i, x, y, z, dx, dy, dz, opt);
+        p[1] = predict<1>(r, i, x, y, z, dx, dy, dz, opt);
+        p[2] = predict<2>(r, i, x, y, z, dx, dy, dz, opt);
+        p[3] = predict<3>(r, i, x, y, z, dx, dy, dz, opt);
+        p[4] = predict<4>(r, i, x, y, z, dx, dy, dz, opt);
+        p[5] = predict<5>(r, i, x, y, z, dx, dy, dz, opt);
+        p[6] = predict<6>(r, i, x, y, z, dx, dy, dz, opt);
     }
+}
+
+// Level kernels are instantiated per mode; a mode with kVarBit set is the
+// instantiation for payloads with predictor options, and reads them from
+// the level arguments (the default instantiations see a constant 0).
+constexpr int kVarBit = 8;
+template <int kMode>
+__device__ __forceinline__ int mode_opt(const LevelArgs& a)
+{
+    return (kMode & kVarBit) ? a.opt : 0;
 }
 
 // Work split of a block grid: tiles cover x in [1, dx-3]; the edge columns
@@ -227,10 +249,11 @@ __device__ __forceinline__ void high_enc_fin(const T p[7], const T* __restrict__
 template <typename T>
 __device__ __forceinline__ void high_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ mid,
                                                const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
-                                               T* __restrict__ side, int x, int y, int z, bool inner, uint32_t q[7])
+                                               T* __restrict__ side, int x, int y, int z, bool inner, uint32_t q[7],
+                                               int opt)
 {
     T p[7];
-    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p, opt);
     high_enc_fin(p, full, px, py, g, a, codes, side, x, y, z, q);
 }
 
@@ -261,10 +284,10 @@ template <typename T>
 __device__ __forceinline__ void low_enc_point(const T* __restrict__ full, int px, int py, const T* __restrict__ base,
                                               const Geom& g, const LevelArgs& a, uint16_t* __restrict__ codes,
                                               T* __restrict__ side, T* __restrict__ mid, int x, int y, int z,
-                                              bool inner, bool vec, uint32_t q[7])
+                                              bool inner, bool vec, uint32_t q[7], int opt)
 {
     T p[7];
-    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p, opt);
     const T c = ldx(base + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
     low_enc_fin(p, c, full, px, py, g, a, codes, side, mid, x, y, z, vec, q);
 }
@@ -324,10 +347,11 @@ __device__ __forceinline__ void low_dec_fin(const T p[7], T centre, const Geom& 
 template <typename T>
 __device__ __forceinline__ void low_dec_point(const T* __restrict__ base, const Geom& g, const LevelArgs& a,
                                               const uint16_t* __restrict__ codes, const T* __restrict__ side,
-                                              T* __restrict__ mid, int x, int y, int z, bool inner, bool vec)
+                                              T* __restrict__ mid, int x, int y, int z, bool inner, bool vec,
+                                              int opt)
 {
     T p[7];
-    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(inner, base, x, y, z, g.dx, g.dy, g.dz, p, opt);
     const T c = ldx(base + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
     low_dec_fin(p, c, g, a, load_codes7(a, codes, stream_pos(g, x, y, z)), side, mid, x, y, z, vec);
 }
@@ -373,11 +397,11 @@ template <typename T>
 __device__ __forceinline__ void high_dec_point(const T* __restrict__ mid, const Geom& g, const LevelArgs& a,
                                                const uint16_t* __restrict__ codes, const T* __restrict__ side,
                                                T* __restrict__ out, int nx, int ny, int nz, int x, int y, int z,
-                                               bool inner, bool vec)
+                                               bool inner, bool vec, int opt)
 {
     if (2 * x >= nx || 2 * y >= ny || 2 * z >= nz) return;
     T p[7];
-    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(inner, mid, x, y, z, g.dx, g.dy, g.dz, p, opt);
     const T c = ldx(mid + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
     high_dec_fin(p, c, g, a, load_codes7(a, codes, stream_pos(g, x, y, z)), side, out, nx, ny, nz, x, y, z, vec);
 }
@@ -463,10 +487,10 @@ template <typename T, int kMode>
 __device__ __forceinline__ void z_point(const LevelK<T>& K, const T p[7], T c, int x, int y, int z, bool vec,
                                         uint32_t q[7], Codes7 cq, const T* dq = nullptr)
 {
-    if constexpr (kMode == kHighEnc) high_enc_fin(p, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, x, y, z, q);
-    else if constexpr (kMode == kLowEnc)
+    if constexpr ((kMode & 7) == kHighEnc) high_enc_fin(p, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, x, y, z, q);
+    else if constexpr ((kMode & 7) == kLowEnc)
         low_enc_fin(p, c, K.full, K.px, K.py, K.g, K.a, K.codes, K.side, K.dst, x, y, z, vec, q);
-    else if constexpr (kMode == kLowDec) low_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, x, y, z, vec, dq);
+    else if constexpr ((kMode & 7) == kLowDec) low_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, x, y, z, vec, dq);
     else high_dec_fin(p, c, K.g, K.a, cq, K.side, K.dst, K.nx, K.ny, K.nz, x, y, z, vec && K.vec != 0, dq);
 }
 
@@ -474,7 +498,7 @@ __device__ __forceinline__ void z_point(const LevelK<T>& K, const T p[7], T c, i
 template <typename T, int kMode>
 __device__ __forceinline__ Codes7 point_codes(const LevelK<T>& K, int x, int y, int z)
 {
-    if constexpr (kMode == kLowDec || kMode == kHighDec) return load_codes7(K.a, K.codes, stream_pos(K.g, x, y, z));
+    if constexpr ((kMode & 7) == kLowDec || (kMode & 7) == kHighDec) return load_codes7(K.a, K.codes, stream_pos(K.g, x, y, z));
     else return Codes7{};
 }
 
@@ -482,11 +506,11 @@ template <typename T, int kMode>
 __device__ __forceinline__ void z_point_checked(const LevelK<T>& K, int x, int y, int z, uint32_t q[7])
 {
     const Geom& g = K.g;
-    if constexpr (kMode == kHighDec) {
+    if constexpr ((kMode & 7) == kHighDec) {
         if (2 * x >= K.nx || 2 * y >= K.ny || 2 * z >= K.nz) return;
     }
     T p[7];
-    predict7(false, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(false, K.ref, x, y, z, g.dx, g.dy, g.dz, p, mode_opt<kMode>(K.a));
     const T c = K.ref[((size_t)z * g.dy + y) * g.dx + x];
     z_point<T, kMode>(K, p, c, x, y, z, false, q, point_codes<T, kMode>(K, x, y, z));
 }
@@ -502,7 +526,7 @@ __device__ __forceinline__ void z_point_checked(const LevelK<T>& K, int x, int y
 template <int kMode>
 constexpr int tile_rows()
 {
-    return (kMode == kHighEnc || kMode == kHighDec) ? 3 : 1;
+    return ((kMode & 7) == kHighEnc || (kMode & 7) == kHighDec) ? 3 : 1;
 }
 
 template <int kTW>
@@ -521,11 +545,11 @@ template <typename T, int kMode>
 __device__ __forceinline__ void tile_point(const LevelK<T>& K, int x, int y, int z, uint32_t q[7])
 {
     const Geom& g = K.g;
-    if constexpr (kMode == kHighDec) {
+    if constexpr ((kMode & 7) == kHighDec) {
         if (2 * x >= K.nx || 2 * y >= K.ny || 2 * z >= K.nz) return;
     }
     T p[7];
-    predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p);
+    predict7(true, K.ref, x, y, z, g.dx, g.dy, g.dz, p, mode_opt<kMode>(K.a));
     const T c = ldx(K.ref + ((size_t)z * g.dy + y) * g.dx, (uint32_t)x * (uint32_t)sizeof(T), 0);
     z_point<T, kMode>(K, p, c, x, y, z, true, q, point_codes<T, kMode>(K, x, y, z));
 }
@@ -533,7 +557,7 @@ __device__ __forceinline__ void tile_point(const LevelK<T>& K, int x, int y, int
 template <typename T, int kMode>
 __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
 {
-    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr bool kEnc = (kMode & 7) == kLowEnc || (kMode & 7) == kHighEnc;
     __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
     uint32_t* hist = K.hist;
     if constexpr (kEnc) {
@@ -551,7 +575,7 @@ __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
     // the base-grid planes a tile's stencil shares with the next ones stay
     // in that XCD's L2 (low encode 104 -> 94 us at 512^3; the high encode
     // measured slower that way, 416 -> 447 us)
-    constexpr bool kSweep = kMode == kLowEnc;
+    constexpr bool kSweep = (kMode & 7) == kLowEnc;
     const uint32_t G = gridDim.x;
     const bool sweep = kSweep && (G % 8u) == 0;
     const uint64_t per = sweep ? (total + 7) / 8 : total;
@@ -602,7 +626,7 @@ __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
 template <typename T, int kMode>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_level_z(const LevelK<T> K)
 {
-    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr bool kEnc = (kMode & 7) == kLowEnc || (kMode & 7) == kHighEnc;
     __shared__ uint32_t sh[kEnc ? 7 : 1][kEnc ? kHW : 1];
     __shared__ T ring[kZSlots * kZPlane];
     __shared__ T dq[kEnc ? 1 : kDqW];
@@ -696,7 +720,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                         const int o = (y - y0 + 1) * kZPitch + lane + 1;
                         auto at = [&](int dy, int dz, int k) -> T { return S[dz + 1][o + dy * kZPitch + k]; };
                         T p[7];
-                        predict7_at<T>(at, p);
+                        predict7_at<T>(at, p, mode_opt<kMode>(K.a));
                         z_point<T, kMode>(K, p, S[1][o], x, y, z, true, q, cq[r], kEnc ? nullptr : dq);
                     }
                 } else if (ok) {
@@ -740,7 +764,7 @@ __device__ __forceinline__ uint32_t find_job(const BoxJob* __restrict__ jobs, ui
 
 // Random access: reference values of sub-block 0 on boxes of the mid grid
 // (F5:2508-2538, low depreprocess restricted to the regions + gather)
-template <typename T>
+template <typename T, bool kVar>
 __global__ __launch_bounds__(256) void k_low_decode_boxes(const T* __restrict__ base, Geom g, LevelArgs a,
                                                           const uint16_t* __restrict__ codes,
                                                           const T* __restrict__ side, T* __restrict__ mid,
@@ -766,14 +790,14 @@ __global__ __launch_bounds__(256) void k_low_decode_boxes(const T* __restrict__ 
             const uint64_t e = ra_index(a, g, b, x, y, z);
             const uint32_t q = codes[e];
             const T r = q ? dequantize<T>(q, a.eb) : side[e];
-            v = r + predict_rt(b, base, i, x, y, z, g.dx, g.dy, g.dz);
+            v = r + predict_rt(b, base, i, x, y, z, g.dx, g.dy, g.dz, kVar ? a.opt : 0);
         }
         mid[((size_t)Z * my + Y) * mx + X] = v;
     }
 }
 
 // Random access: full-resolution values of query boxes (F5:2574-2605)
-template <typename T>
+template <typename T, bool kVar>
 __global__ __launch_bounds__(256) void k_high_decode_boxes(const T* __restrict__ mid, Geom g, LevelArgs a,
                                                            const uint16_t* __restrict__ codes,
                                                            const T* __restrict__ side,
@@ -800,7 +824,7 @@ __global__ __launch_bounds__(256) void k_high_decode_boxes(const T* __restrict__
             const uint64_t e = ra_index(a, g, b, x, y, z);
             const uint32_t q = codes[e];
             const T r = q ? dequantize<T>(q, a.eb) : side[e];
-            v = r + predict_rt(b, mid, i, x, y, z, g.dx, g.dy, g.dz);
+            v = r + predict_rt(b, mid, i, x, y, z, g.dx, g.dy, g.dz, kVar ? a.opt : 0);
         }
         out[k] = v;
     }
@@ -827,26 +851,27 @@ template <typename T, int kMode>
 __device__ __forceinline__ void level_point(const LevelJob& J, uint32_t (*sh)[kHW], int x, int y, int z, bool ok,
                                             bool inner, bool vec)
 {
-    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr bool kEnc = (kMode & 7) == kLowEnc || (kMode & 7) == kHighEnc;
     const Geom& g = J.g;
     if constexpr (kEnc) {
         uint32_t q[7] = {0, 0, 0, 0, 0, 0, 0};
         if (ok) {
-            if (kMode == kHighEnc)
+            if ((kMode & 7) == kHighEnc)
                 high_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side, x, y, z,
-                               inner, q);
+                               inner, q, mode_opt<kMode>(J.a));
             else
                 low_enc_point((const T*)J.full, J.px, J.py, (const T*)J.ref, g, J.a, J.codes, (T*)J.side,
-                              (T*)J.mid, x, y, z, inner, vec, q);
+                              (T*)J.mid, x, y, z, inner, vec, q, mode_opt<kMode>(J.a));
         }
         uint32_t* hist = J.hist;
         STZ_HIST_ADD7(q, ok);
     } else if (ok) {
-        if (kMode == kHighDec)
+        if ((kMode & 7) == kHighDec)
             high_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.out, J.nx, J.ny, J.nz, x, y, z,
-                           inner, vec && J.vec != 0);
+                           inner, vec && J.vec != 0, mode_opt<kMode>(J.a));
         else
-            low_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.mid, x, y, z, inner, vec);
+            low_dec_point((const T*)J.ref, g, J.a, J.codes, (const T*)J.side, (T*)J.mid, x, y, z, inner, vec,
+                          mode_opt<kMode>(J.a));
     }
 }
 
@@ -854,7 +879,7 @@ template <typename T, int kMode, int kWaves>
 __global__ __launch_bounds__(64 * kWaves) void k_level_multi(const LevelJob* __restrict__ jobs, uint32_t njobs,
                                                              uint32_t total, uint32_t* __restrict__ counter)
 {
-    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr bool kEnc = (kMode & 7) == kLowEnc || (kMode & 7) == kHighEnc;
     constexpr int kThreads = 64 * kWaves;
     __shared__ uint32_t sh[kEnc ? 7 : 1][kHW];
     __shared__ uint32_t item_s;
@@ -955,7 +980,7 @@ constexpr bool kZMarchDec = true;
 template <typename T, int kMode>
 void launch_level(hipStream_t s, const LevelK<T>& K, float share = 1.0f)
 {
-    constexpr bool kEnc = kMode == kLowEnc || kMode == kHighEnc;
+    constexpr bool kEnc = (kMode & 7) == kLowEnc || (kMode & 7) == kHighEnc;
     if ((kEnc && kZMarchEnc) || (!kEnc && kZMarchDec)) launch_level_z<T, kMode>(s, K, share);
     else launch_level_tile<T, kMode>(s, K, share);
 }
@@ -998,7 +1023,8 @@ void launch_low_encode(hipStream_t s, const T* full, int px, int py, int pz, con
     LevelK<T> K{};
     K.full = full; K.px = px; K.py = py; K.ref = base; K.g = g; K.a = a;
     K.codes = codes; K.side = side; K.hist = hist; K.dst = mid;
-    launch_level<T, kLowEnc>(s, K);
+    if (a.opt) launch_level<T, kLowEnc | kVarBit>(s, K);
+    else launch_level<T, kLowEnc>(s, K);
 }
 
 template <typename T>
@@ -1009,7 +1035,8 @@ void launch_high_encode(hipStream_t s, const T* full, int px, int py, int pz, co
     LevelK<T> K{};
     K.full = full; K.px = px; K.py = py; K.ref = mid; K.g = g; K.a = a;
     K.codes = codes; K.side = side; K.hist = hist;
-    launch_level<T, kHighEnc>(s, K);
+    if (a.opt) launch_level<T, kHighEnc | kVarBit>(s, K);
+    else launch_level<T, kHighEnc>(s, K);
 }
 
 template <typename T>
@@ -1018,7 +1045,8 @@ void launch_low_decode(hipStream_t s, const T* base, Geom g, LevelArgs a, const 
 {
     LevelK<T> K{};
     K.ref = base; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = mid;
-    launch_level<T, kLowDec>(s, K, share);
+    if (a.opt) launch_level<T, kLowDec | kVarBit>(s, K, share);
+    else launch_level<T, kLowDec>(s, K, share);
 }
 
 template <typename T>
@@ -1029,7 +1057,8 @@ void launch_high_decode(hipStream_t s, const T* mid, Geom g, LevelArgs a, const 
     K.ref = mid; K.g = g; K.a = a; K.codes = (uint16_t*)codes; K.side = (T*)side; K.dst = out;
     K.nx = nx; K.ny = ny; K.nz = nz;
     K.vec = ((nx & 1) == 0) && (((uintptr_t)out % (2 * sizeof(T))) == 0);
-    launch_level<T, kHighDec>(s, K, share);
+    if (a.opt) launch_level<T, kHighDec | kVarBit>(s, K, share);
+    else launch_level<T, kHighDec>(s, K, share);
 }
 
 template <typename T>
@@ -1038,8 +1067,12 @@ void launch_low_decode_boxes(hipStream_t s, const T* base, Geom g, LevelArgs a, 
                              uint32_t nblocks)
 {
     if (!njobs || !nblocks) return;
-    k_low_decode_boxes<T><<<dim3(nblocks), dim3(256), 0, s>>>(base, g, a, codes, side, mid, mx, my, jobs, njobs,
-                                                              nblocks);
+    if (a.opt)
+        k_low_decode_boxes<T, true><<<dim3(nblocks), dim3(256), 0, s>>>(base, g, a, codes, side, mid, mx, my, jobs,
+                                                                        njobs, nblocks);
+    else
+        k_low_decode_boxes<T, false><<<dim3(nblocks), dim3(256), 0, s>>>(base, g, a, codes, side, mid, mx, my, jobs,
+                                                                         njobs, nblocks);
 }
 
 template <typename T>
@@ -1047,7 +1080,11 @@ void launch_high_decode_boxes(hipStream_t s, const T* mid, Geom g, LevelArgs a, 
                               const T* side, const BoxJob* jobs, uint32_t njobs, uint32_t nblocks)
 {
     if (!njobs || !nblocks) return;
-    k_high_decode_boxes<T><<<dim3(nblocks), dim3(256), 0, s>>>(mid, g, a, codes, side, jobs, njobs, nblocks);
+    if (a.opt)
+        k_high_decode_boxes<T, true><<<dim3(nblocks), dim3(256), 0, s>>>(mid, g, a, codes, side, jobs, njobs, nblocks);
+    else
+        k_high_decode_boxes<T, false><<<dim3(nblocks), dim3(256), 0, s>>>(mid, g, a, codes, side, jobs, njobs,
+                                                                          nblocks);
 }
 
 uint32_t level_tiles(const Geom& g, int waves)
@@ -1063,26 +1100,32 @@ uint32_t level_edge_groups(const Geom& g, int waves)
     return (uint32_t)((n + 64u * waves - 1) / (64u * waves));
 }
 
-template <typename T, int kW>
+template <typename T, int kW, int kV>
 void level_multi_w(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total, uint32_t* counter,
                    uint32_t grid)
 {
     const dim3 G(grid), B(64 * kW);
     switch (mode) {
-    case kLowEnc: k_level_multi<T, kLowEnc, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
-    case kHighEnc: k_level_multi<T, kHighEnc, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
-    case kLowDec: k_level_multi<T, kLowDec, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
-    default: k_level_multi<T, kHighDec, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    case kLowEnc: k_level_multi<T, kLowEnc | kV, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    case kHighEnc: k_level_multi<T, kHighEnc | kV, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    case kLowDec: k_level_multi<T, kLowDec | kV, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
+    default: k_level_multi<T, kHighDec | kV, kW><<<G, B, 0, s>>>(jobs, njobs, total, counter); break;
     }
 }
 
+// var: some job has predictor options (each job reads its own)
 template <typename T>
 void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total_items,
-                        uint32_t* counter, int waves, uint32_t grid)
+                        uint32_t* counter, int waves, uint32_t grid, bool var)
 {
     if (!njobs || !total_items || !grid) return;
-    if (waves == 8) level_multi_w<T, 8>(s, mode, jobs, njobs, total_items, counter, grid);
-    else level_multi_w<T, 4>(s, mode, jobs, njobs, total_items, counter, grid);
+    if (var) {
+        if (waves == 8) level_multi_w<T, 8, kVarBit>(s, mode, jobs, njobs, total_items, counter, grid);
+        else level_multi_w<T, 4, kVarBit>(s, mode, jobs, njobs, total_items, counter, grid);
+    } else {
+        if (waves == 8) level_multi_w<T, 8, 0>(s, mode, jobs, njobs, total_items, counter, grid);
+        else level_multi_w<T, 4, 0>(s, mode, jobs, njobs, total_items, counter, grid);
+    }
 }
 
 #define STZ_INST(T)                                                                                              \
@@ -1099,7 +1142,7 @@ void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t 
     template void launch_high_decode_boxes<T>(hipStream_t, const T*, Geom, LevelArgs, const uint16_t*,         \
                                               const T*, const BoxJob*, uint32_t, uint32_t);                      \
     template void launch_level_multi<T>(hipStream_t, int, const LevelJob*, uint32_t, uint32_t, uint32_t*, int,  \
-                                        uint32_t);
+                                        uint32_t, bool);
 STZ_INST(float)
 STZ_INST(double)
 

@@ -219,6 +219,7 @@ struct LevelArgs {
     const uint32_t* cmap;
     uint32_t cm_off[7];
     uint32_t* err;
+    int opt;                // predictor options (kPred* bits of device_math.h), 0 = F5's default build
 };
 constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
 
@@ -274,9 +275,10 @@ struct LevelJob {
     uint32_t item0, ntiles, nedge, pad_;
 };
 // all jobs of one mode; counter: a zeroed device word (the work queue head)
+// var: some job's level arguments carry predictor options
 template <typename T>
 void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t njobs, uint32_t total_items,
-                        uint32_t* counter, int waves, uint32_t grid);
+                        uint32_t* counter, int waves, uint32_t grid, bool var);
 // work items of one field's level for a given number of waves per block
 uint32_t level_tiles(const Geom& g, int waves);
 uint32_t level_edge_groups(const Geom& g, int waves);

@@ -37,7 +37,8 @@ struct FileHeader {
     uint32_t interp, order, lmax, n_streams;
     double alpha, beta;
     uint64_t total_bytes;
-    uint64_t reserved[3];
+    uint64_t pred;  // predictor options (kPred* bits), 0 = F5's default build
+    uint64_t reserved[2];
     uint64_t stream_off[kStreams];
     uint64_t stream_bytes[kStreams];
     uint8_t pad[48];

@@ -732,7 +732,8 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     const stz_params& prm = fe.prm;
     check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
     if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
-        prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0))
+        prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
+        prm.pred > 7)
         throw StzError(STZ_E_ARG, "bad parameters");
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -805,6 +806,7 @@ LevelArgs enc_level_args(const FieldEnc<T>& fe, int level)
     a.eb = fe.ebl[level];
     a.recip = 1.0 / fe.ebl[level];
     for (int b = 0; b < 7; ++b) a.code_off[b] = fe.L.code_off[(level == 1 ? 1 : 8) + b];
+    a.opt = fe.prm.pred;
     return a;
 }
 
@@ -833,6 +835,7 @@ void enc_phase3_headers(FieldEnc<T>& fe)
     fh.sub_size = kSub;
     fh.interp = (uint32_t)prm.interp;
     fh.order = (uint32_t)prm.order;
+    fh.pred = (uint64_t)prm.pred;
     fh.lmax = kLmax;
     fh.n_streams = kStreams;
     fh.alpha = prm.alpha;
@@ -1152,6 +1155,7 @@ LevelArgs level_args(const Meta& m, const Decoded& dd, int level, uint32_t* err)
     }
     a.cmap = dd.compact ? dd.cmap.as<uint32_t>() : nullptr;
     a.err = err;
+    a.opt = (int)m.fh.pred;
     return a;
 }
 
@@ -1486,8 +1490,10 @@ void jobs_launch(stz_ctx* c, int mode, JobSet& js, const stz_ctx::LaunchCfg& cfg
     K(c, "counter_zero", [&] { launch_fill_u32(c->stream, js.counter.as<uint32_t>(), 1, 0u); });
     K(c, mode == kLowEnc ? "low_encode_multi" : mode == kHighEnc ? "high_encode_multi"
          : mode == kLowDec ? "low_decode_multi" : "high_decode_multi", [&] {
+        bool var = false;
+        for (const LevelJob& j : js.h[v]) var = var || j.a.opt != 0;
         launch_level_multi<T>(c->stream, mode, js.d[v].as<LevelJob>(), (uint32_t)js.h[v].size(), js.total[v],
-                              js.counter.as<uint32_t>(), cfg.waves, cfg.grid);
+                              js.counter.as<uint32_t>(), cfg.waves, cfg.grid, var);
     });
 }
 
@@ -2281,6 +2287,7 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     std::memcpy(&m.fh, p, sizeof(FileHeader));
     const FileHeader& fh = m.fh;
     if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
+        fh.pred > 7 ||
         (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
         throw StzError(STZ_E_STREAM, "bad payload header");
     if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))
@@ -2619,6 +2626,7 @@ void stz_default_params(stz_params* p)
     p->beta = 1.0;
     p->zstd_level = 3;    // F5:3048 (SZ3 default)
     p->tuner = 0;
+    p->pred = 0;          // F5's default build: no PRED_* option
 }
 
 const char* stz_strerror(int code)
@@ -2958,6 +2966,7 @@ int stz_dstream_info(const stz_dstream* ds, stz_info* info)
     info->interp = (int)fh.interp; info->order = (int)fh.order;
     info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
     info->alpha = fh.alpha; info->beta = fh.beta;
+    info->pred = (int)fh.pred;
     return STZ_OK;
 }
 
@@ -3119,6 +3128,7 @@ int stz_query_info(const void* stream, size_t bytes, stz_info* info)
         info->interp = (int)fh.interp; info->order = (int)fh.order;
         info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
         info->alpha = fh.alpha; info->beta = fh.beta;
+        info->pred = (int)fh.pred;
     });
 }
 

@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("chunk_high", C.c_int), ("chunk_low", C.c_int),
         ("interp", C.c_int), ("order", C.c_int),
         ("alpha", C.c_double), ("beta", C.c_double),
-        ("zstd_level", C.c_int), ("tuner", C.c_int),
+        ("zstd_level", C.c_int), ("tuner", C.c_int), ("pred", C.c_int),
     ]
 
 
@@ -68,6 +68,7 @@ class Info(C.Structure):
         ("interp", C.c_int), ("order", C.c_int),
         ("mult_base", C.c_double), ("mult_low", C.c_double), ("mult_high", C.c_double),
         ("alpha", C.c_double), ("beta", C.c_double),
+        ("pred", C.c_int),
     ]
 
 

@@ -8,6 +8,11 @@ Covered (F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5
   slice_full_data / merge_sub_blocks_to_full          F5:68-122
   make_query + build_residual_plan + build_low_residual_plan   F5:1602-1830
 
+and tests/golden/ref_pred_vectors.npz: preprocess_block_zslab of F5 built
+with its predictor switches PRED_WPOW2 (F5:177-203), PRED_PLANE_AXIS
+(F5:579-607, F5:627-633) and PRED_CUBE_AXIS (F5:609-622, F5:675-686), one
+build each and one with all three (oracle/_ref/libstz_ref_<name>.so).
+
 Run in a container that holds the reference sources (after build_ref.sh):
     python tests/golden/make_golden.py
 The vectors are data only; nothing of the reference's source is stored.
@@ -133,5 +138,41 @@ def main():
     print("wrote %s (%d arrays, %d bytes)" % (OUT, len(out), os.path.getsize(OUT)))
 
 
+PRED_OUT = os.path.join(HERE, "ref_pred_vectors.npz")
+PRED_SHAPES = [(6, 7, 8), (9, 7, 11), (12, 10, 14)]   # (dz, dy, dx)
+
+
+def pred_inputs(rng, shape, dt):
+    """Rough references with a zero plane and scaled columns (wide variance
+    ranges in the inverse-variance weights)."""
+    r = np.cumsum(rng.normal(size=shape), axis=1)
+    r[:, :, 2::5] *= 40.0
+    s = r + rng.normal(scale=0.3, size=shape)
+    r = r.astype(dt)
+    s = s.astype(dt)
+    r[0] = 0
+    return s, r
+
+
+def main_pred():
+    out = {}
+    for name in ("wpow2", "plane", "cube", "all"):
+        R = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libstz_ref_%s.so" % name))
+        rng = np.random.default_rng(2024)
+        for dt, dtn in ((np.float32, "f32"), (np.float64, "f64")):
+            for si, shape in enumerate(PRED_SHAPES):
+                dz, dy, dx = shape
+                sub, ref = pred_inputs(rng, shape, dt)
+                out["%s-%s-%d-sub" % (name, dtn, si)] = sub
+                out["%s-%s-%d-ref" % (name, dtn, si)] = ref
+                for b in (2, 4, 5, 6):
+                    d = np.empty_like(sub)
+                    getattr(R, "ref_preprocess_" + dtn)(b, _p(sub), _p(ref), _p(d), dx, dy, dz)
+                    out["%s-%s-%d-%d_out" % (name, dtn, si, b)] = d
+    np.savez_compressed(PRED_OUT, **out)
+    print("wrote %s (%d arrays, %d bytes)" % (PRED_OUT, len(out), os.path.getsize(PRED_OUT)))
+
+
 if __name__ == "__main__":
     main()
+    main_pred()

@@ -32,6 +32,11 @@ def gold():
     return np.load(GOLD_PATH)
 
 
+@pytest.fixture(scope="module")
+def pred_gold():
+    return np.load(os.path.join(ROOT, "tests", "golden", "ref_pred_vectors.npz"))
+
+
 def _bits(a):
     a = np.ascontiguousarray(a)
     return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
@@ -123,6 +128,65 @@ def test_reference_predictors_live(oracle, ref, dt):
             assert np.array_equal(_bits(oracle.preprocess(b, s, r)), _bits(out)), (shape, b)
             getattr(ref, "ref_depreprocess_" + suf)(b, _p(s), _p(r), _p(out), dx, dy, dz)
             assert np.array_equal(_bits(oracle.depreprocess(b, s, r)), _bits(out)), (shape, b)
+
+
+# F5 built with its predictor switches (oracle/build_ref.sh): name -> Params.pred bits
+PRED_VARIANTS = {"wpow2": 1, "plane": 2, "cube": 4, "all": 7}
+
+
+def _pred_inputs(rng, shape, dt):
+    """Rough references with a zero plane and a scaled row: every boundary
+    rule, the flush-to-zero mode and wide variance ranges in the weights."""
+    r = np.cumsum(rng.normal(size=shape), axis=1)
+    r[:, :, 2::5] *= 40.0
+    s = r + rng.normal(scale=0.3, size=shape)
+    r = r.astype(dt)
+    s = s.astype(dt)
+    r[0] = 0
+    return s, r
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("name", sorted(PRED_VARIANTS))
+def test_reference_predictor_options_live(oracle, dt, name):
+    """The oracle's PRED_WPOW2 / PRED_PLANE_AXIS / PRED_CUBE_AXIS predictors
+    against F5 compiled with those switches, bit for bit, all 7 blocks (the
+    switches change blocks 2, 4, 5 and 6 only)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libstz_ref_%s.so" % name)
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    R = C.CDLL(path)
+    suf = "f32" if dt == np.float32 else "f64"
+    bits = PRED_VARIANTS[name]
+    rng = np.random.default_rng(91 + bits)
+    for shape in [(17, 9, 33), (6, 7, 8), (20, 20, 20), (1, 3, 4), (4, 4, 4)]:
+        dz, dy, dx = shape
+        s, r = _pred_inputs(rng, shape, dt)
+        out = np.empty_like(s)
+        for b in range(7):
+            getattr(R, "ref_preprocess_" + suf)(b, _p(s), _p(r), _p(out), dx, dy, dz)
+            got = oracle.preprocess(b, s, r, pred=bits)
+            assert np.array_equal(_bits(got), _bits(out)), (name, shape, b)
+            getattr(R, "ref_depreprocess_" + suf)(b, _p(s), _p(r), _p(out), dx, dy, dz)
+            assert np.array_equal(_bits(oracle.depreprocess(b, s, r, pred=bits)), _bits(out)), (name, shape, b)
+        # the option changes something: blocks 2/4/5 (plane, wpow2) or 6 (cube, wpow2)
+        if shape == (20, 20, 20):
+            changed = [b for b in range(7) if not np.array_equal(_bits(oracle.preprocess(b, s, r, pred=bits)),
+                                                                  _bits(oracle.preprocess(b, s, r)))]
+            want = {1: [2, 4, 5, 6], 2: [2, 4, 5], 4: [6], 7: [2, 4, 5, 6]}[bits]
+            assert changed == want, (name, changed)
+
+
+def test_golden_predictor_options(oracle, pred_gold):
+    """The same against the committed vectors of those reference builds."""
+    for key in pred_gold.files:
+        if not key.endswith("_out"):
+            continue
+        name, dtn, si, b = key[:-4].split("-")
+        s = pred_gold["%s-%s-%s-sub" % (name, dtn, si)]
+        r = pred_gold["%s-%s-%s-ref" % (name, dtn, si)]
+        got = oracle.preprocess(int(b), s, r, pred=PRED_VARIANTS[name])
+        assert np.array_equal(_bits(got), _bits(pred_gold[key])), key
 
 
 def test_reference_planner_live(oracle, ref):
