@@ -63,13 +63,16 @@ typedef struct {
     int pred;                              /* STZ_PRED_* bits: F5's predictor build options */
 } stz_params;
 
-/* F5's compile-time predictor options as run-time bits of stz_params.pred
- * (0 = F5's default build).  They change the interior predictions of the
- * plane blocks (2, 4, 5) and the cube block (6), and are written to the
- * stream header, which the decoder follows. */
+/* Predictor options as run-time bits of stz_params.pred (0 = F5's default
+ * build): F5's compile-time options, and stz2.cpp's static predictors.
+ * They change the interior predictions of the plane blocks (2, 4, 5) and
+ * the cube block (6), and are written to the stream header, which the
+ * decoder follows. */
 #define STZ_PRED_WPOW2 1      /* inverse-square variance weights (F5:177-203) */
 #define STZ_PRED_PLANE_AXIS 2 /* two axis-pair candidates more for blocks 2, 4, 5 (F5:579-607, F5:627-633) */
 #define STZ_PRED_CUBE_AXIS 4  /* three axis-pair candidates more for block 6 (F5:609-622, F5:675-686) */
+#define STZ_PRED_BICUBIC12 8  /* blocks 2, 4, 5: 12-point truncated bicubic (stz2.cpp:122-155, 186-199, 222-235) */
+#define STZ_PRED_DIAG16 16    /* block 6: 16-point static predictor (stz2.cpp:271-293) */
 
 typedef struct {
     size_t nx, ny, nz;        /* caller's dims, x fastest */

@@ -7,6 +7,9 @@
 #   F5:24-1253   globals, split/merge, predictors, pre/de-preprocess, quantizer
 #   F5:1577-1839 ROI / slice planners
 #
+# oracle/_ref/libstz_ref_stz2.so: preprocess_block of src/stz2.cpp (its lines
+# 1-8 and 82-306, no SZ3 symbol) with ref2_harness.cpp.
+#
 # F5:18-22 (#include of the SZ3-ncp headers) and every function that calls
 # SZ3-ncp (F5:1255-1576, F5:1841-) are left out: SZ3-ncp is not in this image,
 # so the entropy-coding half of the reference cannot be built here.  The lines
@@ -38,4 +41,12 @@ build libstz_ref_wpow2.so -DPRED_WPOW2 & p1=$!
 build libstz_ref_plane.so -DPRED_PLANE_AXIS & p2=$!
 build libstz_ref_cube.so -DPRED_CUBE_AXIS & p3=$!
 build libstz_ref_all.so -DPRED_WPOW2 -DPRED_PLANE_AXIS -DPRED_CUBE_AXIS & p4=$!
-for p in $p0 $p1 $p2 $p3 $p4; do wait $p || exit 1; done
+# stz2.cpp's static plane / cube predictors (preprocess_block, stz2.cpp:82-306)
+STZ2="$REFROOT/src/stz2.cpp"
+{
+    echo "#line 1 \"stz2.cpp\""; sed -n '1,8p' "$STZ2"
+    echo "#line 82 \"stz2.cpp\""; sed -n '82,306p' "$STZ2"
+    echo "#line 1 \"ref2_harness.cpp\""; cat "$HERE/ref2_harness.cpp"
+} | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w -o "$HERE/_ref/libstz_ref_stz2.so" - & p5=$!
+for p in $p0 $p1 $p2 $p3 $p4 $p5; do wait $p || exit 1; done
+echo "built $HERE/_ref/libstz_ref_stz2.so"

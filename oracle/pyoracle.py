@@ -28,8 +28,9 @@ class Params(C.Structure):
         ("pred", C.c_int),
     ]
 
-# F5's predictor build options as bits of Params.pred (PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS)
-PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS = 1, 2, 4
+# predictor options as bits of Params.pred: F5's PRED_WPOW2, PRED_PLANE_AXIS,
+# PRED_CUBE_AXIS; stz2.cpp's 12-point bicubic (blocks 2/4/5) and 16-point (block 6)
+PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS, PRED_BICUBIC12, PRED_DIAG16 = 1, 2, 4, 8, 16
 
 
 def build():

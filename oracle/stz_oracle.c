@@ -513,7 +513,7 @@ static int ora_open_view(ora_view* v, const uint8_t* p, size_t bytes)
     if (bytes < sizeof(stzf_file_header)) return -1;
     const stzf_file_header* fh = (const stzf_file_header*)p;
     if (fh->magic != STZF_MAGIC || fh->version != STZF_VERSION || fh->n_streams != STZF_NSTREAMS ||
-        fh->pred > 7)
+        fh->pred > ORA_PRED_ALL)
         return -1;
     if (fh->total_bytes > bytes) return -1;
     v->base = p;

@@ -37,10 +37,15 @@ typedef struct {
     int pred;                                /* ORA_PRED_* bits: F5's compile-time predictor options */
 } ora_params;
 
-/* F5's predictor build options, as run-time bits: PRED_WPOW2 (F5:177-203),
- * PRED_PLANE_AXIS (F5:579-607, F5:627-633), PRED_CUBE_AXIS (F5:609-622,
- * F5:675-686) */
-enum { ORA_PRED_WPOW2 = 1, ORA_PRED_PLANE_AXIS = 2, ORA_PRED_CUBE_AXIS = 4 };
+/* Predictor options, as run-time bits: F5's build options PRED_WPOW2
+ * (F5:177-203), PRED_PLANE_AXIS (F5:579-607, F5:627-633), PRED_CUBE_AXIS
+ * (F5:609-622, F5:675-686); and the static predictors of stz2.cpp for the
+ * plane blocks 2/4/5 (12-point truncated bicubic, stz2.cpp:122-155,
+ * 186-199, 222-235) and the cube block 6 (16 points, stz2.cpp:271-293),
+ * which replace the dynamic ones of those blocks */
+enum { ORA_PRED_WPOW2 = 1, ORA_PRED_PLANE_AXIS = 2, ORA_PRED_CUBE_AXIS = 4, ORA_PRED_BICUBIC12 = 8,
+       ORA_PRED_DIAG16 = 16 };
+#define ORA_PRED_ALL 31
 
 void ora_default_params(ora_params* p);
 

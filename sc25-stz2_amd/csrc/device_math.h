@@ -31,12 +31,6 @@ __device__ __forceinline__ void pv(T a, T b, T c, T d, T& p, T& v)
     p = T(0.5f) * (b + c) - T(0.0625f) * (c0 + c1);
 }
 
-// F5's predictor build options as run-time bits (stz_params.pred, the file
-// header's pred): PRED_WPOW2 squares the inverse-variance weights
-// (F5:177-203); PRED_PLANE_AXIS adds two axis-pair candidates to blocks 2, 4
-// and 5 (F5:579-607, F5:627-633); PRED_CUBE_AXIS adds three to block 6
-// (F5:609-622, F5:675-686).  0 is F5's default build.
-constexpr int kPredWpow2 = 1, kPredPlaneAxis = 2, kPredCubeAxis = 4;
 
 // F5:177-186
 template <typename T>
@@ -135,6 +129,31 @@ __device__ __forceinline__ T cube7(const A3& A, T p0, T v0, T p1, T v1, T p2, T 
            (w0 + w1 + w2 + w3 + w4 + w5 + w6);
 }
 
+// stz2.cpp:122-155 (block 2; 186-199 block 4, 222-235 block 5): the
+// 12-point truncated bicubic of a plane, sums in the order written there;
+// f(o1, o2) as for plane_dyn
+template <typename T, typename F2>
+__device__ __forceinline__ T bicubic12(const F2& f)
+{
+    const T w_core = T(0.31640625f);
+    const T core = f(0, 0) + f(1, 0) + f(0, 1) + f(1, 1);
+    const T w_edge = T(-0.033203125f);
+    const T edge = f(-1, 0) + f(2, 0) + f(-1, 1) + f(2, 1) + f(0, -1) + f(1, -1) + f(0, 2) + f(1, 2);
+    return w_core * core + w_edge * edge;
+}
+
+// stz2.cpp:271-293: block 6 from the 8 cell corners and the 8 points one
+// step beyond them along the body diagonals, in the order written there;
+// A(ox, oy, oz) the value at that offset
+template <typename T, typename A3>
+__device__ __forceinline__ T diag16(const A3& A)
+{
+    const T a = T(0.140625f), b = T(0.015625f);
+    return a * A(0, 0, 0) + a * A(1, 1, 1) + a * A(0, 1, 1) + a * A(1, 0, 1) + a * A(1, 1, 0) + a * A(1, 0, 0) +
+           a * A(0, 1, 0) + a * A(0, 0, 1) - b * A(-1, -1, -1) - b * A(2, 2, 2) - b * A(2, -1, 2) -
+           b * A(-1, 2, 2) - b * A(2, 2, -1) - b * A(-1, -1, 2) - b * A(-1, 2, -1) - b * A(2, -1, -1);
+}
+
 // F5:690-748 predict_block_const + F5:624-688 dynamic lines; interior points
 // evaluate exactly what predict_interior_const (F5:828-840) evaluates.
 template <int B, typename T>
@@ -159,6 +178,7 @@ __device__ __forceinline__ T predict(const T* __restrict__ r, int i, int x, int 
         if (x == dx - 1 || y == dy - 1) return r[i];
         if (y == dy - 2 || y == 0 || x == dx - 2 || x == 0)
             return T(0.25f) * (r[i] + r[i + sx + 1] + r[i + sx] + r[i + 1]);
+        if (opt & kPredBicubic12) return bicubic12<T>([&](int o1, int o2) { return A(o1, o2, 0); });
         if (opt & kPredPlaneAxis) return plane_dyn<T>([&](int o1, int o2) { return A(o1, o2, 0); });
         T p0, v0, p1, v1;
         pv(r[i - sx - 1], r[i], r[i + sx + 1], r[i + 2 * sx + 2], p0, v0);
@@ -168,6 +188,7 @@ __device__ __forceinline__ T predict(const T* __restrict__ r, int i, int x, int 
         if (x == dx - 1 || z == dz - 1) return r[i];
         if (z == dz - 2 || z == 0 || x == dx - 2 || x == 0)
             return T(0.25f) * (r[i] + r[i + sxy + 1] + r[i + 1] + r[i + sxy]);
+        if (opt & kPredBicubic12) return bicubic12<T>([&](int o1, int o2) { return A(o1, 0, o2); });
         if (opt & kPredPlaneAxis) return plane_dyn<T>([&](int o1, int o2) { return A(o1, 0, o2); });
         T p0, v0, p1, v1;
         pv(r[i - sxy - 1], r[i], r[i + sxy + 1], r[i + 2 * sxy + 2], p0, v0);
@@ -177,6 +198,7 @@ __device__ __forceinline__ T predict(const T* __restrict__ r, int i, int x, int 
         if (z == dz - 1 || y == dy - 1) return r[i];
         if (y == dy - 2 || y == 0 || z == dz - 2 || z == 0)
             return T(0.25f) * (r[i] + r[i + sxy + sx] + r[i + sxy] + r[i + sx]);
+        if (opt & kPredBicubic12) return bicubic12<T>([&](int o1, int o2) { return A(0, o1, o2); });
         if (opt & kPredPlaneAxis) return plane_dyn<T>([&](int o1, int o2) { return A(0, o1, o2); });
         T p0, v0, p1, v1;
         pv(r[i - sxy - sx], r[i], r[i + sxy + sx], r[i + 2 * sxy + 2 * sx], p0, v0);
@@ -187,6 +209,7 @@ __device__ __forceinline__ T predict(const T* __restrict__ r, int i, int x, int 
         if (z == dz - 2 || z == 0 || y == dy - 2 || y == 0 || x == dx - 2 || x == 0)
             return T(0.125f) * (r[i] + r[i + sxy + sx + 1] + r[i + sxy + sx] + r[i + sxy + 1] +
                                 r[i + sx + 1] + r[i + 1] + r[i + sx] + r[i + sxy]);
+        if (opt & kPredDiag16) return diag16<T>(A);
         T p0, v0, p1, v1, p2, v2, p3, v3;
         pv(r[i - sxy - sx - 1], r[i], r[i + sxy + sx + 1], r[i + 2 * sxy + 2 * sx + 2], p0, v0);
         pv(r[i - sxy - sx + 2], r[i + 1], r[i + sxy + sx], r[i + 2 * sxy + 2 * sx - 1], p1, v1);

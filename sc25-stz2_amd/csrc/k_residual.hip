@@ -102,7 +102,11 @@ __device__ __forceinline__ void predict7_at(const At& at, T p[7], int opt)
     p[1] = cub(at(-1, 0, 0), c, at(1, 0, 0), at(2, 0, 0));
     p[3] = cub(at(0, -1, 0), c, at(0, 1, 0), at(0, 2, 0));
     T p0, v0, p1, v1, p2, v2, p3, v3;
-    if (opt & kPredPlaneAxis) {
+    if (opt & kPredBicubic12) {
+        p[2] = bicubic12<T>([&](int o1, int o2) { return A(o1, o2, 0); });
+        p[4] = bicubic12<T>([&](int o1, int o2) { return A(o1, 0, o2); });
+        p[5] = bicubic12<T>([&](int o1, int o2) { return A(0, o1, o2); });
+    } else if (opt & kPredPlaneAxis) {
         p[2] = plane_dyn<T>([&](int o1, int o2) { return A(o1, o2, 0); });
         p[4] = plane_dyn<T>([&](int o1, int o2) { return A(o1, 0, o2); });
         p[5] = plane_dyn<T>([&](int o1, int o2) { return A(0, o1, o2); });
@@ -116,6 +120,10 @@ __device__ __forceinline__ void predict7_at(const At& at, T p[7], int opt)
         pv(at(-1, -1, 0), c, at(1, 1, 0), at(2, 2, 0), p0, v0);
         pv(at(2, -1, 0), at(1, 0, 0), at(0, 1, 0), at(-1, 2, 0), p1, v1);
         p[5] = blend2(p0, v0, p1, v1, opt);
+    }
+    if (opt & kPredDiag16) {
+        p[6] = diag16<T>(A);
+        return;
     }
     pv(at(-1, -1, -1), c, at(1, 1, 1), at(2, 2, 2), p0, v0);
     pv(at(-1, -1, 2), at(0, 0, 1), at(1, 1, 0), at(2, 2, -1), p1, v1);

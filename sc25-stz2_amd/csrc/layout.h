@@ -25,6 +25,17 @@ constexpr int kBaseChunk = 32768;   // symbols per base-stream chunk
 constexpr int kStreams = 15;
 constexpr int kLutBits = 12;        // decode table window (bits)
 
+// Predictor options as run-time bits (stz_params.pred, the file header's
+// pred).  F5's build options: PRED_WPOW2 squares the inverse-variance
+// weights (F5:177-203); PRED_PLANE_AXIS adds two axis-pair candidates to
+// blocks 2, 4 and 5 (F5:579-607, F5:627-633); PRED_CUBE_AXIS adds three to
+// block 6 (F5:609-622, F5:675-686).  stz2.cpp's static predictors replace
+// the dynamic ones: the 12-point truncated bicubic for blocks 2, 4, 5
+// (stz2.cpp:122-155, 186-199, 222-235) and 16 points for block 6
+// (stz2.cpp:271-293).  0 is F5's default build.
+constexpr int kPredWpow2 = 1, kPredPlaneAxis = 2, kPredCubeAxis = 4, kPredBicubic12 = 8, kPredDiag16 = 16;
+constexpr int kPredAll = 31;
+
 struct FileHeader {
     uint32_t magic, version, dtype, eb_mode;
     uint64_t nx, ny, nz;

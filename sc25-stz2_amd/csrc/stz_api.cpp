@@ -733,7 +733,7 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
     if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
         prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
-        prm.pred > 7)
+        prm.pred > kPredAll)
         throw StzError(STZ_E_ARG, "bad parameters");
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -2287,7 +2287,7 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     std::memcpy(&m.fh, p, sizeof(FileHeader));
     const FileHeader& fh = m.fh;
     if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
-        fh.pred > 7 ||
+        fh.pred > (uint64_t)kPredAll ||
         (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
         throw StzError(STZ_E_STREAM, "bad payload header");
     if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))

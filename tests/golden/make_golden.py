@@ -11,7 +11,9 @@ Covered (F5 = src/zero_all_line_dynamic_random_access_optfast3_optcr_lb_fastdec5
 and tests/golden/ref_pred_vectors.npz: preprocess_block_zslab of F5 built
 with its predictor switches PRED_WPOW2 (F5:177-203), PRED_PLANE_AXIS
 (F5:579-607, F5:627-633) and PRED_CUBE_AXIS (F5:609-622, F5:675-686), one
-build each and one with all three (oracle/_ref/libstz_ref_<name>.so).
+build each and one with all three (oracle/_ref/libstz_ref_<name>.so); and
+preprocess_block of stz2.cpp (stz2.cpp:82-306, its 12-point truncated
+bicubic and 16-point predictors; oracle/_ref/libstz_ref_stz2.so).
 
 Run in a container that holds the reference sources (after build_ref.sh):
     python tests/golden/make_golden.py
@@ -169,6 +171,18 @@ def main_pred():
                     d = np.empty_like(sub)
                     getattr(R, "ref_preprocess_" + dtn)(b, _p(sub), _p(ref), _p(d), dx, dy, dz)
                     out["%s-%s-%d-%d_out" % (name, dtn, si, b)] = d
+    # stz2.cpp's preprocess_block (f32): its plane and cube predictors
+    R = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libstz_ref_stz2.so"))
+    rng = np.random.default_rng(2025)
+    for si, shape in enumerate(PRED_SHAPES):
+        dz, dy, dx = shape
+        sub, ref = pred_inputs(rng, shape, np.float32)
+        out["stz2-f32-%d-sub" % si] = sub
+        out["stz2-f32-%d-ref" % si] = ref
+        for b in (2, 4, 5, 6):
+            d = np.empty_like(sub)
+            R.ref2_preprocess_f32(b, _p(sub), _p(ref), _p(d), dx, dy, dz)
+            out["stz2-f32-%d-%d_out" % (si, b)] = d
     np.savez_compressed(PRED_OUT, **out)
     print("wrote %s (%d arrays, %d bytes)" % (PRED_OUT, len(out), os.path.getsize(PRED_OUT)))
 

@@ -348,16 +348,18 @@ def test_flipped_bitstream_bit_rejected(ctx):
     caught, total = flips(raw, range(1, 15), (0.1, 0.3, 0.5, 0.7, 0.9))
     assert total == 70 and caught >= 60, (caught, total)
 
-# F5's predictor build options (stz_params.pred): each, and all three
-PRED_OPTS = [1, 2, 4, 7]
+# predictor options (stz_params.pred): F5's three build options each and
+# together; stz2.cpp's plane (8) and cube (16) predictors; everything
+PRED_OPTS = [1, 2, 4, 7, 8, 16, 24, 31]
 
 
 @pytest.mark.parametrize("pred", PRED_OPTS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_predictor_options_bitexact(ctx, oracle, dtype, pred):
-    """PRED_WPOW2 / PRED_PLANE_AXIS / PRED_CUBE_AXIS through the whole path:
-    payload bytes equal the oracle's (whose predictors equal F5 built with
-    those switches), the option lands in the header, decode is bit-exact."""
+    """Predictor options through the whole path: payload bytes equal the
+    oracle's (whose predictors equal F5 built with its PRED_* switches and
+    stz2.cpp's preprocess_block), the option lands in the header, decode is
+    bit-exact."""
     _say("predictor options", pred, np.dtype(dtype).name)
     f = _field("lognormal" if dtype == np.float32 else "grf", (40, 44, 52), dtype, seed=31)
     eb = 1e-3 if dtype == np.float32 else 1e-6
@@ -376,10 +378,10 @@ def test_predictor_options_bitexact(ctx, oracle, dtype, pred):
 
 def test_predictor_options_random_access_and_batch(ctx, oracle):
     """Random access (box, slice, levels) and a batch of uneven fields under
-    all three options: equal to the oracle / to one call per field."""
+    the options: equal to the oracle / to one call per field."""
     import torch
     f = _field("lognormal", (48, 40, 56), np.float32, seed=8)
-    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", pred=7, chunk_high=8, chunk_low=4)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", pred=31, chunk_high=8, chunk_low=4)
     assert first_difference(gpu, want) is None
     full = oracle.decompress(want)
     for q in [(3, 5, 7, 50, 30, 40), (10, 10, 10, 10, 39, 47), (0, 0, 0, 55, 39, 47)]:
@@ -400,14 +402,14 @@ def test_predictor_options_random_access_and_batch(ctx, oracle):
     dev = torch.device("cuda:0")
     fs = [torch.from_numpy(_field("grf", sh, np.float32, seed=40 + i)).to(dev)
           for i, sh in enumerate([(33, 47, 61), (16, 20, 24), (40, 36, 52)])]
-    batch = ctx.compress_batch(fs, 1e-3, "rel", pred=7)
+    batch = ctx.compress_batch(fs, 1e-3, "rel", pred=30)
     outs = [torch.empty_like(t) for t in fs]
     ctx.decompress_batch(batch, outs)
     for t, b, o in zip(fs, batch, outs):
-        s = ctx.compress_device(t, 1e-3, "rel", pred=7)
+        s = ctx.compress_device(t, 1e-3, "rel", pred=30)
         pb, ps = b.download(-1), s.download(-1)
         assert pb.size == ps.size and np.array_equal(pb, ps)
-        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, pred=7)) is None
+        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, pred=30)) is None
         _assert_bitexact(o.cpu().numpy(), s.decompress(), "batch decode")
         s.free()
     for b in batch:

@@ -177,6 +177,60 @@ def test_reference_predictor_options_live(oracle, dt, name):
             assert changed == want, (name, changed)
 
 
+def test_reference_stz2_predictors_live(oracle):
+    """The oracle's stz2-style predictors (option bits 8: 12-point truncated
+    bicubic for blocks 2/4/5; 16: 16 points for block 6) against stz2.cpp's
+    own preprocess_block (stz2.cpp:82-306, built by oracle/build_ref.sh), bit
+    for bit in f32 on blocks 2, 4, 5 and 6 (its blocks 0/1/3 use other
+    weights, which the option leaves at F5's)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libstz_ref_stz2.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    R = C.CDLL(path)
+    rng = np.random.default_rng(5)
+    for shape in [(17, 9, 33), (6, 7, 8), (20, 20, 20), (4, 4, 4)]:
+        dz, dy, dx = shape
+        s, r = _pred_inputs(rng, shape, np.float32)
+        out = np.empty_like(s)
+        for b in (2, 4, 5, 6):
+            R.ref2_preprocess_f32(b, _p(s), _p(r), _p(out), dx, dy, dz)
+            got = oracle.preprocess(b, s, r, pred=8 | 16)
+            assert np.array_equal(_bits(got), _bits(out)), (shape, b)
+        # option 8 alone leaves block 6 dynamic, option 16 alone blocks 2/4/5
+        assert np.array_equal(_bits(oracle.preprocess(6, s, r, pred=8)), _bits(oracle.preprocess(6, s, r)))
+        assert np.array_equal(_bits(oracle.preprocess(2, s, r, pred=16)), _bits(oracle.preprocess(2, s, r)))
+
+
+def test_stz2_predictors_reproduce_polynomials(oracle):
+    """Closed-form checks of the two stz2 predictors on exactly representable
+    data: both are weighted sums with weights summing to 1 (the bicubic's
+    0.31640625 x 4 - 0.033203125 x 8, the 16-point's 0.140625 x 8 -
+    0.015625 x 8), so constants are reproduced; and the values they give at
+    a point equal the closed forms for a quadratic in each axis."""
+    for dt in (np.float32, np.float64):
+        c = np.full((8, 8, 8), 3.0, dt)
+        for b in (2, 4, 5, 6):
+            d = oracle.preprocess(b, c, c, pred=8 | 16)
+            assert np.all(d == 0), (dt, b)
+        z, y, x = np.meshgrid(np.arange(8.0), np.arange(8.0), np.arange(8.0), indexing="ij")
+        ref = (x * x + 2 * y * y + 3 * z * z).astype(dt)
+        sub = np.zeros_like(ref)
+        i = (3, 3, 3)          # interior point (z, y, x)
+        # bicubic12 in the (x, y) plane at target offset (+1/2, +1/2):
+        # core points (0|1, 0|1), edges one step outside on one axis
+        want2 = 0.31640625 * (ref[3, 3, 3] + ref[3, 3, 4] + ref[3, 4, 3] + ref[3, 4, 4]) - 0.033203125 * (
+            ref[3, 3, 2] + ref[3, 3, 5] + ref[3, 4, 2] + ref[3, 4, 5] + ref[3, 2, 3] + ref[3, 2, 4] + ref[3, 5, 3] +
+            ref[3, 5, 4])
+        got2 = -oracle.preprocess(2, sub, ref, pred=8)[i]
+        assert abs(got2 - want2) <= 1e-5 * abs(want2), (dt, got2, want2)
+        want6 = 0.140625 * (ref[3, 3, 3] + ref[4, 4, 4] + ref[4, 4, 3] + ref[4, 3, 4] + ref[3, 4, 4] + ref[3, 3, 4] +
+                             ref[3, 4, 3] + ref[4, 3, 3]) - 0.015625 * (
+            ref[2, 2, 2] + ref[5, 5, 5] + ref[5, 2, 5] + ref[5, 5, 2] + ref[2, 5, 5] + ref[5, 2, 2] + ref[2, 5, 2] +
+            ref[2, 2, 5])
+        got6 = -oracle.preprocess(6, sub, ref, pred=16)[i]
+        assert abs(got6 - want6) <= 1e-5 * abs(want6), (dt, got6, want6)
+
+
 def test_golden_predictor_options(oracle, pred_gold):
     """The same against the committed vectors of those reference builds."""
     for key in pred_gold.files:
@@ -185,7 +239,8 @@ def test_golden_predictor_options(oracle, pred_gold):
         name, dtn, si, b = key[:-4].split("-")
         s = pred_gold["%s-%s-%s-sub" % (name, dtn, si)]
         r = pred_gold["%s-%s-%s-ref" % (name, dtn, si)]
-        got = oracle.preprocess(int(b), s, r, pred=PRED_VARIANTS[name])
+        bits = 8 | 16 if name == "stz2" else PRED_VARIANTS[name]
+        got = oracle.preprocess(int(b), s, r, pred=bits)
         assert np.array_equal(_bits(got), _bits(pred_gold[key])), key
 
 
