@@ -298,13 +298,15 @@ def tuner_bench(args, ctx, stream, field, out, N, s, payload_plain, dev):
     configuration of every compress (its statistics + MLP run inside the
     timed step).  (2) The tuner workload: 256x384x384 fp64 turbulence-like
     field (GRF k^-11/3 + 8 tanh fronts of width 2, seed 1003), rel eb 1e-6:
-    learned choice vs the exhaustive trial over all 48 configurations --
-    tuning time and payload bytes -- and vs the untuned configuration."""
+    learned choice vs the exhaustive trial over all 144 configurations --
+    tuning time and payload bytes -- and vs the untuned configuration.  The
+    model runs with its calibrated confidence threshold, below which the
+    learned path falls back to the trial (reported as "fallback")."""
     import numpy as np
     import torch
     from stz_amd import fields, tuner
-    params, meta = tuner.load_params()
-    tuner.set_model(ctx, params, min_confidence=0.0)
+    meta = tuner.load_model(ctx)
+    min_conf = float(meta.get("min_confidence", 0.0))
 
     def step():
         ds = ctx.compress_device(field, args.eb, "rel", tuner=1)
@@ -330,9 +332,11 @@ def tuner_bench(args, ctx, stream, field, out, N, s, payload_plain, dev):
     learned = {"value": round(2.0 * N * s / (ms * 1e-3) / 1e9, 2), "unit": "GB/s", "ms_per_step": round(ms, 4),
                "payload_bytes": int(last.payload_bytes), "cr_payload": round(N * s / last.payload_bytes, 3),
                "cr_vs_plain": round(payload_plain / last.payload_bytes, 4),
-               "choice": {"interp": info.interp, "order": info.order, "mult": [info.mult_base, info.mult_low,
-                                                                                 info.mult_high],
-                          "alpha": info.alpha, "beta": info.beta},
+               "choice": {"pred": info.pred, "interp": info.interp, "order": info.order,
+                          "mult": [info.mult_base, info.mult_low, info.mult_high], "alpha": info.alpha,
+                          "beta": info.beta},
+               "confidence": round(float(info.tune_confidence), 4), "min_confidence": round(min_conf, 4),
+               "fallback": bool(info.tuner == 2),
                "max_err": err, "eb_abs": rel_bound(field, args.eb),
                "error_bound_ok": bool(err <= rel_bound(field, args.eb))}
     last.free()
@@ -352,14 +356,16 @@ def tuner_bench(args, ctx, stream, field, out, N, s, payload_plain, dev):
     t_trial = time.perf_counter() - t0
     nbytes = f3.nbytes
     c3 = {"workload": "256x384x384 fp64 turbulence-like field (GRF k^-11/3 + 8 tanh fronts, seed 1003), rel eb 1e-6",
-          "configs": 48, "tuning_ms_learned": round(t_learned * 1e3, 3), "tuning_ms_trial": round(t_trial * 1e3, 1),
+          "configs": tuner.TUNE_CONFIGS, "tuning_ms_learned": round(t_learned * 1e3, 3), "tuning_ms_trial": round(t_trial * 1e3, 1),
           "tuning_speedup": round(t_trial / t_learned, 1), "choice": int(k), "trial_best": int(sizes.argmin()),
-          "confidence": round(conf, 4), "cr_learned": round(nbytes / sizes[k], 4),
+          "confidence": round(conf, 4), "min_confidence": round(min_conf, 4), "fallback": bool(conf < min_conf),
+          "cr_learned": round(nbytes / sizes[k], 4),
           "cr_trial": round(nbytes / sizes.min(), 4), "cr_untuned": round(nbytes / sizes[tuner.DEFAULT_CONFIG], 4),
           "cr_learned_vs_trial": round(sizes.min() / sizes[k], 5),
           "gain_vs_untuned": round(sizes[tuner.DEFAULT_CONFIG] / sizes[k] - 1.0, 4),
           "model": {"holdout_mean_regret": meta.get("holdout_eval", {}).get("mean_regret"),
-                    "holdout_gain_vs_untuned": meta.get("holdout_eval", {}).get("mean_gain_vs_default")}}
+                    "holdout_gain_vs_untuned": meta.get("holdout_eval", {}).get("mean_gain_vs_default"),
+                    "holdout_fallback_rate": meta.get("calibration", {}).get("fallback_rate")}}
     log("tuner_c3", json.dumps(c3))
     del f3t
     return learned, c3
@@ -438,8 +444,7 @@ def batch_bench(args, dev):
     # learned tuner on the same batch (one statistics pass + one MLP launch for
     # all fields inside the compress call), and the exhaustive trial once
     from stz_amd import tuner
-    tparams, _ = tuner.load_params()
-    tuner.set_model(ctx, tparams, min_confidence=0.0)
+    tuner.load_model(ctx)
     ld = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
     for d in ld:
         d.free()
@@ -453,6 +458,7 @@ def batch_bench(args, dev):
     ctx.decompress_batch(ld, louts)
     t_ld = time.perf_counter() - a
     lpay = sum(d.payload_bytes for d in ld)
+    n_fallback = sum(int(d.info.tuner == 2) for d in ld)
     l_ok = all(float((o.double() - f.double()).abs().max()) <= rel_bound(f, 1e-3) for o, f in zip(louts, fs))
     for d in ld:
         d.free()
@@ -468,7 +474,7 @@ def batch_bench(args, dev):
                "payload_bytes": int(lpay), "cr_payload": round(nbytes / lpay, 3),
                "cr_vs_plain": round(payload / lpay, 4), "error_bound_ok": bool(l_ok),
                "trial_compress_ms": round(t_trial * 1e3, 1), "trial_payload_bytes": int(tpay),
-               "cr_learned_vs_trial": round(tpay / lpay, 5)}
+               "cr_learned_vs_trial": round(tpay / lpay, 5), "fallback_fields": n_fallback}
     out = {"workload": "%d fp32 GRF fields, dims U{32..384}, slopes U[-4,-2] (seeds 1004+i), rel eb 1e-3" % len(fs),
            "points": int(pts), "bytes": int(nbytes), "payload_bytes": int(payload),
            "cr_payload": round(nbytes / payload, 3),

@@ -83,6 +83,10 @@ typedef struct {
     int interp, order;
     double mult_base, mult_low, mult_high, alpha, beta;
     int pred;                 /* STZ_PRED_* bits */
+    int tuner;                /* device streams: 0 parameters given, 1 the learned tuner's
+                                 choice, 2 the exhaustive trial's (asked for, or the
+                                 learned tuner's low-confidence fallback); payloads: 0 */
+    float tune_confidence;    /* the model's confidence when it chose (0 if not asked) */
 } stz_info;
 
 typedef struct stz_ctx stz_ctx;
@@ -194,23 +198,26 @@ int stz_decompress_batch(stz_ctx* ctx, int n, stz_dstream* const* ds, void* cons
 int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
 
 /* ---- learned tuner -------------------------------------------------------
- * Chooses the base interpolator, base axis order and level-eb scaling (the
- * tuning targets of F5:2120-2129 and SZ3's sample-and-trial, F5:1439) among
- * STZ_TUNE_CONFIGS configurations: 2 interpolators x 6 orders x 4 scalings.
+ * Chooses the residual predictor set (F5's; F5's PRED_PLANE_AXIS +
+ * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16), the base interpolator, base
+ * axis order and level-eb scaling (the tuning targets of F5:2120-2129 and
+ * SZ3's sample-and-trial, F5:1439) among STZ_TUNE_CONFIGS configurations:
+ * 3 predictor sets x 2 interpolators x 6 orders x 4 scalings, index
+ * 48 * set + 24 * interp + 4 * order + scaling.
  * The choice lands in the stream header; decoding and the error bound are
  * unchanged.  params.tuner = 1 compresses with the model's choice (falling
  * back to the exhaustive trial below the model's confidence threshold),
  * params.tuner = 2 with the trial's.  Tuning needs an ABS or REL bound. */
-#define STZ_TUNE_CONFIGS 48
+#define STZ_TUNE_CONFIGS 144
 #define STZ_TUNE_FEATURES 16
-#define STZ_MLP_PARAMS 8368 /* 16x64 + 64 + 64x64 + 64 + 64x48 + 48 fp32 */
+#define STZ_MLP_PARAMS 14608 /* 16x64 + 64 + 64x64 + 64 + 64x144 + 144 fp32 */
 int stz_tune_config(int index, stz_params* p);
 /* 16 statistics per 64^3 sub-block (x fastest block order) */
 int stz_tune_features(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,
                       double eb_abs, float* feats, size_t cap_blocks, size_t* n_blocks);
 /* host fp32 parameters; confidence below min_confidence falls back to the trial */
 int stz_tune_set_model(stz_ctx* ctx, const float* params, size_t n, float min_confidence);
-/* the model's configuration: fills interp / order / mult_* / alpha / beta */
+/* the model's configuration: fills pred / interp / order / mult_* / alpha / beta */
 int stz_tune(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz, double eb,
              stz_eb_mode mode, stz_params* inout, float* confidence, int* config);
 /* exhaustive trial: payload bytes of every configuration (sizes may be NULL) */

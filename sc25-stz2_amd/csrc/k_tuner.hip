@@ -1,5 +1,5 @@
 // k_tuner.hip -- the learned tuner: per-sub-block field statistics and a
-// 16 -> 64 -> 64 -> 48 MLP (ReLU) whose forward pass, backward pass and Adam
+// 16 -> 64 -> 64 -> 144 MLP (ReLU) whose forward pass, backward pass and Adam
 // update run here.  Matrix products are bf16 MFMA (v_mfma_f32_16x16x16_bf16)
 // with fp32 accumulation; weights are kept in fp32 (master copy) and rounded
 // to bf16 (round to nearest even) when a workgroup stages them in LDS.
@@ -195,10 +195,10 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, c
                      ld4s(&S.dy[wv][(4 * q) * kOut + 16 * nt + c16], kOut), d);
             for (int i = 0; i < 4; ++i) atomicAdd(&G[oW3 + (16 * mt + 4 * q + i) * kOut + 16 * nt + c16], d[i]);
         }
-    if (lane < kOut) {
+    for (int c = lane; c < kOut; c += 64) {
         float s = 0.f;
-        for (int r = 0; r < 16; ++r) s += bf16_f(S.dy[wv][r * kOut + lane]);
-        atomicAdd(&G[oB3 + lane], s);
+        for (int r = 0; r < 16; ++r) s += bf16_f(S.dy[wv][r * kOut + c]);
+        atomicAdd(&G[oB3 + c], s);
     }
     // dH2 = dY W3^T, masked by ReLU
     for (int nt = 0; nt < kHid / 16; ++nt) {

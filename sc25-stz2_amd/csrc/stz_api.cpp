@@ -353,6 +353,10 @@ struct stz_dstream {
     // decoded base level kept for random access (stz_dstream_prepare)
     DBuf base;
     bool have_base = false;
+    // how the configuration was chosen (0 given, 1 learned, 2 trial) and the
+    // model's confidence when it was asked
+    int tuned = 0;
+    float tune_conf = 0.f;
     // identity for cached launch plans (addresses can be reused)
     uint64_t gen = next_gen();
     static uint64_t next_gen()
@@ -860,7 +864,7 @@ LevelArgs enc_level_args(const FieldEnc<T>& fe, int level);
 
 template <typename T>
 void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
-                     int mode, const stz_params& prm, stz_dstream* const* out, bool* low_conf);
+                     int mode, const stz_params& prm, stz_dstream* const* out, std::vector<char>* low_conf);
 
 template <typename T>
 void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, size_t nz, double eb, int mode,
@@ -870,6 +874,10 @@ void compress_impl(stz_ctx* c, int dtype, const T* field, size_t nx, size_t ny, 
     const size_t dims[3] = {nx, ny, nz};
     compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, prm, &ds, nullptr);
 }
+
+template <typename T>
+int tune_trial(stz_ctx* c, int dtype, const T* d, size_t nx, size_t ny, size_t nz, double eb, int mode,
+               const stz_params& base, stz_dstream* keep, uint64_t* sizes);
 
 // ------------------------------------------------------------ decode helpers
 void ensure_tabs(stz_ctx* c, stz_dstream* ds)
@@ -1399,8 +1407,12 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 }
 
 // ---------------------------------------------------------------- tuner
-// The configurations the tuner chooses among: base interpolator (2) x base
-// axis order (6) x level-eb scaling (4).  Scalings: F5's multipliers
+// The configurations the tuner chooses among: predictor set (3) x base
+// interpolator (2) x base axis order (6) x level-eb scaling (4); index
+// k = 48 * set + 24 * interp + 4 * order + scaling, so 0..47 are F5's own
+// predictors.  Predictor sets: F5's (F5:690-748); F5 built with
+// PRED_PLANE_AXIS and PRED_CUBE_AXIS (F5:579-622, F5:627-686); stz2.cpp's
+// 12-point plane and 16-point cube predictors (stz2.cpp:122-196).  Scalings: F5's multipliers
 // (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
 // every level; F5's multipliers with the base codec's level bounds shrinking
 // by 1.5 per level up to 4x (SZ3-style).  In ABS / REL modes the largest
@@ -1408,11 +1420,14 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 struct TuneScale { double mb, ml, mh, alpha, beta; };
 const TuneScale kScales[4] = {
     {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
-constexpr int kTuneConfigs = 48;
+const int kTunePred[3] = {0, kPredPlaneAxis | kPredCubeAxis, kPredBicubic12 | kPredDiag16};
+constexpr int kTuneConfigs = 144;
+static_assert(kTuneConfigs == kMlpOut, "one MLP output per configuration");
 
 void tune_config(int k, stz_params& p)
 {
-    p.interp = k / 24;
+    p.pred = kTunePred[k / 48];
+    p.interp = (k / 24) % 2;
     p.order = (k / 4) % 6;
     const TuneScale& t = kScales[k % 4];
     p.mult_base = t.mb;
@@ -1645,7 +1660,7 @@ void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, std::vector<
 // marked Huffman encode (lengths, chunk scan, pack).
 template <typename T>
 void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, const size_t* dims, double eb,
-                     int mode, const stz_params& prm, stz_dstream* const* out, bool* low_conf)
+                     int mode, const stz_params& prm, stz_dstream* const* out, std::vector<char>* low_conf)
 {
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -1666,14 +1681,31 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     // every field's range is on the host once the last one is (stream order);
     // the gathers queued after them keep the GPU busy meanwhile
     HIPCHK(hipEventSynchronize(c->ev_range));
+    // fields the caller takes back for its exhaustive trial; kept alive to
+    // the end, after every launch that may still read their buffers
+    std::vector<FieldEnc<T>> dropped;
     if (prm.tuner == 1) {
-        // one field: below the context's confidence threshold the caller
-        // falls back to the exhaustive trial
+        // below the context's confidence threshold a field is marked in
+        // *low_conf and left to the caller's trial; the others go on
         std::vector<float> conf;
         learned_choice_batch<T>(c, fe, &conf);
-        if (low_conf && n == 1 && conf[0] < c->tune_min_conf) {
-            *low_conf = true;
-            return;
+        for (int i = 0; i < n; ++i) {
+            fe[i].ds->tuned = 1;
+            fe[i].ds->tune_conf = conf[i];
+        }
+        if (low_conf) {
+            low_conf->assign(n, 0);
+            std::vector<FieldEnc<T>> keep;
+            for (int i = 0; i < n; ++i) {
+                (*low_conf)[i] = conf[i] < c->tune_min_conf;
+                ((*low_conf)[i] ? dropped : keep).push_back(std::move(fe[i]));
+            }
+            fe.swap(keep);
+            n = (int)fe.size();
+            if (n == 0) {
+                HIPCHK(hipStreamSynchronize(st));
+                return;
+            }
         }
     }
     for (int i = 0; i < n; ++i) enc_phase1(c, fe[i]);
@@ -2018,6 +2050,20 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
             compress_fields<T>(c, dtype, n, fields, dims, eb, mode, pk, traw.data(), nullptr);
             for (int i = 0; i < n; ++i)
                 if (k == 0 || traw[i]->bytes < raw[i]->bytes) std::swap(dss[i], tmp[i]), raw[i] = dss[i].get();
+        }
+        for (int i = 0; i < n; ++i) raw[i]->tuned = 2;
+    } else if (prm.tuner == 1) {
+        // the model's choice for every field in one pass; a field below the
+        // confidence threshold gets its own exhaustive trial
+        std::vector<char> low;
+        compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data(), &low);
+        for (int i = 0; i < n; ++i) {
+            if (!low[i]) continue;
+            const float conf = raw[i]->tune_conf;
+            tune_trial<T>(c, dtype, (const T*)fields[i], dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], eb, mode, prm,
+                          raw[i], nullptr);
+            raw[i]->tuned = 2;
+            raw[i]->tune_conf = conf;
         }
     } else {
         compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data(), nullptr);
@@ -2599,13 +2645,16 @@ void compress_tuned(stz_ctx* c, int dtype, const void* field, size_t nx, size_t 
     if (p.tuner == 1) {
         // the model's choice is made inside the compress, after its value
         // range pass (the same statistics, MLP and vote as stz_tune)
-        bool low = false;
+        std::vector<char> low;
         const void* f = d;
         const size_t dims[3] = {nx, ny, nz};
         compress_fields<T>(c, dtype, 1, &f, dims, eb, mode, p, &ds, &low);
-        if (!low) return;
+        if (!low[0]) return;
     }
+    const float conf = ds->tune_conf;
     tune_trial<T>(c, dtype, d, nx, ny, nz, eb, mode, p, ds, nullptr);
+    ds->tuned = 2;
+    ds->tune_conf = conf;
 }
 
 }  // namespace
@@ -2967,6 +3016,8 @@ int stz_dstream_info(const stz_dstream* ds, stz_info* info)
     info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
     info->alpha = fh.alpha; info->beta = fh.beta;
     info->pred = (int)fh.pred;
+    info->tuner = ds->tuned;
+    info->tune_confidence = ds->tune_conf;
     return STZ_OK;
 }
 
@@ -3129,6 +3180,8 @@ int stz_query_info(const void* stream, size_t bytes, stz_info* info)
         info->mult_base = fh.mult_base; info->mult_low = fh.mult_low; info->mult_high = fh.mult_high;
         info->alpha = fh.alpha; info->beta = fh.beta;
         info->pred = (int)fh.pred;
+        info->tuner = 0;
+        info->tune_confidence = 0.f;
     });
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
     "stz_compress_batch", "stz_decompress_batch", "stz_autotune",
 ]
-TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS = 48, 16, 8368
+TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS = 144, 16, 14608
 
 
 class StzError(RuntimeError):
@@ -68,7 +68,7 @@ class Info(C.Structure):
         ("interp", C.c_int), ("order", C.c_int),
         ("mult_base", C.c_double), ("mult_low", C.c_double), ("mult_high", C.c_double),
         ("alpha", C.c_double), ("beta", C.c_double),
-        ("pred", C.c_int),
+        ("pred", C.c_int), ("tuner", C.c_int), ("tune_confidence", C.c_float),
     ]
 
 

@@ -1,6 +1,7 @@
-"""Learned tuner: a 16 -> 64 -> 64 -> 48 MLP over per-sub-block field
-statistics that picks the base interpolator, base axis order and level-eb
-scaling (48 configurations) in place of the exhaustive trial.
+"""Learned tuner: a 16 -> 64 -> 64 -> 144 MLP over per-sub-block field
+statistics that picks the residual predictor set, base interpolator, base
+axis order and level-eb scaling (144 configurations) in place of the
+exhaustive trial.
 
 Labels come from the exhaustive trial itself (the payload size of every
 configuration on generated fields); the target of a sub-block is the soft
@@ -25,14 +26,14 @@ from .api import EB_MODES, _dtype_id, _ptr
 
 DEFAULT_CONFIG = 24          # cubic, order 0, F5 multipliers: the untuned path
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_model.npz")
-_IN, _HID, _OUT = 16, 64, 48
+_IN, _HID, _OUT = 16, 64, TUNE_CONFIGS
 
 
 def config_params(k):
     p = Params()
     lib().stz_default_params(C.byref(p))
     check(lib().stz_tune_config(int(k), C.byref(p)), "stz_tune_config")
-    return {"interp": p.interp, "order": p.order, "mult_base": p.mult_base, "mult_low": p.mult_low,
+    return {"pred": p.pred, "interp": p.interp, "order": p.order, "mult_base": p.mult_base, "mult_low": p.mult_low,
             "mult_high": p.mult_high, "alpha": p.alpha, "beta": p.beta}
 
 
@@ -148,6 +149,15 @@ def set_model(ctx, params, min_confidence=0.0):
     check(lib().stz_tune_set_model(ctx.h, _ptr(p), p.size, float(min_confidence)), "stz_tune_set_model")
 
 
+def load_model(ctx, path=MODEL_PATH):
+    """Set the trained model on ctx with its calibrated confidence threshold
+    (below it the learned path falls back to the exhaustive trial); returns
+    the model's metadata."""
+    params, meta = load_params(path)
+    set_model(ctx, params, float(meta.get("min_confidence", 0.0)))
+    return meta
+
+
 def choose(ctx, field, eb, mode="rel"):
     """The model's configuration for a field (model set with set_model)."""
     nz, ny, nx = field.shape
@@ -204,10 +214,13 @@ def make_field(rng, device):
     return torch.from_numpy(np.ascontiguousarray(f, dtype=dt)).to(device), eb, str(fam)
 
 
-def build_dataset(ctx, n_fields, seed, device="cuda:0", log=None):
+def build_dataset(ctx, n_fields, seed, device="cuda:0", log=None, keep_from=None):
+    """Features and trial sizes of n_fields generated fields; the fields from
+    index keep_from on are kept (device tensors, with their bounds) for the
+    library-side evaluation."""
     import torch
     rng = np.random.default_rng(seed)
-    X, fid, sizes, meta = [], [], [], []
+    X, fid, sizes, meta, kept = [], [], [], [], {}
     t0 = time.time()
     for i in range(n_fields):
         f, eb, fam = make_field(rng, device)
@@ -218,9 +231,12 @@ def build_dataset(ctx, n_fields, seed, device="cuda:0", log=None):
         fid.append(np.full(X[-1].shape[0], i, np.int32))
         sizes.append(trial_sizes(ctx, f, eb, "rel"))
         meta.append({"family": fam, "shape": list(f.shape), "dtype": str(f.dtype), "eb": eb})
+        if keep_from is not None and i >= keep_from:
+            kept[i] = (f, eb)
         if log and (i + 1) % 50 == 0:
             log("dataset: %d/%d fields, %.1fs" % (i + 1, n_fields, time.time() - t0))
-    return {"X": np.concatenate(X), "fid": np.concatenate(fid), "sizes": np.stack(sizes), "meta": meta}
+    return {"X": np.concatenate(X), "fid": np.concatenate(fid), "sizes": np.stack(sizes), "meta": meta,
+            "kept": kept}
 
 
 def train(model, data, steps=3000, batch=256, lr=3e-3, tau=0.01, seed=0, log=None, fields_idx=None):
@@ -268,6 +284,37 @@ def evaluate(model, data, fields_idx):
             "mean_confidence": float(np.mean(confs))}
 
 
+def library_choices(ctx, data):
+    """The library's own choice and confidence (stz_tune: block-weighted vote)
+    on every kept field, with its regret against the trial."""
+    out = []
+    for i, (f, eb) in sorted(data["kept"].items()):
+        k, conf = choose(ctx, f, eb)
+        s = data["sizes"][i].astype(np.float64)
+        out.append((i, k, conf, float(s[k] / s.min() - 1.0)))
+    return out
+
+
+def calibrate_min_confidence(choices, max_regret=0.01, floor_percentile=2.0):
+    """The confidence threshold below which the learned path falls back to
+    the trial: above the confidence of every held-out field whose regret
+    exceeds max_regret, and no lower than the floor_percentile-th percentile
+    of the held-out confidences (a field less certain than almost every
+    held-out one is treated as unlike the training set)."""
+    conf = np.array([c for _, _, c, _ in choices])
+    reg = np.array([r for _, _, _, r in choices])
+    t = float(np.percentile(conf, floor_percentile))
+    bad = conf[reg > max_regret]
+    if bad.size:
+        t = max(t, float(np.nextafter(np.float32(bad.max()), np.float32(np.inf))))
+    fb = conf < t
+    reg_fb = np.where(fb, 0.0, reg)
+    return t, {"fields": int(conf.size), "fallback_rate": float(fb.mean()), "max_regret_learned": float(reg.max()),
+               "fields_over_max_regret": int((reg > max_regret).sum()), "mean_regret_with_fallback": float(reg_fb.mean()),
+               "max_regret_with_fallback": float(reg_fb.max()), "max_regret_target": max_regret,
+               "confidence_min": float(conf.min()), "confidence_median": float(np.median(conf))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train"])
@@ -285,11 +332,11 @@ def main():
         print(*a, flush=True)
 
     ctx = Context(0)
-    data = build_dataset(ctx, args.fields, args.seed, log=log)
+    n_train = args.fields - args.holdout
+    data = build_dataset(ctx, args.fields, args.seed, log=log, keep_from=n_train)
     if args.data_out:
         np.savez_compressed(args.data_out, X=data["X"], fid=data["fid"], sizes=data["sizes"],
                             meta=json.dumps(data["meta"]))
-    n_train = args.fields - args.holdout
     model = Mlp(ctx, seed=args.seed)
     t0 = time.time()
     train(model, data, steps=args.steps, fields_idx=np.arange(n_train), log=log)
@@ -299,8 +346,13 @@ def main():
     ev_ho = evaluate(model, data, range(n_train, args.fields))
     log("train", json.dumps(ev_tr))
     log("holdout", json.dumps(ev_ho))
+    set_model(ctx, model.params(), 0.0)
+    choices = library_choices(ctx, data)
+    min_conf, cal = calibrate_min_confidence(choices)
+    log("calibration", min_conf, json.dumps(cal))
     model.save(args.out, {"fields": args.fields, "holdout": args.holdout, "steps": args.steps, "seed": args.seed,
-                          "train_seconds": t_train, "train_eval": ev_tr, "holdout_eval": ev_ho})
+                          "configs": TUNE_CONFIGS, "train_seconds": t_train, "train_eval": ev_tr,
+                          "holdout_eval": ev_ho, "min_confidence": min_conf, "calibration": cal})
     log("saved", args.out)
     ctx.close()
 

@@ -94,7 +94,7 @@ def test_batch_argument_checks(stz, ctx):
 
 def test_batch_learned_and_trial_choices(stz, ctx):
     """tuner = 1 in a batch picks what single calls pick (same model, no
-    fallback); tuner = 2 keeps each field's smallest payload over the 48
+    fallback); tuner = 2 keeps each field's smallest payload over the 144
     configurations."""
     import torch
     from stz_amd import tuner
@@ -108,7 +108,23 @@ def test_batch_learned_and_trial_choices(stz, ctx):
         assert np.array_equal(b.download(-1), single.download(-1))
         single.free()
     trial = ctx.compress_batch(fs, 1e-3, "rel", tuner=2)
-    for t, f in zip(trial, fs):
-        assert t.payload_bytes == tuner.trial_sizes(ctx, f, 1e-3).min()
-    for d in batch + trial:
+    sizes = [tuner.trial_sizes(ctx, f, 1e-3) for f in fs]
+    for t, sz in zip(trial, sizes):
+        assert t.payload_bytes == sz.min() and t.info.tuner == 2
+    # a threshold between the fields' confidences: the less confident fields
+    # fall back to their own trial, the others keep the model's choice
+    confs = [b.info.tune_confidence for b in batch]
+    assert all(b.info.tuner == 1 for b in batch) and all(0 < c <= 1 for c in confs)
+    thr = float(np.median(confs))
+    tuner.set_model(ctx, params, min_confidence=thr)
+    mixed = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
+    for m, b, sz, c in zip(mixed, batch, sizes, confs):
+        if c < thr:
+            assert m.info.tuner == 2 and m.payload_bytes == sz.min(), (c, thr)
+        else:
+            assert m.info.tuner == 1 and np.array_equal(m.download(-1), b.download(-1)), (c, thr)
+        assert m.info.tune_confidence == c
+    assert any(c < thr for c in confs) or len(set(confs)) == 1
+    tuner.set_model(ctx, params, min_confidence=0.0)
+    for d in batch + trial + mixed:
         d.free()

@@ -6,7 +6,7 @@
 * 256x384x384 fp64 turbulence-like field (seed 1003) at rel eb 1e-6 with the
   learned tuner and with the exhaustive trial: the payload equals the
   oracle's for the chosen configuration, the trial's choice is the smallest
-  of its 48 sizes and those sizes match the oracle's, decode within the bound;
+  of its 144 sizes and those sizes match the oracle's, decode within the bound;
 * 96 uneven fp32 fields (seeds 1004 + i) in one batch call at rel eb 1e-3:
   every stream and decoded field equals its one-call result, every eighth
   field's payload equals the oracle's, every field within the bound;
@@ -107,7 +107,7 @@ def c3(ctx, dev):
 
 
 def _chosen(info):
-    return dict(interp=int(info.interp), order=int(info.order), mult_base=info.mult_base,
+    return dict(pred=int(info.pred), interp=int(info.interp), order=int(info.order), mult_base=info.mult_base,
                 mult_low=info.mult_low, mult_high=info.mult_high, alpha=info.alpha, beta=info.beta)
 
 
@@ -135,7 +135,7 @@ def test_c3_turbulence_fp64_tuned(oracle, dev, c3, mode):
         sizes = tuner.trial_sizes(c, t, 1e-6)
         assert int(np.argmin(sizes)) == tuner.config_index(prm), (int(np.argmin(sizes)), prm)
         assert gpu.size == int(sizes.min())
-        for k in (0, 17, 40):   # the trial's sizes are the oracle's payload sizes
+        for k in (0, 17, 40, 77, 130):   # the trial's sizes are the oracle's payload sizes
             assert int(sizes[k]) == oracle.compress(f, 1e-6, oracle.EB_REL, **tuner.config_params(k)).size, k
     ds.free()
 

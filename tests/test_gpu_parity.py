@@ -145,20 +145,22 @@ def test_payload_and_decode_bitexact(ctx, oracle, case):
 
 
 def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
-    """All 48 configurations the learned tuner chooses among (base interpolator
-    x axis order x level-bound scaling): payload bytes equal the oracle's, f32
-    and f64, and a spread of them decode bit-exactly."""
+    """All 144 configurations the learned tuner chooses among (predictor set x
+    base interpolator x axis order x level-bound scaling): payload bytes equal
+    the oracle's, f32 and f64, and a spread of them decode bit-exactly."""
     from stz_amd import tuner
     f32 = _field("grf", (36, 40, 44), np.float32, seed=17)
     f64 = _field("sin", (20, 24, 28), np.float64, seed=5)
-    for k in range(48):
+    from stz_amd._lib import TUNE_CONFIGS
+    assert TUNE_CONFIGS == 144
+    for k in range(TUNE_CONFIGS):
         prm = tuner.config_params(k)
         for f, eb in ((f32, 1e-3), (f64, 1e-5)):
             _say("tuner configuration", (k, f.dtype.name))
             ds, gpu, want = _compress_both(ctx, oracle, f, eb, "rel", **prm)
             diff = first_difference(gpu, want)
             assert diff is None, (k, f.dtype.name, diff)
-            if k in (0, 13, 30, 47):
+            if k in (0, 13, 30, 47, 61, 100, 143):
                 _assert_bitexact(ds.decompress(), oracle.decompress(want), "decompress config %d" % k)
             ds.free()
 

@@ -22,10 +22,10 @@ def _problem(n, seed):
     rng = np.random.default_rng(seed)
     P = np.concatenate([rng.normal(0, 0.35, size=16 * 64), rng.normal(0, 0.1, 64),
                         rng.normal(0, 0.18, size=64 * 64), rng.normal(0, 0.1, 64),
-                        rng.normal(0, 0.18, size=64 * 48), rng.normal(0, 0.1, 48)]).astype(np.float32)
+                        rng.normal(0, 0.18, size=64 * mlp_ref.OUT), rng.normal(0, 0.1, mlp_ref.OUT)]).astype(np.float32)
     assert P.size == mlp_ref.N_PARAMS
     X = rng.normal(0, 1, size=(n, 16)).astype(np.float32)
-    T = rng.dirichlet(np.full(48, 0.3), size=n).astype(np.float32)
+    T = rng.dirichlet(np.full(mlp_ref.OUT, 0.3), size=n).astype(np.float32)
     return P, X, T
 
 
@@ -56,7 +56,7 @@ def test_mlp_gradient_matches_cpu_float64(stz, ctx, torch_dev, n):
     # differ in their last bits: normwise 1e-2, per parameter block 2e-2
     assert np.linalg.norm(g - want) <= 1e-2 * np.linalg.norm(want)
     o = 0
-    for k in (16 * 64, 64, 64 * 64, 64, 64 * 48, 48):
+    for k in (16 * 64, 64, 64 * 64, 64, 64 * mlp_ref.OUT, mlp_ref.OUT):
         blk = slice(o, o + k)
         assert np.linalg.norm(g[blk] - want[blk]) <= 2e-2 * np.linalg.norm(want[blk]) + 1e-7, (o, k)
         o += k
@@ -89,11 +89,12 @@ def test_adam_matches_cpu(stz, ctx, torch_dev):
 def test_training_lowers_loss(stz, ctx, torch_dev):
     """A learnable task: the target configuration is a fixed function of the features."""
     from stz_amd.tuner import Mlp
+    from stz_amd._lib import TUNE_CONFIGS as K
     rng = np.random.default_rng(5)
     X = rng.normal(size=(2048, 16)).astype(np.float32)
-    W = rng.normal(size=(16, 48))
+    W = rng.normal(size=(16, K))
     lab = np.argmax(X @ W, axis=1)
-    T = np.full((2048, 48), 0.02 / 47, np.float32)
+    T = np.full((2048, K), 0.02 / (K - 1), np.float32)
     T[np.arange(2048), lab] = 0.98
     m = Mlp(ctx, seed=1)
     first = np.mean([m.step(X[b], T[b], lr=3e-3) for b in np.split(rng.permutation(2048)[:512], 2)])
@@ -205,18 +206,27 @@ def test_compress_with_learned_and_trial_choice(stz, ctx, torch_dev, oracle):
         ds = ctx.compress_device(f, 1e-5, "rel", tuner=mode)
         i = ds.info
         p = ds.download(-1)
-        want = oracle.compress(f, 1e-5, oracle.EB_REL, interp=i.interp, order=i.order, mult_base=i.mult_base,
-                               mult_low=i.mult_low, mult_high=i.mult_high, alpha=i.alpha, beta=i.beta)
+        want = oracle.compress(f, 1e-5, oracle.EB_REL, pred=i.pred, interp=i.interp, order=i.order,
+                               mult_base=i.mult_base, mult_low=i.mult_low, mult_high=i.mult_high, alpha=i.alpha,
+                               beta=i.beta)
         assert np.array_equal(p, want)
         r = ds.decompress()
         assert np.abs(r - f).max() <= 1e-5 * (f.max() - f.min()) * (1 + 1e-12)
+        assert i.tuner == mode
         if mode == 2:
             sizes = tuner.trial_sizes(ctx, f, 1e-5)
             assert i.payload_bytes == sizes.min()
+        else:
+            k, conf = tuner.choose(ctx, f, 1e-5)
+            assert i.tune_confidence == np.float32(conf)
+            assert tuner.config_index(dict(pred=i.pred, interp=i.interp, order=i.order, mult_base=i.mult_base,
+                                           mult_low=i.mult_low, mult_high=i.mult_high, alpha=i.alpha,
+                                           beta=i.beta)) == k
         ds.free()
     # below the confidence threshold the learned path falls back to the trial
     tuner.set_model(ctx, params, min_confidence=1.01)
     ds = ctx.compress_device(f, 1e-5, "rel", tuner=1)
     assert ds.payload_bytes == tuner.trial_sizes(ctx, f, 1e-5).min()
+    assert ds.info.tuner == 2 and 0 < ds.info.tune_confidence <= 1
     ds.free()
     tuner.set_model(ctx, params)

@@ -619,10 +619,10 @@ def test_mlp_reference_gradient_by_finite_differences():
     rng = np.random.default_rng(0)
     P = rng.normal(0, 0.3, size=mlp_ref.N_PARAMS)
     X = rng.normal(size=(24, 16))
-    T = rng.dirichlet(np.ones(48), size=24)
+    T = rng.dirichlet(np.ones(mlp_ref.OUT), size=24)
     _, g = mlp_ref.loss_grad(P, X, T, bf=False)
     eps = 1e-6
-    blocks = np.cumsum([0, 16 * 64, 64, 64 * 64, 64, 64 * 48, 48])
+    blocks = np.cumsum([0, 16 * 64, 64, 64 * 64, 64, 64 * mlp_ref.OUT, mlp_ref.OUT])
     for lo, hi in zip(blocks[:-1], blocks[1:]):     # a few parameters of every block
         for i in rng.choice(np.arange(lo, hi), 6, replace=False):
             Pp, Pm = P.copy(), P.copy()
