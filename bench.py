@@ -29,13 +29,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sc25-stz2_amd"))
 
-# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and
-# two streams on one queue run in order: the serving run's worker streams each
-# need their own queue, or a box waits behind a full-resolution decode (each
-# library context holds up to two streams).  Read once when HIP starts, so it
-# is set before torch is imported.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
-    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# GPU_MAX_HW_QUEUES is left as the caller set it (HIP's default: 4); the
+# serving workers' contexts take hardware queues of their own
+# (STZ_CTX_OWN_QUEUE) and need no more.  Recorded in the result line.
+HW_QUEUES = os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default, unset)")
 
 METRIC = "GB/s comp+decomp at rel eb 1e-3 (learned vs plain, same GPU); ROI p99 ms"
 HBM_PEAK_GBS = 8000.0
@@ -108,6 +105,8 @@ def main():
     ap.add_argument("--serve-n", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[2000.0, 4000.0],
+                    help="further arrival rates, each a trace of max(1000, rate) requests")
     ap.add_argument("--serve-workers", type=int, default=4,
                     help="worker contexts of the request server (one HIP stream each, two for the full-decode worker)")
     args = ap.parse_args()
@@ -279,7 +278,7 @@ def main():
         "payload_bytes": int(payload), "cr_payload": round(cr_payload, 3), "cr_zstd": round(cr_zstd, 3),
         "zstd_ms": round(zstd_ms, 2), "max_err": err, "eb_abs": eb_req, "eb_high_level": eb_high,
         "error_bound_ok": bool(ok),
-        "roofline": roof, "whole_step": whole,
+        "roofline": roof, "whole_step": whole, "gpu_max_hw_queues": HW_QUEUES,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
         "batch": batch,
@@ -513,15 +512,32 @@ def serve_bench(args, dev):
     del g
     torch.cuda.empty_cache()
     ctx.close()  # its streams would take hardware queues the workers need; the stream keeps its memory
-    reqs, arr = serve.poisson_trace((n, n, n), args.serve_requests, args.serve_rate, seed=1005)
-    srv = serve.RoiServer(ds, workers=args.serve_workers, max_batch=32, mem_budget=32 << 30)
-    log("serving %d requests at %.0f/s on %d workers" % (len(reqs), args.serve_rate, args.serve_workers))
-    try:
-        recs, ts = srv.run_trace(reqs, arr)
-    finally:
-        srv.close()
-    stats, gbs, wall = serve.summarize(recs, ts, 4)
-    sizes = [b[1] for b in srv.batches]
+    def run(rate, n_req):
+        reqs, arr = serve.poisson_trace((n, n, n), n_req, rate, seed=1005)
+        srv = serve.RoiServer(ds, workers=args.serve_workers, max_batch=32, mem_budget=32 << 30)
+        log("serving %d requests at %.0f/s on %d workers" % (len(reqs), rate, args.serve_workers))
+        try:
+            recs, ts = srv.run_trace(reqs, arr)
+        finally:
+            srv.close()
+        stats, gbs, wall = serve.summarize(recs, ts, 4)
+        return reqs, stats, gbs, wall, [b[1] for b in srv.batches]
+
+    reqs, stats, gbs, wall, sizes = run(args.serve_rate, args.serve_requests)
+    # rate sweep: shorter traces at higher arrival rates, stopping at the
+    # first failure
+    sweep = []
+    for rate in args.serve_sweep:
+        try:
+            _, st, g_, w_, sz = run(rate, max(1000, int(rate)))
+        except Exception as e:  # recorded, and the sweep ends
+            sweep.append({"rate_per_s": rate, "error": repr(e)})
+            break
+        sweep.append({"rate_per_s": rate, "requests": max(1000, int(rate)), "box_p99_ms": st["box"]["p99_ms"],
+                      "all_p99_ms": st["all"]["p99_ms"], "level3_p99_ms": st.get("level3", {}).get("p99_ms"),
+                      "mean_batch": round(float(np.mean(sz)), 2) if sz else 0, "max_batch_seen": max(sz) if sz else 0,
+                      "delivered_GBps": round(g_, 2)})
+        log("sweep", json.dumps(sweep[-1]))
     out = {"workload": "%d^3 fp32 GRF (k^-3, seed 1005), rel eb 1e-3, compressed once and resident in HBM "
                        "(payload %d bytes, CR %.1f)" % (n, info.payload_bytes, n ** 3 * 4 / info.payload_bytes),
            "requests": len(reqs), "rate_per_s": args.serve_rate,
@@ -529,7 +545,8 @@ def serve_bench(args, dev):
                    "level 1/8": 0.03, "level 1/4": 0.03, "level 1/2": 0.03, "level full": 0.03},
            "workers": args.serve_workers, "max_batch": 32, "batches": len(sizes),
            "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0, "max_batch_seen": max(sizes) if sizes else 0,
-           "wall_s": round(wall, 3), "delivered_GBps": round(gbs, 2), "latency": stats}
+           "wall_s": round(wall, 3), "delivered_GBps": round(gbs, 2), "latency": stats,
+           "gpu_max_hw_queues": HW_QUEUES, "rate_sweep": sweep}
     log("serving", json.dumps(out))
     ds.free()
     ctx.close()

@@ -94,6 +94,16 @@ typedef struct stz_dstream stz_dstream; /* compressed stream resident in HBM */
 
 /* ---- context ----------------------------------------------------------- */
 int stz_create(stz_ctx** ctx, int device);
+/* stz_create with flags.  STZ_CTX_OWN_QUEUE: the context's streams get
+ * hardware queues of their own (streams created with a CU mask over every
+ * CU), so its work never waits behind another stream's on a shared queue
+ * whatever GPU_MAX_HW_QUEUES says (HIP otherwise spreads streams over that
+ * many queues, 4 by default, and streams on one queue run in order).  For
+ * serving contexts whose batches must overlap; each such stream holds one
+ * of the GPU's hardware queue slots while it lives.
+ * No counterpart in the reference, which runs one query at a time. */
+#define STZ_CTX_OWN_QUEUE 1u
+int stz_create_ex(stz_ctx** ctx, int device, unsigned flags);
 void stz_destroy(stz_ctx* ctx);
 /* run subsequent work on a caller-owned hipStream_t (NULL: the library's own) */
 int stz_set_stream(stz_ctx* ctx, void* hip_stream);

@@ -246,6 +246,20 @@ private:
 
 }  // namespace
 
+// A stream whose CU mask covers every CU of the device (current): HIP gives
+// a CU-masked stream a hardware queue of its own instead of sharing one of
+// the GPU_MAX_HW_QUEUES queues with other streams, where work runs in order.
+static hipStream_t full_mask_stream(int device)
+{
+    int n_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    std::vector<uint32_t> mask((size_t)(n_cu + 31) / 32, 0xFFFFFFFFu);
+    if (n_cu % 32) mask.back() = (1u << (n_cu % 32)) - 1u;
+    hipStream_t s = nullptr;
+    HIPCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+    return s;
+}
+
 struct stz_ctx {
     int device = 0;
     // this device's copy of the allocation-free entry point's scratch array
@@ -261,10 +275,16 @@ struct stz_ctx {
     // decode on `stream`; ev_items / ev_high order it
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_items = nullptr, ev_high = nullptr;
+    // STZ_CTX_OWN_QUEUE: the streams are created with a CU mask over every
+    // CU, which gives each its own hardware queue
+    bool own_queue = false;
     hipStream_t second()
     {
         if (!side_stream) {
-            HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+            if (own_queue)
+                side_stream = full_mask_stream(device);
+            else
+                HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
             HIPCHK(hipEventCreateWithFlags(&ev_items, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&ev_high, hipEventDisableTiming));
         }
@@ -2696,9 +2716,11 @@ const char* stz_strerror(int code)
 
 void stz_free(void* p) { std::free(p); }
 
-int stz_create(stz_ctx** out, int device)
+int stz_create(stz_ctx** out, int device) { return stz_create_ex(out, device, 0u); }
+
+int stz_create_ex(stz_ctx** out, int device, unsigned flags)
 {
-    if (!out) return STZ_E_ARG;
+    if (!out || (flags & ~STZ_CTX_OWN_QUEUE)) return STZ_E_ARG;
     *out = nullptr;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
@@ -2711,7 +2733,11 @@ int stz_create(stz_ctx** out, int device)
     return guard(c.get(), [&] {
         // a blocking stream: it orders after work on the legacy default
         // stream, where torch puts the caller's tensors by default
-        HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
+        c->own_queue = (flags & STZ_CTX_OWN_QUEUE) != 0;
+        if (c->own_queue)
+            c->own = full_mask_stream(device);
+        else
+            HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
         HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->ev_range, hipEventDisableTiming));
         c->stream = c->own;

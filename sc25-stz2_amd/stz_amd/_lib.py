@@ -16,7 +16,7 @@ ERRORS = {-1: "STZ_E_ARG", -2: "STZ_E_DIMS", -3: "STZ_E_STREAM", -4: "STZ_E_OOM"
 
 # every entry point include/stz.h declares
 EXPORTS = [
-    "stz_create", "stz_destroy", "stz_set_stream", "stz_get_stream", "stz_set_grid_share", "stz_profile",
+    "stz_create", "stz_create_ex", "stz_destroy", "stz_set_stream", "stz_get_stream", "stz_set_grid_share", "stz_profile",
     "stz_profile_report",
     "stz_default_params", "stz_strerror", "stz_free",
     "stz_compress", "stz_decompress", "stz_decompress_roi", "stz_decompress_slice", "stz_decompress_level",
@@ -90,6 +90,7 @@ def lib():
     vp, sz, i32, dbl = C.c_void_p, C.c_size_t, C.c_int, C.c_double
     P = C.POINTER
     L.stz_create.argtypes = [P(vp), i32]
+    L.stz_create_ex.argtypes = [P(vp), i32, C.c_uint]
     L.stz_destroy.argtypes = [vp]
     L.stz_destroy.restype = None
     L.stz_set_stream.argtypes = [vp, vp]

@@ -203,11 +203,13 @@ class DStream:
 
 
 class Context:
-    """One HIP stream + memory pool on one device (stz_ctx)."""
+    """One HIP stream + memory pool on one device (stz_ctx).  own_queue: the
+    context's streams get hardware queues of their own (STZ_CTX_OWN_QUEUE),
+    so its work overlaps other streams' whatever GPU_MAX_HW_QUEUES is."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, own_queue=False):
         h = C.c_void_p()
-        check(lib().stz_create(C.byref(h), device), "stz_create")
+        check(lib().stz_create_ex(C.byref(h), device, 1 if own_queue else 0), "stz_create_ex")
         self.h = h
 
     def close(self):

@@ -8,20 +8,21 @@ queued requests of its class (worker 0: full-resolution decodes; the
 others: boxes, slices and the 1/8, 1/4, 1/2 levels) that fit its slot limit and the server's
 memory budget and serves them with ONE stz_dstream_decompress_batch call: one Huffman decode of
 the union of their chunks, one launch for all their low regions and one for
-all their high boxes.  Requests that arrive while batches are on the GPU wait
+all their high boxes; identical requests in a batch are decoded once and
+share the result.  Requests that arrive while batches are on the GPU wait
 only for the next free worker; batches of different workers overlap on the
 device (the library call releases the GIL).  Latency is arrival to
 completion of the batch that served the request.
 
 The workers' batches overlap only when their streams sit on different
-hardware queues: HIP spreads streams over GPU_MAX_HW_QUEUES queues (4 by
-default, shared with torch's streams), and streams that share one run in
-order, so a box batch then waits for a full decode queued before it.  A
-context holds up to two streams (the second for the overlapped parts of a
-full decode or compress).  Set GPU_MAX_HW_QUEUES to at least 2 x workers + 4
-before HIP starts (RoiServer warns otherwise).  The full-decode worker's persistent level kernels take only
-`full_share` of the resident workgroup slots (stz_set_grid_share), so a box
-batch finds free CUs while a full decode runs.
+hardware queues: HIP spreads ordinary streams over GPU_MAX_HW_QUEUES queues
+(4 by default, shared with torch's streams), and streams that share one run
+in order, so a box batch would wait for a full decode queued before it.  The
+workers' contexts are therefore created with STZ_CTX_OWN_QUEUE: their streams
+carry a CU mask over every CU, which gives each a hardware queue of its own
+at any GPU_MAX_HW_QUEUES.  The full-decode worker's persistent level kernels
+take only `full_share` of the resident workgroup slots (stz_set_grid_share),
+so a box batch finds free CUs while a full decode runs.
 """
 import collections
 import threading
@@ -50,15 +51,8 @@ class _Pending:
 class RoiServer:
     def __init__(self, ds, workers=3, max_batch=32, mem_budget=16 << 30, device=0, keep_base=True,
                  keep_results=False, full_share=0.5):
-        import os
-        import warnings
         import torch
         from .api import Context
-        hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
-        if workers > 1 and hwq < 2 * workers + 4:
-            warnings.warn("GPU_MAX_HW_QUEUES=%d: %d workers' streams share hardware queues, so batches of "
-                          "different workers run in order; set it to >= %d before HIP starts"
-                          % (hwq, workers, 2 * workers + 4))
         self.torch = torch
         self.ds = ds
         self.dev = torch.device("cuda", device)
@@ -68,7 +62,7 @@ class RoiServer:
         self.max_batch = max_batch
         self.mem_budget = mem_budget
         self.keep_results = keep_results
-        self.ctxs = [Context(device) for _ in range(workers)]
+        self.ctxs = [Context(device, own_queue=True) for _ in range(workers)]
         if workers > 1:
             # the full-resolution decodes' persistent kernels leave part of
             # the GPU to the other workers' batches
@@ -134,14 +128,16 @@ class RoiServer:
                         self.cv.wait(0.01)
             t0 = time.perf_counter()
             try:
-                outs = [torch.empty(self.ds.request_capacity(p.req), dtype=self.tdtype, device=self.dev)
-                        for p in batch]
-                res = self.ds.decompress_batch([p.req for p in batch], outs=outs, ctx=ctx)
+                # identical requests in one batch (the same level, box or
+                # slice) are decoded once and share the result
+                uniq = list(dict.fromkeys(tuple(p.req) for p in batch))
+                outs = [torch.empty(self.ds.request_capacity(r), dtype=self.tdtype, device=self.dev) for r in uniq]
+                res = dict(zip(uniq, self.ds.decompress_batch(uniq, outs=outs, ctx=ctx)))
                 t1 = time.perf_counter()
-                for p, r in zip(batch, res):
+                for p in batch:
                     p.t_done = t1
                     p.batch = len(self.batches)
-                    p.out = r if self.keep_results else None
+                    p.out = res[tuple(p.req)] if self.keep_results else None
             except Exception as e:  # reported by close()
                 t1 = time.perf_counter()
                 self.errors.append(repr(e))

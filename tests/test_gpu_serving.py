@@ -146,6 +146,27 @@ def test_server_results_identical_alone_and_in_mix(stz, ctx):
     ds.free()
 
 
+def test_server_coalesces_identical_requests(stz, ctx):
+    """Identical requests queued together are served by one batch, decoded
+    once, and each gets the request's result."""
+    from stz_amd import serve
+    from stz_amd import fields
+    f = fields.sinusoids(shape=(40, 48, 56), seed=4)
+    ds = ctx.compress_device(f, 1e-3, "rel")
+    full = ds.decompress()
+    reqs = [("level", 3)] * 3 + [("box", 2, 3, 4, 20, 30, 25)] * 4 + [("level", 1)]
+    srv = serve.RoiServer(ds, workers=1, max_batch=16, mem_budget=1 << 30, keep_results=True)
+    try:
+        recs, _ = srv.run_trace(reqs, [0.0] * len(reqs))
+    finally:
+        srv.close()
+    for p in recs:
+        assert np.array_equal(_bits(p.out.cpu().numpy()), _bits(_expected(p.req, full))), p.req
+    # one worker, all queued at once: the duplicates ride in shared batches
+    assert len(srv.batches) < len(reqs) and max(b[1] for b in srv.batches) > 1
+    ds.free()
+
+
 def test_server_tiny_memory_budget_serializes(stz, ctx):
     import torch
     from stz_amd import fields, serve
