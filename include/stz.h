@@ -55,7 +55,7 @@ typedef enum { STZ_EB_ABS = 0, STZ_EB_REL = 1, STZ_EB_REF = 2 } stz_eb_mode;
 typedef struct {
     double mult_base, mult_low, mult_high; /* F5:2127-2129: 0.7 / 2.7 / 8.2 */
     int chunk_high, chunk_low;             /* F5:34-35 marker chunks: 32 / 16 */
-    int interp;                            /* base codec: 0 linear, 1 cubic */
+    int interp;                            /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo */
     int order;                             /* base codec axis order 0..5 */
     double alpha, beta;                    /* base codec level eb: eb/min(alpha^(l-1),beta) */
     int zstd_level;                        /* host zstd level (F5:3048: SZ3 default 3) */
@@ -209,18 +209,22 @@ int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
 
 /* ---- learned tuner -------------------------------------------------------
  * Chooses the residual predictor set (F5's; F5's PRED_PLANE_AXIS +
- * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16), the base interpolator, base
- * axis order and level-eb scaling (the tuning targets of F5:2120-2129 and
- * SZ3's sample-and-trial, F5:1439) among STZ_TUNE_CONFIGS configurations:
- * 3 predictor sets x 2 interpolators x 6 orders x 4 scalings, index
- * 48 * set + 24 * interp + 4 * order + scaling.
+ * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16), the base codec
+ * (interpolator and axis order, or Lorenzo) and the level-eb scaling (the
+ * tuning targets of F5:2120-2129 and SZ3's sample-and-trial between
+ * interpolation and Lorenzo, F5:1439) among STZ_TUNE_CONFIGS configurations:
+ * 3 predictor sets x (2 interpolators x 6 orders x 4 scalings + Lorenzo x 3
+ * scalings), index 51 * set + j, j = 24 * interp + 4 * order + scaling
+ * (j < 48) or 48 + scaling (Lorenzo).  The model has STZ_MLP_OUTPUTS outputs,
+ * the first STZ_TUNE_CONFIGS of which are the configurations.
  * The choice lands in the stream header; decoding and the error bound are
  * unchanged.  params.tuner = 1 compresses with the model's choice (falling
  * back to the exhaustive trial below the model's confidence threshold),
  * params.tuner = 2 with the trial's.  Tuning needs an ABS or REL bound. */
-#define STZ_TUNE_CONFIGS 144
+#define STZ_TUNE_CONFIGS 153
+#define STZ_MLP_OUTPUTS 160
 #define STZ_TUNE_FEATURES 16
-#define STZ_MLP_PARAMS 14608 /* 16x64 + 64 + 64x64 + 64 + 64x144 + 144 fp32 */
+#define STZ_MLP_PARAMS 15648 /* 16x64 + 64 + 64x64 + 64 + 64x160 + 160 fp32 */
 int stz_tune_config(int index, stz_params* p);
 /* 16 statistics per 64^3 sub-block (x fastest block order) */
 int stz_tune_features(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,

@@ -1,4 +1,4 @@
-"""CPU float64 reference of the learned tuner's MLP (16 -> 64 -> 64 -> 144,
+"""CPU float64 reference of the learned tuner's MLP (16 -> 64 -> 64 -> 160,
 ReLU, soft-target cross entropy) and of one Adam step.
 
 TEST INFRASTRUCTURE ONLY: the checker of the library's bf16 MFMA kernels
@@ -10,12 +10,12 @@ is float64 here (fp32 accumulation on the device).  With bf16=False it is
 the plain float64 network, whose gradient the tests pin by finite
 differences.
 
-Parameter layout (fp32 vector of 14608): W1[16][64], b1[64], W2[64][64],
-b2[64], W3[64][144], b3[144]; W[k][n] maps input k to output n.
+Parameter layout (fp32 vector of 15648): W1[16][64], b1[64], W2[64][64],
+b2[64], W3[64][160], b3[160]; W[k][n] maps input k to output n.
 """
 import numpy as np
 
-IN, HID, OUT = 16, 64, 144
+IN, HID, OUT = 16, 64, 160
 N_PARAMS = IN * HID + HID + HID * HID + HID + HID * OUT + OUT
 
 

@@ -31,6 +31,9 @@ class Params(C.Structure):
 # predictor options as bits of Params.pred: F5's PRED_WPOW2, PRED_PLANE_AXIS,
 # PRED_CUBE_AXIS; stz2.cpp's 12-point bicubic (blocks 2/4/5) and 16-point (block 6)
 PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS, PRED_BICUBIC12, PRED_DIAG16 = 1, 2, 4, 8, 16
+# base codec (Params.interp): interpolation linear / cubic, or Lorenzo over
+# independent LZ_TILE^3 tiles
+BASE_LINEAR, BASE_CUBIC, BASE_LORENZO, LZ_TILE = 0, 1, 2, 16
 
 
 def build():
@@ -63,6 +66,7 @@ def lib():
             getattr(L, "ora_dequantize_" + t).argtypes = [vp, sz, dbl, i32, vp, vp]
             getattr(L, "ora_interp_compress_" + t).restype = sz
             getattr(L, "ora_interp_compress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
+            getattr(L, "ora_interp_decompress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
         L.ora_huff_lengths.argtypes = [vp, i32, i32, vp]
         L.ora_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
         L.ora_payload_info.argtypes = [vp, sz, vp, vp, vp]
@@ -114,6 +118,7 @@ def header(payload):
         "eb_base": float(f64[13]), "eb_low": float(f64[14]), "eb_high": float(f64[15]),
         "vmin": float(f64[16]), "vmax": float(f64[17]),
         "total_bytes": int(u64[24]),
+        "interp": int(u32[40]), "order": int(u32[41]), "pred": int(u64[25]),
     }
 
 
@@ -227,6 +232,17 @@ def interp_compress(v, eb, interp=1, order=0, alpha=1.0, beta=1.0):
     unpred = np.empty(v.size, dtype=v.dtype)
     nu = getattr(lib(), "ora_interp_compress_" + _tsuf(v.dtype))(_ptr(v), bx, by, bz, float(eb), interp, order, alpha, beta, _ptr(codes), _ptr(unpred))
     return v, codes, unpred[:nu].copy()
+
+
+def interp_decompress(shape, dtype, eb, codes, unpred, interp=1, order=0, alpha=1.0, beta=1.0):
+    """The base codec's decode: the reconstruction from codes and unpredictables."""
+    bz, by, bx = shape
+    v = np.zeros(shape, dtype=dtype)
+    codes = np.ascontiguousarray(codes, np.uint16)
+    unpred = np.ascontiguousarray(unpred, dtype)
+    getattr(lib(), "ora_interp_decompress_" + _tsuf(np.dtype(dtype)))(_ptr(v), bx, by, bz, float(eb), interp, order,
+                                                                        alpha, beta, _ptr(codes), _ptr(unpred))
+    return v
 
 
 def plan_query(full_dims, q, chunk_high=32, chunk_low=16, cap=1 << 16):

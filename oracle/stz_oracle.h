@@ -26,12 +26,15 @@ extern "C" {
 #endif
 
 enum { ORA_F32 = 0, ORA_F64 = 1 };
+/* base codec (ora_params.interp): interpolation 0 linear / 1 cubic; 2 Lorenzo
+ * over independent ORA_LZ_TILE^3 tiles */
+enum { ORA_BASE_LINEAR = 0, ORA_BASE_CUBIC = 1, ORA_BASE_LORENZO = 2, ORA_LZ_TILE = 16 };
 enum { ORA_EB_ABS = 0, ORA_EB_REL = 1, ORA_EB_REF = 2 };
 
 typedef struct {
     double mult_base, mult_low, mult_high;   /* F5:2127-2129 defaults 0.7/2.7/8.2 */
     int chunk_high, chunk_low;               /* F5:34-35 defaults 32/16 */
-    int interp;                              /* base codec: 0 linear, 1 cubic */
+    int interp;                              /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo */
     int order;                               /* base codec axis order 0..5 */
     double alpha, beta;                      /* base codec level eb scaling */
     int pred;                                /* ORA_PRED_* bits: F5's compile-time predictor options */
@@ -106,6 +109,11 @@ size_t ora_interp_compress_f32(float* v, int bx, int by, int bz, double eb, int 
 size_t ora_interp_compress_f64(double* v, int bx, int by, int bz, double eb, int interp,
                                int order, double alpha, double beta, uint16_t* codes,
                                double* unpred);
+/* its decode: v (zero-filled by the caller) from codes and the unpredictables */
+void ora_interp_decompress_f32(float* v, int bx, int by, int bz, double eb, int interp, int order,
+                               double alpha, double beta, const uint16_t* codes, const float* unpred);
+void ora_interp_decompress_f64(double* v, int bx, int by, int bz, double eb, int interp, int order,
+                               double alpha, double beta, const uint16_t* codes, const double* unpred);
 
 /* ROI planner (F5:1602-1830).  Fills per-block plans for the high and low
  * levels: for each of 7 blocks, [valid, x0,x1,y0,y1,z0,z1, nchunks] and the

@@ -26,8 +26,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib")
-SOURCES = ["k_huff.hip", "k_field.hip", "k_residual.hip", "k_interp.hip", "k_tuner.hip", "k_stage.hip", "host_codec.cpp",
-           "stz_api.cpp"]
+SOURCES = ["k_huff.hip", "k_field.hip", "k_residual.hip", "k_interp.hip", "k_lorenzo.hip", "k_tuner.hip", "k_stage.hip",
+           "host_codec.cpp", "stz_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STZ_OFFLOAD_ARCH", "gfx950")

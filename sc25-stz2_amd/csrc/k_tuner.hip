@@ -1,5 +1,5 @@
 // k_tuner.hip -- the learned tuner: per-sub-block field statistics and a
-// 16 -> 64 -> 64 -> 144 MLP (ReLU) whose forward pass, backward pass and Adam
+// 16 -> 64 -> 64 -> 160 MLP (ReLU) whose forward pass, backward pass and Adam
 // update run here.  Matrix products are bf16 MFMA (v_mfma_f32_16x16x16_bf16)
 // with fp32 accumulation; weights are kept in fp32 (master copy) and rounded
 // to bf16 (round to nearest even) when a workgroup stages them in LDS.

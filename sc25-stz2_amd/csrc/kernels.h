@@ -283,6 +283,27 @@ void launch_level_multi(hipStream_t s, int mode, const LevelJob* jobs, uint32_t 
 uint32_t level_tiles(const Geom& g, int waves);
 uint32_t level_edge_groups(const Geom& g, int waves);
 
+// ---- k_lorenzo.hip ----
+// Lorenzo base codec (interp == 2): one workgroup per kLorenzoTile^3 tile of
+// the base grid.  Encode reconstructs v in place and writes codes, the raw
+// values of unpredictable points (side, stream positions) and the base
+// stream's histogram; decode writes v from codes and side.
+constexpr int kLorenzoTile = 16;
+struct LorenzoJob {
+    void* v;
+    uint16_t* codes;
+    void* side;
+    uint32_t* hist;   // encode only
+    int bx, by, bz;
+    double eb, recip;
+    uint32_t wg0;     // multi-field launch: the field's first workgroup
+};
+uint32_t lorenzo_tiles(int bx, int by, int bz);
+template <typename T>
+void launch_lorenzo(hipStream_t s, const LorenzoJob& j, bool encode);
+template <typename T>
+void launch_lorenzo_multi(hipStream_t s, const LorenzoJob* jobs, uint32_t njobs, uint32_t wgs, bool encode);
+
 // ---- k_interp.hip ----
 struct InterpPass {
     int start[3], step[3], cnt[3];
@@ -366,7 +387,7 @@ void launch_interp_decode(hipStream_t s, const InterpPass& p, T* v, const uint16
                           const T* side);
 
 // ---- k_tuner.hip ----
-constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 144;
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 160;
 constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid * kMlpHid + kMlpHid + kMlpHid * kMlpOut + kMlpOut;
 constexpr int kFeatSums = 18;   // per-sub-block sums of k_block_stats
 void launch_mlp_forward(hipStream_t s, const float* P, const float* X, uint32_t n, float* Y);

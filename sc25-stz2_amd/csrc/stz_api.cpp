@@ -755,7 +755,7 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
 {
     const stz_params& prm = fe.prm;
     check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
-    if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 1 || prm.order < 0 ||
+    if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 2 || prm.order < 0 ||
         prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
         prm.pred > kPredAll)
         throw StzError(STZ_E_ARG, "bad parameters");
@@ -1155,6 +1155,18 @@ struct JoinGuard {
 template <typename T>
 void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& dd, T* v, int stop_level)
 {
+    if (m.fh.interp == 2) {
+        // Lorenzo tiles: the whole base grid whatever stop_level asks
+        LorenzoJob j{};
+        j.v = v;
+        j.codes = dd.codes.as<uint16_t>() + dd.code_base[0];
+        j.side = dd.side.as<T>() + dd.code_base[0];
+        j.bx = L.g[0].dx; j.by = L.g[0].dy; j.bz = L.g[0].dz;
+        j.eb = m.fh.eb_base;
+        j.recip = 1.0 / m.fh.eb_base;
+        K(c, "lorenzo_decode", [&] { launch_lorenzo<T>(c->stream, j, false); });
+        return;
+    }
     const auto passes = make_passes(L.g[0].dx, L.g[0].dy, L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
                                     (int)m.fh.order, m.fh.alpha, m.fh.beta, stop_level);
     run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
@@ -1427,10 +1439,13 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 }
 
 // ---------------------------------------------------------------- tuner
-// The configurations the tuner chooses among: predictor set (3) x base
-// interpolator (2) x base axis order (6) x level-eb scaling (4); index
-// k = 48 * set + 24 * interp + 4 * order + scaling, so 0..47 are F5's own
-// predictors.  Predictor sets: F5's (F5:690-748); F5 built with
+// The configurations the tuner chooses among: predictor set (3) x base codec
+// (interpolation: 2 interpolators x 6 axis orders x 4 level-eb scalings;
+// Lorenzo: 3 scalings -- the fourth differs from the first only in the
+// interpolation levels' alpha / beta), 51 per set; index
+// k = 51 * set + j, j = 24 * interp + 4 * order + scaling below 48 and
+// 48 + scaling for Lorenzo, so 0..47 are F5's own predictors with
+// interpolation.  Predictor sets: F5's (F5:690-748); F5 built with
 // PRED_PLANE_AXIS and PRED_CUBE_AXIS (F5:579-622, F5:627-686); stz2.cpp's
 // 12-point plane and 16-point cube predictors (stz2.cpp:122-196).  Scalings: F5's multipliers
 // (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
@@ -1441,15 +1456,19 @@ struct TuneScale { double mb, ml, mh, alpha, beta; };
 const TuneScale kScales[4] = {
     {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
 const int kTunePred[3] = {0, kPredPlaneAxis | kPredCubeAxis, kPredBicubic12 | kPredDiag16};
-constexpr int kTuneConfigs = 144;
-static_assert(kTuneConfigs == kMlpOut, "one MLP output per configuration");
+constexpr int kTunePerSet = 51;
+constexpr int kTuneConfigs = 3 * kTunePerSet;
+// one MLP output per configuration; the outputs past them (the MFMA tiles
+// are 16 wide) are never chosen
+static_assert(kTuneConfigs <= kMlpOut && kMlpOut - kTuneConfigs < 16, "MLP outputs");
 
 void tune_config(int k, stz_params& p)
 {
-    p.pred = kTunePred[k / 48];
-    p.interp = (k / 24) % 2;
-    p.order = (k / 4) % 6;
-    const TuneScale& t = kScales[k % 4];
+    p.pred = kTunePred[k / kTunePerSet];
+    const int j = k % kTunePerSet;
+    p.interp = j < 48 ? j / 24 : 2;
+    p.order = j < 48 ? (j / 4) % 6 : 0;
+    const TuneScale& t = kScales[j < 48 ? j % 4 : j - 48];
     p.mult_base = t.mb;
     p.mult_low = t.ml;
     p.mult_high = t.mh;
@@ -1645,26 +1664,26 @@ void learned_choice_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, std::vector<
     for (size_t i = 0; i < n; ++i) {
         double wsum = 0;
         for (size_t b = off[i]; b < off[i + 1]; ++b) wsum += hs[b * kFeatSums];
-        std::vector<double> lp(kMlpOut, 0.0), pr(kMlpOut, 0.0);
-        double e[kMlpOut];
+        std::vector<double> lp(kTuneConfigs, 0.0), pr(kTuneConfigs, 0.0);
+        double e[kTuneConfigs];
         for (size_t b = off[i]; b < off[i + 1]; ++b) {
             const double w = wsum > 0 ? hs[b * kFeatSums] / wsum : 1.0 / (double)(off[i + 1] - off[i]);
             const float* r = hy + b * kMlpOut;
             double mx = r[0];
-            for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
+            for (int k = 1; k < kTuneConfigs; ++k) mx = std::max(mx, (double)r[k]);
             double se = 0;
-            for (int k = 0; k < kMlpOut; ++k) {
+            for (int k = 0; k < kTuneConfigs; ++k) {
                 e[k] = std::exp(r[k] - mx);
                 se += e[k];
             }
             const double lse = mx + std::log(se);
-            for (int k = 0; k < kMlpOut; ++k) {
+            for (int k = 0; k < kTuneConfigs; ++k) {
                 lp[k] += w * (r[k] - lse);
                 pr[k] += w * e[k] / se;
             }
         }
         int best = 0;
-        for (int k = 1; k < kMlpOut; ++k)
+        for (int k = 1; k < kTuneConfigs; ++k)
             if (lp[k] > lp[best]) best = k;
         tune_config(best, fe[i].prm);
         if (confs) (*confs)[i] = (float)pr[best];
@@ -1729,13 +1748,30 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         }
     }
     for (int i = 0; i < n; ++i) enc_phase1(c, fe[i]);
-    // base level: coarse passes of every field, then the fine-pass rounds
+    // base level: coarse passes of every field, then the fine-pass rounds;
+    // Lorenzo-coded bases (interp 2) in one tile launch
     {
         std::vector<FusedEncJob> fj;
         std::vector<std::vector<PassEncJob>> rounds;
+        std::vector<LorenzoJob> lz;
+        uint32_t lz_wgs = 0;
         for (int i = 0; i < n; ++i) {
             FieldEnc<T>& f = fe[i];
             const HostLayout& L = f.L;
+            if (f.prm.interp == 2) {
+                LorenzoJob j{};
+                j.v = f.vbase.p;
+                j.codes = f.codes.template as<uint16_t>() + L.code_off[0];
+                j.side = f.side.template as<T>() + L.code_off[0];
+                j.hist = f.hist.template as<uint32_t>();
+                j.bx = L.g[0].dx; j.by = L.g[0].dy; j.bz = L.g[0].dz;
+                j.eb = f.ebl[0];
+                j.recip = 1.0 / f.ebl[0];
+                j.wg0 = lz_wgs;
+                lz_wgs += lorenzo_tiles(j.bx, j.by, j.bz);
+                lz.push_back(j);
+                continue;
+            }
             const auto passes = make_passes(L.g[0].dx, L.g[0].dy, L.g[0].dz, f.ebl[0], f.prm.interp, f.prm.order,
                                             f.prm.alpha, f.prm.beta, 0);
             uint16_t* c0 = f.codes.template as<uint16_t>() + L.code_off[0];
@@ -1763,6 +1799,14 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 rounds[r].push_back(j);
                 ++r;
             }, n > 1);
+        }
+        if (lz.size() == 1) {
+            K(c, "lorenzo_encode", [&] { launch_lorenzo<T>(st, lz[0], true); });
+        } else if (!lz.empty()) {
+            DBuf dlz = upload_vec(c, lz);
+            K(c, "lorenzo_encode", [&] {
+                launch_lorenzo_multi<T>(st, dlz.as<LorenzoJob>(), (uint32_t)lz.size(), lz_wgs, true);
+            });
         }
         DBuf dfj = upload_vec(c, fj);
         K(c, "interp_encode_fused", [&] { launch_interp_encode_fused_multi<T>(st, dfj.as<FusedEncJob>(), (uint32_t)fj.size()); });
@@ -2116,6 +2160,8 @@ struct BatchPlan {
     uint32_t n_st = 0, n_un = 0, wg_st = 0, wg_un = 0, chunks = 0;
     DBuf fused;
     uint32_t n_fused = 0;
+    DBuf lz;                   // Lorenzo-coded bases: tile jobs
+    uint32_t n_lz = 0, lz_wgs = 0;
     std::vector<DBuf> rounds;
     std::vector<std::pair<uint32_t, uint32_t>> round_n;  // jobs, workgroups
     JobSet low, high;
@@ -2180,6 +2226,7 @@ void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
         launch_huff_decode_multi(st, P.dec_un.as<DecJob>(), P.n_un, P.wg_un, kDecDense);
     });
     K(c, "interp_decode_fused", [&] { launch_interp_decode_fused_multi<T>(st, P.fused.as<FusedJob>(), P.n_fused); });
+    K(c, "lorenzo_decode", [&] { launch_lorenzo_multi<T>(st, P.lz.as<LorenzoJob>(), P.n_lz, P.lz_wgs, false); });
     for (size_t r = 0; r < P.rounds.size(); ++r)
         K(c, "interp_decode", [&] {
             launch_interp_decode_multi<T>(st, P.rounds[r].as<PassJob>(), P.round_n[r].first, P.round_n[r].second);
@@ -2199,6 +2246,7 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
     std::vector<DecJob> all, stj, unj;
     std::vector<FusedJob> fj;
     std::vector<std::vector<PassJob>> rounds;
+    std::vector<LorenzoJob> lz;
     uint32_t chunk = 0;
     for (int i = 0; i < n; ++i) {
         BatchPlan::Field& F = P.f[i];
@@ -2220,12 +2268,25 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
             (F.stage ? P.wg_st : P.wg_un) += F.ss.total_wgs;
             (F.stage ? stj : unj).push_back(j);
         }
-        // base passes
+        // base passes (Lorenzo-coded bases: tile jobs)
         const Meta& m = ds->meta;
-        const auto passes = make_passes(F.L.g[0].dx, F.L.g[0].dy, F.L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
-                                        (int)m.fh.order, m.fh.alpha, m.fh.beta, 0);
         const uint16_t* c0 = F.dd.codes.as<uint16_t>() + F.dd.code_base[0];
         const T* s0 = F.dd.side.as<T>() + F.dd.code_base[0];
+        if (m.fh.interp == 2) {
+            LorenzoJob q{};
+            q.v = F.vb.p;
+            q.codes = (uint16_t*)c0;
+            q.side = (void*)s0;
+            q.bx = F.L.g[0].dx; q.by = F.L.g[0].dy; q.bz = F.L.g[0].dz;
+            q.eb = m.fh.eb_base;
+            q.recip = 1.0 / m.fh.eb_base;
+            q.wg0 = P.lz_wgs;
+            P.lz_wgs += lorenzo_tiles(q.bx, q.by, q.bz);
+            lz.push_back(q);
+            continue;
+        }
+        const auto passes = make_passes(F.L.g[0].dx, F.L.g[0].dy, F.L.g[0].dz, m.fh.eb_base, (int)m.fh.interp,
+                                        (int)m.fh.order, m.fh.alpha, m.fh.beta, 0);
         size_t r = 0;
         run_passes(passes, sizeof(T), [&](const InterpPassList& fl) {
             if (fl.n <= 0) return;
@@ -2256,6 +2317,8 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
     P.dec_un = upload_vec(c, unj);
     P.n_fused = (uint32_t)fj.size();
     P.fused = upload_vec(c, fj);
+    P.n_lz = (uint32_t)lz.size();
+    P.lz = upload_vec(c, lz);
     for (auto& rj : rounds) {
         uint32_t wg = 0;
         for (PassJob& q : rj) {
@@ -2353,7 +2416,7 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     std::memcpy(&m.fh, p, sizeof(FileHeader));
     const FileHeader& fh = m.fh;
     if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
-        fh.pred > (uint64_t)kPredAll ||
+        fh.pred > (uint64_t)kPredAll || fh.interp > 2 || fh.order > 5 || !(fh.alpha >= 1.0) || !(fh.beta >= 1.0) ||
         (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
         throw StzError(STZ_E_STREAM, "bad payload header");
     if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))
@@ -2599,25 +2662,25 @@ int tune_learned(stz_ctx* c, const T* d, size_t nx, size_t ny, size_t nz, double
     float* hy = hs + f.size();
     HIPCHK(hipMemcpyAsync(hy, y.p, nb * kMlpOut * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<double> lp(kMlpOut, 0.0), pr(kMlpOut, 0.0);
+    std::vector<double> lp(kTuneConfigs, 0.0), pr(kTuneConfigs, 0.0);
     for (size_t b = 0; b < nb; ++b) {
         const float* r = hy + b * kMlpOut;
         double mx = r[0];
-        for (int k = 1; k < kMlpOut; ++k) mx = std::max(mx, (double)r[k]);
-        double se = 0, e[kMlpOut];
-        for (int k = 0; k < kMlpOut; ++k) {
+        for (int k = 1; k < kTuneConfigs; ++k) mx = std::max(mx, (double)r[k]);
+        double se = 0, e[kTuneConfigs];
+        for (int k = 0; k < kTuneConfigs; ++k) {
             e[k] = std::exp(r[k] - mx);
             se += e[k];
         }
         const double lse = mx + std::log(se);
         const double w = cnt[b] / wsum;
-        for (int k = 0; k < kMlpOut; ++k) {
+        for (int k = 0; k < kTuneConfigs; ++k) {
             lp[k] += w * (r[k] - lse);
             pr[k] += w * e[k] / se;
         }
     }
     int best = 0;
-    for (int k = 1; k < kMlpOut; ++k)
+    for (int k = 1; k < kTuneConfigs; ++k)
         if (lp[k] > lp[best]) best = k;
     conf = (float)pr[best];
     return best;

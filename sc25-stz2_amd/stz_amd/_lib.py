@@ -32,7 +32,7 @@ EXPORTS = [
     "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
     "stz_compress_batch", "stz_decompress_batch", "stz_autotune",
 ]
-TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS = 144, 16, 14608
+TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS, MLP_OUTPUTS = 153, 16, 15648, 160
 
 
 class StzError(RuntimeError):

@@ -1,7 +1,7 @@
-"""Learned tuner: a 16 -> 64 -> 64 -> 144 MLP over per-sub-block field
-statistics that picks the residual predictor set, base interpolator, base
-axis order and level-eb scaling (144 configurations) in place of the
-exhaustive trial.
+"""Learned tuner: a 16 -> 64 -> 64 -> 160 MLP over per-sub-block field
+statistics that picks the residual predictor set, base codec (interpolator
+and axis order, or Lorenzo) and level-eb scaling (153 configurations, the
+first 153 outputs) in place of the exhaustive trial.
 
 Labels come from the exhaustive trial itself (the payload size of every
 configuration on generated fields); the target of a sub-block is the soft
@@ -21,12 +21,12 @@ import time
 
 import numpy as np
 
-from ._lib import MLP_PARAMS, TUNE_CONFIGS, TUNE_FEATURES, Params, check, lib
+from ._lib import MLP_OUTPUTS, MLP_PARAMS, TUNE_CONFIGS, TUNE_FEATURES, Params, check, lib
 from .api import EB_MODES, _dtype_id, _ptr
 
 DEFAULT_CONFIG = 24          # cubic, order 0, F5 multipliers: the untuned path
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_model.npz")
-_IN, _HID, _OUT = 16, 64, TUNE_CONFIGS
+_IN, _HID, _OUT = 16, 64, MLP_OUTPUTS   # outputs past TUNE_CONFIGS are never chosen
 
 
 def config_params(k):
@@ -73,9 +73,13 @@ def trial_sizes(ctx, field, eb, mode="rel"):
 
 
 def soft_target(sizes, tau=0.01):
+    """Target distribution over the MLP's outputs: configurations by payload
+    size, zero on the outputs past them."""
     lg = np.log(sizes.astype(np.float64))
     w = np.exp(-(lg - lg.min()) / tau)
-    return (w / w.sum()).astype(np.float32)
+    t = np.zeros(_OUT, np.float32)
+    t[: sizes.size] = w / w.sum()
+    return t
 
 
 class Mlp:
@@ -176,7 +180,7 @@ def make_field(rng, device):
     import torch
     from . import fields
     shape = tuple(int(v) for v in rng.integers(48, 161, size=3))
-    fam = rng.choice(["grf", "aniso", "lognormal", "sines", "fronts", "rough"])
+    fam = rng.choice(["grf", "aniso", "lognormal", "sines", "fronts", "rough", "blocks"])
     seed = int(rng.integers(1 << 30))
     if fam in ("grf", "rough", "lognormal", "fronts"):
         g = fields.grf(shape, float(rng.uniform(-5.0, -1.5)), seed, device)
@@ -205,8 +209,15 @@ def make_field(rng, device):
         s = float(rng.uniform(-5.0, -2.0))
         with np.errstate(divide="ignore"):
             F *= np.where(k > 0, k ** (s / 2.0), 0.0)
-        f = np.fft.irfftn(F, s=shape)
+        f = np.fft.irfftn(F, s=shape, axes=(0, 1, 2))
         f = (f - f.mean()) / f.std()
+    elif fam == "blocks":
+        # piecewise-constant cells of one random size (AMR-like patches) over
+        # a weak smooth field: where the Lorenzo base beats interpolation
+        b = int(rng.integers(3, 17))
+        vals = rng.normal(size=[(n + b - 1) // b for n in shape])
+        f = np.kron(vals, np.ones((b, b, b)))[: shape[0], : shape[1], : shape[2]]
+        f = f + float(rng.uniform(0.0, 0.1)) * fields.grf(shape, -3.0, seed, device)
     else:
         f = fields.sinusoids(shape=shape, seed=seed, dtype=np.float64)
     dt = np.float64 if rng.random() < 0.3 else np.float32
@@ -259,7 +270,7 @@ def field_choice(model, X_field):
     """argmax of the mean log-softmax over a field's sub-blocks; confidence =
     mean probability of the chosen configuration."""
     X_field = X_field[informative(X_field)]
-    Y = model.forward(X_field).cpu().numpy().astype(np.float64)
+    Y = model.forward(X_field).cpu().numpy().astype(np.float64)[:, :TUNE_CONFIGS]
     Y -= Y.max(axis=1, keepdims=True)
     lp = Y - np.log(np.exp(Y).sum(axis=1, keepdims=True))
     k = int(np.argmax(lp.mean(axis=0)))

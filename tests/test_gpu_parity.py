@@ -145,14 +145,15 @@ def test_payload_and_decode_bitexact(ctx, oracle, case):
 
 
 def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
-    """All 144 configurations the learned tuner chooses among (predictor set x
-    base interpolator x axis order x level-bound scaling): payload bytes equal
-    the oracle's, f32 and f64, and a spread of them decode bit-exactly."""
+    """All 153 configurations the learned tuner chooses among (predictor set x
+    base codec -- interpolator x axis order, or Lorenzo -- x level-bound
+    scaling): payload bytes equal the oracle's, f32 and f64, and a spread of
+    them decode bit-exactly."""
     from stz_amd import tuner
     f32 = _field("grf", (36, 40, 44), np.float32, seed=17)
     f64 = _field("sin", (20, 24, 28), np.float64, seed=5)
     from stz_amd._lib import TUNE_CONFIGS
-    assert TUNE_CONFIGS == 144
+    assert TUNE_CONFIGS == 153
     for k in range(TUNE_CONFIGS):
         prm = tuner.config_params(k)
         for f, eb in ((f32, 1e-3), (f64, 1e-5)):
@@ -160,7 +161,7 @@ def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
             ds, gpu, want = _compress_both(ctx, oracle, f, eb, "rel", **prm)
             diff = first_difference(gpu, want)
             assert diff is None, (k, f.dtype.name, diff)
-            if k in (0, 13, 30, 47, 61, 100, 143):
+            if k in (0, 13, 30, 47, 48, 61, 100, 150, 152):
                 _assert_bitexact(ds.decompress(), oracle.decompress(want), "decompress config %d" % k)
             ds.free()
 
@@ -415,4 +416,89 @@ def test_predictor_options_random_access_and_batch(ctx, oracle):
         _assert_bitexact(o.cpu().numpy(), s.decompress(), "batch decode")
         s.free()
     for b in batch:
+        b.free()
+
+
+# ------------------------------------------------------------------ Lorenzo base
+# interp = 2: the base grid in independent 16^3 Lorenzo tiles (SZ3's
+# ALGO_INTERP_LORENZO alternative).  Base grids from 1 to 2x2x3 tiles, with
+# partial tiles on every axis.
+LZ_CASES = [("lognormal", (64, 56, 64), np.float32, 1e-3, "rel"), ("grf", (48, 40, 56), np.float64, 1e-6, "rel"),
+            ("sin", (9, 130, 70), np.float32, 1e-2, "abs"), ("noise", (20, 24, 28), np.float64, 1e-5, "rel"),
+            ("grf", (4, 4, 4), np.float32, 1e-3, "rel"), ("grf", (128, 96, 160), np.float32, 1e-3, "rel")]
+
+
+@pytest.mark.parametrize("case", LZ_CASES, ids=[("%s-%s-%s-%g-%s" % (c[0], "x".join(map(str, c[1])), c[2].__name__,
+                                                                      c[3], c[4])) for c in LZ_CASES])
+def test_lorenzo_base_bitexact(ctx, oracle, case):
+    kind, shape, dtype, eb, mode = case
+    _say("lorenzo base", case)
+    f = _field(kind, shape, dtype, seed=77)
+    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=2)
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    assert ds.info.interp == 2 and oracle.header(want)["interp"] == 2
+    rec = ds.decompress()
+    _assert_bitexact(rec, oracle.decompress(want), "decompress")
+    assert max_error(rec, f) <= requested_bound(f, eb, mode)
+    ds.free()
+
+
+def test_lorenzo_base_degenerate_fields(ctx, oracle):
+    """Constant, zero, one-point, huge-bound and non-finite fields with the
+    Lorenzo base: payload and decode equal the oracle's."""
+    for kind, eb, mode in DEGENERATE:
+        _say("lorenzo degenerate", kind)
+        f = _degenerate(kind, np.float32)
+        ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=2)
+        got, ref = ds.decompress(), oracle.decompress(want)
+        if kind.startswith("nan"):
+            gpu, want = nan_sign_cleared(gpu, np.float32), nan_sign_cleared(want, np.float32)
+            assert np.array_equal(np.isnan(got), np.isnan(ref))
+            got, ref = nan_sign_cleared_values(got), nan_sign_cleared_values(ref)
+        diff = first_difference(gpu, want)
+        assert diff is None, (kind, diff)
+        _assert_bitexact(got, ref, "decompress " + kind)
+        ds.free()
+
+
+def test_lorenzo_base_random_access_and_batch(ctx, oracle):
+    """Box, slice and progressive-level decodes of a Lorenzo-based stream
+    equal the oracle's; a batch decode mixing Lorenzo and interpolation bases
+    equals one call per stream; a batch compress equals single compresses."""
+    import torch
+    f = _field("lognormal", (64, 56, 64), np.float32, seed=21)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", interp=2, pred=24, chunk_high=8, chunk_low=4)
+    assert first_difference(gpu, want) is None
+    full = oracle.decompress(want)
+    for q in [(3, 5, 7, 50, 30, 40), (10, 10, 10, 10, 55, 63), (0, 0, 0, 63, 55, 63)]:
+        _assert_bitexact(ds.decompress_roi(*q), oracle.decompress_roi(want, *q), "roi %s" % (q,))
+    for lv in range(4):
+        _assert_bitexact(ds.decompress_level(lv), oracle.decompress_level(want, lv), "level %d" % lv)
+    s = ds.download(3)
+    got = ctx.decompress_slice(s, 1, 17)
+    _assert_bitexact(got, full[:, 17:18, :], "slice")
+    ds.prepare(keep_base=True)
+    reqs = [("box", 1, 2, 3, 40, 30, 20), ("slice", 2, 17), ("level", 0)]
+    for r, g in zip(reqs, ds.decompress_batch(reqs)):
+        w = full[3:21, 2:31, 1:41] if r[0] == "box" else full[17:18] if r[0] == "slice" else full[::8, ::8, ::8]
+        _assert_bitexact(g, w, "batch request %s" % (r,))
+    ds.free()
+    dev = torch.device("cuda:0")
+    fs = [torch.from_numpy(_field("grf", sh, np.float32, seed=50 + i)).to(dev)
+          for i, sh in enumerate([(33, 47, 61), (16, 20, 24), (70, 36, 90)])]
+    lz = ctx.compress_batch(fs, 1e-3, "rel", interp=2)
+    it = ctx.compress_batch(fs, 1e-3, "rel", interp=1)
+    mixed = [lz[0], it[1], lz[2], it[0]]
+    outs = [torch.empty_like(fs[i]) for i in (0, 1, 2, 0)]
+    ctx.decompress_batch(mixed, outs)
+    for b, o in zip(mixed, outs):
+        _assert_bitexact(o.cpu().numpy(), b.decompress(), "mixed batch decode")
+    for t, b in zip(fs, lz):
+        single = ctx.compress_device(t, 1e-3, "rel", interp=2)
+        pb = b.download(-1)
+        assert np.array_equal(pb, single.download(-1))
+        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, interp=2)) is None
+        single.free()
+    for b in lz + it:
         b.free()

@@ -89,7 +89,7 @@ def test_adam_matches_cpu(stz, ctx, torch_dev):
 def test_training_lowers_loss(stz, ctx, torch_dev):
     """A learnable task: the target configuration is a fixed function of the features."""
     from stz_amd.tuner import Mlp
-    from stz_amd._lib import TUNE_CONFIGS as K
+    from stz_amd._lib import MLP_OUTPUTS as K
     rng = np.random.default_rng(5)
     X = rng.normal(size=(2048, 16)).astype(np.float32)
     W = rng.normal(size=(16, K))

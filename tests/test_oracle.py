@@ -561,6 +561,97 @@ def test_base_codec_cubic_reproduces_cubics(oracle):
     assert np.any(codes[rules == 5] != 32768) or np.any(codes[rules == 6] != 32768)
 
 
+# ---------------------------------------------------------------- Lorenzo base codec
+def _tile_local(shape):
+    """Per point (stream order) its coordinates inside its 16^3 tile, the
+    tile's origin, and the point's grid coordinates."""
+    bz, by, bx = shape
+    T = 16
+    out = []
+    for tz in range(0, bz, T):
+        for ty in range(0, by, T):
+            for tx in range(0, bx, T):
+                dz, dy, dx = min(T, bz - tz), min(T, by - ty), min(T, bx - tx)
+                for z in range(dz):
+                    for y in range(dy):
+                        for x in range(dx):
+                            out.append((x, y, z, tx + x, ty + y, tz + z))
+    return np.array(out, dtype=np.int64)
+
+
+@pytest.mark.parametrize("shape", [(21, 18, 35), (16, 16, 16), (5, 33, 17), (1, 1, 40)])
+def test_lorenzo_base_closed_form_on_linear_field(oracle, shape):
+    """f = 5 + 2x - 3y + z (integers) with eb = 0.5, where every residual is
+    an integer and quantizes exactly: in a tile's interior and on its faces
+    the 3D / 2D Lorenzo predictors are exact (code = radius); on an edge the
+    prediction is the previous point on that axis (residual = the slope);
+    a tile's corner is predicted by zero (residual = the value).  Pins the
+    stencil, the 16^3 tiles, the zero outside a tile and the stream order."""
+    bz, by, bx = shape
+    z, y, x = np.meshgrid(np.arange(bz), np.arange(by), np.arange(bx), indexing="ij")
+    v = (5 + 2 * x - 3 * y + z).astype(np.float64)
+    rec, codes, side = oracle.interp_compress(v, 0.5, oracle.BASE_LORENZO)
+    P = _tile_local(shape)
+    lx, ly, lz = P[:, 0], P[:, 1], P[:, 2]
+    val = 5 + 2 * P[:, 3] - 3 * P[:, 4] + P[:, 5]
+    zeros = (lx == 0).astype(int) + (ly == 0) + (lz == 0)
+    slope = np.where(lx > 0, 2, np.where(ly > 0, -3, 1))
+    want = np.where(zeros == 3, val, np.where(zeros == 2, slope, 0)) + 32768
+    assert np.array_equal(codes.astype(np.int64), want)
+    assert side.size == 0 and np.array_equal(rec, v)
+
+
+def test_lorenzo_base_exact_on_sums_of_two_axis_functions(oracle):
+    """Inside a tile the residual of the 3D Lorenzo predictor is the mixed
+    third difference: zero on g(x,y) + h(y,z) + k(x,z) for any g, h, k, and
+    exactly 1 per point on xyz (integer data, eb = 0.5 so quantization is
+    exact).  A wrong sign or a missing term of the seven-point stencil
+    leaves a residual on the first field."""
+    rng = np.random.default_rng(3)
+    n = 20
+    g, h, k = (rng.integers(-50, 50, size=(n, n)) for _ in range(3))
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    v = (g[x, y] + h[y, z] + k[x, z]).astype(np.float64)
+    _, codes, _ = oracle.interp_compress(v, 0.5, oracle.BASE_LORENZO)
+    P = _tile_local(v.shape)
+    inner = (P[:, 0] > 0) & (P[:, 1] > 0) & (P[:, 2] > 0)
+    assert inner.sum() > 3000 and np.all(codes[inner] == 32768)
+    w = (x * y * z).astype(np.float64)
+    _, codes, _ = oracle.interp_compress(w, 0.5, oracle.BASE_LORENZO)
+    assert np.all(codes[inner] == 32769)
+
+
+@pytest.mark.parametrize("shape,dt,eb", [((21, 18, 35), np.float32, 1e-2), ((40, 9, 33), np.float64, 1e-4),
+                                         ((3, 4, 5), np.float32, 0.3)])
+def test_lorenzo_base_round_trip_and_bound(oracle, shape, dt, eb):
+    """decode(encode) is the encoder's reconstruction bit for bit, within eb;
+    values beyond the code range are unpredictable and kept exactly."""
+    rng = np.random.default_rng(sum(shape))
+    v = np.cumsum(rng.normal(size=shape), axis=2).astype(dt)
+    v.flat[7] = 1e6
+    rec, codes, side = oracle.interp_compress(v, eb, oracle.BASE_LORENZO)
+    assert side.size >= 1 and np.count_nonzero(codes == 0) == side.size
+    got = oracle.interp_decompress(v.shape, dt, eb, codes, side, oracle.BASE_LORENZO)
+    assert np.array_equal(_bits(got), _bits(rec))
+    assert rec.flat[7] == v.flat[7]
+    assert np.abs(rec.astype(np.float64) - v).max() <= eb
+
+
+def test_lorenzo_base_whole_path(oracle):
+    """interp = 2 through the whole oracle path: the header names it, the
+    decode is within the bound, and the payload differs from interpolation's."""
+    f = _field((40, 36, 44), np.float32, 9)
+    p = oracle.compress(f, 1e-3, oracle.EB_REL, interp=2)
+    assert oracle.header(p)["interp"] == 2
+    r = oracle.decompress(p)
+    from payload_util import max_error, requested_bound
+    E = requested_bound(f, 1e-3, "rel")
+    assert max_error(r, f) <= E
+    assert not np.array_equal(p, oracle.compress(f, 1e-3, oracle.EB_REL, interp=1))
+    with pytest.raises(Exception):
+        oracle.compress(f, 1e-3, oracle.EB_REL, interp=3)
+
+
 # ---------------------------------------------------------------- whole oracle path
 def _field(shape, dt, seed):
     rng = np.random.default_rng(seed)
