@@ -16,6 +16,8 @@
 //               for longer codes
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace stz {
@@ -625,6 +627,23 @@ __global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict
     t.lut[w] = e | (uint64_t)cnt << kDecNShift | (uint64_t)l1 << kDecL1Shift | (uint64_t)used << kDecLtShift;
 }
 
+// Symbol at canonical index idx of a code longer than the table window:
+// from the LDS copy of the first ln of them (lsrt, indexed from llo) or, when
+// not all of them fit (kLdsLong false), from global memory past it.  An index
+// outside both (a corrupt bitstream) sets `bad`.
+template <bool kLdsLong>
+__device__ __forceinline__ uint32_t long_symbol(uint32_t idx, const uint16_t* __restrict__ srt, const uint16_t* lsrt,
+                                                uint32_t llo, uint32_t ln, uint32_t& bad)
+{
+    const uint32_t r = idx - llo;
+    if constexpr (kLdsLong) {
+        bad |= r < ln ? 0u : 1u;
+        return (uint32_t)lsrt[r < ln ? r : 0u];
+    } else {
+        return r < ln ? (uint32_t)lsrt[r] : (uint32_t)srt[idx];
+    }
+}
+
 // Sequential Huffman decode of one lane's sub-chunk.  Bits come from `W`
 // (the workgroup's LDS stage or global memory; word w of the stream is
 // W[w - wbase]); each refill word is fetched one refill ahead.  One table
@@ -632,7 +651,13 @@ __global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict
 // kLutBits bits).  Codes leave in 8-byte groups once the destination is
 // 8-byte aligned (`al` = element offset of co[0]); the first symbols up to
 // that alignment and the last ones go one at a time.
-template <bool kLdsBits>
+// kLdsLong: every symbol of a code longer than the window is in the LDS copy
+// `lsrt` (the workgroup checked), so that path reads LDS only.  Otherwise a
+// long code may read global memory, and on this GPU a global load waits for
+// every store the wave issued before it (the vector memory counter retires
+// loads and stores in order): with a long code every few steps in some lane
+// of a wave, the whole loop then runs at the speed of store completions.
+template <bool kLdsBits, bool kLdsLong>
 __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
                                                const DecItem& it, const uint64_t* __restrict__ lut,
                                                const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
@@ -677,7 +702,7 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
         }
         // the long codes' symbols: the workgroup's LDS copy of their first ln
         const uint32_t idx = fi[l] + (win >> (32 - l)) - fc[l];
-        return idx - llo < ln ? (uint32_t)lsrt[idx - llo] : (uint32_t)srt[idx];
+        return long_symbol<kLdsLong>(idx, srt, lsrt, llo, ln, bad);
     };
     // symbol 0 at stream position `pos` of the sub-chunk: the next raw value
     auto unpred = [&](uint32_t pos) {
@@ -764,12 +789,174 @@ __device__ __forceinline__ uint32_t decode_run(const uint32_t* __restrict__ W, u
     return bad;
 }
 
+// Canonical-code limits of the lengths above the table window, uniform over
+// a workgroup (scalar registers): lim[j] = first_code[l] + count[l] of length
+// l = kLutBits + 1 + j, left-aligned in 32 bits (64-bit values, since a
+// complete code's last limit is 2^32).  Canonical codes of increasing length
+// have increasing left-aligned values, so the length of the code that
+// prefixes a 32-bit window w is kLutBits + 1 + (the number of limits w is at
+// or above): twelve compares, no branches.  The symbol's place in the LDS
+// copy of the long codes' symbols is then loff[j] + (w >> (32 - l)), loff[j]
+// = first_index[l] - first_code[l] - (first long index), from an LDS table.
+constexpr int kLong = kLmax - kLutBits;  // 12 lengths
+struct LongLim {
+    uint64_t lim[kLong];
+};
+
+__device__ __forceinline__ LongLim long_limits(const uint32_t* fc, const uint32_t* cn)
+{
+    LongLim L;
+#pragma unroll
+    for (int j = 0; j < kLong; ++j) {
+        const int l = kLutBits + 1 + j;
+        const uint64_t v = ((uint64_t)fc[l] + cn[l]) << (32 - l);
+        // the builtin returns int: cast both halves back before widening
+        // (a sign-extended low half would set every bit of the high one)
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        L.lim[j] = ((uint64_t)hi << 32) | lo;
+    }
+    return L;
+}
+
+// Sequential Huffman decode of one lane's sub-chunk with its bits in the
+// workgroup's LDS stage (word w of the stream is W[w - wbase]; the stage has
+// kStageSlack readable words past the staged ones, so the reads below need no
+// clamp).  The reader holds two words A, B and k, the bits of A already
+// consumed (1..32): the stream's next 32 bits are alignbit(A, B, 32 - k), one
+// instruction, and 32-bit arithmetic throughout.  Every step reads, beside
+// its table entry, the word two after A (both reads issued before either is
+// waited on); when the step's codes carry k past 32, B moves to A and that
+// word's predecessor to B.  One table lookup takes up to three symbols (the
+// codes that lie wholly in the next kLutBits bits).  A longer code finds its
+// length by the compares above and its symbol in the LDS copy (all of them
+// are there: the caller checked), so the loop reads LDS only; a global load
+// would wait for every store the wave issued before it (the vector memory
+// counter retires loads and stores in order).  Codes leave in 8-byte groups
+// once the destination is 8-byte aligned (`al` = element offset of co[0]);
+// the first symbols up to that alignment and the last ones go one at a time.
+// An unpredictable value (symbol 0) is the only global read, and rare.
+constexpr int kStageSlack = 4;
+
+__device__ __forceinline__ uint32_t decode_run_lds(const uint32_t* W, uint64_t wbase, const DecItem& it,
+                                                   const uint64_t* lut, const LongLim LL, const uint32_t* loff,
+                                                   const uint16_t* lsrt, uint32_t ln, uint16_t* __restrict__ co, uint8_t* __restrict__ so,
+                                                   const uint8_t* __restrict__ un, uint32_t tsize, uint64_t n_unpred,
+                                                   uint64_t al)
+{
+    // bit offset of the sub-chunk in the stage (>= 0: the stage starts at
+    // the workgroup's first word); A = stage word a, k its consumed bits
+    const int64_t rel0 = (int64_t)it.bit0 - (int64_t)(wbase << 5);
+    int32_t a = (int32_t)((rel0 + 31) >> 5) - 1;  // -1 when rel0 = 0 (k = 32, A unused)
+    int32_t k = (int32_t)(rel0 - 32 * (int64_t)a);
+    const int32_t a0 = a, k0 = k;
+    // W[-1] is the word before the stage (the caller's layout keeps it
+    // readable); it only ever stands in for an A that is never read
+    uint32_t A = W[a], B = W[a + 1], N = W[a + 2];
+    uint32_t bad = 0;
+    uint64_t u = it.u0;
+    const uint32_t cnt = it.cnt;
+    auto consume = [&](int32_t l, uint32_t NN) {
+        k += l;
+        const bool t = k > 32;
+        k -= t ? 32 : 0;
+        a += t ? 1 : 0;
+        A = t ? B : A;
+        B = t ? N : B;
+        N = t ? NN : N;
+    };
+    // a code longer than the table window
+    auto slow = [&](uint32_t win, int32_t& l) -> uint32_t {
+        uint32_t j = 0;
+#pragma unroll
+        for (int q = 0; q < kLong; ++q) j += (uint64_t)win >= LL.lim[q] ? 1u : 0u;
+        if (j >= (uint32_t)kLong) {  // no code prefixes the window
+            bad = 1;
+            l = 1;
+            return 0u;
+        }
+        l = kLutBits + 1 + (int32_t)j;
+        const uint32_t r = loff[j] + (win >> (32 - l));
+        bad |= r < ln ? 0u : 1u;
+        return (uint32_t)lsrt[r < ln ? r : 0u];
+    };
+    auto unpred = [&](uint32_t pos) {
+        if (u < n_unpred) {
+            if (tsize == 4) ((uint32_t*)so)[pos] = ((const uint32_t*)un)[u];
+            else ((uint64_t*)so)[pos] = ((const uint64_t*)un)[u];
+        } else {
+            bad = 1;
+        }
+        u++;
+    };
+    // one symbol (the first code of the window)
+    auto single = [&](uint32_t pos) -> uint32_t {
+        const uint32_t NN = W[a + 3];
+        const uint32_t win = __builtin_amdgcn_alignbit(A, B, (uint32_t)(32 - k));
+        const uint64_t e = lut[win >> (32 - kLutBits)];
+        uint32_t sym;
+        int32_t l;
+        if ((e >> kDecNShift) & 3u) {
+            sym = (uint32_t)e & 0xFFFFu;
+            l = (int32_t)((e >> kDecL1Shift) & 15u);
+        } else {
+            sym = slow(win, l);
+        }
+        consume(l, NN);
+        if (sym == 0u) unpred(pos);
+        return sym;
+    };
+    uint32_t i = 0;
+    const uint32_t head = (uint32_t)((4u - (uint32_t)(al & 3u)) & 3u);
+    for (; i < head && i < cnt; ++i) co[i] = (uint16_t)single(i);
+    // co + i is 8-byte aligned; `lo` holds pc decoded codes not yet stored
+    uint64_t lo = 0;
+    uint32_t pc = 0;
+    while (i + pc + 3 <= cnt) {
+        const uint32_t NN = W[a + 3];
+        const uint32_t win = __builtin_amdgcn_alignbit(A, B, (uint32_t)(32 - k));
+        const uint64_t e = lut[win >> (32 - kLutBits)];
+        // the table leaves unused symbol fields zero: the low 48 bits are
+        // the entry's symbols, 16 bits each
+        uint64_t v = e & 0xFFFFFFFFFFFFull;
+        uint32_t n = (uint32_t)(e >> kDecNShift) & 3u;
+        int32_t l = (int32_t)((e >> kDecLtShift) & 15u);
+        // the two rare cases under one branch: a code longer than the
+        // window (n = 0), and symbol 0, which always comes alone (v = 0)
+        if (v == 0ull) {
+            if (n == 0) {
+                v = slow(win, l);
+                n = 1;
+            }
+            if (v == 0ull) unpred(i + pc);
+        }
+        consume(l, NN);
+        const uint64_t spill = pc ? v >> (64 - 16 * pc) : 0ull;
+        lo |= v << (16 * pc);
+        pc += n;
+        if (pc >= 4) {
+            *(uint2*)(co + i) = make_uint2((uint32_t)lo, (uint32_t)(lo >> 32));
+            i += 4;
+            pc -= 4;
+            lo = spill;
+        }
+    }
+    for (uint32_t q = 0; q < pc; ++q) co[i + q] = (uint16_t)(lo >> (16 * q));
+    i += pc;
+    for (; i < cnt; ++i) co[i] = (uint16_t)single(i);
+    // the sub-chunk uses exactly its recorded bit count (else corrupt)
+    const int64_t used = 32 * (int64_t)(a - a0) + (k - k0);
+    bad |= used != (int64_t)it.bits ? 1u : 0u;
+    return bad;
+}
+
 // Single-symbol form for dense payloads (a kLutBits window rarely holds two
 // codes): one table entry (len << 16 | symbol, 0xFFFFFFFF for a longer code)
 // per step from a 16 KiB table, bits from global memory fetched one refill
 // ahead.  Codes leave in 64-byte bursts (32 codes held in registers) when the
-// destination is 64-byte aligned, in 8-byte groups otherwise.
-template <int kGroup>
+// destination is 64-byte aligned, in 8-byte groups otherwise.  kLdsLong as
+// for decode_run.
+template <int kGroup, bool kLdsLong>
 __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, uint64_t wbase, uint64_t nwords,
                                                const DecItem& it, const uint32_t* __restrict__ lut,
                                                const uint32_t* __restrict__ fc, const uint32_t* __restrict__ cn,
@@ -814,7 +1001,7 @@ __device__ __forceinline__ uint32_t decode_run1(const uint32_t* __restrict__ W, 
                 sym = 0;
             } else {
                 const uint32_t idx = fi[l] + (win >> (32 - l)) - fc[l];
-                sym = idx - llo < ln ? (uint32_t)lsrt[idx - llo] : (uint32_t)srt[idx];
+                sym = long_symbol<kLdsLong>(idx, srt, lsrt, llo, ln, bad);
             }
         }
         buf <<= l;
@@ -882,7 +1069,7 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
                                                const DecItem* __restrict__ items, uint16_t* __restrict__ codes,
                                                uint8_t* __restrict__ side, uint32_t* __restrict__ err,
                                                uint32_t wgid, uint64_t* lut, uint32_t* fc, uint32_t* cn,
-                                               uint32_t* fi, uint32_t* stage, uint16_t* lsrt)
+                                               uint32_t* fi, uint32_t* stage, uint16_t* lsrt, uint32_t* loff)
 {
     const int s = find_stream_by_wg(ss, wgid);
     if (s < 0) return;  // uniform across the workgroup
@@ -908,6 +1095,10 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
         cn[threadIdx.x] = t->count[threadIdx.x];
         fi[threadIdx.x] = t->first_index[threadIdx.x];
     }
+    if (threadIdx.x < kLong) {
+        const int l = kLutBits + 1 + (int)threadIdx.x;
+        loff[threadIdx.x] = t->first_index[l] - t->first_code[l] - llo;
+    }
     const uint32_t first = d.item_base + (wgid - d.wg_base) * d.ipw;
     const uint32_t end = d.item_base + d.item_count;
     const uint32_t lim = first + d.ipw < end ? first + d.ipw : end;
@@ -921,19 +1112,22 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     // whose bits lie past it read global memory
     const uint64_t w1 = w1a > w0 + kDecStageWords ? w0 + kDecStageWords : w1a;
     const bool use_lds = kStage && w1 > w0;
+    // staged word w0 + j sits at sw[j]; the stage array keeps one readable
+    // word before sw and kStageSlack after the largest stage
+    uint32_t* sw = stage + (kStage ? 1 : 0);
     if (use_lds) {
         const uint64_t nw = w1 - w0;
         const uint64_t a0 = (w0 + 3) & ~3ull;  // 16-byte aligned body
         const uint64_t h = a0 - w0 < nw ? a0 - w0 : nw;
-        if (threadIdx.x < h) stage[threadIdx.x] = bw[w0 + threadIdx.x];
+        if (threadIdx.x < h) sw[threadIdx.x] = bw[w0 + threadIdx.x];
         const uint64_t nb4 = (nw - h) / 4;
         const uint4* src = (const uint4*)(bw + a0);
         for (uint64_t k = threadIdx.x; k < nb4; k += 256) {
             const uint4 v = src[k];
-            uint32_t* dst = stage + h + 4 * k;
+            uint32_t* dst = sw + h + 4 * k;
             dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
         }
-        for (uint64_t k = h + 4 * nb4 + threadIdx.x; k < nw; k += 256) stage[k] = bw[w0 + k];
+        for (uint64_t k = h + 4 * nb4 + threadIdx.x; k < nw; k += 256) sw[k] = bw[w0 + k];
     }
     __syncthreads();
     const uint32_t item = first + threadIdx.x;
@@ -948,24 +1142,33 @@ __device__ __forceinline__ void huff_decode_wg(const StreamSet& ss, const uint8_
     // two calls rather than one through a pointer chosen per lane: each
     // sees the address space of its bits (LDS reads for the stage, global
     // loads otherwise) instead of generic flat loads
-    uint32_t bad;
-    if constexpr (!kStage) {
-        if ((al & 31u) == 0)
-            bad = decode_run1<32>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
-                                  lsrt, llo, ln);
-        else if ((al & 3u) == 0)
-            bad = decode_run1<4>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
-                                  lsrt, llo, ln);
-        else
-            bad = decode_run1<1>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred,
-                                  lsrt, llo, ln);
-    } else if (in_stage) {
-        bad = decode_run<true>(stage, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al,
-                               lsrt, llo, ln);
-    } else {
-        bad = decode_run<false>(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize, d.n_unpred, al,
-                                lsrt, llo, ln);
-    }
+    // every long-code symbol in the LDS copy (uniform over the workgroup):
+    // the instantiations whose long-code path reads LDS only
+    const bool lds_long = lhi <= llo || lhi - llo <= kCap;
+    auto run = [&](auto long_tag) -> uint32_t {
+        constexpr bool kL = decltype(long_tag)::value;
+        if constexpr (!kStage) {
+            if ((al & 31u) == 0)
+                return decode_run1<32, kL>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize,
+                                           d.n_unpred, lsrt, llo, ln);
+            if ((al & 3u) == 0)
+                return decode_run1<4, kL>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize,
+                                          d.n_unpred, lsrt, llo, ln);
+            return decode_run1<1, kL>(bw, 0, d.bits_words, it, lut1, fc, cn, fi, srt, co, so, un, d.tsize,
+                                      d.n_unpred, lsrt, llo, ln);
+        } else if (in_stage) {
+            if constexpr (kL)
+                return decode_run_lds(sw, w0, it, lut, long_limits(fc, cn), loff, lsrt, ln, co, so, un, d.tsize,
+                                      d.n_unpred, al);
+            else
+                return decode_run<true, false>(sw, w0, w1 - w0, it, lut, fc, cn, fi, srt, co, so, un, d.tsize,
+                                               d.n_unpred, al, lsrt, llo, ln);
+        } else {
+            return decode_run<false, kL>(bw, 0, d.bits_words, it, lut, fc, cn, fi, srt, co, so, un, d.tsize,
+                                         d.n_unpred, al, lsrt, llo, ln);
+        }
+    };
+    const uint32_t bad = lds_long ? run(std::true_type{}) : run(std::false_type{});
     if (bad) atomicOr(err, 1u);
 }
 
@@ -981,11 +1184,11 @@ __global__ __launch_bounds__(256) void k_huff_decode(StreamSet ss, const uint8_t
 {
     constexpr bool kStage = kForm == kDecStaged;
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
-    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
-    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2], loff[kLong];
+    __shared__ uint32_t stage[kStage ? kDecStageWords + 1 + kStageSlack : 1];
     __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
     huff_decode_wg<kForm>(ss, payload, tabs, sorted, items, codes, side, err, blockIdx.x, lut, fc, cn, fi, stage,
-                           lsrt);
+                           lsrt, loff);
 }
 
 // many fields: workgroups wg0 .. of job j decode field j
@@ -1004,12 +1207,12 @@ __global__ __launch_bounds__(256) void k_huff_decode_multi(const DecJob* __restr
 {
     constexpr bool kStage = kForm == kDecStaged;
     __shared__ uint64_t lut[kStage ? (1 << kLutBits) : (1 << kLutBits) / 2];
-    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2];
-    __shared__ uint32_t stage[kStage ? kDecStageWords : 1];
+    __shared__ uint32_t fc[kLmax + 2], cn[kLmax + 2], fi[kLmax + 2], loff[kLong];
+    __shared__ uint32_t stage[kStage ? kDecStageWords + 1 + kStageSlack : 1];
     __shared__ uint16_t lsrt[kStage ? kLongSyms : kLongSyms1];
     const DecJob& J = jobs[find_dec_job(jobs, njobs, blockIdx.x)];
     huff_decode_wg<kForm>(J.ss, J.payload, J.tabs, J.sorted, J.items, J.codes, (uint8_t*)J.side, J.err,
-                           blockIdx.x - J.wg0, lut, fc, cn, fi, stage, lsrt);
+                           blockIdx.x - J.wg0, lut, fc, cn, fi, stage, lsrt, loff);
 }
 
 // many fields, every chunk: thread -> (field, chunk)

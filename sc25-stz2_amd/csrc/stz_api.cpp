@@ -275,16 +275,16 @@ struct stz_ctx {
     // decode on `stream`; ev_items / ev_high order it
     hipStream_t side_stream = nullptr;
     hipEvent_t ev_items = nullptr, ev_high = nullptr;
-    // STZ_CTX_OWN_QUEUE: the streams are created with a CU mask over every
-    // CU, which gives each its own hardware queue
+    // STZ_CTX_OWN_QUEUE: the caller-facing stream is created with a CU mask
+    // over every CU, which gives it its own hardware queue.  The second
+    // stream always gets one: with HIP's default of 4 queues it otherwise
+    // shares a queue with the caller's stream, and the work meant to
+    // overlap runs in order behind it (512^3 decompress 0.77 -> 0.98 ms)
     bool own_queue = false;
     hipStream_t second()
     {
         if (!side_stream) {
-            if (own_queue)
-                side_stream = full_mask_stream(device);
-            else
-                HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+            side_stream = full_mask_stream(device);
             HIPCHK(hipEventCreateWithFlags(&ev_items, hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&ev_high, hipEventDisableTiming));
         }
@@ -983,12 +983,16 @@ int dec_form(double bps)
 }
 
 // sub-chunks per decode workgroup: 256, fewer for the staged form while a
-// stream's expected bits with a 30% margin would overflow the LDS stage
+// stream's expected bits with a 20% margin would overflow the LDS stage (on
+// the 512^3 log-normal field the largest workgroup range of a stream is 1.2x
+// its mean; lanes past the stage read global memory, correct but slower).
+// Two staged workgroups share a CU, so 256 sub-chunks each hold the whole
+// decode of such a field resident in one round.
 uint32_t dec_ipw(int form, double bps)
 {
     uint32_t ipw = 256;
     if (form == kDecStaged)
-        while (ipw > 32 && ipw * (double)kSub * bps * 1.3 > (double)kDecStageWords * 32.0) ipw >>= 1;
+        while (ipw > 32 && ipw * (double)kSub * bps * 1.2 > (double)kDecStageWords * 32.0) ipw >>= 1;
     return ipw;
 }
 
