@@ -246,6 +246,24 @@ private:
 
 }  // namespace
 
+// Waits for an event by polling it: the blocking wait sleeps and its wake-up
+// costs tens of microseconds on a path where the host's next launches are
+// what the GPU waits for (a compress waits twice for small readbacks); after
+// about 20 ms of polling it blocks instead.
+static void wait_event(hipEvent_t e)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0;; ++k) {
+        const hipError_t r = hipEventQuery(e);
+        if (r == hipSuccess) return;
+        if (r != hipErrorNotReady) HIPCHK(r);
+        if ((k & 255) == 255 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+            HIPCHK(hipEventSynchronize(e));
+            return;
+        }
+    }
+}
+
 // A stream whose CU mask covers every CU of the device (current): HIP gives
 // a CU-masked stream a hardware queue of its own instead of sharing one of
 // the GPU_MAX_HW_QUEUES queues with other streams, where work runs in order.
@@ -270,6 +288,8 @@ struct stz_ctx {
     hipEvent_t ev_fork = nullptr;
     // the value range of a compress has reached the host
     hipEvent_t ev_range = nullptr;
+    // a compress's histogram readback has landed
+    hipEvent_t ev_hist = nullptr;
     // second stream of a full decode (created on first use): the high
     // streams' Huffman decode runs there while the base and low levels
     // decode on `stream`; ev_items / ev_high order it
@@ -291,14 +311,20 @@ struct stz_ctx {
         return side_stream;
     }
     std::shared_ptr<Pool> pool;
-    void* pinned = nullptr;
-    size_t pinned_cap = 0;
     bool prof = false;
     std::vector<ProfRec> recs;
     std::map<std::string, std::pair<long, double>> agg;
 
-    size_t stage_off = 0;
-    std::vector<void*> retired;  // outgrown staging buffers, freed at the next call
+    struct Arena {
+        void* pinned = nullptr;
+        size_t cap = 0, off = 0;
+        std::vector<void*> retired;  // outgrown staging buffers, freed when the arena is reused
+        hipEvent_t ev = nullptr;     // the stream position after the call that used it
+        bool pending = false;
+        hipStream_t stream = nullptr;  // the stream its copies went to
+    };
+    Arena arena[2];
+    int slot = 0;
     // launch geometry of the multi-field level kernels, per (dtype, mode,
     // size class), chosen by timing the candidates once
     struct LaunchCfg { int waves = 4; uint32_t grid = 0; float ms = 0.f; std::string trials; };
@@ -327,30 +353,62 @@ struct stz_ctx {
 
     // Pinned staging for async copies: each request gets a region of its own
     // for the rest of the API call (a bump arena reset when a call starts).
+    // Two arenas alternate between calls: a call starts by marking the
+    // stream position after the previous call's copies and waits only for
+    // the call before that one, so the host may run one call ahead of the
+    // GPU (a compress returns while its bit pack still runs, and the next
+    // call's host work overlaps it).
     void* host_stage(size_t n)
     {
+        Arena& A = arena[slot];
         n = align_up(std::max<size_t>(n, 1), 256);
-        if (stage_off + n > pinned_cap) {
-            if (pinned) retired.push_back(pinned);
-            pinned = nullptr;
-            pinned_cap = 0;
-            stage_off = 0;
+        if (A.off + n > A.cap) {
+            if (A.pinned) A.retired.push_back(A.pinned);
+            A.pinned = nullptr;
+            A.cap = 0;
+            A.off = 0;
             const size_t cap = align_up(std::max<size_t>(2 * n, 8u << 20), 1u << 20);
-            HIPCHK(hipHostMalloc(&pinned, cap, hipHostMallocDefault));
-            pinned_cap = cap;
+            HIPCHK(hipHostMalloc(&A.pinned, cap, hipHostMallocDefault));
+            A.cap = cap;
         }
-        void* r = (uint8_t*)pinned + stage_off;
-        stage_off += n;
+        void* r = (uint8_t*)A.pinned + A.off;
+        A.off += n;
         return r;
     }
-    // start of an API call: copies of the previous call have finished
+    // start of an API call
     void stage_reset()
     {
-        if (stage_off == 0 && retired.empty()) return;
-        HIPCHK(hipStreamSynchronize(stream));
-        for (void* p : retired) (void)hipHostFree(p);
-        retired.clear();
-        stage_off = 0;
+        Arena& cur = arena[slot];
+        if (cur.off == 0 && cur.retired.empty()) return;
+        if (!cur.ev) HIPCHK(hipEventCreateWithFlags(&cur.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(cur.ev, cur.stream ? cur.stream : stream));
+        cur.pending = true;
+        slot ^= 1;
+        Arena& nxt = arena[slot];
+        if (nxt.pending) {
+            wait_event(nxt.ev);
+            nxt.pending = false;
+        }
+        for (void* q : nxt.retired) (void)hipHostFree(q);
+        nxt.retired.clear();
+        nxt.off = 0;
+        nxt.stream = stream;
+    }
+    // a call that moves to another stream first lets the staged copies of
+    // the calls before it finish on the old one
+    void stage_drain()
+    {
+        for (Arena& A : arena)
+            if (A.pending) {
+                wait_event(A.ev);
+                A.pending = false;
+            }
+        if (arena[slot].off || !arena[slot].retired.empty()) HIPCHK(hipStreamSynchronize(arena[slot].stream ? arena[slot].stream : stream));
+        for (Arena& A : arena) {
+            for (void* q : A.retired) (void)hipHostFree(q);
+            A.retired.clear();
+        }
+        arena[slot].off = 0;
     }
 };
 
@@ -1723,7 +1781,7 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     }
     // every field's range is on the host once the last one is (stream order);
     // the gathers queued after them keep the GPU busy meanwhile
-    HIPCHK(hipEventSynchronize(c->ev_range));
+    wait_event(c->ev_range);
     // fields the caller takes back for its exhaustive trial; kept alive to
     // the end, after every launch that may still read their buffers
     std::vector<FieldEnc<T>> dropped;
@@ -1954,13 +2012,15 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         HIPCHK(hipStreamWaitEvent(st, c->ev_fork, 0));  // the low readback's buffers, before they are reused
         fork_guard.on = false;
         readback(hs_hi, 8, kStreams - 8, st);
-        HIPCHK(hipEventSynchronize(c->ev_fork));
+        HIPCHK(hipEventRecord(c->ev_hist, st));
+        wait_event(c->ev_fork);
         books(hs_lo, 0, 8);  // overlaps the high level on the GPU
-        HIPCHK(hipStreamSynchronize(st));
+        wait_event(c->ev_hist);
         books(hs_hi, 8, kStreams - 8);
     } else {
         readback(hs_hi, 0, kStreams, st);
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipEventRecord(c->ev_hist, st));
+        wait_event(c->ev_hist);
         books(hs_hi, 0, kStreams);
     }
     for (int i = 0; i < n; ++i) enc_phase3_headers(fe[i]);
@@ -2088,8 +2148,11 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     } else {
         K(c, "huff_encode", [&] { launch_huff_encode_multi(st, dej.as<EncJob>(), (uint32_t)n, chunk0); });
     }
-    // the staging buffers feed async copies: finish before they can be reused
-    HIPCHK(hipStreamSynchronize(st));
+    // No wait here: the bit pack and the copies before it run on in stream
+    // order.  The field was last read by the high level, which the histogram
+    // wait above follows; the staging arena this call filled is reused only
+    // once the stream has passed it (stage_reset); the pool's buffers freed
+    // here are reused only by work ordered after this call on `stream`.
 }
 
 template <typename T>
@@ -2807,6 +2870,7 @@ int stz_create_ex(stz_ctx** out, int device, unsigned flags)
             HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamDefault));
         HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->ev_range, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_hist, hipEventDisableTiming));
         c->stream = c->own;
         c->pool = std::make_shared<Pool>();
         c->pool->device = device;
@@ -2829,11 +2893,16 @@ void stz_destroy(stz_ctx* c)
         (void)hipEventDestroy(r.b);
     }
     c->plan.reset();
-    if (c->pinned) (void)hipHostFree(c->pinned);
-    for (void* p : c->retired) (void)hipHostFree(p);
+    for (auto& A : c->arena) {
+        if (A.stream) (void)hipStreamSynchronize(A.stream);
+        if (A.pinned) (void)hipHostFree(A.pinned);
+        for (void* p : A.retired) (void)hipHostFree(p);
+        if (A.ev) (void)hipEventDestroy(A.ev);
+    }
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_range) (void)hipEventDestroy(c->ev_range);
+    if (c->ev_hist) (void)hipEventDestroy(c->ev_hist);
     if (c->side_stream) {
         (void)hipStreamSynchronize(c->side_stream);
         (void)hipStreamDestroy(c->side_stream);
@@ -2846,7 +2915,15 @@ void stz_destroy(stz_ctx* c)
 int stz_set_stream(stz_ctx* c, void* s)
 {
     if (!c) return STZ_E_ARG;
-    c->stream = s ? (hipStream_t)s : c->own;
+    const hipStream_t ns = s ? (hipStream_t)s : c->own;
+    if (ns == c->stream) return STZ_OK;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        c->stage_drain();
+    } catch (const StzError& e) {
+        return e.code;
+    }
+    c->stream = ns;
     return STZ_OK;
 }
 
