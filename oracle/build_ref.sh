@@ -9,6 +9,9 @@
 #
 # oracle/_ref/libstz_ref_stz2.so: preprocess_block of src/stz2.cpp (its lines
 # 1-8 and 82-306, no SZ3 symbol) with ref2_harness.cpp.
+# oracle/_ref/libstz_ref_dep.so: the staged predictors of src/dependency.cpp
+# (its lines 1-9, 11-29 and 90-432: #includes, globals, interp_axis ..
+# depreprocess_block; line 10 is the SZ3 #include) with ref3_harness.cpp.
 #
 # F5:18-22 (#include of the SZ3-ncp headers) and every function that calls
 # SZ3-ncp (F5:1255-1576, F5:1841-) are left out: SZ3-ncp is not in this image,
@@ -48,5 +51,14 @@ STZ2="$REFROOT/src/stz2.cpp"
     echo "#line 82 \"stz2.cpp\""; sed -n '82,306p' "$STZ2"
     echo "#line 1 \"ref2_harness.cpp\""; cat "$HERE/ref2_harness.cpp"
 } | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w -o "$HERE/_ref/libstz_ref_stz2.so" - & p5=$!
-for p in $p0 $p1 $p2 $p3 $p4 $p5; do wait $p || exit 1; done
+# dependency.cpp's staged inverse-variance predictors (dependency.cpp:90-432)
+DEP="$REFROOT/src/dependency.cpp"
+{
+    echo "#line 1 \"dependency.cpp\""; sed -n '1,9p' "$DEP"
+    echo "#line 11 \"dependency.cpp\""; sed -n '11,29p' "$DEP"
+    echo "#line 90 \"dependency.cpp\""; sed -n '90,432p' "$DEP"
+    echo "#line 1 \"ref3_harness.cpp\""; cat "$HERE/ref3_harness.cpp"
+} | g++ -x c++ -std=c++17 -O3 -fopenmp -fPIC -shared -w -o "$HERE/_ref/libstz_ref_dep.so" - & p6=$!
+for p in $p0 $p1 $p2 $p3 $p4 $p5 $p6; do wait $p || exit 1; done
 echo "built $HERE/_ref/libstz_ref_stz2.so"
+echo "built $HERE/_ref/libstz_ref_dep.so"

@@ -29,8 +29,9 @@ class Params(C.Structure):
     ]
 
 # predictor options as bits of Params.pred: F5's PRED_WPOW2, PRED_PLANE_AXIS,
-# PRED_CUBE_AXIS; stz2.cpp's 12-point bicubic (blocks 2/4/5) and 16-point (block 6)
-PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS, PRED_BICUBIC12, PRED_DIAG16 = 1, 2, 4, 8, 16
+# PRED_CUBE_AXIS; stz2.cpp's 12-point bicubic (blocks 2/4/5) and 16-point (block 6);
+# dependency.cpp's staged inverse-variance predictors (alone)
+PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS, PRED_BICUBIC12, PRED_DIAG16, PRED_STAGED = 1, 2, 4, 8, 16, 32
 # base codec (Params.interp): interpolation linear / cubic, or Lorenzo over
 # independent LZ_TILE^3 tiles
 BASE_LINEAR, BASE_CUBIC, BASE_LORENZO, LZ_TILE = 0, 1, 2, 16
@@ -67,6 +68,7 @@ def lib():
             getattr(L, "ora_interp_compress_" + t).restype = sz
             getattr(L, "ora_interp_compress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
             getattr(L, "ora_interp_decompress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
+            getattr(L, "ora_staged_level_" + t).argtypes = [vp, i32, i32, i32, dbl, vp, vp, vp, vp, vp]
         L.ora_huff_lengths.argtypes = [vp, i32, i32, vp]
         L.ora_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
         L.ora_payload_info.argtypes = [vp, sz, vp, vp, vp]
@@ -199,6 +201,26 @@ def depreprocess(block, de, ref, pred=0):
     out = np.empty_like(de)
     getattr(lib(), "ora_depreprocess_pv_" + _tsuf(de.dtype))(block, pred, _ptr(de), _ptr(ref), _ptr(out), dx, dy, dz)
     return out
+
+
+def staged_level(ref, eb=0.0, targets=None, residuals=None):
+    """One residual level with dependency.cpp's staged predictors.  ref: the
+    coarser grid (dz, dy, dx); targets (encode) or residuals (decode): the 7
+    blocks, shape (7, dz, dy, dx).  Returns the level's 2x grid and, when
+    encoding, the codes and decoded residuals (grid order, per block)."""
+    ref = np.ascontiguousarray(ref)
+    dz, dy, dx = ref.shape
+    g = np.empty((2 * dz, 2 * dy, 2 * dx), dtype=ref.dtype)
+    f = getattr(lib(), "ora_staged_level_" + _tsuf(ref.dtype))
+    if targets is not None:
+        t = np.ascontiguousarray(targets, dtype=ref.dtype)
+        q = np.empty(t.shape, dtype=np.uint16)
+        dv = np.empty_like(t)
+        f(_ptr(ref), dx, dy, dz, float(eb), _ptr(g), _ptr(t), _ptr(q), _ptr(dv), None)
+        return g, q, dv
+    r = np.ascontiguousarray(residuals, dtype=ref.dtype)
+    f(_ptr(ref), dx, dy, dz, float(eb), _ptr(g), None, None, None, _ptr(r))
+    return g
 
 
 def quantize(d, eb, radius=32768):

@@ -513,7 +513,8 @@ static int ora_open_view(ora_view* v, const uint8_t* p, size_t bytes)
     if (bytes < sizeof(stzf_file_header)) return -1;
     const stzf_file_header* fh = (const stzf_file_header*)p;
     if (fh->magic != STZF_MAGIC || fh->version != STZF_VERSION || fh->n_streams != STZF_NSTREAMS ||
-        fh->pred > ORA_PRED_ALL || fh->interp > ORA_BASE_LORENZO || fh->order > 5 || !(fh->alpha >= 1.0) ||
+        fh->pred > ORA_PRED_ALL || ((fh->pred & ORA_PRED_STAGED) && fh->pred != ORA_PRED_STAGED) ||
+        fh->interp > ORA_BASE_LORENZO || fh->order > 5 || !(fh->alpha >= 1.0) ||
         !(fh->beta >= 1.0))
         return -1;
     if (fh->total_bytes > bytes) return -1;
@@ -836,7 +837,8 @@ long long ora_compress(int dtype, const void* field, size_t nx, size_t ny, size_
     ora_params d;
     if (!p) { ora_default_params(&d); p = &d; }
     if (!field || !payload || eb < 0 || p->alpha < 1.0 || p->beta < 1.0 || p->interp < 0 ||
-        p->interp > ORA_BASE_LORENZO || p->order < 0 || p->order > 5)
+        p->interp > ORA_BASE_LORENZO || p->order < 0 || p->order > 5 || p->pred < 0 || p->pred > ORA_PRED_ALL ||
+        ((p->pred & ORA_PRED_STAGED) && p->pred != ORA_PRED_STAGED))
         return -1;
     if (dtype == ORA_F32) return ora_compress_t_f32((const float*)field, nx, ny, nz, eb, eb_mode, p, payload);
     if (dtype == ORA_F64) return ora_compress_t_f64((const double*)field, nx, ny, nz, eb, eb_mode, p, payload);

@@ -45,10 +45,13 @@ typedef struct {
  * (F5:609-622, F5:675-686); and the static predictors of stz2.cpp for the
  * plane blocks 2/4/5 (12-point truncated bicubic, stz2.cpp:122-155,
  * 186-199, 222-235) and the cube block 6 (16 points, stz2.cpp:271-293),
- * which replace the dynamic ones of those blocks */
+ * which replace the dynamic ones of those blocks; and dependency.cpp's
+ * staged inverse-variance predictors (dependency.cpp:162-292, 594-704):
+ * blocks 2, 4, 5 from the reconstructed blocks 0, 1, 3 and block 6 from the
+ * reconstructed 2, 4, 5, alone (no other bit with it) */
 enum { ORA_PRED_WPOW2 = 1, ORA_PRED_PLANE_AXIS = 2, ORA_PRED_CUBE_AXIS = 4, ORA_PRED_BICUBIC12 = 8,
-       ORA_PRED_DIAG16 = 16 };
-#define ORA_PRED_ALL 31
+       ORA_PRED_DIAG16 = 16, ORA_PRED_STAGED = 32 };
+#define ORA_PRED_ALL 63
 
 void ora_default_params(ora_params* p);
 
@@ -114,6 +117,16 @@ void ora_interp_decompress_f32(float* v, int bx, int by, int bz, double eb, int 
                                double alpha, double beta, const uint16_t* codes, const float* unpred);
 void ora_interp_decompress_f64(double* v, int bx, int by, int bz, double eb, int interp, int order,
                                double alpha, double beta, const uint16_t* codes, const double* unpred);
+
+/* one residual level with dependency.cpp's staged predictors: ref the
+ * coarser grid (dx, dy, dz), g its 2x grid (filled: ref at parity 0, each
+ * block's reconstruction at its parity); encode with tgt7 (the 7 target
+ * blocks concatenated, grid order) into q7 / dv7 (codes, decoded residuals),
+ * or decode res7 (the residuals) when tgt7 is NULL */
+void ora_staged_level_f32(const float* ref, int dx, int dy, int dz, double eb, float* g, const float* tgt7,
+                          uint16_t* q7, float* dv7, const float* res7);
+void ora_staged_level_f64(const double* ref, int dx, int dy, int dz, double eb, double* g, const double* tgt7,
+                          uint16_t* q7, double* dv7, const double* res7);
 
 /* ROI planner (F5:1602-1830).  Fills per-block plans for the high and low
  * levels: for each of 7 blocks, [valid, x0,x1,y0,y1,z0,z1, nchunks] and the

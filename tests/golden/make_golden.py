@@ -183,6 +183,26 @@ def main_pred():
             d = np.empty_like(sub)
             R.ref2_preprocess_f32(b, _p(sub), _p(ref), _p(d), dx, dy, dz)
             out["stz2-f32-%d-%d_out" % (si, b)] = d
+    # dependency.cpp's staged level decode (f32), composed in the order of
+    # dependency.cpp:594-636: blocks 0, 1, 3 by depreprocess_block from the
+    # coarser grid, then 2, 4, 5 and last 6 by depreprocess_block_staged
+    # from the reconstructed blocks
+    R = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libstz_ref_dep.so"))
+    rng = np.random.default_rng(2026)
+    for si, shape in enumerate(PRED_SHAPES + [(4, 4, 4), (5, 3, 9)]):
+        dz, dy, dx = shape
+        _, ref = pred_inputs(rng, shape, np.float32)
+        res = rng.normal(scale=0.5, size=(7,) + shape).astype(np.float32)
+        res[:, :, :, 1::4] *= 30.0
+        rec = [np.empty(shape, np.float32) for _ in range(7)]
+        for b in (0, 1, 3):
+            R.ref3_depreprocess_f32(b, _p(res[b]), _p(ref), _p(rec[b]), dx, dy, dz)
+        rb = (C.c_void_p * 7)(*[r.ctypes.data for r in rec])
+        for b in (2, 4, 5, 6):
+            R.ref3_depreprocess_staged_f32(b, _p(np.ascontiguousarray(res[b])), rb, _p(rec[b]), dx, dy, dz)
+        out["staged-f32-%d-ref" % si] = ref
+        out["staged-f32-%d-res" % si] = res
+        out["staged-f32-%d-rec_out" % si] = np.stack(rec)
     np.savez_compressed(PRED_OUT, **out)
     print("wrote %s (%d arrays, %d bytes)" % (PRED_OUT, len(out), os.path.getsize(PRED_OUT)))
 

@@ -231,6 +231,101 @@ def test_stz2_predictors_reproduce_polynomials(oracle):
         assert abs(got6 - want6) <= 1e-5 * abs(want6), (dt, got6, want6)
 
 
+def _parity_blocks(g):
+    """The 7 parity blocks of a level's 2x grid, in block order (block b at
+    the parity of sub-block b + 1: x = bit 0, y = bit 1, z = bit 2)."""
+    return np.stack([g[(k >> 2)::2, ((k >> 1) & 1)::2, (k & 1)::2] for k in range(1, 8)])
+
+
+def test_golden_staged_level(oracle, pred_gold):
+    """The oracle's staged level decode (option bit 32) against dependency.cpp's
+    own depreprocess_block / depreprocess_block_staged composed in its order
+    (dependency.cpp:594-636; vectors by tests/golden/make_golden.py from
+    oracle/_ref/libstz_ref_dep.so), bit for bit: every boundary rule, the
+    inverse-variance blends and which reconstructed block feeds which."""
+    n = 0
+    for key in pred_gold.files:
+        if not (key.startswith("staged-") and key.endswith("-ref")):
+            continue
+        stem = key[:-4]
+        ref, res = pred_gold[stem + "-ref"], pred_gold[stem + "-res"]
+        g = oracle.staged_level(ref, residuals=res)
+        assert np.array_equal(_bits(_parity_blocks(g)), _bits(pred_gold[stem + "-rec_out"])), stem
+        assert np.array_equal(_bits(g[::2, ::2, ::2]), _bits(ref)), stem
+        n += 1
+    assert n >= 5
+
+
+def test_reference_staged_level_live(oracle):
+    """The same against the live dependency.cpp build on more shapes, rough
+    residuals (denormal-range and huge values included)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libstz_ref_dep.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    R = C.CDLL(path)
+    rng = np.random.default_rng(11)
+    for shape in [(3, 3, 3), (8, 5, 2), (13, 17, 9), (20, 16, 24)]:
+        dz, dy, dx = shape
+        _, ref = _pred_inputs(rng, shape, np.float32)
+        res = rng.normal(size=(7,) + shape).astype(np.float32)
+        res[:, ::3] *= 1e-39
+        res[:, :, 1::3] *= 1e20
+        rec = [np.empty(shape, np.float32) for _ in range(7)]
+        for b in (0, 1, 3):
+            R.ref3_depreprocess_f32(b, _p(np.ascontiguousarray(res[b])), _p(ref), _p(rec[b]), dx, dy, dz)
+        rb = (C.c_void_p * 7)(*[r.ctypes.data for r in rec])
+        for b in (2, 4, 5, 6):
+            R.ref3_depreprocess_staged_f32(b, _p(np.ascontiguousarray(res[b])), rb, _p(rec[b]), dx, dy, dz)
+        g = oracle.staged_level(ref, residuals=res)
+        assert np.array_equal(_bits(_parity_blocks(g)), _bits(np.stack(rec))), shape
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_staged_level_closed_loop(oracle, dt):
+    """Encoding a level with the staged predictors and decoding its decoded
+    residuals rebuild the same grid bit for bit (each block is predicted from
+    reconstructions, not originals), and every coded point is within the
+    bound.  The f64 form is the same template as the reference-pinned f32."""
+    rng = np.random.default_rng(12)
+    shape = (9, 11, 14)
+    _, ref = _pred_inputs(rng, shape, dt)
+    tgt = (ref[None] + rng.normal(scale=0.7, size=(7,) + shape)).astype(dt)
+    tgt[3, 2, 2, 2] = np.nan
+    eb = 0.05
+    g, q, dv = oracle.staged_level(ref, eb=eb, targets=tgt)
+    g2 = oracle.staged_level(ref, residuals=dv)
+    assert np.array_equal(_bits(g), _bits(g2))
+    rec = _parity_blocks(g)
+    coded = q != 0
+    assert coded.mean() > 0.9
+    assert np.all(np.abs(rec[coded].astype(np.float64) - tgt[coded]) <= eb * (1 + 1e-6) + 1e-12)
+    assert q[3, 2, 2, 2] == 0
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_staged_payload_roundtrip(oracle, dt):
+    """A whole payload with the staged predictors (option bit 32): the
+    requested bound holds, ROI / slice outputs equal the crop of the full
+    decode, the 1/2 level is the decoded mid grid, and the option cannot be
+    combined with another bit."""
+    rng = np.random.default_rng(13)
+    z, y, x = np.meshgrid(np.arange(22), np.arange(30), np.arange(37), indexing="ij")
+    f = (np.sin(x / 5.0) * np.cos(y / 7.0) + 0.3 * np.sin(z / 3.0) + rng.normal(scale=1e-3, size=x.shape)).astype(dt)
+    eb = 1e-3
+    p = oracle.compress(f, eb, oracle.EB_REL, pred=oracle.PRED_STAGED)
+    assert oracle.header(p)["pred"] == oracle.PRED_STAGED
+    d = oracle.decompress(p)
+    assert np.max(np.abs(d.astype(np.float64) - f)) <= eb * (float(f.max()) - float(f.min()))
+    plain = oracle.compress(f, eb, oracle.EB_REL)
+    assert plain.size != p.size or not np.array_equal(plain, p)
+    roi = oracle.decompress_roi(p, 3, 5, 2, 30, 21, 17)
+    assert np.array_equal(_bits(roi), _bits(d[2:18, 5:22, 3:31]))
+    mid = oracle.decompress_level(p, 2)
+    assert mid.shape == (11, 15, 19)
+    with pytest.raises(Exception):
+        oracle.compress(f, eb, oracle.EB_REL, pred=oracle.PRED_STAGED | oracle.PRED_WPOW2)
+
+
 def test_golden_predictor_options(oracle, pred_gold):
     """The same against the committed vectors of those reference builds."""
     for key in pred_gold.files:
