@@ -64,7 +64,8 @@ typedef struct {
 } stz_params;
 
 /* Predictor options as run-time bits of stz_params.pred (0 = F5's default
- * build): F5's compile-time options, and stz2.cpp's static predictors.
+ * build): F5's compile-time options, stz2.cpp's static predictors and
+ * dependency.cpp's staged ones.
  * They change the interior predictions of the plane blocks (2, 4, 5) and
  * the cube block (6), and are written to the stream header, which the
  * decoder follows. */
@@ -73,6 +74,13 @@ typedef struct {
 #define STZ_PRED_CUBE_AXIS 4  /* three axis-pair candidates more for block 6 (F5:609-622, F5:675-686) */
 #define STZ_PRED_BICUBIC12 8  /* blocks 2, 4, 5: 12-point truncated bicubic (stz2.cpp:122-155, 186-199, 222-235) */
 #define STZ_PRED_DIAG16 16    /* block 6: 16-point static predictor (stz2.cpp:271-293) */
+/* dependency.cpp's staged inverse-variance predictors (dependency.cpp:162-292,
+ * driven as at dependency.cpp:594-704): blocks 2, 4, 5 from the
+ * reconstructed blocks 0, 1, 3, block 6 from the reconstructed 2, 4, 5.
+ * Alone (no other bit with it).  Each level then runs in three stages, and
+ * random access decodes the levels in full (a staged block depends on
+ * reconstructed neighbours of other blocks, past the planner's footprint). */
+#define STZ_PRED_STAGED 32
 
 typedef struct {
     size_t nx, ny, nz;        /* caller's dims, x fastest */

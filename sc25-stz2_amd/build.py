@@ -26,7 +26,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib")
-SOURCES = ["k_huff.hip", "k_field.hip", "k_residual.hip", "k_interp.hip", "k_lorenzo.hip", "k_tuner.hip", "k_stage.hip",
+SOURCES = ["k_huff.hip", "k_field.hip", "k_residual.hip", "k_interp.hip", "k_lorenzo.hip", "k_tuner.hip", "k_stage.hip", "k_staged.hip",
            "host_codec.cpp", "stz_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -32,9 +32,18 @@ constexpr int kLutBits = 12;        // decode table window (bits)
 // block 6 (F5:609-622, F5:675-686).  stz2.cpp's static predictors replace
 // the dynamic ones: the 12-point truncated bicubic for blocks 2, 4, 5
 // (stz2.cpp:122-155, 186-199, 222-235) and 16 points for block 6
-// (stz2.cpp:271-293).  0 is F5's default build.
+// (stz2.cpp:271-293).  PRED_STAGED: dependency.cpp's staged inverse-variance
+// predictors (dependency.cpp:162-292): blocks 2, 4, 5 from the reconstructed
+// blocks 0, 1, 3 and block 6 from the reconstructed 2, 4, 5; alone (no other
+// bit with it).  0 is F5's default build.
 constexpr int kPredWpow2 = 1, kPredPlaneAxis = 2, kPredCubeAxis = 4, kPredBicubic12 = 8, kPredDiag16 = 16;
-constexpr int kPredAll = 31;
+constexpr int kPredStaged = 32;
+constexpr int kPredAll = 63;
+// a valid predictor option set: known bits, the staged predictors alone
+__host__ __device__ inline bool pred_valid(uint64_t p)
+{
+    return p <= (uint64_t)kPredAll && (!(p & kPredStaged) || p == (uint64_t)kPredStaged);
+}
 
 struct FileHeader {
     uint32_t magic, version, dtype, eb_mode;

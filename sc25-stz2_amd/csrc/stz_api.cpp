@@ -815,7 +815,7 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
     if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 2 || prm.order < 0 ||
         prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
-        prm.pred > kPredAll)
+        !pred_valid((uint64_t)prm.pred))
         throw StzError(STZ_E_ARG, "bad parameters");
     hipStream_t st = c->stream;
     auto pool = c->pool;
@@ -1300,16 +1300,32 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     DBuf vb(c->pool, (size_t)lx * ly * lz * sizeof(T)), mid(c->pool, (size_t)L.g[8].dx * L.g[8].dy * L.g[8].dz * sizeof(T));
     decode_base<T>(c, m, L, dd, vb.as<T>(), 0);
     const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
-    K(c, "low_decode", [&] {
-        launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
-                             c->grid_share);
-    });
     OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
-    join_high(c, dd);
-    K(c, "high_decode", [&] {
-        launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
-                              (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, c->grid_share);
-    });
+    if (m.fh.pred & kPredStaged) {
+        // dependency.cpp's staged predictors: three launches per level, the
+        // high level into the padded grid (its stencils read past the crop)
+        DBuf pad(c->pool, (size_t)m.fh.px * m.fh.py * m.fh.pz * sizeof(T));
+        K(c, "low_decode", [&] {
+            launch_staged_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>());
+        });
+        join_high(c, dd);
+        K(c, "high_decode", [&] {
+            launch_staged_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(),
+                                    pad.as<T>());
+            launch_crop<T>(st, pad.as<T>(), (int)m.fh.px, (int)m.fh.py, (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, 0,
+                           0, 0, 1, (T*)ob.ptr);
+        });
+    } else {
+        K(c, "low_decode", [&] {
+            launch_low_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                                 c->grid_share);
+        });
+        join_high(c, dd);
+        K(c, "high_decode", [&] {
+            launch_high_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
+                                  (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, c->grid_share);
+        });
+    }
     ob.copy_back(c);
     err.fetch(c);
     HIPCHK(hipStreamSynchronize(st));
@@ -1416,6 +1432,71 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
         ob[i] = OutBuf(c, outs[i], od[3 * i] * od[3 * i + 1] * od[3 * i + 2] * sizeof(T));
     }
     ErrFlag err(c);
+    if (m.fh.pred & kPredStaged) {
+        // dependency.cpp's staged predictors: a block depends on reconstructed
+        // neighbours of other blocks, past the planner's stencil footprint, so
+        // the requests are crops of the levels decoded in full
+        bool all[kStreams];
+        for (int s = 0; s < kStreams; ++s) all[s] = true;
+        Decoded dd;
+        huff_decode(c, ds, L, all, nullptr, dd, err.ptr());
+        const int lx = L.g[0].dx, ly = L.g[0].dy, lz = L.g[0].dz;
+        const int mx = L.g[8].dx, my = L.g[8].dy, mz = L.g[8].dz;
+        DBuf vbuf;
+        const T* vb;
+        if (ds->have_base) {
+            vb = ds->base.as<T>();
+        } else {
+            vbuf = DBuf(c->pool, (size_t)lx * ly * lz * sizeof(T));
+            decode_base<T>(c, m, L, dd, vbuf.as<T>(), 0);
+            vb = vbuf.as<T>();
+        }
+        DBuf mid(c->pool, (size_t)mx * my * mz * sizeof(T)), pad;
+        const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
+        bool need_full = false, need_mid2 = false;
+        for (int i = 0; i < n; ++i) {
+            const bool lv = rq[i].kind == STZ_REQ_LEVEL;
+            need_full = need_full || !lv || rq[i].a[0] == 3;
+            need_mid2 = need_mid2 || !lv || rq[i].a[0] >= 2;
+        }
+        if (need_mid2)
+            K(c, "low_decode", [&] {
+                launch_staged_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>());
+            });
+        if (need_full) {
+            pad = DBuf(c->pool, (size_t)m.fh.px * m.fh.py * m.fh.pz * sizeof(T));
+            K(c, "high_decode", [&] {
+                launch_staged_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(),
+                                        pad.as<T>());
+            });
+        }
+        for (int i = 0; i < n; ++i) {
+            const size_t* o = od + 3 * i;
+            K(c, "crop", [&] {
+                if (rq[i].kind == STZ_REQ_LEVEL) {
+                    const int level = rq[i].a[0];
+                    if (level == 3)
+                        launch_crop<T>(st, pad.as<T>(), (int)m.fh.px, (int)m.fh.py, (int)o[0], (int)o[1], (int)o[2], 0,
+                                       0, 0, 1, (T*)ob[i].ptr);
+                    else if (level == 2)
+                        launch_crop<T>(st, mid.as<T>(), mx, my, (int)o[0], (int)o[1], (int)o[2], 0, 0, 0, 1,
+                                       (T*)ob[i].ptr);
+                    else
+                        launch_crop<T>(st, vb, lx, ly, (int)o[0], (int)o[1], (int)o[2], 0, 0, 0, level == 0 ? 2 : 1,
+                                       (T*)ob[i].ptr);
+                } else {
+                    const Box& q = plans[i].q;
+                    launch_crop<T>(st, pad.as<T>(), (int)m.fh.px, (int)m.fh.py, (int)o[0], (int)o[1], (int)o[2], q.x0,
+                                   q.y0, q.z0, 1, (T*)ob[i].ptr);
+                }
+            });
+            ob[i].copy_back(c);
+        }
+        err.fetch(c);
+        HIPCHK(hipStreamSynchronize(st));
+        err.check();
+        return;
+    }
     if (need_base) {
         full[0] = !ds->have_base;
         for (int b = 0; b < 7; ++b) full[1 + b] = need_lowfull;
@@ -1966,21 +2047,42 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             HIPCHK(hipEventRecord(c->ev_fork, rs));
             fork_guard.on = rs != st;
         }
-        if (n == 1) {
-            FieldEnc<T>& f = fe[0];
-            const LevelArgs a = enc_level_args(f, level);
-            if (level == 1)
-                K(c, "low_encode", [&] {
-                    launch_low_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.vbase.template as<T>(),
-                                         f.L.g[1], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
-                                         f.hist.template as<uint32_t>() + 65536, f.mid.template as<T>());
-                });
-            else
-                K(c, "high_encode", [&] {
-                    launch_high_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.mid.template as<T>(),
-                                          f.L.g[8], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
-                                          f.hist.template as<uint32_t>() + 8 * 65536);
-                });
+        // one field, or fields under the staged predictors (a batch shares its
+        // parameters): launches of their own per field
+        if (n == 1 || (fe[0].prm.pred & kPredStaged)) {
+            for (int i = 0; i < n; ++i) {
+                FieldEnc<T>& f = fe[i];
+                const LevelArgs a = enc_level_args(f, level);
+                if (f.prm.pred & kPredStaged) {
+                    // dependency.cpp's staged predictors: three launches per level;
+                    // the high level's reconstruction goes to a full-size grid
+                    DBuf rec;
+                    T* out = f.mid.template as<T>();
+                    if (level == 2) {
+                        rec = DBuf(pool, (size_t)f.px * f.py * f.pz * sizeof(T));
+                        out = rec.template as<T>();
+                    }
+                    K(c, level == 1 ? "low_encode" : "high_encode", [&] {
+                        launch_staged_encode<T>(st, f.full, (int)f.px, (int)f.py, level == 1 ? 4 : 2,
+                                                level == 1 ? f.vbase.template as<T>() : f.mid.template as<T>(),
+                                                f.L.g[level == 1 ? 1 : 8], a, f.codes.template as<uint16_t>(),
+                                                f.side.template as<T>(),
+                                                f.hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, out);
+                    });
+                } else if (level == 1) {
+                    K(c, "low_encode", [&] {
+                        launch_low_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.vbase.template as<T>(),
+                                             f.L.g[1], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
+                                             f.hist.template as<uint32_t>() + 65536, f.mid.template as<T>());
+                    });
+                } else {
+                    K(c, "high_encode", [&] {
+                        launch_high_encode<T>(st, f.full, (int)f.px, (int)f.py, (int)f.pz, f.mid.template as<T>(),
+                                              f.L.g[8], a, f.codes.template as<uint16_t>(), f.side.template as<T>(),
+                                              f.hist.template as<uint32_t>() + 8 * 65536);
+                    });
+                }
+            }
             continue;
         }
         const int md = level == 1 ? kLowEnc : kHighEnc;
@@ -2400,6 +2502,18 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
 template <typename T>
 void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* const* outs)
 {
+    // streams under the staged predictors decode one by one (their levels
+    // run in three stages each, outside the persistent multi-field launch)
+    bool staged = false;
+    for (int i = 0; i < n; ++i) staged = staged || (ds[i]->meta.fh.pred & kPredStaged);
+    if (staged) {
+        c->plan.reset();
+        for (int i = 0; i < n; ++i) {
+            if (!is_device_ptr(outs[i])) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
+            decompress_full<T>(c, ds[i], outs[i]);
+        }
+        return;
+    }
     bool same = c->plan && (int)c->plan->ds.size() == n;
     for (int i = 0; same && i < n; ++i)
         same = c->plan->ds[i] == ds[i] && c->plan->gens[i] == ds[i]->gen && c->plan->outs[i] == outs[i];
@@ -2483,7 +2597,7 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     std::memcpy(&m.fh, p, sizeof(FileHeader));
     const FileHeader& fh = m.fh;
     if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
-        fh.pred > (uint64_t)kPredAll || fh.interp > 2 || fh.order > 5 || !(fh.alpha >= 1.0) || !(fh.beta >= 1.0) ||
+        !pred_valid(fh.pred) || fh.interp > 2 || fh.order > 5 || !(fh.alpha >= 1.0) || !(fh.beta >= 1.0) ||
         (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
         throw StzError(STZ_E_STREAM, "bad payload header");
     if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))

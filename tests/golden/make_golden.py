@@ -202,7 +202,7 @@ def main_pred():
             R.ref3_depreprocess_staged_f32(b, _p(np.ascontiguousarray(res[b])), rb, _p(rec[b]), dx, dy, dz)
         out["staged-f32-%d-ref" % si] = ref
         out["staged-f32-%d-res" % si] = res
-        out["staged-f32-%d-rec_out" % si] = np.stack(rec)
+        out["staged-f32-%d-recs" % si] = np.stack(rec)
     np.savez_compressed(PRED_OUT, **out)
     print("wrote %s (%d arrays, %d bytes)" % (PRED_OUT, len(out), os.path.getsize(PRED_OUT)))
 

@@ -419,6 +419,78 @@ def test_predictor_options_random_access_and_batch(ctx, oracle):
         b.free()
 
 
+# dependency.cpp's staged inverse-variance predictors (stz_params.pred = 32):
+# blocks 2/4/5 from the reconstructed 0/1/3, 6 from the reconstructed 2/4/5
+STAGED_CASES = [("lognormal", (40, 44, 52), np.float32, 1e-3, "rel"), ("grf", (36, 30, 42), np.float64, 1e-6, "rel"),
+                ("noise", (21, 18, 27), np.float32, 1e-5, "rel"), ("sin", (9, 70, 66), np.float32, 1e-2, "abs"),
+                ("grf", (4, 4, 4), np.float64, 1e-3, "rel")]
+
+
+@pytest.mark.parametrize("case", STAGED_CASES, ids=[("%s-%s-%s-%g-%s" % (c[0], "x".join(map(str, c[1])),
+                                                                          c[2].__name__, c[3], c[4]))
+                                                     for c in STAGED_CASES])
+def test_staged_predictor_bitexact(ctx, oracle, case):
+    """The staged predictors through the whole path: payload bytes equal the
+    oracle's (whose staged levels equal dependency.cpp's own functions), the
+    option lands in the header, decode is bit-exact, the bound holds; ROI,
+    slice and levels equal the oracle's."""
+    kind, shape, dtype, eb, mode = case
+    _say("staged predictors", case)
+    f = _field(kind, shape, dtype, seed=61)
+    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, pred=32)
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    assert ds.info.pred == 32
+    rec = ds.decompress()
+    _assert_bitexact(rec, oracle.decompress(want), "decompress staged")
+    assert max_error(rec, f) <= requested_bound(f, eb, mode)
+    nz, ny, nx = shape
+    for q in [(1, 2, 0, nx - 2, ny - 3, nz - 1), (nx // 2, 0, nz // 3, nx // 2, ny - 1, nz // 3 + 2)]:
+        _assert_bitexact(ds.decompress_roi(*q), oracle.decompress_roi(want, *q), "roi %s" % (q,))
+    for lv in range(4):
+        _assert_bitexact(ds.decompress_level(lv), oracle.decompress_level(want, lv), "level %d" % lv)
+    ds.free()
+
+
+def test_staged_predictor_requests_and_batch(ctx, oracle):
+    """Batched random-access requests and a batch of uneven fields under the
+    staged predictors: equal to the crops of the full decode and to one call
+    per field; the option does not combine with another bit."""
+    import torch
+    import stz_amd
+    f = _field("lognormal", (48, 40, 56), np.float32, seed=9)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", pred=32, chunk_high=8, chunk_low=4)
+    assert first_difference(gpu, want) is None
+    full = oracle.decompress(want)
+    ds.prepare(keep_base=True)
+    reqs = [("box", 1, 2, 3, 40, 30, 20), ("slice", 2, 17), ("level", 2), ("level", 0), ("level", 3)]
+    for r, got in zip(reqs, ds.decompress_batch(reqs)):
+        if r[0] == "box":
+            want_r = full[3:21, 2:31, 1:41]
+        elif r[0] == "slice":
+            want_r = full[17:18]
+        elif r[1] == 3:
+            want_r = full
+        else:
+            want_r = oracle.decompress_level(want, r[1])
+        _assert_bitexact(got, want_r, "batch request %s" % (r,))
+    ds.free()
+    dev = torch.device("cuda:0")
+    fs = [torch.from_numpy(_field("grf", sh, np.float32, seed=50 + i)).to(dev)
+          for i, sh in enumerate([(33, 47, 61), (16, 20, 24)])]
+    batch = ctx.compress_batch(fs, 1e-3, "rel", pred=32)
+    outs = [torch.empty_like(t) for t in fs]
+    ctx.decompress_batch(batch, outs)
+    for t, b, o in zip(fs, batch, outs):
+        pb = b.download(-1)
+        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, pred=32)) is None
+        _assert_bitexact(o.cpu().numpy(), b.decompress(), "batch decode")
+    for b in batch:
+        b.free()
+    with pytest.raises(stz_amd.StzError):
+        ctx.compress_device(fs[1], 1e-3, "rel", pred=32 | 1)
+
+
 # ------------------------------------------------------------------ Lorenzo base
 # interp = 2: the base grid in independent 16^3 Lorenzo tiles (SZ3's
 # ALGO_INTERP_LORENZO alternative).  Base grids from 1 to 2x2x3 tiles, with

@@ -250,7 +250,7 @@ def test_golden_staged_level(oracle, pred_gold):
         stem = key[:-4]
         ref, res = pred_gold[stem + "-ref"], pred_gold[stem + "-res"]
         g = oracle.staged_level(ref, residuals=res)
-        assert np.array_equal(_bits(_parity_blocks(g)), _bits(pred_gold[stem + "-rec_out"])), stem
+        assert np.array_equal(_bits(_parity_blocks(g)), _bits(pred_gold[stem + "-recs"])), stem
         assert np.array_equal(_bits(g[::2, ::2, ::2]), _bits(ref)), stem
         n += 1
     assert n >= 5
