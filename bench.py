@@ -297,7 +297,7 @@ def tuner_bench(args, ctx, stream, field, out, N, s, payload_plain, dev):
     configuration of every compress (its statistics + MLP run inside the
     timed step).  (2) The tuner workload: 256x384x384 fp64 turbulence-like
     field (GRF k^-11/3 + 8 tanh fronts of width 2, seed 1003), rel eb 1e-6:
-    learned choice vs the exhaustive trial over all 153 configurations --
+    learned choice vs the exhaustive trial over all 204 configurations --
     tuning time and payload bytes -- and vs the untuned configuration.  The
     model runs with its calibrated confidence threshold, below which the
     learned path falls back to the trial (reported as "fallback")."""

@@ -217,11 +217,12 @@ int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
 
 /* ---- learned tuner -------------------------------------------------------
  * Chooses the residual predictor set (F5's; F5's PRED_PLANE_AXIS +
- * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16), the base codec
+ * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16; dependency.cpp's staged
+ * inverse-variance predictors, STZ_PRED_STAGED), the base codec
  * (interpolator and axis order, or Lorenzo) and the level-eb scaling (the
  * tuning targets of F5:2120-2129 and SZ3's sample-and-trial between
  * interpolation and Lorenzo, F5:1439) among STZ_TUNE_CONFIGS configurations:
- * 3 predictor sets x (2 interpolators x 6 orders x 4 scalings + Lorenzo x 3
+ * 4 predictor sets x (2 interpolators x 6 orders x 4 scalings + Lorenzo x 3
  * scalings), index 51 * set + j, j = 24 * interp + 4 * order + scaling
  * (j < 48) or 48 + scaling (Lorenzo).  The model has STZ_MLP_OUTPUTS outputs,
  * the first STZ_TUNE_CONFIGS of which are the configurations.
@@ -229,10 +230,10 @@ int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
  * unchanged.  params.tuner = 1 compresses with the model's choice (falling
  * back to the exhaustive trial below the model's confidence threshold),
  * params.tuner = 2 with the trial's.  Tuning needs an ABS or REL bound. */
-#define STZ_TUNE_CONFIGS 153
-#define STZ_MLP_OUTPUTS 160
+#define STZ_TUNE_CONFIGS 204
+#define STZ_MLP_OUTPUTS 208
 #define STZ_TUNE_FEATURES 16
-#define STZ_MLP_PARAMS 15648 /* 16x64 + 64 + 64x64 + 64 + 64x160 + 160 fp32 */
+#define STZ_MLP_PARAMS 18768 /* 16x64 + 64 + 64x64 + 64 + 64x208 + 208 fp32 */
 int stz_tune_config(int index, stz_params* p);
 /* 16 statistics per 64^3 sub-block (x fastest block order) */
 int stz_tune_features(stz_ctx* ctx, stz_dtype dtype, const void* field, size_t nx, size_t ny, size_t nz,

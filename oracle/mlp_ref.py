@@ -1,4 +1,4 @@
-"""CPU float64 reference of the learned tuner's MLP (16 -> 64 -> 64 -> 160,
+"""CPU float64 reference of the learned tuner's MLP (16 -> 64 -> 64 -> 208,
 ReLU, soft-target cross entropy) and of one Adam step.
 
 TEST INFRASTRUCTURE ONLY: the checker of the library's bf16 MFMA kernels
@@ -11,11 +11,11 @@ the plain float64 network, whose gradient the tests pin by finite
 differences.
 
 Parameter layout (fp32 vector of 15648): W1[16][64], b1[64], W2[64][64],
-b2[64], W3[64][160], b3[160]; W[k][n] maps input k to output n.
+b2[64], W3[64][208], b3[208]; W[k][n] maps input k to output n.
 """
 import numpy as np
 
-IN, HID, OUT = 16, 64, 160
+IN, HID, OUT = 16, 64, 208
 N_PARAMS = IN * HID + HID + HID * HID + HID + HID * OUT + OUT
 
 

@@ -1,5 +1,5 @@
 // k_tuner.hip -- the learned tuner: per-sub-block field statistics and a
-// 16 -> 64 -> 64 -> 160 MLP (ReLU) whose forward pass, backward pass and Adam
+// 16 -> 64 -> 64 -> 208 MLP (ReLU) whose forward pass, backward pass and Adam
 // update run here.  Matrix products are bf16 MFMA (v_mfma_f32_16x16x16_bf16)
 // with fp32 accumulation; weights are kept in fp32 (master copy) and rounded
 // to bf16 (round to nearest even) when a workgroup stages them in LDS.
@@ -53,6 +53,11 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// waves per MLP workgroup (16 sample rows each): two keep the weights and
+// both waves' activations within the 160 KiB of LDS at 208 outputs
+constexpr int kMlpWaves = 2;
+constexpr int kMlpThreads = 64 * kMlpWaves;
+
 struct MlpLds {
     uint16_t w1t[kHid * kIn];   // W1^T: [n][k]
     uint16_t w2t[kHid * kHid];  // W2^T
@@ -60,38 +65,38 @@ struct MlpLds {
     uint16_t w3t[kOut * kHid];  // W3^T
     uint16_t w3[kHid * kOut];   // W3
     float b1[kHid], b2[kHid], b3[kOut];
-    uint16_t x[4][16 * kIn];
-    uint16_t h1[4][16 * kHid];
-    uint16_t h2[4][16 * kHid];
-    float y[4][16 * kOut];
-    uint16_t dy[4][16 * kOut];
-    uint16_t dh2[4][16 * kHid];
-    uint16_t dh1[4][16 * kHid];
+    uint16_t x[kMlpWaves][16 * kIn];
+    uint16_t h1[kMlpWaves][16 * kHid];
+    uint16_t h2[kMlpWaves][16 * kHid];
+    float y[kMlpWaves][16 * kOut];
+    uint16_t dy[kMlpWaves][16 * kOut];
+    uint16_t dh2[kMlpWaves][16 * kHid];
+    uint16_t dh1[kMlpWaves][16 * kHid];
 };
 
 __device__ void stage_weights(MlpLds& S, const float* __restrict__ P)
 {
-    for (int e = threadIdx.x; e < kIn * kHid; e += 256) {
+    for (int e = threadIdx.x; e < kIn * kHid; e += blockDim.x) {
         const int k = e / kHid, n = e % kHid;
         S.w1t[n * kIn + k] = bf16_rne(P[oW1 + e]);
     }
-    for (int e = threadIdx.x; e < kHid * kHid; e += 256) {
+    for (int e = threadIdx.x; e < kHid * kHid; e += blockDim.x) {
         const int k = e / kHid, n = e % kHid;
         const uint16_t h = bf16_rne(P[oW2 + e]);
         S.w2[e] = h;
         S.w2t[n * kHid + k] = h;
     }
-    for (int e = threadIdx.x; e < kHid * kOut; e += 256) {
+    for (int e = threadIdx.x; e < kHid * kOut; e += blockDim.x) {
         const int k = e / kOut, n = e % kOut;
         const uint16_t h = bf16_rne(P[oW3 + e]);
         S.w3[e] = h;
         S.w3t[n * kHid + k] = h;
     }
-    for (int e = threadIdx.x; e < kHid; e += 256) {
+    for (int e = threadIdx.x; e < kHid; e += blockDim.x) {
         S.b1[e] = P[oB1 + e];
         S.b2[e] = P[oB2 + e];
     }
-    for (int e = threadIdx.x; e < kOut; e += 256) S.b3[e] = P[oB3 + e];
+    for (int e = threadIdx.x; e < kOut; e += blockDim.x) S.b3[e] = P[oB3 + e];
 }
 
 // rows r0 .. r0+15 of X (zero past n) -> logits in S.y[wave]
@@ -135,13 +140,13 @@ __device__ void forward16(MlpLds& S, const float* __restrict__ X, uint32_t r0, u
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P, const float* __restrict__ X,
+__global__ __launch_bounds__(kMlpThreads) void k_mlp_forward(const float* __restrict__ P, const float* __restrict__ X,
                                                      uint32_t n, float* __restrict__ Y)
 {
     __shared__ MlpLds S;
     stage_weights(S, P);
     __syncthreads();
-    const uint32_t r0 = blockIdx.x * 64u + (threadIdx.x >> 6) * 16u;
+    const uint32_t r0 = blockIdx.x * (16u * kMlpWaves) + (threadIdx.x >> 6) * 16u;
     forward16(S, X, r0, n);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int e = lane; e < 16 * kOut; e += 64) {
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P
 
 // Forward, soft-target cross entropy (mean over the n samples) and backward;
 // gradients and loss are accumulated with fp32 atomics into G and *loss.
-__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
+__global__ __launch_bounds__(kMlpThreads) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
                                                   const float* __restrict__ T, uint32_t n, float* __restrict__ G,
                                                   float* __restrict__ loss)
 {
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, c
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c16 = lane & 15, q = lane >> 4;
-    const uint32_t r0 = blockIdx.x * 64u + (uint32_t)wv * 16u;
+    const uint32_t r0 = blockIdx.x * (16u * kMlpWaves) + (uint32_t)wv * 16u;
     forward16(S, X, r0, n);
     const float inv_n = 1.0f / (float)n;
     // softmax + loss + dY, one lane per row
@@ -370,13 +375,13 @@ __global__ void k_block_stats_combine(const double* __restrict__ part, uint32_t 
 void launch_mlp_forward(hipStream_t s, const float* P, const float* X, uint32_t n, float* Y)
 {
     if (!n) return;
-    k_mlp_forward<<<dim3((n + 63) / 64), dim3(256), 0, s>>>(P, X, n, Y);
+    k_mlp_forward<<<dim3((n + 16 * kMlpWaves - 1) / (16 * kMlpWaves)), dim3(kMlpThreads), 0, s>>>(P, X, n, Y);
 }
 
 void launch_mlp_grad(hipStream_t s, const float* P, const float* X, const float* T, uint32_t n, float* G, float* loss)
 {
     if (!n) return;
-    k_mlp_grad<<<dim3((n + 63) / 64), dim3(256), 0, s>>>(P, X, T, n, G, loss);
+    k_mlp_grad<<<dim3((n + 16 * kMlpWaves - 1) / (16 * kMlpWaves)), dim3(kMlpThreads), 0, s>>>(P, X, T, n, G, loss);
 }
 
 void launch_adam(hipStream_t s, float* P, float* G, float* M, float* V, uint32_t n, float lr, float b1, float b2,

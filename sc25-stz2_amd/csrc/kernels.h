@@ -401,7 +401,7 @@ void launch_interp_decode(hipStream_t s, const InterpPass& p, T* v, const uint16
                           const T* side);
 
 // ---- k_tuner.hip ----
-constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 160;
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 208;
 constexpr int kMlpParams = kMlpIn * kMlpHid + kMlpHid + kMlpHid * kMlpHid + kMlpHid + kMlpHid * kMlpOut + kMlpOut;
 constexpr int kFeatSums = 18;   // per-sub-block sums of k_block_stats
 void launch_mlp_forward(hipStream_t s, const float* P, const float* X, uint32_t n, float* Y);

@@ -1582,7 +1582,7 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 }
 
 // ---------------------------------------------------------------- tuner
-// The configurations the tuner chooses among: predictor set (3) x base codec
+// The configurations the tuner chooses among: predictor set (4) x base codec
 // (interpolation: 2 interpolators x 6 axis orders x 4 level-eb scalings;
 // Lorenzo: 3 scalings -- the fourth differs from the first only in the
 // interpolation levels' alpha / beta), 51 per set; index
@@ -1590,7 +1590,9 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 // 48 + scaling for Lorenzo, so 0..47 are F5's own predictors with
 // interpolation.  Predictor sets: F5's (F5:690-748); F5 built with
 // PRED_PLANE_AXIS and PRED_CUBE_AXIS (F5:579-622, F5:627-686); stz2.cpp's
-// 12-point plane and 16-point cube predictors (stz2.cpp:122-196).  Scalings: F5's multipliers
+// 12-point plane and 16-point cube predictors (stz2.cpp:122-196);
+// dependency.cpp's staged inverse-variance predictors (dependency.cpp:162-292).
+// Scalings: F5's multipliers
 // (F5:2127-2129); float.cpp's 1 / 2.5 / 6.25 (float.cpp:598); one bound for
 // every level; F5's multipliers with the base codec's level bounds shrinking
 // by 1.5 per level up to 4x (SZ3-style).  In ABS / REL modes the largest
@@ -1598,9 +1600,9 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 struct TuneScale { double mb, ml, mh, alpha, beta; };
 const TuneScale kScales[4] = {
     {0.7, 2.7, 8.2, 1.0, 1.0}, {1.0, 2.5, 6.25, 1.0, 1.0}, {1.0, 1.0, 1.0, 1.0, 1.0}, {0.7, 2.7, 8.2, 1.5, 4.0}};
-const int kTunePred[3] = {0, kPredPlaneAxis | kPredCubeAxis, kPredBicubic12 | kPredDiag16};
+const int kTunePred[4] = {0, kPredPlaneAxis | kPredCubeAxis, kPredBicubic12 | kPredDiag16, kPredStaged};
 constexpr int kTunePerSet = 51;
-constexpr int kTuneConfigs = 3 * kTunePerSet;
+constexpr int kTuneConfigs = 4 * kTunePerSet;
 // one MLP output per configuration; the outputs past them (the MFMA tiles
 // are 16 wide) are never chosen
 static_assert(kTuneConfigs <= kMlpOut && kMlpOut - kTuneConfigs < 16, "MLP outputs");

@@ -32,7 +32,7 @@ EXPORTS = [
     "stz_mlp_forward", "stz_mlp_grad", "stz_mlp_adam",
     "stz_compress_batch", "stz_decompress_batch", "stz_autotune",
 ]
-TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS, MLP_OUTPUTS = 153, 16, 15648, 160
+TUNE_CONFIGS, TUNE_FEATURES, MLP_PARAMS, MLP_OUTPUTS = 204, 16, 18768, 208
 
 
 class StzError(RuntimeError):

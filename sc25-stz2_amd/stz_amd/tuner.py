@@ -1,7 +1,7 @@
-"""Learned tuner: a 16 -> 64 -> 64 -> 160 MLP over per-sub-block field
+"""Learned tuner: a 16 -> 64 -> 64 -> 208 MLP over per-sub-block field
 statistics that picks the residual predictor set, base codec (interpolator
-and axis order, or Lorenzo) and level-eb scaling (153 configurations, the
-first 153 outputs) in place of the exhaustive trial.
+and axis order, or Lorenzo) and level-eb scaling (204 configurations, the
+first 204 outputs) in place of the exhaustive trial.
 
 Labels come from the exhaustive trial itself (the payload size of every
 configuration on generated fields); the target of a sub-block is the soft

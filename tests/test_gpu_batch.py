@@ -94,7 +94,7 @@ def test_batch_argument_checks(stz, ctx):
 
 def test_batch_learned_and_trial_choices(stz, ctx):
     """tuner = 1 in a batch picks what single calls pick (same model, no
-    fallback); tuner = 2 keeps each field's smallest payload over the 153
+    fallback); tuner = 2 keeps each field's smallest payload over the 204
     configurations."""
     import torch
     from stz_amd import tuner

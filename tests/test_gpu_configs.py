@@ -6,7 +6,7 @@
 * 256x384x384 fp64 turbulence-like field (seed 1003) at rel eb 1e-6 with the
   learned tuner and with the exhaustive trial: the payload equals the
   oracle's for the chosen configuration, the trial's choice is the smallest
-  of its 153 sizes and those sizes match the oracle's, decode within the bound;
+  of its 204 sizes and those sizes match the oracle's, decode within the bound;
 * 96 uneven fp32 fields (seeds 1004 + i) in one batch call at rel eb 1e-3:
   every stream and decoded field equals its one-call result, every eighth
   field's payload equals the oracle's, every field within the bound;

@@ -145,7 +145,7 @@ def test_payload_and_decode_bitexact(ctx, oracle, case):
 
 
 def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
-    """All 153 configurations the learned tuner chooses among (predictor set x
+    """All 204 configurations the learned tuner chooses among (predictor set x
     base codec -- interpolator x axis order, or Lorenzo -- x level-bound
     scaling): payload bytes equal the oracle's, f32 and f64, and a spread of
     them decode bit-exactly."""
@@ -153,7 +153,7 @@ def test_every_tuner_configuration_bitexact(stz, ctx, oracle):
     f32 = _field("grf", (36, 40, 44), np.float32, seed=17)
     f64 = _field("sin", (20, 24, 28), np.float64, seed=5)
     from stz_amd._lib import TUNE_CONFIGS
-    assert TUNE_CONFIGS == 153
+    assert TUNE_CONFIGS == 204
     for k in range(TUNE_CONFIGS):
         prm = tuner.config_params(k)
         for f, eb in ((f32, 1e-3), (f64, 1e-5)):
