@@ -13,6 +13,53 @@
 
 namespace stz {
 
+// Histogram counts: an LDS bin (workgroup scope) and a global bin (device
+// scope).  The scopes differ on purpose: two identical atomics on either
+// side of a branch get merged into one through a selected generic pointer,
+// a flat atomic that waits on both the vector-memory and the LDS counters.
+__device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void glb_add(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The digits (a, b, c) of work items t = t0, t0 + step, t0 + 2 step, ... in
+// the mixed radix t = (c nb + b) na + a, kept up to date by additions: a
+// grid-stride item loop then divides once, not once per item (a 64-bit
+// division by a run-time value is a long scalar routine).  Every quantity is
+// wave-uniform.
+struct Walk3 {
+    uint32_t a, b, c;     // digits of the current item
+    uint32_t sa, sb, sc;  // digits of the step
+    uint32_t na, nb;
+    __device__ __forceinline__ void init(uint64_t t0, uint64_t step, uint32_t na_, uint32_t nb_)
+    {
+        na = na_;
+        nb = nb_;
+        const uint64_t nab = (uint64_t)na * nb;
+        a = (uint32_t)(t0 % na);
+        b = (uint32_t)((t0 / na) % nb);
+        c = (uint32_t)(t0 / nab);
+        sa = (uint32_t)(step % na);
+        sb = (uint32_t)((step / na) % nb);
+        sc = (uint32_t)(step / nab);
+    }
+    __device__ __forceinline__ void next()
+    {
+        a += sa;
+        const uint32_t ca = a >= na ? 1u : 0u;
+        a -= ca ? na : 0u;
+        b += sb + ca;
+        const uint32_t cb = b >= nb ? 1u : 0u;
+        b -= cb ? nb : 0u;
+        c += sc + cb;
+    }
+};
+
 // F5:141-146 cubic_line_pred
 template <typename T>
 __device__ __forceinline__ T cub(T a, T b, T c, T d)

@@ -75,8 +75,8 @@ __device__ __forceinline__ PassPoint pass_point(const InterpPass& p, uint64_t k6
 
 __device__ __forceinline__ void hist_put(uint32_t* sh, uint32_t* gh, uint32_t q, uint32_t n)
 {
-    if (q - kHLo < (uint32_t)kHW) atomicAdd(&sh[q - kHLo], n);
-    else atomicAdd(&gh[q], n);
+    if (q - kHLo < (uint32_t)kHW) lds_add(&sh[q - kHLo], n);
+    else glb_add(&gh[q], n);
 }
 
 __device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t* gh, uint32_t q, bool valid)

@@ -31,8 +31,8 @@ constexpr uint32_t kHLo = kRadius - kHW / 2;
 
 __device__ __forceinline__ void hist_put(uint32_t* sh, uint32_t* gh, uint32_t q, uint32_t n)
 {
-    if (q - kHLo < (uint32_t)kHW) atomicAdd(&sh[q - kHLo], n);
-    else atomicAdd(&gh[q], n);
+    if (q - kHLo < (uint32_t)kHW) lds_add(&sh[q - kHLo], n);
+    else glb_add(&gh[q], n);
 }
 
 // one wave's codes into the LDS histogram: lanes holding the most common

@@ -47,8 +47,8 @@ __device__ __forceinline__ uint32_t xcd_block()
 
 __device__ __forceinline__ void hist_put(uint32_t* sh, uint32_t* gh, uint32_t q, uint32_t n)
 {
-    if (q - kHLo < (uint32_t)kHW) atomicAdd(&sh[q - kHLo], n);
-    else atomicAdd(&gh[q], n);
+    if (q - kHLo < (uint32_t)kHW) lds_add(&sh[q - kHLo], n);
+    else glb_add(&gh[q], n);
 }
 
 // Histogram update by every active lane of the wave (`valid` says which
@@ -61,7 +61,7 @@ __device__ __forceinline__ void hist_add(uint32_t* sh, uint32_t* gh, uint32_t q,
     const unsigned long long m = __ballot(centre);
     const unsigned long long act = __ballot(1);
     if (m && __lane_id() == (unsigned)(__ffsll((long long)act) - 1))
-        atomicAdd(&sh[kRadius - kHLo], (uint32_t)__popcll(m));
+        lds_add(&sh[kRadius - kHLo], (uint32_t)__popcll(m));
     if (valid && !centre) hist_put(sh, gh, q, 1u);
 }
 
@@ -589,12 +589,14 @@ __global__ __launch_bounds__(256) void k_level_tile(const LevelK<T> K)
     const uint64_t per = sweep ? (total + 7) / 8 : total;
     const uint64_t base = sweep ? (uint64_t)(blockIdx.x % 8u) * per : 0;
     const uint64_t first = sweep ? blockIdx.x / 8u : xcd_block(), step = sweep ? G / 8u : G;
-    for (uint64_t j = first; j < per; j += step) {
+    Walk3 wk;
+    wk.init(base + first, step, (uint32_t)(ntx > 0 ? ntx : 1), (uint32_t)nty);
+    for (uint64_t j = first; j < per; j += step, wk.next()) {
         const uint64_t t = base + j;
         if (t >= total) break;
         if (t < ntiles) {
-            const int tx = (int)(t % ntx), ty = (int)((t / ntx) % nty);
-            const int z = (int)(t / ((uint64_t)ntx * nty));
+            const int tx = (int)wk.a, ty = (int)wk.b;
+            const int z = (int)wk.c;
             const int x = 1 + tx * 64 + lane;
             const int y0 = ty * 4 * kTW + kTW * row;
             const bool okx = x <= g.dx - 3;

@@ -224,17 +224,21 @@ struct LevelArgs {
 constexpr uint32_t kNoChunk = 0xFFFFFFFFu;
 
 // ---- k_staged.hip (dependency.cpp's staged predictors, option bit kPredStaged) ----
-// One residual level in three launches (blocks 0/1/3, 2/4/5, 6).  out: the
-// level's 2x grid (2 g.dx x 2 g.dy x 2 g.dz), filled with ref at parity 0 and
-// every block's reconstruction.  Encode: targets from the padded field at
-// stride tstep (2 high level, 4 low level); codes, unpredictable residuals
-// and the 7 streams' histograms (hist: 7 x 65536) as the other level kernels.
+// One residual level in three launches (blocks 0/1/3, 2/4/5, 6).  rec:
+// scratch for the reconstructions of blocks 0..5, 6 x g.dx x g.dy x g.dz
+// values.  Encode: targets from the padded field at stride tstep (2 high
+// level, 4 low level); codes, unpredictable residuals and the 7 streams'
+// histograms (hist: 7 x 65536) as the other level kernels; out (or null):
+// the level's 2x grid (2 g.dx x 2 g.dy x 2 g.dz) with ref at parity 0 and
+// every block's reconstruction.  Decode: the same 2x grid into out with dims
+// nx x ny x nz (points past them are not written: the padded grid, or the
+// field it crops to).
 template <typename T>
 void launch_staged_encode(hipStream_t s, const T* full, int px, int py, int tstep, const T* ref, Geom g, LevelArgs a,
-                          uint16_t* codes, T* side, uint32_t* hist, T* out);
+                          uint16_t* codes, T* side, uint32_t* hist, T* out, T* rec);
 template <typename T>
 void launch_staged_decode(hipStream_t s, const T* ref, Geom g, LevelArgs a, const uint16_t* codes, const T* side,
-                          T* out);
+                          T* out, int nx, int ny, int nz, T* rec);
 
 
 // one box of a batched random-access decode (inclusive bounds); workgroups

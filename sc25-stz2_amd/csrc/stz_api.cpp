@@ -542,6 +542,13 @@ void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, siz
     L.total_codes = off;
 }
 
+// scratch bytes of the staged predictors' reconstructions of blocks 0..5 of
+// either residual level (the high level's block grid is the larger)
+inline size_t staged_scratch(const HostLayout& L, size_t elem)
+{
+    return 6 * (size_t)L.n[8] * elem;
+}
+
 // Layouts by shape: building one walks every chunk of every stream (tens of
 // microseconds at 512^3), and fields of one shape come back again and again
 // (steps, serving, batches), so the process keeps the recent ones.
@@ -1303,17 +1310,16 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
     OutBuf ob(c, out, m.fh.nx * m.fh.ny * m.fh.nz * sizeof(T));
     if (m.fh.pred & kPredStaged) {
         // dependency.cpp's staged predictors: three launches per level, the
-        // high level into the padded grid (its stencils read past the crop)
-        DBuf pad(c->pool, (size_t)m.fh.px * m.fh.py * m.fh.pz * sizeof(T));
+        // high level's cells cropped to the field as they are written
+        DBuf rec(c->pool, staged_scratch(L, sizeof(T)));
         K(c, "low_decode", [&] {
-            launch_staged_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>());
+            launch_staged_decode<T>(st, vb.as<T>(), L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(),
+                                    L.g[8].dx, L.g[8].dy, L.g[8].dz, rec.as<T>());
         });
         join_high(c, dd);
         K(c, "high_decode", [&] {
-            launch_staged_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(),
-                                    pad.as<T>());
-            launch_crop<T>(st, pad.as<T>(), (int)m.fh.px, (int)m.fh.py, (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, 0,
-                           0, 0, 1, (T*)ob.ptr);
+            launch_staged_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(), (T*)ob.ptr,
+                                    (int)m.fh.nx, (int)m.fh.ny, (int)m.fh.nz, rec.as<T>());
         });
     } else {
         K(c, "low_decode", [&] {
@@ -1451,7 +1457,7 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
             decode_base<T>(c, m, L, dd, vbuf.as<T>(), 0);
             vb = vbuf.as<T>();
         }
-        DBuf mid(c->pool, (size_t)mx * my * mz * sizeof(T)), pad;
+        DBuf mid(c->pool, (size_t)mx * my * mz * sizeof(T)), pad, rec(c->pool, staged_scratch(L, sizeof(T)));
         const LevelArgs la = level_args(m, dd, 1, err.ptr()), ha = level_args(m, dd, 2, err.ptr());
         bool need_full = false, need_mid2 = false;
         for (int i = 0; i < n; ++i) {
@@ -1461,13 +1467,14 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
         }
         if (need_mid2)
             K(c, "low_decode", [&] {
-                launch_staged_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>());
+                launch_staged_decode<T>(st, vb, L.g[1], la, dd.codes.as<uint16_t>(), dd.side.as<T>(), mid.as<T>(), mx,
+                                        my, mz, rec.as<T>());
             });
         if (need_full) {
             pad = DBuf(c->pool, (size_t)m.fh.px * m.fh.py * m.fh.pz * sizeof(T));
             K(c, "high_decode", [&] {
                 launch_staged_decode<T>(st, mid.as<T>(), L.g[8], ha, dd.codes.as<uint16_t>(), dd.side.as<T>(),
-                                        pad.as<T>());
+                                        pad.as<T>(), (int)m.fh.px, (int)m.fh.py, (int)m.fh.pz, rec.as<T>());
             });
         }
         for (int i = 0; i < n; ++i) {
@@ -2056,20 +2063,18 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 FieldEnc<T>& f = fe[i];
                 const LevelArgs a = enc_level_args(f, level);
                 if (f.prm.pred & kPredStaged) {
-                    // dependency.cpp's staged predictors: three launches per level;
-                    // the high level's reconstruction goes to a full-size grid
-                    DBuf rec;
-                    T* out = f.mid.template as<T>();
-                    if (level == 2) {
-                        rec = DBuf(pool, (size_t)f.px * f.py * f.pz * sizeof(T));
-                        out = rec.template as<T>();
-                    }
+                    // dependency.cpp's staged predictors: three launches per
+                    // level; only the low level's 2x grid is written (the
+                    // high level's coarser grid)
+                    T* out = level == 1 ? f.mid.template as<T>() : nullptr;
+                    DBuf rec(pool, staged_scratch(f.L, sizeof(T)));
                     K(c, level == 1 ? "low_encode" : "high_encode", [&] {
                         launch_staged_encode<T>(st, f.full, (int)f.px, (int)f.py, level == 1 ? 4 : 2,
                                                 level == 1 ? f.vbase.template as<T>() : f.mid.template as<T>(),
                                                 f.L.g[level == 1 ? 1 : 8], a, f.codes.template as<uint16_t>(),
                                                 f.side.template as<T>(),
-                                                f.hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, out);
+                                                f.hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, out,
+                                                rec.template as<T>());
                     });
                 } else if (level == 1) {
                     K(c, "low_encode", [&] {
