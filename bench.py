@@ -87,6 +87,62 @@ def measured_traffic(name, n, s):
     return {"bytes_per_launch": int(k["bytes_per_launch"]), "method": d.get("method", "")}
 
 
+def host_cpu():
+    """The host CPU the baseline runs on: model name, sockets, logical CPUs
+    visible to this process, and the OpenMP thread binding in effect."""
+    model, sockets = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model is None:
+                    model = v.strip()
+                elif k == "physical id":
+                    sockets.add(v.strip())
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        visible = os.cpu_count()
+    return {"cpu_model": model, "sockets": len(sockets) or None, "visible_cpus": visible,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", "unset"),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS", "unset")}
+
+
+def copy_peak(torch, dev, mib=1024, reps=10):
+    """Device-to-device copy rate (read + write bytes per second) of a
+    `mib` MiB buffer: the practical HBM ceiling beside the 8 TB/s peak."""
+    a = torch.empty(mib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return round(2.0 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
+
+
+def psnr_db(ref, rec):
+    """PSNR of rec against ref over the value range (the reference's Global
+    PSNR line, F5:3003-3016): 20 log10(range) - 10 log10(MSE)."""
+    import math
+    d = (rec.double() - ref.double())
+    mse = float((d * d).mean())
+    rng = float(ref.max() - ref.min())
+    if mse == 0.0:
+        return float("inf")
+    return round(20.0 * math.log10(rng) - 10.0 * math.log10(mse), 3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,9 +240,10 @@ def main():
     payload = last.payload_bytes
     eb_req = rel_bound(field, args.eb)
     ok = err <= eb_req
+    psnr = psnr_db(field, out)
     log("step %.3f ms (device), wall %.3f ms; compress %.3f ms, decompress %.3f ms (host clock); max err %.3g "
-        "eb %.3g ok=%s" % (ms_per_step, wall * 1e3 / args.steps, tc * 1e3 / args.steps, td * 1e3 / args.steps,
-                           err, eb_high, ok))
+        "eb %.3g ok=%s psnr %.2f dB" % (ms_per_step, wall * 1e3 / args.steps, tc * 1e3 / args.steps,
+                                        td * 1e3 / args.steps, err, eb_high, ok, psnr))
 
     # host zstd of the payload (timed separately, D2H included)
     z0 = time.perf_counter()
@@ -211,11 +268,14 @@ def main():
         dom_name, (dom_launches, dom_total_ms) = dom
         dom_avg_ms = dom_total_ms / dom_launches
         kb = kernel_bytes(dom_name, N, s, payload)
+    copy_gbps = copy_peak(torch, dev)
+    log("device copy: %.1f GB/s (read + write)" % copy_gbps)
     if kb is not None:
         ach = kb / (dom_avg_ms * 1e-3) / 1e9
         roof = {"kernel": dom_name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(dom_avg_ms, 4), "algorithmic_bytes": int(kb)}
+                "avg_launch_ms": round(dom_avg_ms, 4), "algorithmic_bytes": int(kb),
+                "measured_copy_GBps": copy_gbps, "frac_of_copy": round(ach / copy_gbps, 4)}
         tr = measured_traffic(dom_name, args.n, s)
         if tr is not None:
             roof["traffic"] = tr["bytes_per_launch"]
@@ -224,7 +284,8 @@ def main():
     step_alg_bytes = 2.0 * (N * s + payload)
     whole = {"algorithmic_bytes_per_step": int(step_alg_bytes),
              "achieved_GBps": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-             "frac_of_hbm_peak": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+             "frac_of_hbm_peak": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "frac_of_copy": round(step_alg_bytes / (ms_per_step * 1e-3) / 1e9 / copy_gbps, 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -245,7 +306,7 @@ def main():
             tcs.append(c1 - c0)
             tds.append(c2 - c1)
         med = float(np.median(rates))
-        cpu = {"value": round(med, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(med, 4), "unit": "GB/s", "cores": threads, "kind": "port", **host_cpu(),
                "sample": "the whole %d^3 field, compress + decompress (pre-zstd payload) by the C/OpenMP oracle, "
                          "%d repetitions in %.1f s, median; compress %.2f s, decompress %.2f s (medians)"
                          % (n, len(rates), time.perf_counter() - t_start, float(np.median(tcs)),
@@ -277,7 +338,7 @@ def main():
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
         "payload_bytes": int(payload), "cr_payload": round(cr_payload, 3), "cr_zstd": round(cr_zstd, 3),
         "zstd_ms": round(zstd_ms, 2), "max_err": err, "eb_abs": eb_req, "eb_high_level": eb_high,
-        "error_bound_ok": bool(ok),
+        "error_bound_ok": bool(ok), "psnr_db": psnr,
         "roofline": roof, "whole_step": whole, "gpu_max_hw_queues": HW_QUEUES,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1][1])},
         "cpu_baseline": cpu,
