@@ -907,12 +907,17 @@ __device__ __forceinline__ uint32_t decode_run_lds(const uint32_t* W, uint64_t w
         return sym;
     };
     uint32_t i = 0;
-    const uint32_t head = (uint32_t)((4u - (uint32_t)(al & 3u)) & 3u);
+    const uint32_t head = (uint32_t)((8u - (uint32_t)(al & 7u)) & 7u);
     for (; i < head && i < cnt; ++i) co[i] = (uint16_t)single(i);
-    // co + i is 8-byte aligned; `lo` holds pc decoded codes not yet stored
-    uint64_t lo = 0;
-    uint32_t pc = 0;
-    while (i + pc + 3 <= cnt) {
+    // co + i is 16-byte aligned.  Codes leave in 16-byte pairs of 4-code
+    // groups: `pv` holds the group at co + i when `hp`, `lo` the pc codes
+    // after it.  A lane's stores land on lines of their own (one sub-chunk
+    // each), so every store instruction addresses 64 lines; halving their
+    // number halves that address work, which stalled the lanes' chains
+    // (8-byte stores took a quarter of the decode's time).
+    uint64_t lo = 0, pv = 0;
+    uint32_t pc = 0, hp = 0;
+    while (i + hp + pc + 3 <= cnt) {
         const uint32_t NN = W[a + 3];
         const uint32_t win = __builtin_amdgcn_alignbit(A, B, (uint32_t)(32 - k));
         const uint64_t e = lut[win >> (32 - kLutBits)];
@@ -928,18 +933,28 @@ __device__ __forceinline__ uint32_t decode_run_lds(const uint32_t* W, uint64_t w
                 v = slow(win, l);
                 n = 1;
             }
-            if (v == 0ull) unpred(i + pc);
+            if (v == 0ull) unpred(i + hp + pc);
         }
         consume(l, NN);
         const uint64_t spill = pc ? v >> (64 - 16 * pc) : 0ull;
         lo |= v << (16 * pc);
         pc += n;
         if (pc >= 4) {
-            *(uint2*)(co + i) = make_uint2((uint32_t)lo, (uint32_t)(lo >> 32));
-            i += 4;
+            if (hp) {
+                *(uint4*)(co + i) = make_uint4((uint32_t)pv, (uint32_t)(pv >> 32), (uint32_t)lo, (uint32_t)(lo >> 32));
+                i += 8;
+                hp = 0;
+            } else {
+                pv = lo;
+                hp = 4;
+            }
             pc -= 4;
             lo = spill;
         }
+    }
+    if (hp) {
+        *(uint2*)(co + i) = make_uint2((uint32_t)pv, (uint32_t)(pv >> 32));
+        i += 4;
     }
     for (uint32_t q = 0; q < pc; ++q) co[i + q] = (uint16_t)(lo >> (16 * q));
     i += pc;
