@@ -2,7 +2,8 @@
 
 * 512^3 fp32 log-normal field (seed 1002) at rel eb 1e-2, 1e-3 and 1e-4:
   payload bytes equal the oracle's, decode bit-exact, error within the
-  requested bound (strictly, as every check below);
+  requested bound (strictly, as every check below); the same at 1e-3 under
+  dependency.cpp's staged predictors;
 * 256x384x384 fp64 turbulence-like field (seed 1003) at rel eb 1e-6 with the
   learned tuner and with the exhaustive trial: the payload equals the
   oracle's for the chosen configuration, the trial's choice is the smallest
@@ -86,6 +87,33 @@ def test_c2_512cubed_lognormal_bitexact(ctx, oracle, dev, c2_field, eb):
     ref = oracle.decompress(want)
     _equal_bits(rec, ref, "512^3 decompress")
     ok, err, bound = _bound_ok(rec, f, eb)
+    assert ok, (err, bound)
+    ds.free()
+
+
+# dependency.cpp's staged predictors at the same size: the default base and
+# multipliers, and the configuration the learned tuner picks for this field
+# (linear base, axis order 1, multipliers 1 / 1 / 1)
+@pytest.mark.parametrize("params", [dict(pred=32),
+                                    dict(pred=32, interp=0, order=1, mult_base=1.0, mult_low=1.0, mult_high=1.0)],
+                         ids=["staged", "staged-linear-o1-m1"])
+def test_c2_512cubed_staged_bitexact(ctx, oracle, dev, c2_field, params):
+    import torch
+    f = c2_field
+    _say("512^3 log-normal, rel eb 1e-3, staged predictors", params)
+    t = torch.from_numpy(f).to(dev)
+    ds = ctx.compress_device(t, 1e-3, "rel", **params)
+    gpu = ds.download(zstd_level=-1)
+    t0 = time.time()
+    want = oracle.compress(f, 1e-3, oracle.EB_REL, **params)
+    _say("  oracle compress %.1fs, payload %d bytes" % (time.time() - t0, want.size))
+    diff = first_difference(gpu, want)
+    assert diff is None, diff
+    out = torch.empty_like(t)
+    ds.decompress(out)
+    rec = out.cpu().numpy()
+    _equal_bits(rec, oracle.decompress(want), "512^3 staged decompress")
+    ok, err, bound = _bound_ok(rec, f, 1e-3)
     assert ok, (err, bound)
     ds.free()
 
