@@ -203,9 +203,9 @@ int stz_dstream_decompress_batch(stz_ctx* ctx, stz_dstream* ds, int n, const stz
  * persistent launch per level fed by a device work queue; its launch
  * geometry (waves per workgroup, workgroups per CU) is timed per dtype,
  * level and size class the first time and kept in the context.  Streams and
- * decoded fields are bit-identical to one call per field.  The batch decode
- * is captured as a HIP graph and replayed while a call names the same
- * streams and outputs (device memory).  Replaces per-field runs of the
+ * decoded fields are bit-identical to one call per field.  A new set of
+ * streams decodes eagerly; a second call naming the same streams and outputs
+ * (device memory) captures the decode as a HIP graph, replayed from then on.  Replaces per-field runs of the
  * compress / decompress halves of run_typed (F5:2102-2451). */
 int stz_compress_batch(stz_ctx* ctx, stz_dtype dtype, int n, const void* const* fields,
                        const size_t* dims /* 3 per field: nx ny nz */, double eb, stz_eb_mode mode,

@@ -1703,10 +1703,9 @@ void jobs_launch(stz_ctx* c, int mode, JobSet& js, const stz_ctx::LaunchCfg& cfg
     });
 }
 
-// the tuned geometry for this launch; times the candidates the first time
-// (reset() restores the launch's inputs before each candidate)
-template <typename T, typename R>
-stz_ctx::LaunchCfg level_cfg(stz_ctx* c, int mode, JobSet& js, R&& reset)
+// the autotuner's key of a level launch: dtype, mode and size class
+template <typename T>
+std::string level_cfg_key(int mode, const JobSet& js)
 {
     uint64_t pts = 0;
     for (const LevelJob& j : js.h[0]) pts += (uint64_t)j.g.dx * j.g.dy * j.g.dz;
@@ -1716,6 +1715,22 @@ stz_ctx::LaunchCfg level_cfg(stz_ctx* c, int mode, JobSet& js, R&& reset)
     std::snprintf(key, sizeof(key), "%s/%s/2^%d", sizeof(T) == 4 ? "f32" : "f64",
                   mode == kLowEnc ? "low_encode" : mode == kHighEnc ? "high_encode" : mode == kLowDec ? "low_decode"
                                                                                            : "high_decode", lg);
+    return key;
+}
+
+// whether level_cfg would time candidates for this launch
+template <typename T>
+bool level_cfg_pending(stz_ctx* c, int mode, const JobSet& js)
+{
+    return c->autotune && c->tuned.find(level_cfg_key<T>(mode, js)) == c->tuned.end();
+}
+
+// the tuned geometry for this launch; times the candidates the first time
+// (reset() restores the launch's inputs before each candidate)
+template <typename T, typename R>
+stz_ctx::LaunchCfg level_cfg(stz_ctx* c, int mode, JobSet& js, R&& reset)
+{
+    const std::string key = level_cfg_key<T>(mode, js);
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) return it->second;
     stz_ctx::LaunchCfg best;
@@ -2565,31 +2580,46 @@ void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* cons
         plan_decode_jobs<T>(c, *P);
         jobs_finish(c, P->low);
         jobs_finish(c, P->high);
-        // eager run (also the inputs of the timing runs), then the geometry
-        plan_enqueue<T>(c, *P, false);
-        P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
-        jobs_launch<T>(c, kLowDec, P->low, P->cfg_low);
-        P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
-        HIPCHK(hipStreamSynchronize(c->stream));
-        // capture the whole sequence
-        const bool prof = c->prof;
-        c->prof = false;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        try {
+        // A new set of streams runs eagerly: most batches are decoded once,
+        // and a capture plus instantiation costs more host time than the
+        // launches.  The first batch of a size class also times the level
+        // geometries, the eager run's results being their inputs (a decode
+        // is idempotent, so a candidate that runs again writes the same).
+        if (level_cfg_pending<T>(c, kLowDec, P->low) || level_cfg_pending<T>(c, kHighDec, P->high)) {
+            plan_enqueue<T>(c, *P, false);
+            P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
+            jobs_launch<T>(c, kLowDec, P->low, P->cfg_low);
+            P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
+            jobs_launch<T>(c, kHighDec, P->high, P->cfg_high);
+        } else {
+            P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
+            P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
             plan_enqueue<T>(c, *P, true);
-        } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(c->stream, &g);
-            if (g) (void)hipGraphDestroy(g);
-            c->prof = prof;
-            throw;
         }
-        HIPCHK(hipStreamEndCapture(c->stream, &P->graph));
-        c->prof = prof;
-        HIPCHK(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
         c->plan = std::move(P);
+    } else {
+        // the same streams and outputs again: the sequence is captured as a
+        // graph once and replayed from then on
+        BatchPlan& P = *c->plan;
+        if (!P.exec) {
+            const bool prof = c->prof;
+            c->prof = false;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+            try {
+                plan_enqueue<T>(c, P, true);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                c->prof = prof;
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(c->stream, &P.graph));
+            c->prof = prof;
+            HIPCHK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(P.exec, c->stream));
     }
-    HIPCHK(hipGraphLaunch(c->plan->exec, c->stream));
     uint32_t* herr = (uint32_t*)c->host_stage((size_t)n * 4);
     HIPCHK(hipMemcpyAsync(herr, c->plan->err.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
