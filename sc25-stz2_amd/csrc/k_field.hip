@@ -231,6 +231,99 @@ __global__ __launch_bounds__(1024) void k_dequant_list(const uint16_t* __restric
     }
 }
 
+// ---------------------------------------------------------------- many fields
+
+// the job whose range of one kind holds workgroup b: the last job with
+// wg0 <= b (jobs with no workgroups of that kind share the next one's wg0)
+template <typename F>
+__device__ __forceinline__ uint32_t find_pro_job(const ProJob* __restrict__ jobs, uint32_t n, uint32_t b, F wg0)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (wg0(jobs[m]) <= b) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+// k_pad per field, a workgroup per range of output rows
+template <typename T>
+__global__ __launch_bounds__(256) void k_pad_multi(const ProJob* __restrict__ jobs, uint32_t n)
+{
+    const ProJob& J = jobs[find_pro_job(jobs, n, blockIdx.x, [](const ProJob& q) { return q.pad_wg0; })];
+    const uint32_t rows = (uint32_t)J.py * (uint32_t)J.pz, per = (rows + J.pad_nwg - 1) / J.pad_nwg;
+    const uint32_t r0 = (blockIdx.x - J.pad_wg0) * per, r1 = min(rows, r0 + per);
+    const T* in = (const T*)J.in;
+    T* out = (T*)J.padded;
+    for (uint32_t r = r0; r < r1; ++r) {
+        const int z = (int)(r / (uint32_t)J.py), y = (int)(r % (uint32_t)J.py);
+        const int sz = z < J.nz ? z : J.nz - 1, sy = y < J.ny ? y : J.ny - 1;
+        const T* src = in + ((uint64_t)sz * J.ny + sy) * J.nx;
+        T* dst = out + (uint64_t)r * J.px;
+        for (int x = threadIdx.x; x < J.px; x += 256) dst[x] = src[x < J.nx ? x : J.nx - 1];
+    }
+}
+
+// k_minmax per field: the job's workgroups stride over its values
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax_multi(const ProJob* __restrict__ jobs, uint32_t n)
+{
+    __shared__ double smn[256], smx[256];
+    const ProJob& J = jobs[find_pro_job(jobs, n, blockIdx.x, [](const ProJob& q) { return q.mm_wg0; })];
+    const uint32_t local = blockIdx.x - J.mm_wg0;
+    const T* d = (const T*)J.in;
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    constexpr int V = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
+    const uint64_t nv = ((uintptr_t)d % 16 == 0) ? J.nin / V : 0;
+    const vec* dv = (const vec*)d;
+    const uint64_t stride = (uint64_t)J.mm_nwg * 256ull;
+    for (uint64_t i = local * 256ull + threadIdx.x; i < nv; i += stride) {
+        const vec v = dv[i];
+#pragma unroll
+        for (int k = 0; k < V; ++k) mm_upd<T>((double)v[k], mn, mx);
+    }
+    for (uint64_t i = nv * V + local * 256ull + threadIdx.x; i < J.nin; i += stride) mm_upd<T>((double)d[i], mn, mx);
+    block_minmax<T>(mn, mx, smn, smx);
+    if (threadIdx.x == 0) {
+        J.partial[2 * local] = smn[0];
+        J.partial[2 * local + 1] = smx[0];
+    }
+}
+
+// one workgroup per field: its partials -> mm_out[2 j], mm_out[2 j + 1]
+__global__ __launch_bounds__(256) void k_minmax_final_multi(const ProJob* __restrict__ jobs, double* __restrict__ out)
+{
+    __shared__ double smn[256], smx[256];
+    const ProJob& J = jobs[blockIdx.x];
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    for (uint32_t b = threadIdx.x; b < J.mm_nwg; b += 256) {
+        if (J.partial[2 * b] < mn) mn = J.partial[2 * b];
+        if (J.partial[2 * b + 1] > mx) mx = J.partial[2 * b + 1];
+    }
+    block_minmax<double>(mn, mx, smn, smx);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = smn[0];
+        out[2 * blockIdx.x + 1] = smx[0];
+    }
+}
+
+// k_gather_low0 per field
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_multi(const ProJob* __restrict__ jobs, uint32_t n)
+{
+    const ProJob& J = jobs[find_pro_job(jobs, n, blockIdx.x, [](const ProJob& q) { return q.ga_wg0; })];
+    const uint32_t lx = (uint32_t)J.px / 4, ly = (uint32_t)J.py / 4, lz = (uint32_t)J.pz / 4;
+    const uint64_t nb = (uint64_t)lx * ly * lz;
+    const T* full = (const T*)J.full;
+    T* v = (T*)J.vbase;
+    const uint64_t stride = (uint64_t)J.ga_nwg * 256ull;
+    for (uint64_t i = (uint64_t)(blockIdx.x - J.ga_wg0) * 256ull + threadIdx.x; i < nb; i += stride) {
+        const uint32_t x = (uint32_t)(i % lx), t = (uint32_t)(i / lx), y = t % ly, z = t / ly;
+        v[i] = full[((uint64_t)(4 * z) * J.py + 4 * y) * J.px + 4 * x];
+    }
+}
+
 inline unsigned grid_for(uint64_t n, unsigned cap = 16384)
 {
     uint64_t b = (n + 255) / 256;
@@ -259,6 +352,19 @@ void launch_gather_low0(hipStream_t s, const T* full, int px, int py, int pz, T*
 {
     const uint64_t n = (uint64_t)(px / 4) * (py / 4) * (pz / 4);
     k_gather_low0<T><<<dim3(grid_for(n)), dim3(256), 0, s>>>(full, px, py, pz, v);
+}
+
+template <typename T>
+void launch_prologue_multi(hipStream_t s, const ProJob* jobs, uint32_t n, uint32_t pad_wgs, uint32_t mm_wgs,
+                           uint32_t ga_wgs, double* mm_out)
+{
+    if (!n) return;
+    if (pad_wgs) k_pad_multi<T><<<dim3(pad_wgs), dim3(256), 0, s>>>(jobs, n);
+    if (mm_wgs) {
+        k_minmax_multi<T><<<dim3(mm_wgs), dim3(256), 0, s>>>(jobs, n);
+        k_minmax_final_multi<<<dim3(n), dim3(256), 0, s>>>(jobs, mm_out);
+    }
+    if (ga_wgs) k_gather_multi<T><<<dim3(ga_wgs), dim3(256), 0, s>>>(jobs, n);
 }
 
 template <typename T>
@@ -331,6 +437,8 @@ void launch_dequant_list(hipStream_t s, const uint16_t* codes, const T* unpred, 
     template void launch_minmax<T>(hipStream_t, const T*, uint64_t, double*, double*);                \
     template void launch_pad<T>(hipStream_t, const T*, int, int, int, T*, int, int, int);             \
     template void launch_gather_low0<T>(hipStream_t, const T*, int, int, int, T*);                    \
+    template void launch_prologue_multi<T>(hipStream_t, const ProJob*, uint32_t, uint32_t, uint32_t, uint32_t, \
+                                           double*);                                                   \
     template void launch_split<T>(hipStream_t, const T*, int, int, int, T* const*);                   \
     template void launch_merge<T>(hipStream_t, const T* const*, int, int, int, T*);                   \
     template void launch_residual_predict<T>(hipStream_t, int, const T*, const T*, int, int, int, T*, \

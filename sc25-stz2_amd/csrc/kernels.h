@@ -167,6 +167,26 @@ template <typename T>
 void launch_pad(hipStream_t s, const T* in, int nx, int ny, int nz, T* out, int px, int py, int pz);
 template <typename T>
 void launch_gather_low0(hipStream_t s, const T* full, int px, int py, int pz, T* v);
+
+// The first steps of a batch compress for many fields in one launch each:
+// padding to multiples of 4, the value range (partials, then one workgroup
+// per field writing min and max to mm_out + 2 j), the base-grid gather.
+// Job j's workgroups of each kind are [kind_wg0, kind_wg0 + kind_nwg), the
+// ranges back to back in job order (a kind's total is the last job's
+// wg0 + nwg).
+struct ProJob {
+    const void* in;       // the caller's field (nx x ny x nz)
+    void* padded;         // its padded copy (px x py x pz), or null
+    const void* full;     // the padded field (padded, or in)
+    void* vbase;          // base grid out: full[4z][4y][4x]
+    double* partial;      // 2 x mm_nwg value-range partials
+    int nx, ny, nz, px, py, pz;
+    uint64_t nin;
+    uint32_t pad_wg0, pad_nwg, mm_wg0, mm_nwg, ga_wg0, ga_nwg;
+};
+template <typename T>
+void launch_prologue_multi(hipStream_t s, const ProJob* jobs, uint32_t n, uint32_t pad_wgs, uint32_t mm_wgs,
+                           uint32_t ga_wgs, double* mm_out);
 template <typename T>
 void launch_split(hipStream_t s, const T* full, int nx, int ny, int nz, T* const* sub);
 template <typename T>

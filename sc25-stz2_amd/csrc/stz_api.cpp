@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <chrono>
+#include <sched.h>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -340,8 +341,11 @@ struct stz_ctx {
     DBuf tuner_w;
     bool have_tuner = false;
     float tune_min_conf = 0.f;
-    // host workers (created on first use)
-    std::unique_ptr<TaskPool> workers;
+    // host workers (created on first use): three for a call's few streams
+    // (more only cost wake-ups there), and a wide pool for a batch's many
+    // (96 fields: 1440 codebooks, 11-15 ms on four threads), sized to the
+    // CPUs this process may run on, at most 16
+    std::unique_ptr<TaskPool> workers, workers_wide;
     TaskPool& tasks()
     {
         if (!workers) {
@@ -349,6 +353,17 @@ struct stz_ctx {
             workers.reset(new TaskPool((int)std::min<unsigned>(hc > 1 ? hc - 1 : 0, 3)));
         }
         return *workers;
+    }
+    TaskPool& tasks_wide()
+    {
+        if (!workers_wide) {
+            cpu_set_t set;
+            int cpus = 0;
+            if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = CPU_COUNT(&set);
+            if (cpus < 1) cpus = (int)std::thread::hardware_concurrency();
+            workers_wide.reset(new TaskPool(std::max(0, std::min(cpus, 16) - 1)));
+        }
+        return *workers_wide;
     }
 
     // Pinned staging for async copies: each request gets a region of its own
@@ -876,6 +891,97 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     K(c, "gather_low0", [&] {
         launch_gather_low0<T>(st, fe.full, (int)fe.px, (int)fe.py, (int)fe.pz, fe.vbase.template as<T>());
     });
+}
+
+template <typename V>
+DBuf upload_vec(stz_ctx* c, const std::vector<V>& v);
+
+// enc_phase0 for many fields: the checks, input copies and buffers per
+// field, then the padding, value ranges, histogram clear and base-grid
+// gathers of all of them in one launch each (a batch of 96 fields otherwise
+// issues some 700 small launches and copies here).  hist_all receives every
+// field's histograms (15 x 65536 bins each, views into it) and must outlive
+// their use.
+template <typename T>
+void enc_phase0_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, const void* const* fields, DBuf& hist_all,
+                      DBuf& mm_all)
+{
+    hipStream_t st = c->stream;
+    auto pool = c->pool;
+    const int n = (int)fe.size();
+    const size_t hist_words = (size_t)kStreams * 65536;
+    hist_all = DBuf(pool, (size_t)n * hist_words * 4);
+    constexpr uint32_t kMaxWgs = 256;  // per field and kind
+    std::vector<ProJob> jobs(n);
+    uint32_t pad_wgs = 0, mm_wgs = 0, ga_wgs = 0;
+    for (int i = 0; i < n; ++i) {
+        FieldEnc<T>& f = fe[i];
+        const stz_params& prm = f.prm;
+        check_dims(f.nx, f.ny, f.nz, prm.chunk_high, prm.chunk_low);
+        if (!(f.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 2 || prm.order < 0 ||
+            prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
+            !pred_valid((uint64_t)prm.pred))
+            throw StzError(STZ_E_ARG, "bad parameters");
+        f.px = pad4(f.nx);
+        f.py = pad4(f.ny);
+        f.pz = pad4(f.nz);
+        f.nin = (uint64_t)f.nx * f.ny * f.nz;
+        const T* field = (const T*)fields[i];
+        f.din = field;
+        const bool padded = f.px != f.nx || f.py != f.ny || f.pz != f.nz;
+        if (!is_device_ptr(field)) {
+            f.in_copy = DBuf(pool, f.nin * sizeof(T));
+            HIPCHK(hipMemcpyAsync(f.in_copy.p, field, f.nin * sizeof(T), hipMemcpyHostToDevice, st));
+            f.din = f.in_copy.template as<T>();
+        } else if (((uintptr_t)field % (2 * sizeof(T))) != 0 && !padded) {
+            f.in_copy = DBuf(pool, f.nin * sizeof(T));
+            HIPCHK(hipMemcpyAsync(f.in_copy.p, field, f.nin * sizeof(T), hipMemcpyDeviceToDevice, st));
+            f.din = f.in_copy.template as<T>();
+        }
+        f.full = f.din;
+        if (padded) {
+            f.padded = DBuf(pool, f.px * f.py * f.pz * sizeof(T));
+            f.full = f.padded.template as<T>();
+        }
+        make_layout(f.L, f.nx, f.ny, f.nz, f.px, f.py, f.pz, prm.chunk_high, prm.chunk_low);
+        const HostLayout& L = f.L;
+        const size_t lb = (f.px / 4) * (f.py / 4) * (f.pz / 4);
+        f.codes = DBuf(pool, L.total_codes * 2);
+        f.side = DBuf(pool, L.total_codes * sizeof(T));
+        f.hist.p = hist_all.as<uint32_t>() + (size_t)i * hist_words;  // a view (no pool: not freed)
+        f.vbase = DBuf(pool, lb * sizeof(T));
+        f.mid = DBuf(pool, (f.px / 2) * (f.py / 2) * (f.pz / 2) * sizeof(T));
+        f.mm_part = DBuf(pool, 2 * kMaxWgs * sizeof(double));
+        ProJob& j = jobs[i];
+        j.in = f.din;
+        j.padded = padded ? f.padded.p : nullptr;
+        j.full = f.full;
+        j.vbase = f.vbase.p;
+        j.partial = f.mm_part.template as<double>();
+        j.nx = (int)f.nx; j.ny = (int)f.ny; j.nz = (int)f.nz;
+        j.px = (int)f.px; j.py = (int)f.py; j.pz = (int)f.pz;
+        j.nin = f.nin;
+        const uint64_t rows = (uint64_t)f.py * f.pz;
+        j.pad_wg0 = pad_wgs;
+        j.pad_nwg = padded ? (uint32_t)std::min<uint64_t>((rows + 7) / 8, kMaxWgs) : 0u;
+        pad_wgs += j.pad_nwg;
+        j.mm_wg0 = mm_wgs;
+        j.mm_nwg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(f.nin / (256 * 64), 1), kMaxWgs);
+        mm_wgs += j.mm_nwg;
+        j.ga_wg0 = ga_wgs;
+        j.ga_nwg = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((lb + 2047) / 2048, 1), kMaxWgs);
+        ga_wgs += j.ga_nwg;
+    }
+    DBuf dj = upload_vec(c, jobs);
+    mm_all = DBuf(pool, (size_t)n * 2 * sizeof(double) + 64);
+    K(c, "hist_zero", [&] { launch_fill_u32(st, hist_all.as<uint32_t>(), (uint64_t)n * hist_words, 0u); });
+    K(c, "minmax", [&] {
+        launch_prologue_multi<T>(st, dj.as<ProJob>(), (uint32_t)n, pad_wgs, mm_wgs, ga_wgs, mm_all.as<double>());
+    });
+    double* mm_host = (double*)c->host_stage((size_t)n * 2 * sizeof(double));
+    HIPCHK(hipMemcpyAsync(mm_host, mm_all.p, (size_t)n * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev_range, st));
+    for (int i = 0; i < n; ++i) fe[i].mm_host = mm_host + 2 * i;
 }
 
 // after the wait: level bounds from the range
@@ -1870,6 +1976,7 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
 {
     hipStream_t st = c->stream;
     auto pool = c->pool;
+    DBuf hist_all, mm_all;  // a batch's histograms and value ranges (outlive fe's views)
     std::vector<FieldEnc<T>> fe(n);
     for (int i = 0; i < n; ++i) {
         if (!fields[i]) throw StzError(STZ_E_ARG, "null field");
@@ -1882,8 +1989,9 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         f.ny = dims[3 * i + 1];
         f.nz = dims[3 * i + 2];
         f.ds = out[i];
-        enc_phase0(c, f, (const T*)fields[i]);
     }
+    if (n == 1) enc_phase0(c, fe[0], (const T*)fields[0]);
+    else enc_phase0_batch(c, fe, fields, hist_all, mm_all);
     // every field's range is on the host once the last one is (stream order);
     // the gathers queued after them keep the GPU busy meanwhile
     wait_event(c->ev_range);
@@ -2026,23 +2134,17 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
     // after the copies have landed: symbol lists, then codebooks on the host workers
     auto books = [&](const uint8_t* hs, int s0, int ns) {
         const uint64_t* hf = (const uint64_t*)(hs + (size_t)n * 256);
-        for (int i = 0; i < n; ++i) {
+        (n * ns >= 64 ? c->tasks_wide() : c->tasks()).run(n * ns, [&](int k) {
+            const int i = k / ns, s2 = s0 + k % ns;
             FieldEnc<T>& f = fe[i];
-            const uint32_t* hc = (const uint32_t*)(hs + (size_t)i * 256);
-            for (int s2 = s0; s2 < s0 + ns; ++s2) {
-                const uint32_t m = hc[s2];
-                f.lists[s2].resize(m);
-                const uint64_t* src = hf + ((size_t)i * kStreams + s2) * kFirst;
-                if (m <= kFirst) std::memcpy(f.lists[s2].data(), src, (size_t)m * 8);
-                else
-                    HIPCHK(hipMemcpy(f.lists[s2].data(), compact.as<uint64_t>() + ((size_t)i * kStreams + s2) * 65536,
-                                     (size_t)m * 8, hipMemcpyDeviceToHost));
-            }
-        }
-        c->tasks().run(n * ns, [&](int k) {
-            FieldEnc<T>& f = fe[k / ns];
-            const int s2 = s0 + k % ns;
-            build_codebook(f.lists[s2].data(), (uint32_t)f.lists[s2].size(), kLmax, f.ds->meta.cb[s2]);
+            const uint32_t m = ((const uint32_t*)(hs + (size_t)i * 256))[s2];
+            f.lists[s2].resize(m);
+            const uint64_t* src = hf + ((size_t)i * kStreams + s2) * kFirst;
+            if (m <= kFirst) std::memcpy(f.lists[s2].data(), src, (size_t)m * 8);
+            else
+                HIPCHK(hipMemcpy(f.lists[s2].data(), compact.as<uint64_t>() + ((size_t)i * kStreams + s2) * 65536,
+                                 (size_t)m * 8, hipMemcpyDeviceToHost));
+            build_codebook(f.lists[s2].data(), m, kLmax, f.ds->meta.cb[s2]);
         });
     };
     // on an error after the low readback forked to the second stream,
