@@ -581,12 +581,8 @@ __global__ __launch_bounds__(64) void k_dec_prep(StreamSet ss, const uint8_t* __
 // -- codes of length l are first_code[l] .. first_code[l] + count[l] - 1 --
 // which names the unique code that prefixes the window.  Up to three codes
 // are chained while they fit in the window; symbol 0 only ever stands alone.
-__global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict__ heads,
-                                                       const uint16_t* __restrict__ sorted, DecTab* __restrict__ tabs)
+__device__ __forceinline__ void build_dectab(const DecHead& h, const uint16_t* __restrict__ sorted, DecTab& t)
 {
-    const int s = blockIdx.y;
-    const DecHead& h = heads[s];
-    DecTab& t = tabs[s];
     if (blockIdx.x == 0 && threadIdx.x < kLmax + 2) {
         t.first_code[threadIdx.x] = h.first_code[threadIdx.x];
         t.count[threadIdx.x] = h.count[threadIdx.x];
@@ -625,6 +621,19 @@ __global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict
         used += lk;
     }
     t.lut[w] = e | (uint64_t)cnt << kDecNShift | (uint64_t)l1 << kDecL1Shift | (uint64_t)used << kDecLtShift;
+}
+
+__global__ __launch_bounds__(256) void k_build_dectabs(const DecHead* __restrict__ heads,
+                                                       const uint16_t* __restrict__ sorted, DecTab* __restrict__ tabs)
+{
+    build_dectab(heads[blockIdx.y], sorted, tabs[blockIdx.y]);
+}
+
+// many stream sets: set blockIdx.z, stream blockIdx.y
+__global__ __launch_bounds__(256) void k_build_dectabs_multi(const DecTabJob* __restrict__ jobs)
+{
+    const DecTabJob& J = jobs[blockIdx.z];
+    build_dectab(J.heads[blockIdx.y], J.sorted, J.tabs[blockIdx.y]);
 }
 
 // Symbol at canonical index idx of a code longer than the table window:
@@ -1352,6 +1361,12 @@ void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* s
 {
     if (nstreams > 0)
         k_build_dectabs<<<dim3((1u << kLutBits) / 256u, (unsigned)nstreams), dim3(256), 0, s>>>(heads, sorted, tabs);
+}
+
+void launch_build_dectabs_multi(hipStream_t s, const DecTabJob* jobs, uint32_t njobs, int nstreams)
+{
+    if (njobs && nstreams > 0)
+        k_build_dectabs_multi<<<dim3((1u << kLutBits) / 256u, (unsigned)nstreams, njobs), dim3(256), 0, s>>>(jobs);
 }
 
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload, const DecTab* tabs,

@@ -156,6 +156,13 @@ void launch_dec_prep(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                      const uint32_t* list, uint32_t nlist, DecItem* items, uint32_t* cmap);
 // the decode tables of nstreams streams from their canonical descriptions
 void launch_build_dectabs(hipStream_t s, const DecHead* heads, const uint16_t* sorted, DecTab* tabs, int nstreams);
+// the tables of many stream sets in one launch: job j's nstreams streams
+struct DecTabJob {
+    const DecHead* heads;
+    const uint16_t* sorted;
+    DecTab* tabs;
+};
+void launch_build_dectabs_multi(hipStream_t s, const DecTabJob* jobs, uint32_t njobs, int nstreams);
 void launch_huff_decode(hipStream_t s, const StreamSet& ss, const uint8_t* payload,
                         const DecTab* tabs, const uint16_t* sorted, const DecItem* items,
                         uint16_t* codes, void* side, uint32_t* err, int form);

@@ -1104,6 +1104,65 @@ void ensure_tabs(stz_ctx* c, stz_dstream* ds)
     ds->have_tabs = true;
 }
 
+// ensure_tabs for many streams at once: their canonical descriptions on the
+// wide host pool, one upload per stream object, one table launch for all
+void ensure_tabs_batch(stz_ctx* c, stz_dstream* const* dss, int n)
+{
+    std::vector<stz_dstream*> todo;
+    for (int i = 0; i < n; ++i)
+        if (!dss[i]->have_tabs) todo.push_back(dss[i]);
+    const int m = (int)todo.size();
+    if (m < 2) {
+        for (stz_dstream* d : todo) ensure_tabs(c, d);
+        return;
+    }
+    std::vector<DecHead> heads((size_t)m * kStreams);
+    std::vector<std::vector<uint16_t>> sorted((size_t)m * kStreams);
+    c->tasks_wide().run(m * kStreams, [&](int k) {
+        const Meta& me = todo[k / kStreams]->meta;
+        const int s = k % kStreams;
+        build_dechead(me.cb[s].sym.data(), me.cb[s].len.data(), (uint32_t)me.cb[s].sym.size(), heads[k], sorted[k]);
+    });
+    std::vector<size_t> sb(m), off(m);
+    size_t total = 0;
+    for (int i = 0; i < m; ++i) {
+        size_t ns = 0;
+        for (int s = 0; s < kStreams; ++s) {
+            heads[(size_t)i * kStreams + s].sorted_off = (uint32_t)ns;
+            ns += sorted[(size_t)i * kStreams + s].size();
+        }
+        sb[i] = align_up(ns * 2, 16);
+        off[i] = total;
+        total += align_up(sb[i] + sizeof(DecHead) * kStreams, 16);
+    }
+    const size_t jobs_off = total;
+    uint8_t* stage = (uint8_t*)c->host_stage(total + (size_t)m * sizeof(DecTabJob) + 64);
+    std::vector<DecTabJob> jobs(m);
+    for (int i = 0; i < m; ++i) {
+        stz_dstream* d = todo[i];
+        const size_t hb = sizeof(DecHead) * kStreams;
+        d->tabs = DBuf(d->pool, sizeof(DecTab) * kStreams);
+        d->sorted = DBuf(d->pool, sb[i] + hb + 16);
+        uint8_t* b = stage + off[i];
+        size_t o = 0;
+        for (int s = 0; s < kStreams; ++s) {
+            const std::vector<uint16_t>& v = sorted[(size_t)i * kStreams + s];
+            std::memcpy(b + o, v.data(), v.size() * 2);
+            o += v.size() * 2;
+        }
+        std::memcpy(b + sb[i], &heads[(size_t)i * kStreams], hb);
+        HIPCHK(hipMemcpyAsync(d->sorted.p, b, sb[i] + hb, hipMemcpyHostToDevice, c->stream));
+        jobs[i].heads = (const DecHead*)(d->sorted.as<uint8_t>() + sb[i]);
+        jobs[i].sorted = d->sorted.as<uint16_t>();
+        jobs[i].tabs = d->tabs.as<DecTab>();
+        d->have_tabs = true;
+    }
+    std::memcpy(stage + jobs_off, jobs.data(), (size_t)m * sizeof(DecTabJob));
+    DBuf dj(c->pool, (size_t)m * sizeof(DecTabJob));
+    HIPCHK(hipMemcpyAsync(dj.p, stage + jobs_off, (size_t)m * sizeof(DecTabJob), hipMemcpyHostToDevice, c->stream));
+    K(c, "dectabs", [&] { launch_build_dectabs_multi(c->stream, dj.as<DecTabJob>(), (uint32_t)m, kStreams); });
+}
+
 void layout_of(const Meta& m, HostLayout& L)
 {
     make_layout(L, m.fh.nx, m.fh.ny, m.fh.nz, m.fh.px, m.fh.py, m.fh.pz, (int)m.fh.chunk_high, (int)m.fh.chunk_low);
@@ -2649,10 +2708,10 @@ void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* cons
         P->outs.assign(outs, outs + n);
         P->f.resize(n);
         P->err = DBuf(c->pool, (size_t)n * 4 + 64);
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n; ++i)
             if (!is_device_ptr(outs[i])) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
-            plan_huff_setup(c, ds[i], P->f[i]);
-        }
+        ensure_tabs_batch(c, ds, n);
+        for (int i = 0; i < n; ++i) plan_huff_setup(c, ds[i], P->f[i]);
         for (int i = 0; i < n; ++i) {
             BatchPlan::Field& F = P->f[i];
             const Meta& m = ds[i]->meta;
