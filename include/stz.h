@@ -55,7 +55,8 @@ typedef enum { STZ_EB_ABS = 0, STZ_EB_REL = 1, STZ_EB_REF = 2 } stz_eb_mode;
 typedef struct {
     double mult_base, mult_low, mult_high; /* F5:2127-2129: 0.7 / 2.7 / 8.2 */
     int chunk_high, chunk_low;             /* F5:34-35 marker chunks: 32 / 16 */
-    int interp;                            /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo */
+    int interp;                            /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo,
+                                              3 Lorenzo or linear regression per 8^3 sub-block */
     int order;                             /* base codec axis order 0..5 */
     double alpha, beta;                    /* base codec level eb: eb/min(alpha^(l-1),beta) */
     int zstd_level;                        /* host zstd level (F5:3048: SZ3 default 3) */
@@ -219,12 +220,13 @@ int stz_autotune(stz_ctx* ctx, int enable, char* report, size_t cap);
  * Chooses the residual predictor set (F5's; F5's PRED_PLANE_AXIS +
  * PRED_CUBE_AXIS; stz2.cpp's BICUBIC12 + DIAG16; dependency.cpp's staged
  * inverse-variance predictors, STZ_PRED_STAGED), the base codec
- * (interpolator and axis order, or Lorenzo) and the level-eb scaling (the
- * tuning targets of F5:2120-2129 and SZ3's sample-and-trial between
- * interpolation and Lorenzo, F5:1439) among STZ_TUNE_CONFIGS configurations:
- * 4 predictor sets x (2 interpolators x 6 orders x 4 scalings + Lorenzo x 3
+ * (interpolator and axis order, or Lorenzo / regression, interp 3) and the
+ * level-eb scaling (the tuning targets of F5:2120-2129 and SZ3's
+ * sample-and-trial between interpolation and Lorenzo / regression, F5:1439,
+ * sz.cpp:63-76) among STZ_TUNE_CONFIGS configurations: 4 predictor sets x
+ * (2 interpolators x 6 orders x 4 scalings + Lorenzo / regression x 3
  * scalings), index 51 * set + j, j = 24 * interp + 4 * order + scaling
- * (j < 48) or 48 + scaling (Lorenzo).  The model has STZ_MLP_OUTPUTS outputs,
+ * (j < 48) or 48 + scaling (Lorenzo / regression).  The model has STZ_MLP_OUTPUTS outputs,
  * the first STZ_TUNE_CONFIGS of which are the configurations.
  * The choice lands in the stream header; decoding and the error bound are
  * unchanged.  params.tuner = 1 compresses with the model's choice (falling

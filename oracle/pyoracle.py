@@ -34,7 +34,7 @@ class Params(C.Structure):
 PRED_WPOW2, PRED_PLANE_AXIS, PRED_CUBE_AXIS, PRED_BICUBIC12, PRED_DIAG16, PRED_STAGED = 1, 2, 4, 8, 16, 32
 # base codec (Params.interp): interpolation linear / cubic, or Lorenzo over
 # independent LZ_TILE^3 tiles
-BASE_LINEAR, BASE_CUBIC, BASE_LORENZO, LZ_TILE = 0, 1, 2, 16
+BASE_LINEAR, BASE_CUBIC, BASE_LORENZO, BASE_LZREG, LZ_TILE, LZ_HDR = 0, 1, 2, 3, 16, 32
 
 
 def build():
@@ -66,8 +66,10 @@ def lib():
             getattr(L, "ora_quantize_" + t).argtypes = [vp, sz, dbl, i32, vp, vp, vp]
             getattr(L, "ora_dequantize_" + t).argtypes = [vp, sz, dbl, i32, vp, vp]
             getattr(L, "ora_interp_compress_" + t).restype = sz
-            getattr(L, "ora_interp_compress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
-            getattr(L, "ora_interp_decompress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, vp, vp]
+            getattr(L, "ora_interp_compress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, dbl, dbl,
+                                                               vp, vp]
+            getattr(L, "ora_interp_decompress_" + t).argtypes = [vp, i32, i32, i32, dbl, i32, i32, dbl, dbl, dbl, dbl,
+                                                                 vp, vp]
             getattr(L, "ora_staged_level_" + t).argtypes = [vp, i32, i32, i32, dbl, vp, vp, vp, vp, vp]
         L.ora_huff_lengths.argtypes = [vp, i32, i32, vp]
         L.ora_plan_query.argtypes = [i32] * 11 + [vp, vp, vp, vp, vp, C.c_int64]
@@ -247,23 +249,42 @@ def huff_lengths(freq, lmax=24):
     return out
 
 
-def interp_compress(v, eb, interp=1, order=0, alpha=1.0, beta=1.0):
+def base_symbols(shape, interp):
+    """Symbols of the base stream of a grid: its points, plus LZ_HDR header
+    symbols per 16^3 tile for the Lorenzo/regression base."""
+    n = int(np.prod(shape))
+    if interp == BASE_LZREG:
+        n += LZ_HDR * int(np.prod([(d + LZ_TILE - 1) // LZ_TILE for d in shape]))
+    return n
+
+
+def interp_compress(v, eb, interp=1, order=0, alpha=1.0, beta=1.0, lo=None, hi=None):
+    """The base codec on one grid: (reconstruction, codes, unpredictables);
+    lo/hi clamp the Lorenzo/regression base's plane predictions (default:
+    the grid's range)."""
     v = np.ascontiguousarray(v).copy()
     bz, by, bx = v.shape
-    codes = np.empty(v.size, dtype=np.uint16)
-    unpred = np.empty(v.size, dtype=v.dtype)
-    nu = getattr(lib(), "ora_interp_compress_" + _tsuf(v.dtype))(_ptr(v), bx, by, bz, float(eb), interp, order, alpha, beta, _ptr(codes), _ptr(unpred))
+    lo = float(v.min()) if lo is None else float(lo)
+    hi = float(v.max()) if hi is None else float(hi)
+    n = base_symbols(v.shape, interp)
+    codes = np.empty(n, dtype=np.uint16)
+    unpred = np.empty(n, dtype=v.dtype)
+    nu = getattr(lib(), "ora_interp_compress_" + _tsuf(v.dtype))(_ptr(v), bx, by, bz, float(eb), interp, order, alpha,
+                                                                 beta, lo, hi, _ptr(codes), _ptr(unpred))
     return v, codes, unpred[:nu].copy()
 
 
-def interp_decompress(shape, dtype, eb, codes, unpred, interp=1, order=0, alpha=1.0, beta=1.0):
-    """The base codec's decode: the reconstruction from codes and unpredictables."""
+def interp_decompress(shape, dtype, eb, codes, unpred, interp=1, order=0, alpha=1.0, beta=1.0, lo=0.0, hi=0.0):
+    """The base codec's decode: the reconstruction from codes and
+    unpredictables (lo/hi: the clamp range the encode used)."""
     bz, by, bx = shape
     v = np.zeros(shape, dtype=dtype)
     codes = np.ascontiguousarray(codes, np.uint16)
+    assert codes.size == base_symbols(shape, interp)
     unpred = np.ascontiguousarray(unpred, dtype)
     getattr(lib(), "ora_interp_decompress_" + _tsuf(np.dtype(dtype)))(_ptr(v), bx, by, bz, float(eb), interp, order,
-                                                                        alpha, beta, _ptr(codes), _ptr(unpred))
+                                                                        alpha, beta, float(lo), float(hi), _ptr(codes),
+                                                                        _ptr(unpred))
     return v
 
 

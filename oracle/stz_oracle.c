@@ -70,11 +70,13 @@ static inline uint64_t ora_stream_pos(const ora_geom* g, int x, int y, int z)
     return start + ((uint64_t)(z - cz * C) * wy + (uint64_t)(y - cy * C)) * wx + (uint64_t)(x - cx * C);
 }
 
-static void geom_init(ora_geom* g, int dx, int dy, int dz, int chunk)
+/* extra: symbols the stream holds beyond its grid points (the header
+ * symbols of a Lorenzo/regression base) */
+static void geom_init(ora_geom* g, int dx, int dy, int dz, int chunk, uint64_t extra)
 {
     memset(g, 0, sizeof(*g));
     g->dx = dx; g->dy = dy; g->dz = dz; g->chunk = chunk;
-    g->n = (uint64_t)dx * dy * dz;
+    g->n = (uint64_t)dx * dy * dz + extra;
     if (chunk > 0) {
         g->ncx = (dx + chunk - 1) / chunk;
         g->ncy = (dy + chunk - 1) / chunk;
@@ -115,17 +117,27 @@ static void layout_free(ora_layout* L)
 
 static size_t pad4(size_t n) { size_t p = (n + 3) & ~(size_t)3; return p < 4 ? 4 : p; }
 
+/* header symbols of a Lorenzo/regression base grid: ORA_LZ_HDR per tile */
+static uint64_t ora_base_extra(int interp, int bx, int by, int bz)
+{
+    if (interp != ORA_BASE_LZREG) return 0;
+    const uint64_t t = (uint64_t)((bx + ORA_LZ_TILE - 1) / ORA_LZ_TILE) * ((by + ORA_LZ_TILE - 1) / ORA_LZ_TILE) *
+                       ((bz + ORA_LZ_TILE - 1) / ORA_LZ_TILE);
+    return t * ORA_LZ_HDR;
+}
+
 static int ora_layout_dims(ora_layout* L, size_t nx, size_t ny, size_t nz, size_t px, size_t py,
-                           size_t pz, int chunk_high, int chunk_low)
+                           size_t pz, int chunk_high, int chunk_low, int interp)
 {
     memset(L, 0, sizeof(*L));
     L->nx = nx; L->ny = ny; L->nz = nz;
     L->px = px; L->py = py; L->pz = pz;
     L->chunk_high = chunk_high; L->chunk_low = chunk_low;
-    geom_init(&L->geom[0], (int)(px / 4), (int)(py / 4), (int)(pz / 4), 0);
+    const int lx = (int)(px / 4), ly = (int)(py / 4), lz = (int)(pz / 4);
+    geom_init(&L->geom[0], lx, ly, lz, 0, ora_base_extra(interp, lx, ly, lz));
     for (int b = 0; b < 7; ++b) {
-        geom_init(&L->geom[1 + b], (int)(px / 4), (int)(py / 4), (int)(pz / 4), chunk_low);
-        geom_init(&L->geom[8 + b], (int)(px / 2), (int)(py / 2), (int)(pz / 2), chunk_high);
+        geom_init(&L->geom[1 + b], lx, ly, lz, chunk_low, 0);
+        geom_init(&L->geom[8 + b], (int)(px / 2), (int)(py / 2), (int)(pz / 2), chunk_high, 0);
     }
     return 0;
 }
@@ -134,7 +146,7 @@ static int ora_make_layout(ora_layout* L, size_t nx, size_t ny, size_t nz, const
 {
     if (nx == 0 || ny == 0 || nz == 0) return -1;
     if (p->chunk_high <= 0 || p->chunk_low <= 0) return -1;
-    return ora_layout_dims(L, nx, ny, nz, pad4(nx), pad4(ny), pad4(nz), p->chunk_high, p->chunk_low);
+    return ora_layout_dims(L, nx, ny, nz, pad4(nx), pad4(ny), pad4(nz), p->chunk_high, p->chunk_low, p->interp);
 }
 
 /* axis orders of the base codec: axes listed first-to-last inside a level (0=x,1=y,2=z) */
@@ -179,7 +191,10 @@ void ora_default_params(ora_params* p)
  * difference the quantizer checks add at most 3.25 u (M + E), u the unit
  * roundoff of T and M = max |value|, so the levels share E - 4 u (M + E)
  * (0 when the bound is below that), and the level unit is that over
- * max(mult).  REF keeps F5's CLI meaning (argv[1] times the level
+ * max(mult).  A Lorenzo prediction sums seven reconstructed values
+ * (|prediction| <= 7 (M + E)), so the Lorenzo bases (interp 2 and 3, whose
+ * regression predictions are clamped to the value range) take
+ * E - 10 u (M + E).  REF keeps F5's CLI meaning (argv[1] times the level
  * multiplier, F5:2139, F5:2201, F5:2259). */
 static void ora_level_ebs(double eb, int mode, double vmin, double vmax, int dtype, const ora_params* p,
                           double ebl[3], double* unit)
@@ -190,7 +205,7 @@ static void ora_level_ebs(double eb, int mode, double vmin, double vmax, int dty
         const double E = (mode == ORA_EB_REL) ? eb * (vmax - vmin) : eb;
         const double M = fabs(vmin) > fabs(vmax) ? fabs(vmin) : fabs(vmax);
         const double u = dtype == ORA_F32 ? 0x1p-24 : 0x1p-53;
-        double Es = E - 4.0 * u * (M + E);
+        double Es = E - (p->interp >= ORA_BASE_LORENZO ? 10.0 : 4.0) * u * (M + E);
         if (!(Es > 0)) Es = 0;
         double mm = p->mult_base;
         if (p->mult_low > mm) mm = p->mult_low;
@@ -514,7 +529,7 @@ static int ora_open_view(ora_view* v, const uint8_t* p, size_t bytes)
     const stzf_file_header* fh = (const stzf_file_header*)p;
     if (fh->magic != STZF_MAGIC || fh->version != STZF_VERSION || fh->n_streams != STZF_NSTREAMS ||
         fh->pred > ORA_PRED_ALL || ((fh->pred & ORA_PRED_STAGED) && fh->pred != ORA_PRED_STAGED) ||
-        fh->interp > ORA_BASE_LORENZO || fh->order > 5 || !(fh->alpha >= 1.0) ||
+        fh->interp > ORA_BASE_LZREG || fh->order > 5 || !(fh->alpha >= 1.0) ||
         !(fh->beta >= 1.0))
         return -1;
     if (fh->total_bytes > bytes) return -1;
@@ -529,7 +544,7 @@ static int ora_open_view(ora_view* v, const uint8_t* p, size_t bytes)
         if (v->sh[s]->magic != STZF_STREAM_MAGIC) return -1;
     }
     ora_layout_dims(&v->lay, fh->nx, fh->ny, fh->nz, fh->px, fh->py, fh->pz, (int)fh->chunk_high,
-                    (int)fh->chunk_low);
+                    (int)fh->chunk_low, (int)fh->interp);
     for (int s = 0; s < STZF_NSTREAMS; ++s)
         if (v->sh[s]->n_symbols != v->lay.geom[s].n || v->sh[s]->n_chunks != v->lay.geom[s].n_chunks) {
             layout_free(&v->lay);
@@ -784,7 +799,7 @@ int ora_plan_query(int fx, int fy, int fz, int chunk_high, int chunk_low, int ax
     ora_layout L;
     if (fx <= 0 || fy <= 0 || fz <= 0 || chunk_high <= 0 || chunk_low <= 0) return -1;
     ora_layout_dims(&L, (size_t)fx, (size_t)fy, (size_t)fz, pad4((size_t)fx), pad4((size_t)fy),
-                    pad4((size_t)fz), chunk_high, chunk_low);
+                    pad4((size_t)fz), chunk_high, chunk_low, ORA_BASE_CUBIC);
     ora_plan pl;
     ora_make_plan(&pl, &L, ax, ay, az, bx, by, bz);
     query[0] = pl.q.x0; query[1] = pl.q.x1; query[2] = pl.q.y0; query[3] = pl.q.y1;
@@ -837,7 +852,7 @@ long long ora_compress(int dtype, const void* field, size_t nx, size_t ny, size_
     ora_params d;
     if (!p) { ora_default_params(&d); p = &d; }
     if (!field || !payload || eb < 0 || p->alpha < 1.0 || p->beta < 1.0 || p->interp < 0 ||
-        p->interp > ORA_BASE_LORENZO || p->order < 0 || p->order > 5 || p->pred < 0 || p->pred > ORA_PRED_ALL ||
+        p->interp > ORA_BASE_LZREG || p->order < 0 || p->order > 5 || p->pred < 0 || p->pred > ORA_PRED_ALL ||
         ((p->pred & ORA_PRED_STAGED) && p->pred != ORA_PRED_STAGED))
         return -1;
     if (dtype == ORA_F32) return ora_compress_t_f32((const float*)field, nx, ny, nz, eb, eb_mode, p, payload);

@@ -27,14 +27,16 @@ extern "C" {
 
 enum { ORA_F32 = 0, ORA_F64 = 1 };
 /* base codec (ora_params.interp): interpolation 0 linear / 1 cubic; 2 Lorenzo
- * over independent ORA_LZ_TILE^3 tiles */
-enum { ORA_BASE_LINEAR = 0, ORA_BASE_CUBIC = 1, ORA_BASE_LORENZO = 2, ORA_LZ_TILE = 16 };
+ * over independent ORA_LZ_TILE^3 tiles; 3 Lorenzo or linear regression per
+ * 8^3 sub-block of those tiles, ORA_LZ_HDR header symbols per tile */
+enum { ORA_BASE_LINEAR = 0, ORA_BASE_CUBIC = 1, ORA_BASE_LORENZO = 2, ORA_BASE_LZREG = 3, ORA_LZ_TILE = 16,
+       ORA_LZ_HDR = 32 };
 enum { ORA_EB_ABS = 0, ORA_EB_REL = 1, ORA_EB_REF = 2 };
 
 typedef struct {
     double mult_base, mult_low, mult_high;   /* F5:2127-2129 defaults 0.7/2.7/8.2 */
     int chunk_high, chunk_low;               /* F5:34-35 defaults 32/16 */
-    int interp;                              /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo */
+    int interp;                              /* base codec: 0 linear, 1 cubic interpolation, 2 Lorenzo, 3 Lorenzo/regression */
     int order;                               /* base codec axis order 0..5 */
     double alpha, beta;                      /* base codec level eb scaling */
     int pred;                                /* ORA_PRED_* bits: F5's compile-time predictor options */
@@ -105,18 +107,21 @@ void ora_dequantize_f64(const uint16_t* codes, size_t n, double eb, int radius,
                         const double* unpred, double* out);
 /* canonical, length-limited Huffman code lengths for a 65536-symbol histogram */
 int ora_huff_lengths(const uint64_t* freq, int nsym, int lmax, uint8_t* len);
-/* base codec on one grid (in-place reconstruction), codes in stream order */
+/* base codec on one grid (in-place reconstruction), codes in stream order;
+ * [lo, hi] clamps the regression predictions of interp 3 (the field's range) */
 size_t ora_interp_compress_f32(float* v, int bx, int by, int bz, double eb, int interp,
-                               int order, double alpha, double beta, uint16_t* codes,
-                               float* unpred);
+                               int order, double alpha, double beta, double lo, double hi,
+                               uint16_t* codes, float* unpred);
 size_t ora_interp_compress_f64(double* v, int bx, int by, int bz, double eb, int interp,
-                               int order, double alpha, double beta, uint16_t* codes,
-                               double* unpred);
+                               int order, double alpha, double beta, double lo, double hi,
+                               uint16_t* codes, double* unpred);
 /* its decode: v (zero-filled by the caller) from codes and the unpredictables */
 void ora_interp_decompress_f32(float* v, int bx, int by, int bz, double eb, int interp, int order,
-                               double alpha, double beta, const uint16_t* codes, const float* unpred);
+                               double alpha, double beta, double lo, double hi, const uint16_t* codes,
+                               const float* unpred);
 void ora_interp_decompress_f64(double* v, int bx, int by, int bz, double eb, int interp, int order,
-                               double alpha, double beta, const uint16_t* codes, const double* unpred);
+                               double alpha, double beta, double lo, double hi, const uint16_t* codes,
+                               const double* unpred);
 
 /* one residual level with dependency.cpp's staged predictors: ref the
  * coarser grid (dx, dy, dz), g its 2x grid (filled: ref at parity 0, each

@@ -329,21 +329,32 @@ uint32_t level_tiles(const Geom& g, int waves);
 uint32_t level_edge_groups(const Geom& g, int waves);
 
 // ---- k_lorenzo.hip ----
-// Lorenzo base codec (interp == 2): one workgroup per kLorenzoTile^3 tile of
-// the base grid.  Encode reconstructs v in place and writes codes, the raw
-// values of unpredictable points (side, stream positions) and the base
-// stream's histogram; decode writes v from codes and side.
+// Lorenzo base codec (interp == 2) and Lorenzo / regression base (interp ==
+// 3): one workgroup per kLorenzoTile^3 tile of the base grid.  Encode
+// reconstructs v in place and writes codes, the raw values of unpredictable
+// points (side, stream positions) and the base stream's histogram; decode
+// writes v from codes and side.  With reg set every tile's stream starts
+// with kLorenzoHdr header symbols (four per 8^3 sub-block: its plane's
+// coefficients as unpredictables, or four zero residuals for Lorenzo).
 constexpr int kLorenzoTile = 16;
+constexpr int kLorenzoHdr = 32;
 struct LorenzoJob {
     void* v;
     uint16_t* codes;
     void* side;
     uint32_t* hist;   // encode only
     int bx, by, bz;
+    int reg;          // 1: Lorenzo / regression per sub-block (interp 3)
     double eb, recip;
+    double lo, hi;    // regression predictions are clamped to [lo, hi] (the field's range)
     uint32_t wg0;     // multi-field launch: the field's first workgroup
 };
 uint32_t lorenzo_tiles(int bx, int by, int bz);
+// base stream symbols beyond the grid's points: the tiles' header symbols
+inline uint64_t base_extra_symbols(int interp, int bx, int by, int bz)
+{
+    return interp == 3 ? (uint64_t)kLorenzoHdr * lorenzo_tiles(bx, by, bz) : 0;
+}
 template <typename T>
 void launch_lorenzo(hipStream_t s, const LorenzoJob& j, bool encode);
 template <typename T>

@@ -536,7 +536,10 @@ struct HostLayout {
     uint64_t total_codes;
 };
 
-void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
+// interp: the base codec (a Lorenzo/regression base, 3, adds its tiles'
+// header symbols to the base stream)
+void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl,
+                  int interp)
 {
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.px = px; L.py = py; L.pz = pz;
@@ -549,7 +552,7 @@ void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, siz
         else if (s < 8) L.g[s] = make_geom(lx, ly, lz, cl);
         else L.g[s] = make_geom(mx, my, mz, ch);
         const Geom& g = L.g[s];
-        L.n[s] = (uint64_t)g.dx * g.dy * g.dz;
+        L.n[s] = (uint64_t)g.dx * g.dy * g.dz + (s == 0 ? base_extra_symbols(interp, g.dx, g.dy, g.dz) : 0);
         L.n_sub[s] = chunk_subs(g, L.n[s], L.chunk_sub[s]);
         L.code_off[s] = off;
         off += align_up(L.n[s], 64);
@@ -567,12 +570,14 @@ inline size_t staged_scratch(const HostLayout& L, size_t elem)
 // Layouts by shape: building one walks every chunk of every stream (tens of
 // microseconds at 512^3), and fields of one shape come back again and again
 // (steps, serving, batches), so the process keeps the recent ones.
-void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl)
+void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size_t py, size_t pz, int ch, int cl,
+                 int interp)
 {
-    typedef std::array<uint64_t, 8> Key;
+    typedef std::array<uint64_t, 9> Key;
     static std::mutex mu;
     static std::map<Key, std::shared_ptr<const HostLayout>> cache;
-    const Key k = {nx, ny, nz, px, py, pz, (uint64_t)(uint32_t)ch, (uint64_t)(uint32_t)cl};
+    const uint64_t ib = interp == 3 ? 1 : 0;  // the only base codec with a layout of its own
+    const Key k = {nx, ny, nz, px, py, pz, (uint64_t)(uint32_t)ch, (uint64_t)(uint32_t)cl, ib};
     std::shared_ptr<const HostLayout> hit;
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -584,7 +589,7 @@ void make_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, size
         return;
     }
     auto fresh = std::make_shared<HostLayout>();
-    build_layout(*fresh, nx, ny, nz, px, py, pz, ch, cl);
+    build_layout(*fresh, nx, ny, nz, px, py, pz, ch, cl, interp);
     L = *fresh;
     std::lock_guard<std::mutex> lk(mu);
     if (cache.size() >= 64) cache.clear();
@@ -608,7 +613,10 @@ void check_dims(size_t nx, size_t ny, size_t nz, int ch, int cl)
 // weighted means of reconstructed values with weights summing to 1 and
 // absolute sum <= 1.25).  The levels get E - 4 u (M + E), so the error is
 // strictly at most E; a bound below that slack leaves 0 (every value
-// stored).  REF keeps F5's meaning: level bounds are mult_* x eb.
+// stored).  A Lorenzo prediction sums seven reconstructed values
+// (|prediction| <= 7 (M + E)), so the Lorenzo bases (interp 2 and 3, whose
+// regression predictions are clamped to the value range) take
+// E - 10 u (M + E).  REF keeps F5's meaning: level bounds are mult_* x eb.
 void level_ebs(double eb, int mode, double vmin, double vmax, int dtype, const stz_params& p, double ebl[3],
                double& unit)
 {
@@ -618,7 +626,7 @@ void level_ebs(double eb, int mode, double vmin, double vmax, int dtype, const s
         const double E = (mode == STZ_EB_REL) ? eb * (vmax - vmin) : eb;
         const double M = std::max(std::fabs(vmin), std::fabs(vmax));
         const double u = dtype == STZ_F32 ? 0x1p-24 : 0x1p-53;
-        double Es = E - 4.0 * u * (M + E);
+        double Es = E - (p.interp >= 2 ? 10.0 : 4.0) * u * (M + E);
         if (!(Es > 0)) Es = 0;
         double mm = p.mult_base;
         if (p.mult_low > mm) mm = p.mult_low;
@@ -835,7 +843,7 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
 {
     const stz_params& prm = fe.prm;
     check_dims(fe.nx, fe.ny, fe.nz, prm.chunk_high, prm.chunk_low);
-    if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 2 || prm.order < 0 ||
+    if (!(fe.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 3 || prm.order < 0 ||
         prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
         !pred_valid((uint64_t)prm.pred))
         throw StzError(STZ_E_ARG, "bad parameters");
@@ -876,7 +884,7 @@ void enc_phase0(stz_ctx* c, FieldEnc<T>& fe, const T* field)
     // the work that does not depend on the range is queued before the
     // caller waits for it (on ev_range): scratch, histograms cleared, base
     // grid gathered
-    make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low);
+    make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, prm.chunk_high, prm.chunk_low, prm.interp);
     const HostLayout& L = fe.L;
     const int lx = (int)(fe.px / 4), ly = (int)(fe.py / 4), lz = (int)(fe.pz / 4);
     const int mx = (int)(fe.px / 2), my = (int)(fe.py / 2), mz = (int)(fe.pz / 2);
@@ -918,7 +926,7 @@ void enc_phase0_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, const void* cons
         FieldEnc<T>& f = fe[i];
         const stz_params& prm = f.prm;
         check_dims(f.nx, f.ny, f.nz, prm.chunk_high, prm.chunk_low);
-        if (!(f.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 2 || prm.order < 0 ||
+        if (!(f.eb >= 0) || prm.alpha < 1.0 || prm.beta < 1.0 || prm.interp < 0 || prm.interp > 3 || prm.order < 0 ||
             prm.order > 5 || !(prm.mult_base > 0) || !(prm.mult_low > 0) || !(prm.mult_high > 0) || prm.pred < 0 ||
             !pred_valid((uint64_t)prm.pred))
             throw StzError(STZ_E_ARG, "bad parameters");
@@ -943,7 +951,7 @@ void enc_phase0_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, const void* cons
             f.padded = DBuf(pool, f.px * f.py * f.pz * sizeof(T));
             f.full = f.padded.template as<T>();
         }
-        make_layout(f.L, f.nx, f.ny, f.nz, f.px, f.py, f.pz, prm.chunk_high, prm.chunk_low);
+        make_layout(f.L, f.nx, f.ny, f.nz, f.px, f.py, f.pz, prm.chunk_high, prm.chunk_low, prm.interp);
         const HostLayout& L = f.L;
         const size_t lb = (f.px / 4) * (f.py / 4) * (f.pz / 4);
         f.codes = DBuf(pool, L.total_codes * 2);
@@ -1165,7 +1173,8 @@ void ensure_tabs_batch(stz_ctx* c, stz_dstream* const* dss, int n)
 
 void layout_of(const Meta& m, HostLayout& L)
 {
-    make_layout(L, m.fh.nx, m.fh.ny, m.fh.nz, m.fh.px, m.fh.py, m.fh.pz, (int)m.fh.chunk_high, (int)m.fh.chunk_low);
+    make_layout(L, m.fh.nx, m.fh.ny, m.fh.nz, m.fh.px, m.fh.py, m.fh.pz, (int)m.fh.chunk_high, (int)m.fh.chunk_low,
+                (int)m.fh.interp);
 }
 
 // Decoded symbols.  Full layout (every chunk of every stream): stream s
@@ -1389,9 +1398,12 @@ struct JoinGuard {
 template <typename T>
 void decode_base(stz_ctx* c, const Meta& m, const HostLayout& L, const Decoded& dd, T* v, int stop_level)
 {
-    if (m.fh.interp == 2) {
+    if (m.fh.interp >= 2) {
         // Lorenzo tiles: the whole base grid whatever stop_level asks
         LorenzoJob j{};
+        j.reg = m.fh.interp == 3;
+        j.lo = m.fh.vmin;
+        j.hi = m.fh.vmax;
         j.v = v;
         j.codes = dd.codes.as<uint16_t>() + dd.code_base[0];
         j.side = dd.side.as<T>() + dd.code_base[0];
@@ -1756,11 +1768,12 @@ void decompress_batch(stz_ctx* c, stz_dstream* ds, int n, const stz_request* rq,
 // ---------------------------------------------------------------- tuner
 // The configurations the tuner chooses among: predictor set (4) x base codec
 // (interpolation: 2 interpolators x 6 axis orders x 4 level-eb scalings;
-// Lorenzo: 3 scalings -- the fourth differs from the first only in the
+// the Lorenzo / regression base (interp 3, SZ3's configuration in
+// sz.cpp:63-76): 3 scalings -- the fourth differs from the first only in the
 // interpolation levels' alpha / beta), 51 per set; index
 // k = 51 * set + j, j = 24 * interp + 4 * order + scaling below 48 and
-// 48 + scaling for Lorenzo, so 0..47 are F5's own predictors with
-// interpolation.  Predictor sets: F5's (F5:690-748); F5 built with
+// 48 + scaling for Lorenzo / regression, so 0..47 are F5's own predictors
+// with interpolation.  Predictor sets: F5's (F5:690-748); F5 built with
 // PRED_PLANE_AXIS and PRED_CUBE_AXIS (F5:579-622, F5:627-686); stz2.cpp's
 // 12-point plane and 16-point cube predictors (stz2.cpp:122-196);
 // dependency.cpp's staged inverse-variance predictors (dependency.cpp:162-292).
@@ -1783,7 +1796,7 @@ void tune_config(int k, stz_params& p)
 {
     p.pred = kTunePred[k / kTunePerSet];
     const int j = k % kTunePerSet;
-    p.interp = j < 48 ? j / 24 : 2;
+    p.interp = j < 48 ? j / 24 : 3;
     p.order = j < 48 ? (j / 4) % 6 : 0;
     const TuneScale& t = kScales[j < 48 ? j % 4 : j - 48];
     p.mult_base = t.mb;
@@ -2092,8 +2105,11 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
         for (int i = 0; i < n; ++i) {
             FieldEnc<T>& f = fe[i];
             const HostLayout& L = f.L;
-            if (f.prm.interp == 2) {
+            if (f.prm.interp >= 2) {
                 LorenzoJob j{};
+                j.reg = f.prm.interp == 3;
+                j.lo = f.vmin;
+                j.hi = f.vmax;
                 j.v = f.vbase.p;
                 j.codes = f.codes.template as<uint16_t>() + L.code_off[0];
                 j.side = f.side.template as<T>() + L.code_off[0];
@@ -2624,8 +2640,11 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
         const Meta& m = ds->meta;
         const uint16_t* c0 = F.dd.codes.as<uint16_t>() + F.dd.code_base[0];
         const T* s0 = F.dd.side.as<T>() + F.dd.code_base[0];
-        if (m.fh.interp == 2) {
+        if (m.fh.interp >= 2) {
             LorenzoJob q{};
+            q.reg = m.fh.interp == 3;
+            q.lo = m.fh.vmin;
+            q.hi = m.fh.vmax;
             q.v = F.vb.p;
             q.codes = (uint16_t*)c0;
             q.side = (void*)s0;
@@ -2795,14 +2814,14 @@ void parse_meta(const uint8_t* p, size_t bytes, Meta& m)
     std::memcpy(&m.fh, p, sizeof(FileHeader));
     const FileHeader& fh = m.fh;
     if (fh.magic != kMagic || fh.version != kVersion || fh.n_streams != kStreams || fh.total_bytes > bytes ||
-        !pred_valid(fh.pred) || fh.interp > 2 || fh.order > 5 || !(fh.alpha >= 1.0) || !(fh.beta >= 1.0) ||
+        !pred_valid(fh.pred) || fh.interp > 3 || fh.order > 5 || !(fh.alpha >= 1.0) || !(fh.beta >= 1.0) ||
         (fh.dtype != STZ_F32 && fh.dtype != STZ_F64) || fh.radius != (uint32_t)kRadius || fh.sub_size != (uint32_t)kSub)
         throw StzError(STZ_E_STREAM, "bad payload header");
     if (fh.nx == 0 || fh.ny == 0 || fh.nz == 0 || fh.px != pad4(fh.nx) || fh.py != pad4(fh.ny) || fh.pz != pad4(fh.nz))
         throw StzError(STZ_E_STREAM, "bad dims");
     check_dims(fh.nx, fh.ny, fh.nz, (int)fh.chunk_high, (int)fh.chunk_low);
     HostLayout L;
-    make_layout(L, fh.nx, fh.ny, fh.nz, fh.px, fh.py, fh.pz, (int)fh.chunk_high, (int)fh.chunk_low);
+    make_layout(L, fh.nx, fh.ny, fh.nz, fh.px, fh.py, fh.pz, (int)fh.chunk_high, (int)fh.chunk_low, (int)fh.interp);
     // streams are packed back to back after the file header (the writer's
     // layout); every bound below is checked without overflow
     uint64_t next = sizeof(FileHeader);

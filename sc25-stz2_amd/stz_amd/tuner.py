@@ -1,6 +1,6 @@
 """Learned tuner: a 16 -> 64 -> 64 -> 208 MLP over per-sub-block field
 statistics that picks the residual predictor set, base codec (interpolator
-and axis order, or Lorenzo) and level-eb scaling (204 configurations, the
+and axis order, or Lorenzo / regression) and level-eb scaling (204 configurations, the
 first 204 outputs) in place of the exhaustive trial.
 
 Labels come from the exhaustive trial itself (the payload size of every
@@ -213,7 +213,7 @@ def make_field(rng, device):
         f = (f - f.mean()) / f.std()
     elif fam == "blocks":
         # piecewise-constant cells of one random size (AMR-like patches) over
-        # a weak smooth field: where the Lorenzo base beats interpolation
+        # a weak smooth field: where the Lorenzo / regression base beats interpolation
         b = int(rng.integers(3, 17))
         vals = rng.normal(size=[(n + b - 1) // b for n in shape])
         f = np.kron(vals, np.ones((b, b, b)))[: shape[0], : shape[1], : shape[2]]

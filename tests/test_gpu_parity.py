@@ -493,36 +493,40 @@ def test_staged_predictor_requests_and_batch(ctx, oracle):
 
 # ------------------------------------------------------------------ Lorenzo base
 # interp = 2: the base grid in independent 16^3 Lorenzo tiles (SZ3's
-# ALGO_INTERP_LORENZO alternative).  Base grids from 1 to 2x2x3 tiles, with
-# partial tiles on every axis.
+# ALGO_INTERP_LORENZO alternative); interp = 3: Lorenzo or a fitted plane per
+# 8^3 sub-block of those tiles (SZ3's Lorenzo + regression, sz.cpp:63-76).
+# Base grids from 1 to 2x2x3 tiles, with partial tiles (and empty sub-blocks)
+# on every axis.
 LZ_CASES = [("lognormal", (64, 56, 64), np.float32, 1e-3, "rel"), ("grf", (48, 40, 56), np.float64, 1e-6, "rel"),
             ("sin", (9, 130, 70), np.float32, 1e-2, "abs"), ("noise", (20, 24, 28), np.float64, 1e-5, "rel"),
             ("grf", (4, 4, 4), np.float32, 1e-3, "rel"), ("grf", (128, 96, 160), np.float32, 1e-3, "rel")]
 
 
+@pytest.mark.parametrize("interp", [2, 3])
 @pytest.mark.parametrize("case", LZ_CASES, ids=[("%s-%s-%s-%g-%s" % (c[0], "x".join(map(str, c[1])), c[2].__name__,
                                                                       c[3], c[4])) for c in LZ_CASES])
-def test_lorenzo_base_bitexact(ctx, oracle, case):
+def test_lorenzo_base_bitexact(ctx, oracle, case, interp):
     kind, shape, dtype, eb, mode = case
-    _say("lorenzo base", case)
+    _say("lorenzo base", interp, case)
     f = _field(kind, shape, dtype, seed=77)
-    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=2)
+    ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=interp)
     diff = first_difference(gpu, want)
     assert diff is None, diff
-    assert ds.info.interp == 2 and oracle.header(want)["interp"] == 2
+    assert ds.info.interp == interp and oracle.header(want)["interp"] == interp
     rec = ds.decompress()
     _assert_bitexact(rec, oracle.decompress(want), "decompress")
     assert max_error(rec, f) <= requested_bound(f, eb, mode)
     ds.free()
 
 
-def test_lorenzo_base_degenerate_fields(ctx, oracle):
+@pytest.mark.parametrize("interp", [2, 3])
+def test_lorenzo_base_degenerate_fields(ctx, oracle, interp):
     """Constant, zero, one-point, huge-bound and non-finite fields with the
-    Lorenzo base: payload and decode equal the oracle's."""
+    Lorenzo bases: payload and decode equal the oracle's."""
     for kind, eb, mode in DEGENERATE:
-        _say("lorenzo degenerate", kind)
+        _say("lorenzo degenerate", interp, kind)
         f = _degenerate(kind, np.float32)
-        ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=2)
+        ds, gpu, want = _compress_both(ctx, oracle, f, eb, mode, interp=interp)
         got, ref = ds.decompress(), oracle.decompress(want)
         if kind.startswith("nan"):
             gpu, want = nan_sign_cleared(gpu, np.float32), nan_sign_cleared(want, np.float32)
@@ -534,13 +538,15 @@ def test_lorenzo_base_degenerate_fields(ctx, oracle):
         ds.free()
 
 
-def test_lorenzo_base_random_access_and_batch(ctx, oracle):
+@pytest.mark.parametrize("interp", [2, 3])
+def test_lorenzo_base_random_access_and_batch(ctx, oracle, interp):
     """Box, slice and progressive-level decodes of a Lorenzo-based stream
     equal the oracle's; a batch decode mixing Lorenzo and interpolation bases
     equals one call per stream; a batch compress equals single compresses."""
     import torch
+    _say("lorenzo random access and batch", interp)
     f = _field("lognormal", (64, 56, 64), np.float32, seed=21)
-    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", interp=2, pred=24, chunk_high=8, chunk_low=4)
+    ds, gpu, want = _compress_both(ctx, oracle, f, 1e-3, "rel", interp=interp, pred=24, chunk_high=8, chunk_low=4)
     assert first_difference(gpu, want) is None
     full = oracle.decompress(want)
     for q in [(3, 5, 7, 50, 30, 40), (10, 10, 10, 10, 55, 63), (0, 0, 0, 63, 55, 63)]:
@@ -559,7 +565,7 @@ def test_lorenzo_base_random_access_and_batch(ctx, oracle):
     dev = torch.device("cuda:0")
     fs = [torch.from_numpy(_field("grf", sh, np.float32, seed=50 + i)).to(dev)
           for i, sh in enumerate([(33, 47, 61), (16, 20, 24), (70, 36, 90)])]
-    lz = ctx.compress_batch(fs, 1e-3, "rel", interp=2)
+    lz = ctx.compress_batch(fs, 1e-3, "rel", interp=interp)
     it = ctx.compress_batch(fs, 1e-3, "rel", interp=1)
     mixed = [lz[0], it[1], lz[2], it[0]]
     outs = [torch.empty_like(fs[i]) for i in (0, 1, 2, 0)]
@@ -567,10 +573,10 @@ def test_lorenzo_base_random_access_and_batch(ctx, oracle):
     for b, o in zip(mixed, outs):
         _assert_bitexact(o.cpu().numpy(), b.decompress(), "mixed batch decode")
     for t, b in zip(fs, lz):
-        single = ctx.compress_device(t, 1e-3, "rel", interp=2)
+        single = ctx.compress_device(t, 1e-3, "rel", interp=interp)
         pb = b.download(-1)
         assert np.array_equal(pb, single.download(-1))
-        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, interp=2)) is None
+        assert first_difference(pb, oracle.compress(t.cpu().numpy(), 1e-3, 1, interp=interp)) is None
         single.free()
     for b in lz + it:
         b.free()

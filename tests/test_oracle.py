@@ -744,7 +744,149 @@ def test_lorenzo_base_whole_path(oracle):
     assert max_error(r, f) <= E
     assert not np.array_equal(p, oracle.compress(f, 1e-3, oracle.EB_REL, interp=1))
     with pytest.raises(Exception):
-        oracle.compress(f, 1e-3, oracle.EB_REL, interp=3)
+        oracle.compress(f, 1e-3, oracle.EB_REL, interp=4)
+
+
+# ------------------------------------------------- Lorenzo / regression base codec
+def _lzreg_layout(shape):
+    """Stream positions of the Lorenzo/regression base: per tile (raster
+    order) its 32 header symbols (4 per 8^3 sub-block, sub-block index
+    x>=8 | (y>=8)<<1 | (z>=8)<<2), then its points; returns (header rows:
+    tile origin, sub-block, extents, first position), and per point its
+    position, tile-local and grid coordinates."""
+    bz, by, bx = shape
+    T = 16
+    hdr, pts = [], []
+    pos = 0
+    for tz in range(0, bz, T):
+        for ty in range(0, by, T):
+            for tx in range(0, bx, T):
+                dz, dy, dx = min(T, bz - tz), min(T, by - ty), min(T, bx - tx)
+                for b in range(8):
+                    sx, sy, sz = (b & 1) * 8, ((b >> 1) & 1) * 8, (b >> 2) * 8
+                    hdr.append((tx, ty, tz, b, min(8, dx - sx), min(8, dy - sy), min(8, dz - sz), pos + 4 * b))
+                pos += 32
+                for z in range(dz):
+                    for y in range(dy):
+                        for x in range(dx):
+                            pts.append((pos, x, y, z, tx + x, ty + y, tz + z))
+                            pos += 1
+    return np.array(hdr, dtype=np.int64), np.array(pts, dtype=np.int64)
+
+
+@pytest.mark.parametrize("shape", [(21, 18, 35), (16, 16, 16), (5, 33, 17), (1, 1, 40)])
+def test_lzreg_base_planes_on_linear_field(oracle, shape):
+    """On f = 5 + 2.5x - 3.25y + 1.5z (exact in binary) every non-empty
+    sub-block takes its plane: the header carries code 0 and the slopes and
+    the sub-block origin's value (an axis of extent 1 gets slope 0, its
+    d absorbing nothing), every point's residual is zero (code = radius), and
+    the unpredictables are exactly those coefficients in stream order.  Pins
+    the least-squares fit (a dropped centring term or a wrong sign moves d
+    or a slope), the header layout, the tiles and sub-blocks, and the decode."""
+    bz, by, bx = shape
+    z, y, x = np.meshgrid(np.arange(bz), np.arange(by), np.arange(bx), indexing="ij")
+    v = (5 + 2.5 * x - 3.25 * y + 1.5 * z).astype(np.float32)
+    rec, codes, side = oracle.interp_compress(v, 1e-2, oracle.BASE_LZREG)
+    H, P = _lzreg_layout(shape)
+    assert codes.size == oracle.base_symbols(shape, oracle.BASE_LZREG) == len(P) + 4 * len(H)
+    want_side = []
+    for tx, ty, tz, b, nx, ny, nz, p0 in H:
+        if nx <= 0 or ny <= 0 or nz <= 0:
+            assert np.all(codes[p0:p0 + 4] == 32768)
+            continue
+        assert np.all(codes[p0:p0 + 4] == 0)
+        ox, oy, oz = tx + (b & 1) * 8, ty + ((b >> 1) & 1) * 8, tz + (b >> 2) * 8
+        want_side += [2.5 if nx > 1 else 0.0, -3.25 if ny > 1 else 0.0, 1.5 if nz > 1 else 0.0,
+                      5 + 2.5 * ox - 3.25 * oy + 1.5 * oz]
+    assert np.all(codes[P[:, 0]] == 32768)
+    assert np.array_equal(side, np.array(want_side, np.float32))
+    assert np.array_equal(rec, v)
+    back = oracle.interp_decompress(v.shape, np.float32, 1e-2, codes, side, oracle.BASE_LZREG, lo=v.min(), hi=v.max())
+    assert np.array_equal(back, v)
+
+
+def test_lzreg_base_clamps_planes_to_the_range(oracle):
+    """A plane prediction above hi is hi: with hi set below the field's top,
+    the points whose exact plane value exceeds hi get residual v - hi
+    (quantized exactly at eb = 0.5 on integer data), the others zero."""
+    n = 16
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    v = (x + 2 * y + 3 * z).astype(np.float64)
+    hi = 40.0
+    rec, codes, side = oracle.interp_compress(v, 0.5, oracle.BASE_LZREG, lo=0.0, hi=hi)
+    H, P = _lzreg_layout(v.shape)
+    val = P[:, 4] + 2 * P[:, 5] + 3 * P[:, 6]
+    reg = {int(b) for (_, _, _, b, _, _, _, p0) in H if codes[p0] == 0}
+    sb = (P[:, 1] >= 8) | ((P[:, 2] >= 8) << 1) | ((P[:, 3] >= 8) << 2)
+    on = np.isin(sb, sorted(reg))
+    assert 0 in reg and on.sum() > 0
+    q = codes[P[:, 0]].astype(np.int64) - 32768
+    assert np.array_equal(q[on], np.where(val[on] > hi, val[on] - hi, 0))
+    assert np.array_equal(rec, v)
+
+
+def test_lzreg_base_keeps_lorenzo_where_it_is_exact(oracle):
+    """On g(x,y) + h(y,z) + k(x,z) (random integers) the Lorenzo residual is
+    zero wherever all seven neighbours lie in the tile, so a sub-block away
+    from the tile's low faces (index 7) keeps Lorenzo (no plane fits such a
+    field better than zero) and its points code zero residuals."""
+    rng = np.random.default_rng(3)
+    n = 32
+    g, h, k = (rng.integers(-50, 50, size=(n, n)) for _ in range(3))
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    v = (g[x, y] + h[y, z] + k[x, z]).astype(np.float64)
+    _, codes, _ = oracle.interp_compress(v, 0.5, oracle.BASE_LZREG)
+    H, P = _lzreg_layout(v.shape)
+    seven = H[H[:, 3] == 7]
+    assert len(seven) == 8 and np.all(codes[seven[:, 7]] == 32768)
+    inner = (P[:, 1] >= 8) & (P[:, 2] >= 8) & (P[:, 3] >= 8)
+    assert inner.sum() == 8 * 512 and np.all(codes[P[inner, 0]] == 32768)
+
+
+def test_lzreg_base_prefers_planes_on_noise(oracle):
+    """Independent noise: a plane's error is about the noise, Lorenzo's
+    prediction sums seven noisy neighbours (residual about sqrt(8) times the
+    noise), so nearly every full sub-block takes its plane and the codes'
+    entropy drops below the pure Lorenzo base's."""
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(32, 32, 32)).astype(np.float32)
+    _, c2, _ = oracle.interp_compress(v, 1e-2, oracle.BASE_LORENZO)
+    _, c3, s3 = oracle.interp_compress(v, 1e-2, oracle.BASE_LZREG)
+    H, _ = _lzreg_layout(v.shape)
+    assert np.mean(c3[H[:, 7]] == 0) > 0.9
+
+    def bits(c):
+        p = np.bincount(c, minlength=65536).astype(float)
+        p = p[p > 0] / c.size
+        return -(p * np.log2(p)).sum() * c.size
+
+    assert bits(c3) < 0.9 * bits(c2)
+
+
+@pytest.mark.parametrize("shape,dt,eb", [((21, 18, 35), np.float32, 1e-2), ((40, 9, 33), np.float64, 1e-4),
+                                         ((3, 4, 5), np.float32, 0.3)])
+def test_lzreg_base_round_trip_and_bound(oracle, shape, dt, eb):
+    """decode(encode) is the encoder's reconstruction bit for bit, within eb."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    v = (np.cumsum(rng.normal(size=shape), axis=2) + rng.normal(scale=3, size=shape)).astype(dt)
+    v.flat[7] = 1e6
+    rec, codes, side = oracle.interp_compress(v, eb, oracle.BASE_LZREG)
+    got = oracle.interp_decompress(v.shape, dt, eb, codes, side, oracle.BASE_LZREG, lo=v.min(), hi=v.max())
+    assert np.array_equal(_bits(got), _bits(rec))
+    assert rec.flat[7] == v.flat[7]
+    assert np.abs(rec.astype(np.float64) - v).max() <= eb
+
+
+def test_lzreg_base_whole_path(oracle):
+    """interp = 3 through the whole oracle path: the header names it, the
+    base stream holds the header symbols, the decode is within the bound."""
+    f = _field((40, 36, 44), np.float32, 9)
+    p = oracle.compress(f, 1e-3, oracle.EB_REL, interp=3)
+    assert oracle.header(p)["interp"] == 3
+    r = oracle.decompress(p)
+    from payload_util import max_error, requested_bound
+    assert max_error(r, f) <= requested_bound(f, 1e-3, "rel")
+    assert not np.array_equal(p, oracle.compress(f, 1e-3, oracle.EB_REL, interp=2))
 
 
 # ---------------------------------------------------------------- whole oracle path
