@@ -10,7 +10,7 @@ is float64 here (fp32 accumulation on the device).  With bf16=False it is
 the plain float64 network, whose gradient the tests pin by finite
 differences.
 
-Parameter layout (fp32 vector of 15648): W1[16][64], b1[64], W2[64][64],
+Parameter layout (fp32 vector of 18768): W1[16][64], b1[64], W2[64][64],
 b2[64], W3[64][208], b3[208]; W[k][n] maps input k to output n.
 """
 import numpy as np

@@ -128,7 +128,9 @@ __host__ __device__ inline uint64_t chunk_volume(const Geom& g, uint32_t c, uint
     return (uint64_t)imin_(C, g.dx - cx * C) * imin_(C, g.dy - cy * C) * imin_(C, g.dz - cz * C);
 }
 
-inline Geom make_geom(int dx, int dy, int dz, int chunk)
+// extra: symbols a linearly chunked stream holds beyond its grid points (the
+// tiles' header symbols of a Lorenzo/regression base)
+inline Geom make_geom(int dx, int dy, int dz, int chunk, uint64_t extra = 0)
 {
     Geom g{};
     g.dx = dx; g.dy = dy; g.dz = dz; g.chunk = chunk;
@@ -141,7 +143,7 @@ inline Geom make_geom(int dx, int dy, int dz, int chunk)
         g.ncz = (dz + chunk - 1) / chunk;
         g.n_chunks = (uint32_t)g.ncx * g.ncy * g.ncz;
     } else {
-        const uint64_t n = (uint64_t)dx * dy * dz;
+        const uint64_t n = (uint64_t)dx * dy * dz + extra;
         g.n_chunks = (uint32_t)((n + kBaseChunk - 1) / kBaseChunk);
     }
     return g;

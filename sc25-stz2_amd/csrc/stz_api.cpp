@@ -548,7 +548,7 @@ void build_layout(HostLayout& L, size_t nx, size_t ny, size_t nz, size_t px, siz
     const int mx = (int)(px / 2), my = (int)(py / 2), mz = (int)(pz / 2);
     uint64_t off = 0;
     for (int s = 0; s < kStreams; ++s) {
-        if (s == 0) L.g[s] = make_geom(lx, ly, lz, 0);
+        if (s == 0) L.g[s] = make_geom(lx, ly, lz, 0, base_extra_symbols(interp, lx, ly, lz));
         else if (s < 8) L.g[s] = make_geom(lx, ly, lz, cl);
         else L.g[s] = make_geom(mx, my, mz, ch);
         const Geom& g = L.g[s];
@@ -992,14 +992,23 @@ void enc_phase0_batch(stz_ctx* c, std::vector<FieldEnc<T>>& fe, const void* cons
     for (int i = 0; i < n; ++i) fe[i].mm_host = mm_host + 2 * i;
 }
 
-// after the wait: level bounds from the range
+// after the wait: level bounds from the range.  A learned choice made after
+// the layout may have switched to or from the Lorenzo/regression base, whose
+// base stream holds header symbols: then the layout and the code buffers
+// (which no launch has touched yet) are made again.
 template <typename T>
 void enc_phase1(stz_ctx* c, FieldEnc<T>& fe)
 {
-    (void)c;
     fe.vmin = fe.mm_host[0];
     fe.vmax = fe.mm_host[1];
     level_ebs(fe.eb, fe.mode, fe.vmin, fe.vmax, fe.dtype, fe.prm, fe.ebl, fe.unit);
+    const Geom& g0 = fe.L.g[0];
+    if (fe.L.n[0] != (uint64_t)g0.dx * g0.dy * g0.dz + base_extra_symbols(fe.prm.interp, g0.dx, g0.dy, g0.dz)) {
+        make_layout(fe.L, fe.nx, fe.ny, fe.nz, fe.px, fe.py, fe.pz, fe.prm.chunk_high, fe.prm.chunk_low,
+                    fe.prm.interp);
+        fe.codes = DBuf(c->pool, fe.L.total_codes * 2);
+        fe.side = DBuf(c->pool, fe.L.total_codes * sizeof(T));
+    }
 }
 
 template <typename T>
@@ -2248,11 +2257,12 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
             HIPCHK(hipEventRecord(c->ev_fork, rs));
             fork_guard.on = rs != st;
         }
-        // one field, or fields under the staged predictors (a batch shares its
-        // parameters): launches of their own per field
-        if (n == 1 || (fe[0].prm.pred & kPredStaged)) {
+        // one field, and fields under the staged predictors (a learned batch
+        // mixes them with the others): launches of their own per field
+        {
             for (int i = 0; i < n; ++i) {
                 FieldEnc<T>& f = fe[i];
+                if (n > 1 && !(f.prm.pred & kPredStaged)) continue;
                 const LevelArgs a = enc_level_args(f, level);
                 if (f.prm.pred & kPredStaged) {
                     // dependency.cpp's staged predictors: three launches per
@@ -2282,13 +2292,15 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                     });
                 }
             }
-            continue;
+            if (n == 1) continue;
         }
+        // the other fields: one persistent launch for all of them
         const int md = level == 1 ? kLowEnc : kHighEnc;
         JobSet js;
         for (int v = 0; v < 2; ++v)
             for (int i = 0; i < n; ++i) {
                 FieldEnc<T>& f = fe[i];
+                if (f.prm.pred & kPredStaged) continue;
                 LevelJob j{};
                 j.full = f.full;
                 j.px = (int)f.px;
@@ -2302,10 +2314,13 @@ void compress_fields(stz_ctx* c, int dtype, int n, const void* const* fields, co
                 j.a = enc_level_args(f, level);
                 js.h[v].push_back(j);
             }
+        if (js.h[0].empty()) continue;
         jobs_finish(c, js);
         const stz_ctx::LaunchCfg cfg = level_cfg<T>(c, md, js, [&] {
             for (int i = 0; i < n; ++i)
-                launch_fill_u32(st, fe[i].hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, 7 * 65536, 0u);
+                if (!(fe[i].prm.pred & kPredStaged))
+                    launch_fill_u32(st, fe[i].hist.template as<uint32_t>() + (level == 1 ? 1 : 8) * 65536, 7 * 65536,
+                                    0u);
         });
         jobs_launch<T>(c, md, js, cfg);
     }

@@ -496,10 +496,12 @@ def test_staged_predictor_requests_and_batch(ctx, oracle):
 # ALGO_INTERP_LORENZO alternative); interp = 3: Lorenzo or a fitted plane per
 # 8^3 sub-block of those tiles (SZ3's Lorenzo + regression, sz.cpp:63-76).
 # Base grids from 1 to 2x2x3 tiles, with partial tiles (and empty sub-blocks)
-# on every axis.
+# on every axis; a 64x32x16 base (one 32768-symbol chunk of points) whose
+# tiles' header symbols start a second base-stream chunk.
 LZ_CASES = [("lognormal", (64, 56, 64), np.float32, 1e-3, "rel"), ("grf", (48, 40, 56), np.float64, 1e-6, "rel"),
             ("sin", (9, 130, 70), np.float32, 1e-2, "abs"), ("noise", (20, 24, 28), np.float64, 1e-5, "rel"),
-            ("grf", (4, 4, 4), np.float32, 1e-3, "rel"), ("grf", (128, 96, 160), np.float32, 1e-3, "rel")]
+            ("grf", (4, 4, 4), np.float32, 1e-3, "rel"), ("grf", (128, 96, 160), np.float32, 1e-3, "rel"),
+            ("grf", (64, 128, 256), np.float32, 1e-3, "rel")]
 
 
 @pytest.mark.parametrize("interp", [2, 3])

@@ -230,3 +230,130 @@ def test_compress_with_learned_and_trial_choice(stz, ctx, torch_dev, oracle):
     assert ds.info.tuner == 2 and 0 < ds.info.tune_confidence <= 1
     ds.free()
     tuner.set_model(ctx, params)
+
+
+def _forced_model(k):
+    """A model whose every sub-block scores configuration k far above the
+    rest: zero last-layer weights, biases +10 at k and -10 elsewhere."""
+    from stz_amd import tuner
+    from stz_amd._lib import MLP_OUTPUTS, MLP_PARAMS
+    p = np.zeros(MLP_PARAMS, np.float32)
+    b3 = np.full(MLP_OUTPUTS, -10.0, np.float32)
+    b3[k] = 10.0
+    p[-MLP_OUTPUTS:] = b3
+    return p
+
+
+@pytest.mark.parametrize("k", [48, 3 * 51 + 49])
+def test_learned_choice_of_the_lorenzo_regression_base(stz, ctx, torch_dev, oracle, k):
+    """A learned choice of the Lorenzo/regression base (whose base stream
+    holds header symbols, so the layout made before the choice is made
+    again): single and batch compresses carry the choice in the header,
+    equal the oracle's payload for it and decode within the bound."""
+    import torch
+    from stz_amd import fields, tuner
+    params, meta = tuner.load_params()
+    prm = tuner.config_params(k)
+    assert prm["interp"] == 3
+    tuner.set_model(ctx, _forced_model(k))
+    try:
+        fs = [fields.grf(sh, -3.0, 900 + i, None).astype(np.float32) for i, sh in
+              enumerate([(40, 36, 44), (17, 70, 33)])]
+        for f in fs:
+            print("learned lorenzo/regression", k, f.shape, flush=True)
+            ds = ctx.compress_device(f, 1e-3, "rel", tuner=1)
+            i = ds.info
+            assert i.interp == 3 and i.tuner == 1
+            want = oracle.compress(f, 1e-3, oracle.EB_REL, pred=i.pred, interp=i.interp, order=i.order,
+                                   mult_base=i.mult_base, mult_low=i.mult_low, mult_high=i.mult_high, alpha=i.alpha,
+                                   beta=i.beta)
+            assert np.array_equal(ds.download(-1), want)
+            r = ds.decompress()
+            assert np.abs(r.astype(np.float64) - f).max() <= 1e-3 * (float(f.max()) - float(f.min()))
+            ds.free()
+        dev = torch.device("cuda:0")
+        ts = [torch.from_numpy(f).to(dev) for f in fs]
+        bs = ctx.compress_batch(ts, 1e-3, "rel", tuner=1)
+        for f, b in zip(fs, bs):
+            assert b.info.interp == 3
+            want = oracle.compress(f, 1e-3, oracle.EB_REL, pred=prm["pred"], interp=3, order=prm["order"],
+                                   mult_base=prm["mult_base"], mult_low=prm["mult_low"], mult_high=prm["mult_high"],
+                                   alpha=prm["alpha"], beta=prm["beta"])
+            assert np.array_equal(b.download(-1), want)
+        outs = [torch.empty_like(t) for t in ts]
+        ctx.decompress_batch(bs, outs)
+        for f, o in zip(fs, outs):
+            assert np.abs(o.cpu().numpy().astype(np.float64) - f).max() <= 1e-3 * (float(f.max()) - float(f.min()))
+        for b in bs:
+            b.free()
+    finally:
+        tuner.set_model(ctx, params, float(meta.get("min_confidence", 0.0)))
+
+
+def _two_way_model(k_rough, k_smooth):
+    """A model that picks k_rough for fields whose sub-blocks have a large
+    spread in units of the bound (feature 0, log2(std/eb)/8, above 0.5) and
+    k_smooth otherwise: one hidden unit relu(f0 - 0.5) passed through, output
+    k_rough = 40 x that unit, k_smooth a bias of 10, the rest -10."""
+    from stz_amd._lib import MLP_OUTPUTS, MLP_PARAMS
+    IN, HID, OUT = 16, 64, MLP_OUTPUTS
+    p = np.zeros(MLP_PARAMS, np.float32)
+    o_b1 = IN * HID
+    o_w2 = o_b1 + HID
+    o_b2 = o_w2 + HID * HID
+    o_w3 = o_b2 + HID
+    o_b3 = o_w3 + HID * OUT
+    p[0 * HID + 0] = 1.0          # W1[f0][h0]
+    p[o_b1 + 0] = -0.5            # b1[h0]
+    p[o_w2 + 0 * HID + 0] = 1.0   # W2[h0][h0]
+    p[o_w3 + 0 * OUT + k_rough] = 40.0
+    b3 = np.full(OUT, -10.0, np.float32)
+    b3[k_smooth] = 10.0
+    b3[k_rough] = 0.0
+    p[o_b3:] = b3
+    assert o_b3 + OUT == MLP_PARAMS
+    return p
+
+
+def test_learned_batch_mixing_staged_and_other_predictors(stz, ctx, torch_dev, oracle):
+    """One learned batch whose fields get different predictor sets, the first
+    one not staged: each field's payload is the oracle's for its own choice
+    (staged fields run their three-stage launches, the others the shared
+    persistent launch), and the batch decode is within the bound."""
+    import torch
+    from stz_amd import fields, tuner
+    params, meta = tuner.load_params()
+    k_staged = 3 * 51 + 24     # staged predictors, cubic base, order 0, F5 multipliers
+    k_other = 48               # F5's predictors, Lorenzo / regression base
+    assert tuner.config_params(k_staged)["pred"] == 32 and tuner.config_params(k_other)["interp"] == 3
+    tuner.set_model(ctx, _two_way_model(k_staged, k_other))
+    try:
+        rng = np.random.default_rng(5)
+        spiky = []
+        for i, sh in enumerate([(72, 80, 96), (70, 66, 130)]):
+            f = 0.01 * fields.grf(sh, -3.0, 700 + i, None)
+            f.flat[rng.integers(f.size)] = 1e3
+            spiky.append(f.astype(np.float32))
+        noise = rng.uniform(-1, 1, size=(40, 48, 56)).astype(np.float32)
+        fs = [spiky[0], noise, spiky[1]]
+        want_k = [k_other, k_staged, k_other]
+        dev = torch.device("cuda:0")
+        ts = [torch.from_numpy(f).to(dev) for f in fs]
+        bs = ctx.compress_batch(ts, 1e-3, "rel", tuner=1)
+        for f, b, k in zip(fs, bs, want_k):
+            i = b.info
+            print("mixed learned batch", f.shape, i.pred, i.interp, flush=True)
+            prm = tuner.config_params(k)
+            assert (i.pred, i.interp, i.order) == (prm["pred"], prm["interp"], prm["order"])
+            want = oracle.compress(f, 1e-3, oracle.EB_REL, pred=prm["pred"], interp=prm["interp"], order=prm["order"],
+                                   mult_base=prm["mult_base"], mult_low=prm["mult_low"], mult_high=prm["mult_high"],
+                                   alpha=prm["alpha"], beta=prm["beta"])
+            assert np.array_equal(b.download(-1), want)
+        outs = [torch.empty_like(t) for t in ts]
+        ctx.decompress_batch(bs, outs)
+        for f, o in zip(fs, outs):
+            assert np.abs(o.cpu().numpy().astype(np.float64) - f).max() <= 1e-3 * (float(f.max()) - float(f.min()))
+        for b in bs:
+            b.free()
+    finally:
+        tuner.set_model(ctx, params, float(meta.get("min_confidence", 0.0)))
