@@ -2500,17 +2500,47 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
         }
         for (int i = 0; i < n; ++i) raw[i]->tuned = 2;
     } else if (prm.tuner == 1) {
-        // the model's choice for every field in one pass; a field below the
-        // confidence threshold gets its own exhaustive trial
+        // the model's choice for every field in one pass; the fields below
+        // the confidence threshold get the exhaustive trial, as one batch per
+        // configuration (each keeps its first smallest payload, as a
+        // single-field trial does)
         std::vector<char> low;
         compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data(), &low);
-        for (int i = 0; i < n; ++i) {
-            if (!low[i]) continue;
-            const float conf = raw[i]->tune_conf;
-            tune_trial<T>(c, dtype, (const T*)fields[i], dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], eb, mode, prm,
-                          raw[i], nullptr);
-            raw[i]->tuned = 2;
-            raw[i]->tune_conf = conf;
+        std::vector<int> lows;
+        for (int i = 0; i < n; ++i)
+            if (low[i]) lows.push_back(i);
+        const int m = (int)lows.size();
+        if (m > 0) {
+            std::vector<const void*> lf(m);
+            std::vector<size_t> ld(3 * (size_t)m);
+            std::vector<float> conf(m);
+            std::vector<uint64_t> best(m, ~0ull);
+            for (int j = 0; j < m; ++j) {
+                lf[j] = fields[lows[j]];
+                for (int a = 0; a < 3; ++a) ld[3 * j + a] = dims[3 * lows[j] + a];
+                conf[j] = raw[lows[j]]->tune_conf;
+            }
+            for (int k = 0; k < kTuneConfigs; ++k) {
+                stz_params pk = prm;
+                pk.tuner = 0;
+                tune_config(k, pk);
+                std::vector<std::unique_ptr<stz_dstream>> tmp(m);
+                std::vector<stz_dstream*> traw(m);
+                for (int j = 0; j < m; ++j) {
+                    tmp[j].reset(new stz_dstream());
+                    traw[j] = tmp[j].get();
+                }
+                compress_fields<T>(c, dtype, m, lf.data(), ld.data(), eb, mode, pk, traw.data(), nullptr);
+                for (int j = 0; j < m; ++j)
+                    if (traw[j]->bytes < best[j]) {
+                        best[j] = traw[j]->bytes;
+                        std::swap(*raw[lows[j]], *traw[j]);
+                    }
+            }
+            for (int j = 0; j < m; ++j) {
+                raw[lows[j]]->tuned = 2;
+                raw[lows[j]]->tune_conf = conf[j];
+            }
         }
     } else {
         compress_fields<T>(c, dtype, n, fields, dims, eb, mode, prm, raw.data(), nullptr);

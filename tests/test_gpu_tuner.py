@@ -357,3 +357,28 @@ def test_learned_batch_mixing_staged_and_other_predictors(stz, ctx, torch_dev, o
             b.free()
     finally:
         tuner.set_model(ctx, params, float(meta.get("min_confidence", 0.0)))
+
+
+def test_learned_batch_fallback_is_the_trial(stz, ctx, torch_dev, oracle):
+    """Below the confidence threshold every field of a learned batch falls
+    back to the exhaustive trial (run as one batch per configuration): each
+    payload equals the single-field trial's, and the header records the
+    trial (tuner 2) with the model's confidence."""
+    import torch
+    from stz_amd import fields, tuner
+    params, meta = tuner.load_params()
+    tuner.set_model(ctx, params, min_confidence=1.01)
+    try:
+        dev = torch.device("cuda:0")
+        fs = [torch.from_numpy(fields.grf(sh, -3.0, 600 + i, None).astype(np.float32)).to(dev)
+              for i, sh in enumerate([(24, 20, 28), (12, 33, 17)])]
+        bs = ctx.compress_batch(fs, 1e-3, "rel", tuner=1)
+        for t, b in zip(fs, bs):
+            single = ctx.compress_device(t, 1e-3, "rel", tuner=2)
+            assert np.array_equal(b.download(-1), single.download(-1))
+            assert b.info.tuner == 2 and 0 < b.info.tune_confidence <= 1
+            single.free()
+        for b in bs:
+            b.free()
+    finally:
+        tuner.set_model(ctx, params, float(meta.get("min_confidence", 0.0)))
