@@ -1473,15 +1473,19 @@ struct OutBuf {
     }
 };
 
+// shared: an error flag of the caller's, which then fetches and checks it
+// after all its decodes (no wait per stream)
 template <typename T>
-void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
+void decompress_full(stz_ctx* c, stz_dstream* ds, void* out, ErrFlag* shared = nullptr)
 {
     const Meta& m = ds->meta;
     HostLayout L;
     layout_of(m, L);
     bool all[kStreams];
     for (int s = 0; s < kStreams; ++s) all[s] = true;
-    ErrFlag err(c);
+    std::unique_ptr<ErrFlag> own;
+    if (!shared) own.reset(new ErrFlag(c));
+    ErrFlag& err = shared ? *shared : *own;
     Decoded dd;
     // the high streams' Huffman decode overlaps the base and low levels
     // (one stream when profiling, so per-kernel events stay meaningful, and
@@ -1519,6 +1523,7 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
         });
     }
     ob.copy_back(c);
+    if (shared) return;
     err.fetch(c);
     HIPCHK(hipStreamSynchronize(st));
     err.check();
@@ -2750,15 +2755,27 @@ template <typename T>
 void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* const* outs)
 {
     // streams under the staged predictors decode one by one (their levels
-    // run in three stages each, outside the persistent multi-field launch)
-    bool staged = false;
-    for (int i = 0; i < n; ++i) staged = staged || (ds[i]->meta.fh.pred & kPredStaged);
-    if (staged) {
-        c->plan.reset();
-        for (int i = 0; i < n; ++i) {
-            if (!is_device_ptr(outs[i])) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
-            decompress_full<T>(c, ds[i], outs[i]);
-        }
+    // run in three stages each, outside the persistent multi-field launch),
+    // with their tables built together and one wait at the end; the others
+    // as a batch of their own
+    std::vector<stz_dstream*> rest, stg;
+    std::vector<void*> rest_out, stg_out;
+    for (int i = 0; i < n; ++i) {
+        const bool sf = ds[i]->meta.fh.pred & kPredStaged;
+        (sf ? stg : rest).push_back(ds[i]);
+        (sf ? stg_out : rest_out).push_back(outs[i]);
+    }
+    if (!stg.empty()) {
+        for (void* o : stg_out)
+            if (!is_device_ptr(o)) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
+        if (!rest.empty()) decompress_batch_impl<T>(c, (int)rest.size(), rest.data(), rest_out.data());
+        else c->plan.reset();
+        ensure_tabs_batch(c, stg.data(), (int)stg.size());
+        ErrFlag err(c);
+        for (size_t i = 0; i < stg.size(); ++i) decompress_full<T>(c, stg[i], stg_out[i], &err);
+        err.fetch(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        err.check();
         return;
     }
     bool same = c->plan && (int)c->plan->ds.size() == n;
