@@ -1473,19 +1473,15 @@ struct OutBuf {
     }
 };
 
-// shared: an error flag of the caller's, which then fetches and checks it
-// after all its decodes (no wait per stream)
 template <typename T>
-void decompress_full(stz_ctx* c, stz_dstream* ds, void* out, ErrFlag* shared = nullptr)
+void decompress_full(stz_ctx* c, stz_dstream* ds, void* out)
 {
     const Meta& m = ds->meta;
     HostLayout L;
     layout_of(m, L);
     bool all[kStreams];
     for (int s = 0; s < kStreams; ++s) all[s] = true;
-    std::unique_ptr<ErrFlag> own;
-    if (!shared) own.reset(new ErrFlag(c));
-    ErrFlag& err = shared ? *shared : *own;
+    ErrFlag err(c);
     Decoded dd;
     // the high streams' Huffman decode overlaps the base and low levels
     // (one stream when profiling, so per-kernel events stay meaningful, and
@@ -1523,7 +1519,6 @@ void decompress_full(stz_ctx* c, stz_dstream* ds, void* out, ErrFlag* shared = n
         });
     }
     ob.copy_back(c);
-    if (shared) return;
     err.fetch(c);
     HIPCHK(hipStreamSynchronize(st));
     err.check();
@@ -2554,9 +2549,11 @@ void compress_batch_impl(stz_ctx* c, int dtype, int n, const void* const* fields
 }
 
 // Batch decode: per field the Huffman decode and the base passes, then the
-// low and high levels of all fields as one persistent launch each.  The
-// launch sequence is captured as a HIP graph on the first call and replayed
-// while the call names the same streams and outputs.
+// low and high levels of all fields as one persistent launch each (fields
+// under the staged predictors: their three stage launches per level, one
+// field after another).  The launch sequence is captured as a HIP graph on
+// the first call and replayed while the call names the same streams and
+// outputs.
 struct BatchPlan {
     std::vector<stz_dstream*> ds;
     std::vector<uint64_t> gens;
@@ -2584,6 +2581,20 @@ struct BatchPlan {
     std::vector<std::pair<uint32_t, uint32_t>> round_n;  // jobs, workgroups
     JobSet low, high;
     stz_ctx::LaunchCfg cfg_low, cfg_high;
+    // fields under the staged predictors: their levels in three stage
+    // launches each (one scratch for all: they run one after another)
+    struct Staged {
+        const void* vb;
+        void* mid;
+        Geom g1, g8;
+        LevelArgs la, ha;
+        const uint16_t* codes;
+        const void* side;
+        void* out;
+        int nx, ny, nz;
+    };
+    std::vector<Staged> stg;
+    DBuf rec;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~BatchPlan()
@@ -2633,6 +2644,21 @@ void plan_huff_setup(stz_ctx* c, stz_dstream* ds, BatchPlan::Field& F)
 }
 
 template <typename T>
+void plan_staged(stz_ctx* c, BatchPlan& P)
+{
+    for (const BatchPlan::Staged& q : P.stg) {
+        K(c, "low_decode", [&] {
+            launch_staged_decode<T>(c->stream, (const T*)q.vb, q.g1, q.la, q.codes, (const T*)q.side, (T*)q.mid,
+                                    q.g8.dx, q.g8.dy, q.g8.dz, P.rec.as<T>());
+        });
+        K(c, "high_decode", [&] {
+            launch_staged_decode<T>(c->stream, (const T*)q.mid, q.g8, q.ha, q.codes, (const T*)q.side, (T*)q.out,
+                                    q.nx, q.ny, q.nz, P.rec.as<T>());
+        });
+    }
+}
+
+template <typename T>
 void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
 {
     hipStream_t st = c->stream;
@@ -2650,8 +2676,11 @@ void plan_enqueue(stz_ctx* c, BatchPlan& P, bool levels)
             launch_interp_decode_multi<T>(st, P.rounds[r].as<PassJob>(), P.round_n[r].first, P.round_n[r].second);
         });
     if (!levels) return;
-    jobs_launch<T>(c, kLowDec, P.low, P.cfg_low);
-    jobs_launch<T>(c, kHighDec, P.high, P.cfg_high);
+    if (!P.low.h[0].empty()) {
+        jobs_launch<T>(c, kLowDec, P.low, P.cfg_low);
+        jobs_launch<T>(c, kHighDec, P.high, P.cfg_high);
+    }
+    plan_staged<T>(c, P);
 }
 
 // upload a host vector into a pool buffer (staged copy on the context stream)
@@ -2754,30 +2783,6 @@ void plan_decode_jobs(stz_ctx* c, BatchPlan& P)
 template <typename T>
 void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* const* outs)
 {
-    // streams under the staged predictors decode one by one (their levels
-    // run in three stages each, outside the persistent multi-field launch),
-    // with their tables built together and one wait at the end; the others
-    // as a batch of their own
-    std::vector<stz_dstream*> rest, stg;
-    std::vector<void*> rest_out, stg_out;
-    for (int i = 0; i < n; ++i) {
-        const bool sf = ds[i]->meta.fh.pred & kPredStaged;
-        (sf ? stg : rest).push_back(ds[i]);
-        (sf ? stg_out : rest_out).push_back(outs[i]);
-    }
-    if (!stg.empty()) {
-        for (void* o : stg_out)
-            if (!is_device_ptr(o)) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
-        if (!rest.empty()) decompress_batch_impl<T>(c, (int)rest.size(), rest.data(), rest_out.data());
-        else c->plan.reset();
-        ensure_tabs_batch(c, stg.data(), (int)stg.size());
-        ErrFlag err(c);
-        for (size_t i = 0; i < stg.size(); ++i) decompress_full<T>(c, stg[i], stg_out[i], &err);
-        err.fetch(c);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        err.check();
-        return;
-    }
     bool same = c->plan && (int)c->plan->ds.size() == n;
     for (int i = 0; same && i < n; ++i)
         same = c->plan->ds[i] == ds[i] && c->plan->gens[i] == ds[i]->gen && c->plan->outs[i] == outs[i];
@@ -2793,9 +2798,29 @@ void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* cons
             if (!is_device_ptr(outs[i])) throw StzError(STZ_E_ARG, "batch outputs must be device memory");
         ensure_tabs_batch(c, ds, n);
         for (int i = 0; i < n; ++i) plan_huff_setup(c, ds[i], P->f[i]);
+        size_t rec_bytes = 0;
         for (int i = 0; i < n; ++i) {
             BatchPlan::Field& F = P->f[i];
             const Meta& m = ds[i]->meta;
+            if (m.fh.pred & kPredStaged) {
+                // dependency.cpp's staged predictors: stage launches of their own
+                BatchPlan::Staged q{};
+                q.vb = F.vb.p;
+                q.mid = F.mid.p;
+                q.g1 = F.L.g[1];
+                q.g8 = F.L.g[8];
+                q.la = level_args(m, F.dd, 1, P->err.as<uint32_t>() + i);
+                q.ha = level_args(m, F.dd, 2, P->err.as<uint32_t>() + i);
+                q.codes = F.dd.codes.as<uint16_t>();
+                q.side = F.dd.side.p;
+                q.out = outs[i];
+                q.nx = (int)m.fh.nx;
+                q.ny = (int)m.fh.ny;
+                q.nz = (int)m.fh.nz;
+                P->stg.push_back(q);
+                rec_bytes = std::max(rec_bytes, staged_scratch(F.L, sizeof(T)));
+                continue;
+            }
             for (int v = 0; v < 2; ++v) {
                 LevelJob lo{};
                 lo.ref = F.vb.p;
@@ -2819,23 +2844,30 @@ void decompress_batch_impl(stz_ctx* c, int n, stz_dstream* const* ds, void* cons
                 P->high.h[v].push_back(hi);
             }
         }
+        if (rec_bytes) P->rec = DBuf(c->pool, rec_bytes);
         plan_decode_jobs<T>(c, *P);
-        jobs_finish(c, P->low);
-        jobs_finish(c, P->high);
+        const bool multi = !P->low.h[0].empty();
+        if (multi) {
+            jobs_finish(c, P->low);
+            jobs_finish(c, P->high);
+        }
         // A new set of streams runs eagerly: most batches are decoded once,
         // and a capture plus instantiation costs more host time than the
         // launches.  The first batch of a size class also times the level
         // geometries, the eager run's results being their inputs (a decode
         // is idempotent, so a candidate that runs again writes the same).
-        if (level_cfg_pending<T>(c, kLowDec, P->low) || level_cfg_pending<T>(c, kHighDec, P->high)) {
+        if (multi && (level_cfg_pending<T>(c, kLowDec, P->low) || level_cfg_pending<T>(c, kHighDec, P->high))) {
             plan_enqueue<T>(c, *P, false);
             P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
             jobs_launch<T>(c, kLowDec, P->low, P->cfg_low);
             P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
             jobs_launch<T>(c, kHighDec, P->high, P->cfg_high);
+            plan_staged<T>(c, *P);
         } else {
-            P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
-            P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
+            if (multi) {
+                P->cfg_low = level_cfg<T>(c, kLowDec, P->low, [] {});
+                P->cfg_high = level_cfg<T>(c, kHighDec, P->high, [] {});
+            }
             plan_enqueue<T>(c, *P, true);
         }
         c->plan = std::move(P);

@@ -52,6 +52,34 @@ def test_batch_bit_identical_to_single(stz, ctx, dt):
         d.free()
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_batch_decode_with_staged_streams(stz, ctx, dt):
+    """A batch decode mixing streams under the staged predictors (their
+    levels as stage launches inside the batch plan) with plain ones: eager,
+    captured and replayed decodes all equal one call per stream, bit for bit,
+    with eager work in between."""
+    import torch
+    dev = torch.device("cuda:0")
+    fs = _fields(dt, dev, seed=40)
+    eb = 1e-3 if dt == np.float32 else 1e-6
+    st = ctx.compress_batch(fs[:4], eb, "rel", pred=32)
+    pl = ctx.compress_batch(fs[4:], eb, "rel")
+    mixed = [st[0], pl[0], st[1], st[2], pl[1], st[3], pl[2]]
+    want = [d.decompress() for d in mixed]
+    outs = [torch.empty(w.shape, dtype=torch.float32 if dt == np.float32 else torch.float64, device=dev)
+            for w in want]
+    for rep in range(4):
+        for o in outs:
+            o.fill_(-7.0)
+        ctx.decompress_batch(mixed, outs)
+        for k, (o, w) in enumerate(zip(outs, want)):
+            assert np.array_equal(_bits(o), _bits(w)), (rep, k)
+        extra = st[1].decompress()
+        assert np.array_equal(_bits(extra), _bits(want[2]))
+    for d in st + pl:
+        d.free()
+
+
 def test_batch_autotune_and_new_plan(stz, ctx):
     import torch
     dev = torch.device("cuda:0")
