@@ -124,3 +124,17 @@ def test_container_passes_and_wrapped_sizes_refused(stz, oracle):
     bad = good.copy()
     bad[16 + 8 * 17 + 8 * 3: 16 + 8 * 17 + 8 * 4].view(np.uint64)[0] = np.uint64(1 << 40)
     assert _rc(stz, bad) == STZ_E_STREAM
+
+
+@pytest.mark.parametrize("shape", [(64, 128, 256), (37, 50, 70)])
+def test_every_base_codec_layout_matches_the_oracle(stz, oracle, shape):
+    """The library's host layout accepts the oracle's payload for every base
+    codec, including the Lorenzo/regression base whose tiles' header symbols
+    lengthen the base stream (on 64x128x256 they open a second 32768-symbol
+    base chunk: 64x32x16 points plus 8 tiles x 32 symbols)."""
+    rng = np.random.default_rng(5)
+    f = np.cumsum(rng.normal(size=shape), axis=2).astype(np.float32)
+    for interp in (0, 1, 2, 3):
+        p = oracle.compress(f, 1e-3, oracle.EB_REL, interp=interp)
+        assert oracle.header(p)["interp"] == interp
+        assert _rc(stz, p) == 0, interp
