@@ -491,6 +491,29 @@ def test_staged_predictor_requests_and_batch(ctx, oracle):
         ctx.compress_device(fs[1], 1e-3, "rel", pred=32 | 1)
 
 
+def test_repeated_compress_of_one_field(ctx, oracle):
+    """Compressing the same field again and again on one context (its
+    buffers come back from the pool, its layout from the cache): every
+    payload equals the oracle's, including after a different field and a
+    profiled call in between."""
+    import torch
+    dev = torch.device("cuda:0")
+    f = _field("lognormal", (64, 56, 72), np.float32, seed=5)
+    g = _field("grf", (40, 44, 36), np.float32, seed=6)
+    want_f = oracle.compress(f, 1e-3, oracle.EB_REL)
+    want_g = oracle.compress(g, 1e-3, oracle.EB_REL)
+    tf, tg = torch.from_numpy(f).to(dev), torch.from_numpy(g).to(dev)
+    for k, (t, want) in enumerate([(tf, want_f)] * 4 + [(tg, want_g)] + [(tf, want_f)] * 3):
+        _say("repeated compress", k)
+        if k == 6:
+            ctx.profile(True)
+        ds = ctx.compress_device(t, 1e-3, "rel")
+        if k == 6:
+            ctx.profile(False)
+        assert first_difference(ds.download(-1), want) is None, k
+        ds.free()
+
+
 # ------------------------------------------------------------------ Lorenzo base
 # interp = 2: the base grid in independent 16^3 Lorenzo tiles (SZ3's
 # ALGO_INTERP_LORENZO alternative); interp = 3: Lorenzo or a fitted plane per
