@@ -201,13 +201,17 @@ int stz_dstream_decompress_batch(stz_ctx* ctx, stz_dstream* ds, int n, const stz
                                  void* const* outs, size_t* out_dims);
 /* ---- many fields in one call ---------------------------------------------
  * Uneven fields of one dtype.  The residual levels of all fields run as one
- * persistent launch per level fed by a device work queue; its launch
+ * persistent launch per level fed by a device work queue (fields under
+ * STZ_PRED_STAGED: their own three stage launches per level); its launch
  * geometry (waves per workgroup, workgroups per CU) is timed per dtype,
  * level and size class the first time and kept in the context.  Streams and
- * decoded fields are bit-identical to one call per field.  A new set of
- * streams decodes eagerly; a second call naming the same streams and outputs
- * (device memory) captures the decode as a HIP graph, replayed from then on.  Replaces per-field runs of the
- * compress / decompress halves of run_typed (F5:2102-2451). */
+ * decoded fields are bit-identical to one call per field.  With params.tuner
+ * = 1 every field gets its own choice; the fields below the confidence
+ * threshold take the exhaustive trial together.  A new set of streams
+ * decodes eagerly; a second call naming the same streams and outputs (device
+ * memory) captures the decode as a HIP graph, replayed from then on.
+ * Replaces per-field runs of the compress / decompress halves of run_typed
+ * (F5:2102-2451). */
 int stz_compress_batch(stz_ctx* ctx, stz_dtype dtype, int n, const void* const* fields,
                        const size_t* dims /* 3 per field: nx ny nz */, double eb, stz_eb_mode mode,
                        const stz_params* params, stz_dstream** out);
